@@ -1,0 +1,145 @@
+/* lysh_cpu_bench — CPU baseline harness around the reference scanner (test infrastructure).
+ *
+ * Links the reference's own lysh objects (built by oracle/Makefile from engine/tools/lysh/src) and runs
+ * its phase-1 Seed Check exactly as `lysh scan` does for one seed: lysh_scan_rect (search.c:198) with
+ * threads = 1 and no phase-2 hook, which is the first pass of cmd_scan (main.c:139-216). Seeds come from
+ * the sequential SplitMix64 stream (Vigna) and are handed out one at a time to T worker threads, so a
+ * T-core host runs T seeds at once (lysh caps threads per seed at the number of rz rows, search.c:205-207).
+ *
+ *   lysh_cpu_bench --stream S --count N [--offset O] --rx0 A --rx1 B --rz0 C --rz1 D
+ *                  [--max-y M] [--version V] [--threads T] [--time-limit SEC]
+ *
+ * Prints one JSON line: seeds and region checks done, seconds, checks/s, and the summed funnel
+ * (reject kinds, main.c:96-100 order) and erosion tier histogram, which must equal the GPU's counters.
+ */
+#include <inttypes.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "search.h"
+#include "biome.h"
+#include "structure.h"
+
+static uint64_t g_stream, g_offset;
+static long long g_count;
+static int g_rx0, g_rx1, g_rz0, g_rz1, g_maxy, g_threads = 1;
+static double g_limit = 0.0;
+static lysh_phase1_opts g_opts;
+static long long g_next;
+static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;
+static double g_t0;
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + (double)ts.tv_nsec * 1e-9;
+}
+
+/* Sequential SplitMix64: the i-th output (i from 0) of the generator seeded with s. */
+static uint64_t splitmix64_at(uint64_t s, uint64_t i) {
+    uint64_t z = s + (i + 1) * UINT64_C(0x9E3779B97F4A7C15);
+    z = (z ^ (z >> 30)) * UINT64_C(0xBF58476D1CE4E5B9);
+    z = (z ^ (z >> 27)) * UINT64_C(0x94D049BB133111EB);
+    return z ^ (z >> 31);
+}
+
+typedef struct {
+    long long seeds, scanned, accepted, reached;
+    long long reject[LYSH_P1_REJECT_KIND_N];
+    long long tier[LYSH_TIER_MAX];
+} acc_t;
+
+static void *worker(void *arg) {
+    acc_t *a = (acc_t *)arg;
+    for (;;) {
+        long long i;
+        pthread_mutex_lock(&g_mu);
+        i = g_next;
+        if (i < g_count && (g_limit <= 0.0 || now_s() - g_t0 < g_limit)) g_next++;
+        else i = -1;
+        pthread_mutex_unlock(&g_mu);
+        if (i < 0) break;
+
+        lysh_scan_opts o;
+        memset(&o, 0, sizeof o);
+        o.seed = splitmix64_at(g_stream, g_offset + (uint64_t)i);
+        o.opts = g_opts;
+        o.max_height = g_maxy;
+        o.salt = LYSH_SWAMP_HUT_SALT;
+        o.threads = 1;
+        lysh_scan_result r;
+        if (lysh_scan_rect(&o, g_rx0, g_rx1, g_rz0, g_rz1, &r)) continue;
+        a->seeds++;
+        a->scanned += r.scanned;
+        a->accepted += r.accepted;
+        a->reached += r.reached_ladder;
+        for (int k = 0; k < LYSH_P1_REJECT_KIND_N; k++) a->reject[k] += r.reject_hist[k];
+        for (int k = 0; k < LYSH_TIER_MAX; k++) a->tier[k] += r.tier_hist[k];
+    }
+    return NULL;
+}
+
+/* Version name -> phase-1 flags: the game versions lysh_biome_version knows, then main.c:111-125's aliases. */
+static int set_version(const char *s) {
+    int tr, is118, pre;
+    if (!lysh_biome_version(s, &tr, &is118, &pre)) { g_opts.mc_1_18_2 = is118; g_opts.pre_26_2 = pre; g_opts.biome_tree = tr + 1; return 0; }
+    if (!strcmp(s, "1.18.x")) { g_opts.mc_1_18_2 = 1; g_opts.pre_26_2 = 1; return 0; }
+    if (!strcmp(s, "1.19.x") || !strcmp(s, "1.20.x") || !strcmp(s, "1.21.x~26.1") || !strcmp(s, "1.21.5~26.1")
+        || !strcmp(s, "1.19.x~26.1")) { g_opts.mc_1_18_2 = 0; g_opts.pre_26_2 = 1; return 0; }
+    if (!strcmp(s, "26.2~26.3")) { g_opts.mc_1_18_2 = 0; g_opts.pre_26_2 = 0; return 0; }
+    return 1;
+}
+
+int main(int argc, char **argv) {
+    g_maxy = -50;
+    g_rx0 = g_rz0 = -2; g_rx1 = g_rz1 = 2;
+    g_count = 1;
+    memset(&g_opts, 0, sizeof g_opts);
+    set_version("1.20");
+    for (int i = 1; i < argc; i++) {
+        const char *a = argv[i], *v = i + 1 < argc ? argv[i + 1] : NULL;
+        if (!v) { fprintf(stderr, "missing value for %s\n", a); return 2; }
+        i++;
+        if (!strcmp(a, "--stream")) g_stream = strtoull(v, NULL, 10);
+        else if (!strcmp(a, "--offset")) g_offset = strtoull(v, NULL, 10);
+        else if (!strcmp(a, "--count")) g_count = strtoll(v, NULL, 10);
+        else if (!strcmp(a, "--rx0")) g_rx0 = atoi(v);
+        else if (!strcmp(a, "--rx1")) g_rx1 = atoi(v);
+        else if (!strcmp(a, "--rz0")) g_rz0 = atoi(v);
+        else if (!strcmp(a, "--rz1")) g_rz1 = atoi(v);
+        else if (!strcmp(a, "--max-y")) g_maxy = atoi(v);
+        else if (!strcmp(a, "--threads")) g_threads = atoi(v);
+        else if (!strcmp(a, "--time-limit")) g_limit = atof(v);
+        else if (!strcmp(a, "--version")) { if (set_version(v)) { fprintf(stderr, "bad --version %s\n", v); return 2; } }
+        else { fprintf(stderr, "unknown option %s\n", a); return 2; }
+    }
+    if (g_maxy < -50) g_maxy = -50;   /* phase1_check_height, main.c:135-137 */
+    if (g_threads < 1) g_threads = 1;
+
+    acc_t *acc = calloc((size_t)g_threads, sizeof(acc_t));
+    pthread_t *th = calloc((size_t)g_threads, sizeof(pthread_t));
+    g_t0 = now_s();
+    for (int t = 0; t < g_threads; t++) pthread_create(&th[t], NULL, worker, &acc[t]);
+    acc_t s;
+    memset(&s, 0, sizeof s);
+    for (int t = 0; t < g_threads; t++) {
+        pthread_join(th[t], NULL);
+        s.seeds += acc[t].seeds; s.scanned += acc[t].scanned; s.accepted += acc[t].accepted; s.reached += acc[t].reached;
+        for (int k = 0; k < LYSH_P1_REJECT_KIND_N; k++) s.reject[k] += acc[t].reject[k];
+        for (int k = 0; k < LYSH_TIER_MAX; k++) s.tier[k] += acc[t].tier[k];
+    }
+    double secs = now_s() - g_t0;
+    printf("{\"seeds\": %lld, \"checks\": %lld, \"seconds\": %.6f, \"checks_per_s\": %.1f, \"threads\": %d, "
+           "\"accepted\": %lld, \"reached_ladder\": %lld, \"funnel\": [",
+           s.seeds, s.scanned, secs, secs > 0 ? (double)s.scanned / secs : 0.0, g_threads, s.accepted, s.reached);
+    for (int k = 0; k < LYSH_P1_REJECT_KIND_N; k++) printf("%s%lld", k ? ", " : "", s.reject[k]);
+    printf("], \"tiers\": [");
+    for (int k = 0; k < 7; k++) printf("%s%lld", k ? ", " : "", s.tier[k]);
+    printf("]}\n");
+    free(acc);
+    free(th);
+    return 0;
+}

@@ -1,0 +1,185 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libwhf_oracle.so) for the tests.
+
+The oracle is the checker: tests, the smoke check and the benchmark's CPU-baseline leg use it to decide
+whether the HIP path is right. Nothing in the product path imports this module.
+"""
+import ctypes
+import os
+import struct
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(REPO, "oracle", "_build", "libwhf_oracle.so")
+
+KIND_N = 11
+V_118, V_262, V_LARGE, V_SINGLE = 1, 2, 4, 8
+
+(N_TEMPERATURE, N_VEGETATION, N_CONTINENTALNESS, N_EROSION, N_TEMPERATURE_LARGE, N_VEGETATION_LARGE,
+ N_CONTINENTALNESS_LARGE, N_EROSION_LARGE, N_RIDGE, N_FLOODEDNESS, N_CAVE_ENTRANCE, N_SPAG3D_RARITY,
+ N_SPAG3D_THICKNESS, N_SPAG3D_1, N_SPAG3D_2, N_SPAG_ROUGH_MOD, N_SPAG_ROUGH, N_CAVE_LAYER, N_CAVE_CHEESE) = range(19)
+
+
+class Funnel(ctypes.Structure):
+    _fields_ = [("checks", ctypes.c_uint64), ("accepted", ctypes.c_uint64), ("reached_ladder", ctypes.c_uint64),
+                ("funnel", ctypes.c_uint64 * KIND_N), ("tiers", ctypes.c_uint64 * 8)]
+
+
+class Hit(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int64), ("rx", ctypes.c_int32), ("rz", ctypes.c_int32),
+                ("cx", ctypes.c_int32), ("cz", ctypes.c_int32)]
+
+
+class XR(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: run `make -C oracle oracle` (or __graft_entry__.build())")
+        L = ctypes.CDLL(ORACLE_SO)
+        u64, i64, i32, dbl = ctypes.c_uint64, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        L.wo_splitmix64_at.restype = u64
+        L.wo_splitmix64_at.argtypes = [u64, u64]
+        L.wo_scramble_index.restype = u64
+        L.wo_scramble_index.argtypes = [u64]
+        L.wo_md5.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p]
+        L.wo_xr_seed.argtypes = [ctypes.POINTER(XR), u64]
+        L.wo_xr_next.restype = u64
+        L.wo_xr_next.argtypes = [ctypes.POINTER(XR)]
+        L.wo_xr_next_int.restype = i32
+        L.wo_xr_next_int.argtypes = [ctypes.POINTER(XR), i32]
+        L.wo_xr_next_double.restype = dbl
+        L.wo_xr_next_double.argtypes = [ctypes.POINTER(XR)]
+        L.wo_lcg_next_int_seeded.restype = i32
+        L.wo_lcg_next_int_seeded.argtypes = [ctypes.POINTER(u64), i32]
+        L.wo_lcg_next_long.restype = u64
+        L.wo_lcg_next_long.argtypes = [ctypes.POINTER(u64)]
+        L.wo_hut_chunk.argtypes = [u64, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.wo_version_flags.restype = i32
+        L.wo_version_flags.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]
+        L.wo_scan.restype = i32
+        L.wo_scan.argtypes = [ctypes.POINTER(i64), i64, i32, i32, i32, i32, i32, i32, i32, ctypes.POINTER(Funnel),
+                              ctypes.POINTER(Hit), i64, ctypes.POINTER(i64)]
+        L.wo_scan_regions.restype = i32
+        L.wo_scan_regions.argtypes = [i64, i32, i32, i32, i32, i32, i32, ctypes.c_void_p, ctypes.c_void_p]
+        L.wo_sampler_fingerprint.restype = u64
+        L.wo_sampler_fingerprint.argtypes = [u64, i32, i32, i32]
+        L.wo_probe_alloc.restype = ctypes.c_void_p
+        L.wo_probe_alloc.argtypes = [u64, i32]
+        L.wo_probe_free.argtypes = [ctypes.c_void_p]
+        L.wo_phase1_probe.argtypes = [ctypes.c_void_p, i32, i32, i32, ctypes.POINTER(dbl)]
+        L.wo_noise_alloc.restype = ctypes.c_void_p
+        L.wo_noise_alloc.argtypes = [u64, i32]
+        L.wo_noise_amplitude.restype = dbl
+        L.wo_noise_amplitude.argtypes = [ctypes.c_void_p]
+        L.wo_noise_sample.restype = dbl
+        L.wo_noise_sample.argtypes = [ctypes.c_void_p, dbl, dbl, dbl]
+        L.wo_noise_sample_y0.restype = dbl
+        L.wo_noise_sample_y0.argtypes = [ctypes.c_void_p, dbl, dbl]
+        L.wo_erosion_tiered.restype = i32
+        L.wo_erosion_tiered.argtypes = [ctypes.c_void_p, dbl, dbl, ctypes.POINTER(dbl), ctypes.POINTER(i32)]
+        L.wo_noise_perm.argtypes = [ctypes.c_void_p, i32, i32, ctypes.c_char_p]
+        _lib = L
+    return _lib
+
+
+def u64(v):
+    return v & 0xFFFFFFFFFFFFFFFF
+
+
+def s64(v):
+    v &= 0xFFFFFFFFFFFFFFFF
+    return v - (1 << 64) if v >= 1 << 63 else v
+
+
+def splitmix(stream, index):
+    return lib().wo_splitmix64_at(u64(stream), index)
+
+
+def seeds_splitmix(stream, count, offset=0):
+    return np.array([s64(splitmix(stream, offset + i)) for i in range(count)], dtype=np.int64)
+
+
+def version_flags(name):
+    v = ctypes.c_int(0)
+    if lib().wo_version_flags(name.encode(), ctypes.byref(v)):
+        raise ValueError("unknown version %r" % name)
+    return v.value
+
+
+def scan(seeds, rect, max_y=-50, vflags=0, threads=8, cap=1 << 16):
+    seeds = np.ascontiguousarray(seeds, dtype=np.int64)
+    f = Funnel()
+    hits = (Hit * cap)()
+    nh = ctypes.c_int64(0)
+    rc = lib().wo_scan(seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(seeds), rect[0], rect[1], rect[2],
+                       rect[3], max_y, vflags, threads, ctypes.byref(f), hits, cap, ctypes.byref(nh))
+    assert rc == 0
+    out = [(hits[i].seed, hits[i].rx, hits[i].rz, hits[i].cx, hits[i].cz) for i in range(min(nh.value, cap))]
+    return dict(checks=f.checks, accepted=f.accepted, reached_ladder=f.reached_ladder, funnel=list(f.funnel),
+                tiers=list(f.tiers)[:7], hits=sorted(out))
+
+
+def scan_regions(seed, rect, max_y=-50, vflags=0):
+    n = (rect[1] - rect[0]) * (rect[3] - rect[2])
+    kind = np.zeros(n, np.int8)
+    tier = np.zeros(n, np.int8)
+    assert lib().wo_scan_regions(s64(seed), rect[0], rect[1], rect[2], rect[3], max_y, vflags,
+                                 kind.ctypes.data, tier.ctypes.data) == 0
+    return kind, tier
+
+
+def hut_chunk(seed, rx, rz, salt=14357620):
+    cx, cz = ctypes.c_int(0), ctypes.c_int(0)
+    lib().wo_hut_chunk(u64(seed), rx, rz, salt, ctypes.byref(cx), ctypes.byref(cz))
+    return cx.value, cz.value
+
+
+def probe(seed, x, z, max_y, vflags=0):
+    L = lib()
+    h = L.wo_probe_alloc(u64(seed), vflags)
+    out = (ctypes.c_double * 38)()
+    L.wo_phase1_probe(h, x, z, max_y, out)
+    L.wo_probe_free(h)
+    return [struct.unpack("<Q", struct.pack("<d", v))[0] for v in out]
+
+
+class Noise:
+    def __init__(self, seed, noise_id):
+        self.h = lib().wo_noise_alloc(u64(seed), noise_id)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().wo_probe_free(self.h)
+
+    @property
+    def amplitude(self):
+        return lib().wo_noise_amplitude(self.h)
+
+    def sample(self, x, y, z):
+        return lib().wo_noise_sample(self.h, x, y, z)
+
+    def sample_y0(self, x, z):
+        return lib().wo_noise_sample_y0(self.h, x, z)
+
+    def tiered(self, x, z):
+        v, t = ctypes.c_double(0), ctypes.c_int(0)
+        fired = lib().wo_erosion_tiered(self.h, x, z, ctypes.byref(v), ctypes.byref(t))
+        return fired, t.value, v.value
+
+    def perm(self, sub, slot):
+        buf = ctypes.create_string_buffer(256)
+        lib().wo_noise_perm(self.h, sub, slot, buf)
+        return bytes(buf.raw)
+
+
+def md5(data):
+    out = ctypes.create_string_buffer(16)
+    lib().wo_md5(data, len(data), out)
+    return out.raw.hex()
