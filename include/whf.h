@@ -1,0 +1,125 @@
+/* whf.h — C ABI of the MI355X witch-hut Seed Check library (libwhf_hip.so).
+ *
+ * Plain C types only: callers own every input array and output buffer, the library owns device memory,
+ * streams and kernels. Errors are negative return codes; the library never exits the process.
+ *
+ * Each entry point replaces a piece of the reference's GPU engine (engine/gpu/low54/phase1_scan.cu) or of
+ * its CPU scanner (engine/tools/lysh/src), cited per function. The reference has no C ABI for this path:
+ * hunt.py drives a `phase1_scan` process over argv/stdin (phase1_scan.cu:561-601, 1365-1387); this library
+ * is what that process (low-witch-finder_amd/host/phase1_scan.cpp) now calls.
+ */
+#ifndef WHF_H
+#define WHF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WHF_ABI_VERSION 1
+
+/* Return codes. */
+#define WHF_OK          0
+#define WHF_EINVAL     (-1)   /* bad argument: empty rect, null pointer, unknown mode */
+#define WHF_ENOMEM     (-2)   /* device or host allocation failed, or the request exceeds the HBM budget */
+#define WHF_EDEVICE    (-3)   /* a HIP call failed (no device, launch error, fault) */
+#define WHF_EOVERFLOW  (-4)   /* more survivors than the sink holds; counters are still complete */
+#define WHF_ENOTSUP    (-5)   /* option not supported by this build */
+
+/* Seed sources of a request. */
+#define WHF_SEEDS_LIST      0   /* seeds[0..n_seeds) as given */
+#define WHF_SEEDS_SPLITMIX  1   /* i-th output (offset + i) of sequential SplitMix64 seeded with `stream` (Vigna) */
+#define WHF_SEEDS_COUNTER   2   /* SplitMix64 finalizer of (stream + offset + i + 0x9E3779B97F4A7C15): the reference's
+                                   --seed-gen base count form (device_construct.h:39-46, phase1_scan.cu:87) */
+
+/* Phase-1 rule flags: which game-version rules apply (phase1.h:36-48). */
+#define WHF_V_118      1   /* 1.18.x: temperature only has its lower bound -0.45 */
+#define WHF_V_262      2   /* 26.2+: ridge <= -0.91 also rejects */
+#define WHF_V_LARGE    4   /* LARGE_BIOMES preset: the _large climate noises */
+#define WHF_V_SINGLE   8   /* SINGLE_BIOME preset: climate gates skipped */
+
+/* Arithmetic of the noise bulk. */
+#define WHF_PREC_FP64      0   /* plain FP64 path, bit-exact with the reference CPU scanner */
+
+typedef struct whf_ctx whf_ctx;
+
+typedef struct {
+    int device;                 /* HIP device ordinal */
+    int flags;                  /* reserved, 0 */
+    uint64_t hbm_budget;        /* bytes of device memory the context may hold; 0 = 8 GiB */
+} whf_config;
+
+typedef struct {
+    int seed_mode;              /* WHF_SEEDS_* */
+    int precision;              /* WHF_PREC_* */
+    const int64_t *seeds;       /* WHF_SEEDS_LIST: host array of n_seeds world seeds */
+    uint64_t n_seeds;
+    uint64_t stream, offset;    /* WHF_SEEDS_SPLITMIX / _COUNTER */
+    int rx0, rx1, rz0, rz1;     /* hut regions, half-open [rx0,rx1) x [rz0,rz1) */
+    int max_height;             /* phase-1 maxHeight (lysh_phase1_check_ex's max_height, used as given) */
+    int vflags;                 /* WHF_V_* */
+    int salt;                   /* structure salt; 0 = swamp hut 14357620 */
+    int reserved;
+} whf_request;
+
+/* One phase-1 survivor: a hut position that passed every phase-1 gate. */
+typedef struct {
+    int64_t seed;
+    int32_t rx, rz;             /* region */
+    int32_t cx, cz;             /* hut chunk; the hut's block is (16*cx, 16*cz) */
+    int32_t floor_y;            /* not computed by phase 1: INT32_MIN */
+    int32_t flags;              /* reserved, 0 */
+} whf_hit;
+
+typedef struct {
+    whf_hit *hits;              /* caller-owned array */
+    uint64_t capacity;          /* entries in hits */
+    uint64_t count;             /* out: survivors found (may exceed capacity -> WHF_EOVERFLOW) */
+} whf_sink;
+
+/* Counters of one scan; the funnel follows lysh's reject kinds (phase1.h:53-66, main.c:96-100). */
+#define WHF_KIND_N 11
+typedef struct {
+    uint64_t seeds;
+    uint64_t checks;            /* hut-region checks (seeds x regions) */
+    uint64_t accepted;          /* phase-1 survivors */
+    uint64_t reached_ladder;    /* regions past the climate gates (lysh scan's "reached cave ladder") */
+    uint64_t funnel[WHF_KIND_N];/* per reject kind; [10] = accepted */
+    uint64_t tiers[8];          /* erosion tier where the tiered gate fired (noise.c:362-377); [7] unused */
+    uint64_t samplers_built;    /* Perlin samplers constructed on the device */
+    double device_ms;           /* device time of the scan (HIP events around the kernels) */
+    double kernel_ms[8];        /* per kernel family: seeds, erosion build, erosion gate, compact, late build, late gate */
+} whf_stats;
+
+/* Opens a context on a device and allocates its working buffers (replaces phase1_scan's allocateBuffers,
+ * phase1_scan.cu:820-868). */
+int whf_open(const whf_config *cfg, whf_ctx **out);
+
+/* Releases everything the context holds (replaces freeBuffers, phase1_scan.cu:870-879). */
+void whf_close(whf_ctx *ctx);
+
+/* Synchronous Seed Check, one request per call: the plain FP64 drop-in. Scans every (seed, region) of the
+ * request through the full phase-1 chain of the reference CPU scanner (lysh_scan_rect search.c:198 ->
+ * lysh_phase1_check_ex phase1.c:130-200) and returns the survivors and the funnel. Replaces runScan
+ * (phase1_scan.cu:881-1345), whose candidates were a superset of these. Not re-entrant per context. */
+int whf_scan(whf_ctx *ctx, const whf_request *req, whf_sink *sink, whf_stats *stats);
+
+/* Raw phase-1 values at hut positions: the 38 doubles of lysh_phase1_probe (phase1.c:39-80) for each
+ * (seeds[i], hut_x[i], hut_z[i]), computed by the device kernels; out has 38*n doubles. For parity checks. */
+int whf_probe(whf_ctx *ctx, const int64_t *seeds, const int32_t *hut_x, const int32_t *hut_z, uint64_t n,
+              int max_height, int vflags, double *out);
+
+/* Version name -> WHF_V_* flags, accepting lysh's version names (main.c:104-125, biome_trees.h:70499-70526). */
+int whf_version_flags(const char *name, int *vflags);
+
+/* Text of a return code. */
+const char *whf_strerror(int code);
+
+/* Name of the device the context runs on (for logs), or "" when unknown. */
+const char *whf_device_name(whf_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WHF_H */

@@ -1,0 +1,232 @@
+// whf_device.h — device-side building blocks of the witch-hut Seed Check for gfx950 (CDNA4, wave64).
+//
+// Integer parts (SplitMix64, Xoroshiro128++, java.util.Random placement) are exact by construction. The
+// noise parts are IEEE binary64 and must reproduce the reference CPU scanner bit for bit, so this code is
+// compiled with -ffp-contract=off and keeps the reference's association order in every expression
+// (lysh noise.c:36-58, 224-342; caves.c:58-130; mcmath.h), including x / s as a true division.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace whf {
+
+// ---------------------------------------------------------------------------------------------- seeds
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+
+// ---------------------------------------------------------------------------------------------- Xoroshiro128++
+// The game's XoroshiroRandomSource (lysh rng.c:75-181).
+
+struct Xr {
+    uint64_t lo, hi;
+};
+
+__device__ __forceinline__ uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+__device__ __forceinline__ Xr xr_make(uint64_t lo, uint64_t hi) {
+    if ((lo | hi) == 0) { lo = kGolden; hi = 0x6A09E667F3BCC909ull; }
+    return Xr{lo, hi};
+}
+
+__device__ __forceinline__ Xr xr_seed(uint64_t seed) {
+    uint64_t l = seed ^ 0x6A09E667F3BCC909ull;
+    return Xr{mix64(l), mix64(l - 0x61C8864680B583EBull)};
+}
+
+__device__ __forceinline__ uint64_t xr_next(Xr &s) {
+    uint64_t lo = s.lo, hi = s.hi;
+    uint64_t out = rotl(lo + hi, 17) + lo;
+    hi ^= lo;
+    s.lo = rotl(lo, 49) ^ hi ^ (hi << 21);
+    s.hi = rotl(hi, 28);
+    return out;
+}
+
+// Java 17 Random.nextInt(bound) over a 64-bit source: Lemire multiply-shift with rejection (rng.c:124-146).
+__device__ __forceinline__ uint32_t xr_next_bounded(Xr &s, uint32_t bound) {
+    uint64_t m = (xr_next(s) & 0xFFFFFFFFull) * (uint64_t)bound;
+    if ((uint32_t)m < bound) {
+        uint32_t lim = (0u - bound) % bound;
+        while ((uint32_t)m < lim) m = (xr_next(s) & 0xFFFFFFFFull) * (uint64_t)bound;
+    }
+    return (uint32_t)(m >> 32);
+}
+
+__device__ __forceinline__ double xr_next_double(Xr &s) {
+    return (double)(xr_next(s) >> 11) * 0x1.0p-53;
+}
+
+// createRandomDeriver(): two longs of the parent (rng.c:157-161).
+__device__ __forceinline__ Xr xr_fork(Xr &parent) {
+    uint64_t a = xr_next(parent);
+    uint64_t b = xr_next(parent);
+    return xr_make(a, b);
+}
+
+// ---------------------------------------------------------------------------------------------- placement
+// java.util.Random region placement of the swamp hut (structure.c:4-24, rng.c:187-227).
+
+constexpr uint64_t kLcgMul = 0x5DEECE66Dull;
+constexpr uint64_t kLcgM48 = 0xFFFFFFFFFFFFull;
+
+__device__ __forceinline__ int lcg_next31(uint64_t &st) {
+    st = (st * kLcgMul + 0xBull) & kLcgM48;
+    return (int)(st >> 17);
+}
+
+__device__ __forceinline__ int lcg_int24(uint64_t &st) {
+    for (;;) {
+        int bits = lcg_next31(st);
+        int val = bits % 24;
+        if ((int32_t)((uint32_t)bits - (uint32_t)val + 23u) >= 0) return val;
+    }
+}
+
+__device__ __forceinline__ void hut_chunk(uint64_t seed, int rx, int rz, int salt, int &cx, int &cz) {
+    uint64_t s = (uint64_t)((int64_t)rx * 341873128712ll) + (uint64_t)((int64_t)rz * 132897987541ll) + seed +
+                 (uint64_t)(int64_t)salt;
+    uint64_t st = (s ^ kLcgMul) & kLcgM48;
+    int j = lcg_int24(st);
+    int k = lcg_int24(st);
+    cx = rx * 32 + j;
+    cz = rz * 32 + k;
+}
+
+// ---------------------------------------------------------------------------------------------- Perlin
+// A sampler is a 256-byte permutation plus its origin; y-constants for the y = 0 climate path are
+// precomputed at build time exactly as the reference does (noise.c:73-77).
+
+struct SPar {
+    double ox, oy, oz;
+    double hy, hy1, fy;   // oy - floor(oy), hy - 1, fade(hy)
+    int jy;               // (int)floor(oy)
+    int pad;
+};
+static_assert(sizeof(SPar) == 56, "SPar layout");
+
+// Gradient table of the game's improved noise (noise.c:15-20) as bit masks over the 4-bit hash: for each
+// axis, which hashes give -1 and which give 0. Built at compile time from the table itself.
+struct GradMasks {
+    uint32_t neg[3], zero[3];
+};
+constexpr GradMasks make_grad_masks() {
+    const int g[16][3] = {{1, 1, 0}, {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1}, {-1, 0, 1},
+                          {1, 0, -1}, {-1, 0, -1}, {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1},
+                          {1, 1, 0}, {0, -1, 1}, {-1, 1, 0}, {0, -1, -1}};
+    GradMasks m{{0, 0, 0}, {0, 0, 0}};
+    for (int h = 0; h < 16; h++)
+        for (int a = 0; a < 3; a++) {
+            if (g[h][a] < 0) m.neg[a] |= 1u << h;
+            if (g[h][a] == 0) m.zero[a] |= 1u << h;
+        }
+    return m;
+}
+constexpr GradMasks kGrad = make_grad_masks();
+
+// The gradient component as a double (-1.0, 0.0 or 1.0) built from its bit pattern, so g * x below is the
+// reference's (double)g[a] * x exactly, signed zeros included.
+__device__ __forceinline__ double grad_comp(uint32_t h, uint32_t neg, uint32_t zero) {
+    uint32_t hi = (((zero >> h) & 1u) ? 0u : 0x3FF00000u) | (((neg >> h) & 1u) << 31);
+    return __hiloint2double((int)hi, 0);
+}
+
+__device__ __forceinline__ double grad(int hash, double x, double y, double z) {
+    uint32_t h = (uint32_t)hash & 15u;
+    return grad_comp(h, kGrad.neg[0], kGrad.zero[0]) * x + grad_comp(h, kGrad.neg[1], kGrad.zero[1]) * y +
+           grad_comp(h, kGrad.neg[2], kGrad.zero[2]) * z;
+}
+
+__device__ __forceinline__ double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
+__device__ __forceinline__ double lerp(double t, double a, double b) { return a + t * (b - a); }
+
+// Corner blend shared by the y = 0 and 3D samplers (noise.c:224-244): P is the permutation, (i, j, k) the
+// lattice cell, (x, y, z) the fractional position and ty the y value whose fade is used.
+template <typename Perm>
+__device__ __forceinline__ double perlin_cell(const Perm &P, int i, int j, int k, double x, double y, double z,
+                                              double fy) {
+    int a = P(i), b = P(i + 1);
+    int c = P(a + j), d = P(a + j + 1), e = P(b + j), f = P(b + j + 1);
+    double x1 = x - 1, y1 = y - 1, z1 = z - 1;
+    double v000 = grad(P(c + k), x, y, z);
+    double v100 = grad(P(e + k), x1, y, z);
+    double v010 = grad(P(d + k), x, y1, z);
+    double v110 = grad(P(f + k), x1, y1, z);
+    double v001 = grad(P(c + k + 1), x, y, z1);
+    double v101 = grad(P(e + k + 1), x1, y, z1);
+    double v011 = grad(P(d + k + 1), x, y1, z1);
+    double v111 = grad(P(f + k + 1), x1, y1, z1);
+    double fx = fade(x), fz = fade(z);
+    double lo = lerp(fy, lerp(fx, v000, v100), lerp(fx, v010, v110));
+    double hi = lerp(fy, lerp(fx, v001, v101), lerp(fx, v011, v111));
+    return lerp(fz, lo, hi);
+}
+
+// y = 0 sample with the precomputed y constants (noise.c:292-318).
+template <typename Perm>
+__device__ __forceinline__ double perlin_y0(const Perm &P, const SPar &s, double x, double z) {
+    double d = x + s.ox, f = z + s.oz;
+    double fi = floor(d), fk = floor(f);
+    int i = (int)fi, k = (int)fk;
+    double g = d - fi, l = f - fk;
+    // The y-row uses hy and hy - 1 as its two corner offsets and fade(hy) as its weight.
+    int a = P(i), b = P(i + 1);
+    int j = s.jy;
+    int c = P(a + j), dd = P(a + j + 1), e = P(b + j), ff = P(b + j + 1);
+    double h = s.hy, h1 = s.hy1, g1 = g - 1, l1 = l - 1;
+    double v000 = grad(P(c + k), g, h, l);
+    double v100 = grad(P(e + k), g1, h, l);
+    double v010 = grad(P(dd + k), g, h1, l);
+    double v110 = grad(P(ff + k), g1, h1, l);
+    double v001 = grad(P(c + k + 1), g, h, l1);
+    double v101 = grad(P(e + k + 1), g1, h, l1);
+    double v011 = grad(P(dd + k + 1), g, h1, l1);
+    double v111 = grad(P(ff + k + 1), g1, h1, l1);
+    double fx = fade(g), fz = fade(l), fy = s.fy;
+    double lo = lerp(fy, lerp(fx, v000, v100), lerp(fx, v010, v110));
+    double hi = lerp(fy, lerp(fx, v001, v101), lerp(fx, v011, v111));
+    return lerp(fz, lo, hi);
+}
+
+// 3D sample with yScale = yMax = 0 (noise.c:246-261).
+template <typename Perm>
+__device__ __forceinline__ double perlin_3d(const Perm &P, const SPar &s, double x, double y, double z) {
+    double d = x + s.ox, e = y + s.oy, f = z + s.oz;
+    double fi = floor(d), fj = floor(e), fk = floor(f);
+    double h = e - fj;
+    return perlin_cell(P, (int)fi, (int)fj, (int)fk, d - fi, h, f - fk, fade(h));
+}
+
+// OctavePerlinNoiseSampler.maintainPrecision (noise.c:28-34): identity inside +-1e7.
+__device__ __forceinline__ double keep_precision(double v) {
+    if (v > -1.0e7 && v < 1.0e7) return v;
+    return v - floor(v * 2.9802322387695312E-08 + 0.5) * 3.3554432E7;
+}
+
+// ---------------------------------------------------------------------------------------------- Java math
+// Math.min / Math.max / MathHelper.clamp with Java's NaN and signed-zero rules (mcmath.h).
+
+__device__ __forceinline__ bool is_neg_zero(double v) {
+    return __double_as_longlong(v) == (long long)0x8000000000000000ull;
+}
+__device__ __forceinline__ double jmax(double a, double b) {
+    if (a != a) return a;
+    if (a == 0.0 && b == 0.0 && is_neg_zero(a)) return b;
+    return a >= b ? a : b;
+}
+__device__ __forceinline__ double jmin(double a, double b) {
+    if (a != a) return a;
+    if (a == 0.0 && b == 0.0 && is_neg_zero(b)) return b;
+    return a <= b ? a : b;
+}
+__device__ __forceinline__ double jclamp(double v, double lo, double hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+}  // namespace whf

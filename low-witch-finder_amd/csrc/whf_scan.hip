@@ -1,0 +1,1051 @@
+// whf_scan.hip — the MI355X (gfx950) witch-hut phase-1 Seed Check: kernels and the C ABI (include/whf.h).
+//
+// Pipeline per batch of seeds x regions (one HIP stream, no host round trip inside a scan):
+//   k_seeds      seed values of the batch (SplitMix64 / counter form on the device, or an uploaded list)
+//   k_build      Perlin samplers, one lane per (seed slot, sampler): Xoroshiro derivation + Fisher-Yates in LDS,
+//                tables and origins written to HBM. Run for set E (all seeds), set C (seeds with erosion
+//                survivors) and set L (seeds with Entrance@50/60 survivors).
+//   k_erosion    placement + tiered erosion per (seed, region) with the seed tile's set-E tables in LDS; a
+//                per-lane work queue keeps lanes busy through the tier early exits; survivors -> queue Q1.
+//   k_compact    seeds flagged by a gate -> dense slot list for the next build.
+//   k_gate2      temperature, ridge, Entrance@50, Entrance@60 per Q1 entry; survivors -> Q2.
+//   k_gate3      cave@maxHeight, both ladders, continentalness, floodedness per Q2 entry; survivors -> hits.
+// Every decision follows lysh_phase1_check_ex (engine/tools/lysh/src/phase1.c:130-200) bit for bit, so the
+// survivors and the funnel counters equal the reference CPU scanner's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/whf.h"
+#include "whf_kernels.h"
+#include "whf_tables.h"
+
+using namespace whf;
+
+// ================================================================================================ device
+
+namespace {
+
+constexpr int kWave = 64;
+
+// Device control words of a batch.
+enum Ctl { CTL_Q1 = 0, CTL_Q2, CTL_NC, CTL_NL, CTL_HITS, CTL_N };
+
+// Device counters of a scan (u64).
+enum Cnt {
+    CNT_TIER0 = 0,                       // 0..6 tiers
+    CNT_KIND0 = 8,                       // 8 + kind, kinds 1..10 (kind 0 = sum of tiers)
+    CNT_LADDER = 20,
+    CNT_BUILT = 21,
+    CNT_N = 24
+};
+
+struct DevHit {
+    int64_t seed;
+    int32_t rx, rz, cx, cz, floor_y, flags;
+};
+static_assert(sizeof(DevHit) == sizeof(whf_hit), "hit layout");
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-aggregated append of `pred` lanes to a queue: returns this lane's slot (valid where pred).
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter) {
+    unsigned long long mask = __ballot(pred);
+    uint32_t base = 0;
+    if (mask) {
+        int leader = __ffsll((long long)mask) - 1;
+        if ((int)lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+    }
+    unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
+    return base + (uint32_t)__popcll(below);
+}
+
+// ---------------------------------------------------------------------------------------------- seeds
+
+__global__ void k_seeds(int mode, uint64_t stream, uint64_t offset, uint32_t n, uint64_t *out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t idx = offset + i;
+    out[i] = mode == WHF_SEEDS_SPLITMIX ? mix64(stream + (idx + 1) * kGolden) : mix64(stream + idx + kGolden);
+}
+
+// Zero the per-batch queue counts and gate flags (a fill kernel, so the batch stays capturable). The hit
+// count is not reset: hits of all batches of a scan accumulate in one buffer.
+__global__ void k_reset(uint32_t *ctl, uint32_t *needC, uint32_t *needL, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < CTL_HITS) ctl[i] = 0;
+    if (i < n) { needC[i] = 0; needL[i] = 0; }
+}
+
+// ---------------------------------------------------------------------------------------------- build
+// One lane per (slot, sampler). The permutation is shuffled in LDS in a lane-interleaved layout (word w of
+// lane l at word w * 64 + l), so every byte access of the 64 lanes hits 64 distinct banks whatever the
+// random swap index. Job j writes table j of the output (slot-major, set_n tables per slot).
+
+__global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds, const BuildDesc *__restrict__ bd,
+                                              int set_n, const uint32_t *__restrict__ slot_seed,
+                                              const uint32_t *__restrict__ n_slots_dev, uint32_t n_slots_host,
+                                              uint8_t *__restrict__ out_tab, SPar *__restrict__ out_par,
+                                              unsigned long long *__restrict__ cnt) {
+    __shared__ uint32_t words[64 * 64];
+    uint8_t *bytes = reinterpret_cast<uint8_t *>(words);
+    const uint32_t lane = threadIdx.x;
+    const uint32_t n_slots = n_slots_dev ? *n_slots_dev : n_slots_host;
+    const uint64_t n_jobs = (uint64_t)n_slots * (uint32_t)set_n;
+    uint32_t built = 0;
+    for (uint64_t j0 = (uint64_t)blockIdx.x * 64; j0 < n_jobs; j0 += (uint64_t)gridDim.x * 64) {
+        const uint64_t j = j0 + lane;
+        if (j >= n_jobs) break;
+        const uint32_t slot = (uint32_t)(j / (uint32_t)set_n);
+        const int sid = (int)(j - (uint64_t)slot * (uint32_t)set_n);
+        const uint64_t seed = seeds[slot_seed ? slot_seed[slot] : slot];
+        const BuildDesc d = bd[sid];
+
+        // worldSeed -> deriver -> createRandom(identifier) -> octave deriver (second one for sub 1)
+        // -> createRandom("octave_<n>")  (noise.c:80-143, rng.c:157-170)
+        Xr r = xr_seed(seed);
+        Xr der = xr_fork(r);
+        Xr nr = xr_make(d.na ^ der.lo, d.nb ^ der.hi);
+        if (d.sub) { xr_next(nr); xr_next(nr); }
+        Xr od = xr_fork(nr);
+        Xr pr = xr_make(d.oa ^ od.lo, d.ob ^ od.hi);
+
+        // PerlinNoiseSampler(random): origin, then Fisher-Yates over 0..255 (noise.c:61-78)
+        const double ox = xr_next_double(pr) * 256.0;
+        const double oy = xr_next_double(pr) * 256.0;
+        const double oz = xr_next_double(pr) * 256.0;
+#pragma unroll 8
+        for (int w = 0; w < 64; w++) words[w * 64 + lane] = 0x03020100u + (uint32_t)w * 0x04040404u;
+        uint32_t ai = lane * 4;   // byte address of position i (i = 0)
+        for (int i = 0; i < 256; i++) {
+            const uint32_t jj = (uint32_t)i + xr_next_bounded(pr, 256u - (uint32_t)i);
+            const uint32_t aj = ((jj >> 2) << 8) + lane * 4 + (jj & 3);
+            const uint8_t vi = bytes[ai];
+            const uint8_t vj = bytes[aj];
+            bytes[ai] = vj;
+            bytes[aj] = vi;
+            ai += ((i & 3) == 3) ? 256 - 3 : 1;
+        }
+        uint4 *dst = reinterpret_cast<uint4 *>(out_tab + j * 256);
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            dst[q] = make_uint4(words[(4 * q) * 64 + lane], words[(4 * q + 1) * 64 + lane],
+                                words[(4 * q + 2) * 64 + lane], words[(4 * q + 3) * 64 + lane]);
+        SPar sp;
+        sp.ox = ox;
+        sp.oy = oy;
+        sp.oz = oz;
+        const double fj = floor(oy);
+        sp.jy = (int)fj;
+        sp.hy = oy - fj;
+        sp.hy1 = sp.hy - 1.0;
+        sp.fy = fade(sp.hy);
+        sp.pad = 0;
+        out_par[j] = sp;
+        built++;
+    }
+    // one atomic per wave
+    unsigned long long tot = built;
+    for (int o = 32; o > 0; o >>= 1) tot += __shfl_down(tot, o);
+    if (lane == 0 && tot) atomicAdd(&cnt[CNT_BUILT], tot);
+}
+
+// ---------------------------------------------------------------------------------------------- erosion gate
+
+struct LPerm {
+    const uint8_t *p;
+    __device__ __forceinline__ int operator()(int i) const { return p[i & 255]; }
+};
+
+constexpr int kEroWG = 256;
+constexpr int kEroItems = 1024;         // items placed per pass (4 per lane)
+constexpr int kCzBias = 1 << 22;        // chunk z is stored biased so that 4 tile bits fit above it
+
+// ST seeds per workgroup. Items of a pass are (seed-in-tile, region) pairs laid out seed-major.
+template <int ST>
+__global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__ seeds, uint32_t n_seeds,
+                                                    const uint8_t *__restrict__ tabE, const SPar *__restrict__ parE,
+                                                    const GateTables *__restrict__ gt, int rx0, int rz0, uint32_t W,
+                                                    uint64_t flat0, uint32_t flat_n, uint32_t chunk,
+                                                    uint32_t n_chunks, int salt, int single,
+                                                    uint4 *__restrict__ q1, uint32_t *__restrict__ ctl,
+                                                    uint32_t *__restrict__ needC,
+                                                    unsigned long long *__restrict__ cnt) {
+    __shared__ uint4 s_tab[ST * 8 * 16];
+    __shared__ SPar s_par[ST * 8];
+    __shared__ int2 s_item[kEroItems];
+    __shared__ double s_sc[8], s_w[8], s_thr[8];
+    __shared__ uint32_t s_head;
+    __shared__ uint32_t s_cnt[8];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t tile = blockIdx.x / n_chunks;
+    const uint32_t ck = blockIdx.x - tile * n_chunks;
+    const uint32_t seed0 = tile * ST;
+    const uint32_t ns = min((uint32_t)ST, n_seeds - seed0);
+    const uint32_t rbase = ck * chunk;
+    const uint32_t rn = min(chunk, flat_n - rbase);
+
+    {   // stage the tile's erosion tables and origins into LDS
+        const uint4 *src = reinterpret_cast<const uint4 *>(tabE + (size_t)seed0 * 8 * 256);
+        for (uint32_t w = tid; w < ns * 8 * 16; w += kEroWG) s_tab[w] = src[w];
+        for (uint32_t w = tid; w < ns * 8; w += kEroWG) s_par[w] = parE[(size_t)seed0 * 8 + w];
+        if (tid < 8) {
+            s_sc[tid] = gt->ero.sc[tid] * 1.0;
+            s_w[tid] = gt->ero.w[tid];
+            s_thr[tid] = tid >= 1 ? gt->ero.thr[tid - 1] : 0.0;
+            s_cnt[tid] = 0;
+        }
+    }
+    const double amplitude = gt->ero.amplitude;
+    // lacunarity of evaluation k: sc[k] is 1 or 1.0181268882175227, lac doubles per slot
+    __shared__ double s_lac[8];
+    if (tid < 8) s_lac[tid] = gt->ero.lac[tid];
+    __syncthreads();
+
+    const uint8_t *tabb = reinterpret_cast<const uint8_t *>(s_tab);
+    const uint64_t total = (uint64_t)ns * rn;
+
+    for (uint64_t pass0 = 0; pass0 < total; pass0 += kEroItems) {
+        const uint32_t npass = (uint32_t)min((uint64_t)kEroItems, total - pass0);
+        // ---- placement of the pass's items (uniform work) ----
+        for (uint32_t i = tid; i < npass; i += kEroWG) {
+            const uint64_t it = pass0 + i;
+            const uint32_t sl = (uint32_t)(it / rn);
+            const uint32_t ro = (uint32_t)(it - (uint64_t)sl * rn);
+            const uint64_t flat = flat0 + rbase + ro;
+            const uint64_t row = flat / W;
+            const int rz = rz0 + (int)row;
+            const int rx = rx0 + (int)(flat - row * W);
+            int cx, cz;
+            hut_chunk(seeds[seed0 + sl], rx, rz, salt, cx, cz);
+            s_item[i] = make_int2(cx, (cz + kCzBias) | (int)(sl << 23));
+        }
+        if (tid == 0) s_head = kEroWG;
+        __syncthreads();
+
+        // ---- tiered erosion with a per-lane work queue: one Perlin evaluation per lane per step ----
+        uint32_t cur = tid;            // first item is static, later ones come from s_head
+        bool have = cur < npass;
+        int k = 0, sl = 0, cx = 0, cz = 0;
+        double x = 0.0, z = 0.0, sum = 0.0;
+        if (have) {
+            int2 e = s_item[cur];
+            cx = e.x;
+            sl = e.y >> 23;
+            cz = (e.y & ((1 << 23) - 1)) - kCzBias;
+            x = (double)(cx * 16 + 8) / 4.0;
+            z = (double)(cz * 16 + 8) / 4.0;
+        }
+        for (;;) {
+            bool pass_now = false;
+            if (have) {
+                if (single) {
+                    pass_now = true;
+                    have = false;
+                } else {
+                    const double sc = s_sc[k], lac = s_lac[k];
+                    const double xs = keep_precision(x * sc * lac);
+                    const double zs = keep_precision(z * sc * lac);
+                    const uint32_t t = (uint32_t)sl * 8 + k;
+                    const double v = perlin_y0(LPerm{tabb + t * 256}, s_par[t], xs, zs);
+                    sum += s_w[k] * v;
+                    if (k >= 1) {
+                        if (sum * amplitude < s_thr[k]) {
+                            atomicAdd(&s_cnt[k - 1], 1u);
+                            have = false;
+                        } else if (k == 7) {
+                            pass_now = true;
+                            have = false;
+                        }
+                    }
+                    k++;
+                }
+            }
+            // survivors of the erosion gate -> Q1
+            const uint32_t pos = wave_append(pass_now, &ctl[CTL_Q1]);
+            if (pass_now) {
+                q1[pos] = make_uint4(seed0 + (uint32_t)sl, (uint32_t)cx, (uint32_t)cz, 0u);
+                needC[seed0 + sl] = 1u;
+            }
+            // refill lanes whose item finished
+            const bool need = !have && cur < npass;
+            unsigned long long nmask = __ballot(need);
+            if (nmask) {
+                const int leader = __ffsll((long long)nmask) - 1;
+                uint32_t base = 0;
+                if ((int)lane_id() == leader) base = atomicAdd(&s_head, (uint32_t)__popcll(nmask));
+                base = __shfl(base, leader);
+                if (need) {
+                    cur = base + (uint32_t)__popcll(nmask & ((1ull << lane_id()) - 1ull));
+                    if (cur < npass) {
+                        const int2 e = s_item[cur];
+                        cx = e.x;
+                        sl = e.y >> 23;
+                        cz = (e.y & ((1 << 23) - 1)) - kCzBias;
+                        x = (double)(cx * 16 + 8) / 4.0;
+                        z = (double)(cz * 16 + 8) / 4.0;
+                        k = 0;
+                        sum = 0.0;
+                        have = true;
+                    }
+                }
+            }
+            if (!__ballot(have)) break;
+        }
+        __syncthreads();
+    }
+    if (tid < 7 && s_cnt[tid]) atomicAdd(&cnt[CNT_TIER0 + tid], (unsigned long long)s_cnt[tid]);
+}
+
+// ---------------------------------------------------------------------------------------------- compaction
+
+__global__ void k_compact(const uint32_t *__restrict__ need, uint32_t n, uint32_t *__restrict__ slot_of,
+                          uint32_t *__restrict__ seed_of, uint32_t *__restrict__ count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool f = i < n && need[i];
+    uint32_t pos = wave_append(f, count);
+    if (f) {
+        slot_of[i] = pos;
+        seed_of[pos] = i;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------- gates 2 and 3
+
+__device__ __forceinline__ void block_count(uint32_t *s_cnt, int kind) { atomicAdd(&s_cnt[kind], 1u); }
+
+constexpr int kGateWG = 256;
+
+__global__ __launch_bounds__(kGateWG) void k_gate2(const uint64_t *__restrict__ seeds, const GateTables *__restrict__ gt,
+                                                    const uint8_t *__restrict__ tabC, const SPar *__restrict__ parC,
+                                                    const uint32_t *__restrict__ slotC, const uint4 *__restrict__ q1,
+                                                    uint4 *__restrict__ q2, uint32_t *__restrict__ ctl,
+                                                    uint32_t *__restrict__ needL, int vflags,
+                                                    unsigned long long *__restrict__ cnt) {
+    __shared__ uint32_t s_cnt[16];
+    if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t n = ctl[CTL_Q1];
+    const SetView C{tabC, parC, kSetSize[SET_C]};
+    const bool single = (vflags & WHF_V_SINGLE) != 0;
+    for (uint32_t q0 = blockIdx.x * kGateWG + (threadIdx.x & ~63u); q0 < n; q0 += gridDim.x * kGateWG) {
+        const uint32_t q = q0 + lane_id();
+        bool survive = false;
+        uint4 e = make_uint4(0, 0, 0, 0);
+        if (q < n) {
+            e = q1[q];
+            const uint32_t slot = slotC[e.x];
+            const int hut_x = (int)e.y * 16, hut_z = (int)e.z * 16;
+            int kind = -1;
+            if (!single) {
+                const double qx = (hut_x + 8) / 4.0, qz = (hut_z + 8) / 4.0;
+                const double t = noise_y0(gt->noise[G_TEMPERATURE], C, slot, qx, qz);
+                const bool t_bad = (vflags & WHF_V_118) ? (t < -0.45) : (t > 0.2 || t < -0.45);
+                if (t_bad) kind = 1;
+                else {
+                    const double r = noise_y0(gt->noise[G_RIDGE], C, slot, qx, qz);
+                    if ((r > 0.42 && r < 0.91) || (r < -0.42 && r > -0.91)) kind = 2;
+                    else if ((vflags & WHF_V_262) && r <= -0.91) kind = 2;
+                }
+            }
+            if (kind < 0) {
+                atomicAdd(&s_cnt[15], 1u);   // reached the cave ladder
+                CaveCtx cv{gt, C, SetView{nullptr, nullptr, 0}, slot, 0};
+                const int x = hut_x + 3, z = hut_z + 3;
+                if (cv.entrance_nonneg(x, 50, z)) kind = 3;
+                else if (cv.entrance_nonneg(x, 60, z)) kind = 4;
+            }
+            if (kind > 0) block_count(s_cnt, kind);
+            else survive = true;
+        }
+        const uint32_t pos = wave_append(survive, &ctl[CTL_Q2]);
+        if (survive) {
+            q2[pos] = e;
+            needL[e.x] = 1u;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 && s_cnt[threadIdx.x]) {
+        const int i = threadIdx.x;
+        atomicAdd(&cnt[i == 15 ? CNT_LADDER : CNT_KIND0 + i], (unsigned long long)s_cnt[i]);
+    }
+}
+
+__global__ __launch_bounds__(kGateWG) void k_gate3(const uint64_t *__restrict__ seeds, const GateTables *__restrict__ gt,
+                                                    const uint8_t *__restrict__ tabC, const SPar *__restrict__ parC,
+                                                    const uint8_t *__restrict__ tabL, const SPar *__restrict__ parL,
+                                                    const uint32_t *__restrict__ slotC,
+                                                    const uint32_t *__restrict__ slotL, const uint4 *__restrict__ q2,
+                                                    uint32_t *__restrict__ ctl, int vflags, int max_h,
+                                                    DevHit *__restrict__ hits, uint32_t hit_cap,
+                                                    unsigned long long *__restrict__ cnt) {
+    __shared__ uint32_t s_cnt[16];
+    if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t n = ctl[CTL_Q2];
+    const bool single = (vflags & WHF_V_SINGLE) != 0;
+    for (uint32_t q0 = blockIdx.x * kGateWG + (threadIdx.x & ~63u); q0 < n; q0 += gridDim.x * kGateWG) {
+        const uint32_t q = q0 + lane_id();
+        bool accept = false;
+        uint4 e = make_uint4(0, 0, 0, 0);
+        if (q < n) {
+            e = q2[q];
+            const CaveCtx cv{gt, SetView{tabC, parC, kSetSize[SET_C]}, SetView{tabL, parL, kSetSize[SET_L]},
+                             slotC[e.x], slotL[e.x]};
+            const int hut_x = (int)e.y * 16, hut_z = (int)e.z * 16;
+            const int x = hut_x + 3, z = hut_z + 3;
+            int kind = -1;
+            if (cv.spaghetti(x, max_h, z) >= 0 && cv.cheese(x, max_h, z) >= 0) kind = 5;
+            if (kind < 0) {
+                int floor_y = max_h < -40 ? max_h : -40;
+                if (floor_y < -50) floor_y = -50;
+                for (int y = 0; y >= floor_y; y -= 10)
+                    if (max_h < y && cv.spaghetti(x, y, z) >= 0 && cv.cheese(x, y, z) >= 0) { kind = 6; break; }
+            }
+            if (kind < 0)
+                for (int y = 10; y <= 40; y += 10)
+                    if (cv.entrance_nonneg(x, y, z) && cv.cheese(x, y, z) >= 0) { kind = 7; break; }
+            if (kind < 0 && !single) {
+                const NoiseDesc &d = gt->noise[G_CONTINENTALNESS];
+                const double c = noise_y0(d, cv.L, cv.slotL, (hut_x + 8) / 4.0, (hut_z + 8) / 4.0);
+                if (c < -0.11) kind = 8;
+            }
+            if (kind < 0)
+                for (int y = max_h; y <= 60; y += 10)
+                    if (cv.flooded(hut_x, hut_z, y) > 0.41) { kind = 9; break; }
+            if (kind < 0) kind = 10;
+            block_count(s_cnt, kind);
+            accept = kind == 10;
+        }
+        const uint32_t pos = wave_append(accept, &ctl[CTL_HITS]);
+        if (accept && pos < hit_cap) {
+            DevHit h;
+            h.seed = (int64_t)seeds[e.x];
+            h.cx = (int)e.y;
+            h.cz = (int)e.z;
+            h.rx = h.cx >> 5;
+            h.rz = h.cz >> 5;
+            h.floor_y = INT32_MIN;
+            h.flags = 0;
+            hits[pos] = h;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 && threadIdx.x > 0 && s_cnt[threadIdx.x])
+        atomicAdd(&cnt[CNT_KIND0 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------------------------- probe
+// The 38 values of lysh_phase1_probe (phase1.c:39-80) per (seed slot, hut position); all three sets are
+// built for every slot. Erosion is the exact double-noise sample here, not the tiered gate.
+
+__global__ void k_probe(const GateTables *__restrict__ gt, const uint8_t *tabE, const SPar *parE,
+                        const uint8_t *tabC, const SPar *parC, const uint8_t *tabL, const SPar *parL,
+                        const int32_t *__restrict__ hx, const int32_t *__restrict__ hz, uint32_t n, int max_h,
+                        double *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SetView E{tabE, parE, kSetSize[SET_E]};
+    const CaveCtx cv{gt, SetView{tabC, parC, kSetSize[SET_C]}, SetView{tabL, parL, kSetSize[SET_L]}, i, i};
+    const int hut_x = hx[i], hut_z = hz[i];
+    const double qx = (hut_x + 8) / 4.0, qz = (hut_z + 8) / 4.0;
+    const int x = hut_x + 3, z = hut_z + 3;
+    double *o = out + (size_t)i * 38;
+    int k = 0;
+    o[k++] = noise_y0(gt->noise[G_EROSION], E, i, qx, qz);
+    o[k++] = noise_y0(gt->noise[G_TEMPERATURE], cv.C, i, qx, qz);
+    o[k++] = noise_y0(gt->noise[G_RIDGE], cv.C, i, qx, qz);
+    o[k++] = noise_y0(gt->noise[G_CONTINENTALNESS], cv.L, i, qx, qz);
+    o[k++] = cv.entrance(x, 50, z);
+    o[k++] = cv.entrance(x, 60, z);
+    o[k++] = cv.spaghetti(x, max_h, z);
+    o[k++] = cv.cheese(x, max_h, z);
+    for (int y = 0; y >= -40; y -= 10) { o[k++] = cv.spaghetti(x, y, z); o[k++] = cv.cheese(x, y, z); }
+    for (int y = 10; y <= 40; y += 10) { o[k++] = cv.entrance(x, y, z); o[k++] = cv.cheese(x, y, z); }
+    for (int y = -50; y <= 60; y += 10) o[k++] = cv.flooded(hut_x, hut_z, y);
+}
+
+}  // namespace
+
+// ================================================================================================ host
+
+namespace {
+
+// ---- MD5 (RFC 1321), used once per context to key the sampler derivations ----
+struct Md5 {
+    static uint32_t rol(uint32_t v, int s) { return (v << s) | (v >> (32 - s)); }
+    static void block(uint32_t st[4], const uint8_t *p, const uint32_t K[64]) {
+        static const int S[4][4] = {{7, 12, 17, 22}, {5, 9, 14, 20}, {4, 11, 16, 23}, {6, 10, 15, 21}};
+        uint32_t m[16];
+        for (int i = 0; i < 16; i++) m[i] = p[4 * i] | p[4 * i + 1] << 8 | p[4 * i + 2] << 16 | (uint32_t)p[4 * i + 3] << 24;
+        uint32_t a = st[0], b = st[1], c = st[2], d = st[3];
+        for (int i = 0; i < 64; i++) {
+            uint32_t f;
+            int g, r = i >> 4;
+            if (r == 0) { f = (b & c) | (~b & d); g = i; }
+            else if (r == 1) { f = (d & b) | (~d & c); g = (5 * i + 1) & 15; }
+            else if (r == 2) { f = b ^ c ^ d; g = (3 * i + 5) & 15; }
+            else { f = c ^ (b | ~d); g = (7 * i) & 15; }
+            uint32_t t = d;
+            d = c;
+            c = b;
+            b = b + rol(a + f + K[i] + m[g], S[r][i & 3]);
+            a = t;
+        }
+        st[0] += a; st[1] += b; st[2] += c; st[3] += d;
+    }
+    static void digest(const std::string &s, uint64_t &hi, uint64_t &lo) {
+        uint32_t K[64];
+        for (int i = 0; i < 64; i++) K[i] = (uint32_t)std::floor(std::fabs(std::sin((double)(i + 1))) * 4294967296.0);
+        uint32_t st[4] = {0x67452301u, 0xEFCDAB89u, 0x98BADCFEu, 0x10325476u};
+        std::vector<uint8_t> msg(s.begin(), s.end());
+        uint64_t bits = (uint64_t)msg.size() * 8;
+        msg.push_back(0x80);
+        while (msg.size() % 64 != 56) msg.push_back(0);
+        for (int i = 0; i < 8; i++) msg.push_back((uint8_t)(bits >> (8 * i)));
+        for (size_t o = 0; o < msg.size(); o += 64) block(st, msg.data() + o, K);
+        uint8_t out[16];
+        for (int i = 0; i < 16; i++) out[i] = (uint8_t)(st[i / 4] >> (8 * (i % 4)));
+        hi = lo = 0;
+        for (int i = 0; i < 8; i++) hi = hi << 8 | out[i];
+        for (int i = 8; i < 16; i++) lo = lo << 8 | out[i];
+    }
+};
+
+#define HIP_TRY(expr)                                                                                       \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) {                                                                             \
+            std::snprintf(g_last_error, sizeof g_last_error, "%s: %s", #expr, hipGetErrorString(e_));      \
+            return WHF_EDEVICE;                                                                             \
+        }                                                                                                   \
+    } while (0)
+
+thread_local char g_last_error[512];
+
+// Host description of one sampler of a set.
+struct SamplerRef {
+    int noise, sub, slot;
+};
+
+// Which noise stands in each gate role for these rule flags (LARGE_BIOMES swaps three climate noises).
+int gate_noise_param(int g, int vflags) {
+    const bool large = (vflags & WHF_V_LARGE) != 0;
+    switch (g) {
+    case G_EROSION: return large ? N_EROSION_LARGE : N_EROSION;
+    case G_TEMPERATURE: return large ? N_TEMPERATURE_LARGE : N_TEMPERATURE;
+    case G_RIDGE: return N_RIDGE;
+    case G_CAVE_ENTRANCE: return N_CAVE_ENTRANCE;
+    case G_SPAG_RARITY: return N_SPAG3D_RARITY;
+    case G_SPAG_THICKNESS: return N_SPAG3D_THICKNESS;
+    case G_SPAG_1: return N_SPAG3D_1;
+    case G_SPAG_2: return N_SPAG3D_2;
+    case G_ROUGH_MOD: return N_SPAG_ROUGH_MOD;
+    case G_ROUGH: return N_SPAG_ROUGH;
+    case G_CAVE_LAYER: return N_CAVE_LAYER;
+    case G_CAVE_CHEESE: return N_CAVE_CHEESE;
+    case G_CONTINENTALNESS: return large ? N_CONTINENTALNESS_LARGE : N_CONTINENTALNESS;
+    case G_FLOODEDNESS: return N_FLOODEDNESS;
+    }
+    return -1;
+}
+
+int gate_set(int g) {
+    switch (g) {
+    case G_EROSION: return SET_E;
+    case G_TEMPERATURE: case G_RIDGE: case G_CAVE_ENTRANCE: case G_SPAG_RARITY: case G_SPAG_THICKNESS:
+    case G_SPAG_1: case G_SPAG_2: case G_ROUGH_MOD: case G_ROUGH: return SET_C;
+    default: return SET_L;
+    }
+}
+
+// Table layouts of the host-side gate tables and build descriptors for a rule-flag set.
+struct Layout {
+    std::vector<SamplerRef> set[SET_COUNT];
+    GateTables gt;
+    std::vector<BuildDesc> bd[SET_COUNT];
+};
+
+void build_layout(int vflags, Layout &L) {
+    for (int s = 0; s < SET_COUNT; s++) { L.set[s].clear(); L.bd[s].clear(); }
+    std::memset(&L.gt, 0, sizeof L.gt);
+    // set E: erosion in tier order A0 B0 A1 B1 ... over its non-empty slots
+    {
+        const NoiseParam &np = kNoise[gate_noise_param(G_EROSION, vflags)];
+        for (int i = 0; i < np.n; i++)
+            if (np.amps[i] != 0.0) { L.set[SET_E].push_back({gate_noise_param(G_EROSION, vflags), 0, i});
+                                     L.set[SET_E].push_back({gate_noise_param(G_EROSION, vflags), 1, i}); }
+    }
+    // sets C and L: per gate noise, all non-empty slots of sub 0 then of sub 1
+    for (int g = 1; g < G_COUNT; g++) {
+        const int pid = gate_noise_param(g, vflags);
+        const NoiseParam &np = kNoise[pid];
+        for (int s = 0; s < 2; s++)
+            for (int i = 0; i < np.n; i++)
+                if (np.amps[i] != 0.0) L.set[gate_set(g)].push_back({pid, s, i});
+    }
+    // noise descriptors
+    for (int g = 0; g < G_COUNT; g++) {
+        const int pid = gate_noise_param(g, vflags);
+        const NoiseParam &np = kNoise[pid];
+        NoiseDesc &d = L.gt.noise[g];
+        d.set = gate_set(g);
+        const std::vector<SamplerRef> &set = L.set[d.set];
+        int p = 0, first = -1, last = -1;
+        const double per0 = std::ldexp(1.0, np.n - 1) / (std::ldexp(1.0, np.n) - 1.0);
+        for (int i = 0; i < np.n; i++) {
+            if (np.amps[i] == 0.0) continue;
+            if (first < 0) first = i;
+            last = i;
+            for (int s = 0; s < 2; s++)
+                for (size_t t = 0; t < set.size(); t++)
+                    if (set[t].noise == pid && set[t].sub == s && set[t].slot == i) d.tab[s][p] = (int)t;
+            d.lac[p] = std::ldexp(1.0, np.first_octave + i);
+            d.amp[p] = np.amps[i];
+            d.per[p] = std::ldexp(per0, -i);   // persistence halves per slot (noise.c:277-278), exactly
+            p++;
+        }
+        d.np = p;
+        d.amplitude = 0.16666666666666666 / (0.1 * (1.0 + 1.0 / ((double)(last - first) + 1.0)));
+    }
+    // tiered erosion
+    {
+        const NoiseDesc &d = L.gt.noise[G_EROSION];
+        EroDesc &e = L.gt.ero;
+        for (int k = 0; k < 8; k++) {
+            const int p = k >> 1;
+            e.sc[k] = (k & 1) ? 1.0181268882175227 : 1.0;
+            e.lac[k] = d.lac[p];
+            e.w[k] = d.amp[p] * d.per[p];
+        }
+        e.amplitude = d.amplitude;
+        const double thr[7] = {0.05, 0.22, 0.42, 0.45, 0.49, 0.51, 0.55};
+        for (int t = 0; t < 7; t++) e.thr[t] = thr[t];
+    }
+    // build descriptors
+    for (int s = 0; s < SET_COUNT; s++)
+        for (const SamplerRef &r : L.set[s]) {
+            BuildDesc b{};
+            Md5::digest(kNoise[r.noise].name, b.na, b.nb);
+            Md5::digest("octave_" + std::to_string(kNoise[r.noise].first_octave + r.slot), b.oa, b.ob);
+            b.sub = r.sub;
+            L.bd[s].push_back(b);
+        }
+}
+
+constexpr uint32_t kMaxSeeds = 1u << 17;     // seed slots per batch
+constexpr uint32_t kMaxItems = 1u << 25;     // (seed, region) items per batch
+constexpr uint32_t kHitCap = 1u << 20;
+constexpr int kLayouts = 16;                 // every combination of the four rule flags
+
+}  // namespace
+
+struct whf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char name[256] = {0};
+    bool profile = false;
+    uint64_t *d_seeds = nullptr;
+    uint8_t *d_tab[SET_COUNT] = {nullptr, nullptr, nullptr};
+    SPar *d_par[SET_COUNT] = {nullptr, nullptr, nullptr};
+    uint4 *d_q1 = nullptr, *d_q2 = nullptr;
+    uint32_t *d_needC = nullptr, *d_needL = nullptr, *d_slotC = nullptr, *d_slotL = nullptr;
+    uint32_t *d_seedC = nullptr, *d_seedL = nullptr, *d_ctl = nullptr;
+    unsigned long long *d_cnt = nullptr;
+    DevHit *d_hits = nullptr;
+    GateTables *d_gt = nullptr;      // [kLayouts]
+    BuildDesc *d_bd = nullptr;       // [kLayouts][8 + 28 + 36]
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> prof_ev;  // pairs per kernel launch when profiling
+    std::vector<int> prof_fam;
+    int n_cu = 256;
+};
+
+namespace {
+
+constexpr int kBdPerLayout = 8 + 28 + 36;
+
+int ctx_alloc(whf_ctx *c) {
+    const size_t S = kMaxSeeds;
+    HIP_TRY(hipMalloc(&c->d_seeds, S * sizeof(uint64_t)));
+    for (int s = 0; s < SET_COUNT; s++) {
+        HIP_TRY(hipMalloc(&c->d_tab[s], S * kSetSize[s] * 256));
+        HIP_TRY(hipMalloc(&c->d_par[s], S * kSetSize[s] * sizeof(SPar)));
+    }
+    HIP_TRY(hipMalloc(&c->d_q1, (size_t)kMaxItems * sizeof(uint4)));
+    HIP_TRY(hipMalloc(&c->d_q2, (size_t)kMaxItems * sizeof(uint4)));
+    HIP_TRY(hipMalloc(&c->d_needC, S * 4));
+    HIP_TRY(hipMalloc(&c->d_needL, S * 4));
+    HIP_TRY(hipMalloc(&c->d_slotC, S * 4));
+    HIP_TRY(hipMalloc(&c->d_slotL, S * 4));
+    HIP_TRY(hipMalloc(&c->d_seedC, S * 4));
+    HIP_TRY(hipMalloc(&c->d_seedL, S * 4));
+    HIP_TRY(hipMalloc(&c->d_ctl, 64));
+    HIP_TRY(hipMalloc(&c->d_cnt, CNT_N * sizeof(unsigned long long)));
+    HIP_TRY(hipMalloc(&c->d_hits, (size_t)kHitCap * sizeof(DevHit)));
+    HIP_TRY(hipMalloc(&c->d_gt, kLayouts * sizeof(GateTables)));
+    HIP_TRY(hipMalloc(&c->d_bd, kLayouts * kBdPerLayout * sizeof(BuildDesc)));
+    // layouts for every rule-flag combination, uploaded once
+    std::vector<GateTables> gts(kLayouts);
+    std::vector<BuildDesc> bds(kLayouts * kBdPerLayout);
+    for (int v = 0; v < kLayouts; v++) {
+        Layout L;
+        build_layout(v, L);
+        gts[v] = L.gt;
+        size_t o = (size_t)v * kBdPerLayout;
+        for (int s = 0; s < SET_COUNT; s++) {
+            if ((int)L.bd[s].size() != kSetSize[s]) {
+                std::snprintf(g_last_error, sizeof g_last_error, "sampler set %d has %zu entries", s, L.bd[s].size());
+                return WHF_EINVAL;
+            }
+            std::copy(L.bd[s].begin(), L.bd[s].end(), bds.begin() + o);
+            o += kSetSize[s];
+        }
+    }
+    HIP_TRY(hipMemcpy(c->d_gt, gts.data(), gts.size() * sizeof(GateTables), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_bd, bds.data(), bds.size() * sizeof(BuildDesc), hipMemcpyHostToDevice));
+    HIP_TRY(hipEventCreate(&c->ev0));
+    HIP_TRY(hipEventCreate(&c->ev1));
+    return WHF_OK;
+}
+
+const BuildDesc *bd_of(whf_ctx *c, int layout, int set) {
+    size_t o = (size_t)layout * kBdPerLayout;
+    for (int s = 0; s < set; s++) o += kSetSize[s];
+    return c->d_bd + o;
+}
+
+// Kernel families for the profile breakdown.
+enum Fam { F_SEEDS = 0, F_BUILD_E, F_EROSION, F_COMPACT, F_BUILD_C, F_GATE2, F_BUILD_L, F_GATE3 };
+
+struct Prof {
+    whf_ctx *c;
+    int fam;
+    Prof(whf_ctx *c_, int f) : c(c_), fam(f) {
+        if (c->profile) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            (void)hipEventRecord(e, c->stream);
+            c->prof_ev.push_back(e);
+            c->prof_fam.push_back(fam);
+        }
+    }
+    ~Prof() {
+        if (c->profile) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            (void)hipEventRecord(e, c->stream);
+            c->prof_ev.push_back(e);
+        }
+    }
+};
+
+int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        std::snprintf(g_last_error, sizeof g_last_error, "kernel launch: %s", hipGetErrorString(e));
+        return WHF_EDEVICE;
+    }
+    return WHF_OK;
+}
+
+// One batch: seeds [s0, s0 + ns) of the request, flat region range [f0, f0 + fn).
+int run_batch(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_t f0, uint32_t fn, uint32_t W) {
+    const hipStream_t st = c->stream;
+    const uint32_t blocks_n = (ns + 255) / 256;
+    {
+        Prof p(c, F_SEEDS);
+        hipLaunchKernelGGL(k_reset, dim3(std::max(1u, blocks_n)), dim3(256), 0, st, c->d_ctl, c->d_needC, c->d_needL, ns);
+    }
+    {
+        Prof p(c, F_BUILD_E);
+        const uint32_t jobs = ns * 8;
+        hipLaunchKernelGGL(k_build, dim3((jobs + 63) / 64), dim3(64), 0, st, c->d_seeds, bd_of(c, layout, SET_E), 8,
+                           (const uint32_t *)nullptr, (const uint32_t *)nullptr, ns, c->d_tab[SET_E], c->d_par[SET_E],
+                           c->d_cnt);
+    }
+    {
+        Prof p(c, F_EROSION);
+        const int salt = req->salt ? req->salt : 14357620;
+        const int single = (req->vflags & WHF_V_SINGLE) ? 1 : 0;
+        const GateTables *gt = c->d_gt + layout;
+        // seeds per tile: narrow rects want many seeds per workgroup to fill the lanes
+        const bool wide = fn >= 64;
+        const int ST = wide ? 4 : 16;
+        const uint32_t chunk = std::min<uint32_t>(fn, 1u << 16);
+        const uint32_t n_chunks = (fn + chunk - 1) / chunk;
+        const uint32_t tiles = (ns + ST - 1) / ST;
+        const dim3 grid(tiles * n_chunks);
+        if (wide)
+            hipLaunchKernelGGL(k_erosion<4>, grid, dim3(kEroWG), 0, st, c->d_seeds, ns, c->d_tab[SET_E],
+                               c->d_par[SET_E], gt, req->rx0, req->rz0, W, f0, fn, chunk, n_chunks, salt, single,
+                               c->d_q1, c->d_ctl, c->d_needC, c->d_cnt);
+        else
+            hipLaunchKernelGGL(k_erosion<16>, grid, dim3(kEroWG), 0, st, c->d_seeds, ns, c->d_tab[SET_E],
+                               c->d_par[SET_E], gt, req->rx0, req->rz0, W, f0, fn, chunk, n_chunks, salt, single,
+                               c->d_q1, c->d_ctl, c->d_needC, c->d_cnt);
+    }
+    {
+        Prof p(c, F_COMPACT);
+        hipLaunchKernelGGL(k_compact, dim3(std::max(1u, blocks_n)), dim3(256), 0, st, c->d_needC, ns, c->d_slotC,
+                           c->d_seedC, c->d_ctl + CTL_NC);
+    }
+    const uint32_t persist = (uint32_t)c->n_cu * 8;
+    {
+        Prof p(c, F_BUILD_C);
+        hipLaunchKernelGGL(k_build, dim3(std::min<uint32_t>(persist * 4, (ns * 28 + 63) / 64 + 1)), dim3(64), 0, st,
+                           c->d_seeds, bd_of(c, layout, SET_C), 28, c->d_seedC, c->d_ctl + CTL_NC, 0u,
+                           c->d_tab[SET_C], c->d_par[SET_C], c->d_cnt);
+    }
+    {
+        Prof p(c, F_GATE2);
+        hipLaunchKernelGGL(k_gate2, dim3(persist), dim3(kGateWG), 0, st, c->d_seeds, c->d_gt + layout,
+                           c->d_tab[SET_C], c->d_par[SET_C], c->d_slotC, c->d_q1, c->d_q2, c->d_ctl, c->d_needL,
+                           req->vflags, c->d_cnt);
+    }
+    {
+        Prof p(c, F_COMPACT);
+        hipLaunchKernelGGL(k_compact, dim3(std::max(1u, blocks_n)), dim3(256), 0, st, c->d_needL, ns, c->d_slotL,
+                           c->d_seedL, c->d_ctl + CTL_NL);
+    }
+    {
+        Prof p(c, F_BUILD_L);
+        hipLaunchKernelGGL(k_build, dim3(std::min<uint32_t>(persist, (ns * 36 + 63) / 64 + 1)), dim3(64), 0, st,
+                           c->d_seeds, bd_of(c, layout, SET_L), 36, c->d_seedL, c->d_ctl + CTL_NL, 0u,
+                           c->d_tab[SET_L], c->d_par[SET_L], c->d_cnt);
+    }
+    {
+        Prof p(c, F_GATE3);
+        hipLaunchKernelGGL(k_gate3, dim3(persist), dim3(kGateWG), 0, st, c->d_seeds, c->d_gt + layout,
+                           c->d_tab[SET_C], c->d_par[SET_C], c->d_tab[SET_L], c->d_par[SET_L], c->d_slotC,
+                           c->d_slotL, c->d_q2, c->d_ctl, req->vflags, req->max_height, c->d_hits, kHitCap,
+                           c->d_cnt);
+    }
+    return check_launch();
+}
+
+}  // namespace
+
+// ================================================================================================ C ABI
+
+extern "C" {
+
+const char *whf_strerror(int code) {
+    switch (code) {
+    case WHF_OK: return "ok";
+    case WHF_EINVAL: return "invalid argument";
+    case WHF_ENOMEM: return "out of memory or over the HBM budget";
+    case WHF_EDEVICE: return g_last_error[0] ? g_last_error : "device error";
+    case WHF_EOVERFLOW: return "more survivors than the sink holds";
+    case WHF_ENOTSUP: return "not supported";
+    }
+    return "unknown error";
+}
+
+int whf_version_flags(const char *name, int *vflags) {
+    // game versions (biome_trees.h:70499-70526) and lysh's aliases (main.c:104-125)
+    static const struct { const char *n; int f; } V[] = {
+        {"1.18", WHF_V_118}, {"1.18.1", WHF_V_118}, {"1.18.2", WHF_V_118}, {"1.18.x", WHF_V_118},
+        {"1.19", 0}, {"1.19.2", 0}, {"1.19.3", 0}, {"1.19.4", 0}, {"1.19.x", 0}, {"1.20", 0}, {"1.20.1", 0},
+        {"1.20.2", 0}, {"1.20.4", 0}, {"1.20.5", 0}, {"1.20.6", 0}, {"1.20.x", 0}, {"1.21", 0}, {"1.21.1", 0},
+        {"1.21.2", 0}, {"1.21.3", 0}, {"1.21.4", 0}, {"1.21.5", 0}, {"1.21.8", 0}, {"1.21.11", 0},
+        {"1.21.x~26.1", 0}, {"1.21.5~26.1", 0}, {"1.19.x~26.1", 0}, {"26.1", 0}, {"26.2", WHF_V_262},
+        {"26.3", WHF_V_262}, {"26.2~26.3", WHF_V_262},
+    };
+    if (!name || !vflags) return WHF_EINVAL;
+    for (const auto &v : V)
+        if (!std::strcmp(v.n, name)) { *vflags = v.f; return WHF_OK; }
+    return WHF_EINVAL;
+}
+
+int whf_open(const whf_config *cfg, whf_ctx **out) {
+    if (!out) return WHF_EINVAL;
+    *out = nullptr;
+    g_last_error[0] = 0;
+    int dev = cfg ? cfg->device : 0;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (dev < 0 || dev >= n) {
+        std::snprintf(g_last_error, sizeof g_last_error, "device %d not present (%d visible)", dev, n);
+        return WHF_EDEVICE;
+    }
+    HIP_TRY(hipSetDevice(dev));
+    whf_ctx *c = new (std::nothrow) whf_ctx();
+    if (!c) return WHF_ENOMEM;
+    c->device = dev;
+    c->profile = cfg && (cfg->flags & 1);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        std::snprintf(c->name, sizeof c->name, "%s (%s)", prop.name, prop.gcnArchName);
+        c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return WHF_EDEVICE;
+    }
+    int rc = ctx_alloc(c);
+    if (rc != WHF_OK) {
+        whf_close(c);
+        return rc == WHF_EDEVICE ? WHF_ENOMEM : rc;
+    }
+    *out = c;
+    return WHF_OK;
+}
+
+void whf_close(whf_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *bufs[] = {c->d_seeds, c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_par[0], c->d_par[1], c->d_par[2],
+                    c->d_q1, c->d_q2, c->d_needC, c->d_needL, c->d_slotC, c->d_slotL, c->d_seedC, c->d_seedL,
+                    c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *whf_device_name(whf_ctx *c) { return c ? c->name : ""; }
+
+int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stats) {
+    if (!c || !req) return WHF_EINVAL;
+    g_last_error[0] = 0;
+    if (req->rx1 <= req->rx0 || req->rz1 <= req->rz0) return WHF_EINVAL;
+    if (req->seed_mode < WHF_SEEDS_LIST || req->seed_mode > WHF_SEEDS_COUNTER) return WHF_EINVAL;
+    if (req->seed_mode == WHF_SEEDS_LIST && req->n_seeds && !req->seeds) return WHF_EINVAL;
+    if (req->precision != WHF_PREC_FP64) return WHF_ENOTSUP;
+    if (req->vflags < 0 || req->vflags >= kLayouts) return WHF_EINVAL;
+    // chunk coordinates travel biased in 23 bits through the erosion kernel
+    const int64_t lim = 65536;
+    if (req->rx0 < -lim || req->rx1 > lim || req->rz0 < -lim || req->rz1 > lim) return WHF_EINVAL;
+    HIP_TRY(hipSetDevice(c->device));
+
+    const uint32_t W = (uint32_t)(req->rx1 - req->rx0);
+    const uint64_t R = (uint64_t)W * (uint64_t)(req->rz1 - req->rz0);
+    const int layout = req->vflags;
+    const hipStream_t st = c->stream;
+    HIP_TRY(hipMemsetAsync(c->d_cnt, 0, CNT_N * sizeof(unsigned long long), st));
+    HIP_TRY(hipMemsetAsync(c->d_ctl, 0, 64, st));
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    c->prof_ev.clear();
+    c->prof_fam.clear();
+    HIP_TRY(hipEventRecord(c->ev0, st));
+
+    // batch plan: seeds per batch x regions per batch <= kMaxItems
+    uint32_t seeds_per, regions_per;
+    if (R <= kMaxItems) {
+        regions_per = (uint32_t)R;
+        seeds_per = (uint32_t)std::min<uint64_t>(kMaxSeeds, kMaxItems / R);
+    } else {
+        regions_per = kMaxItems;
+        seeds_per = 1;
+    }
+    for (uint64_t s0 = 0; s0 < req->n_seeds; s0 += seeds_per) {
+        const uint32_t ns = (uint32_t)std::min<uint64_t>(seeds_per, req->n_seeds - s0);
+        for (uint64_t f0 = 0; f0 < R; f0 += regions_per) {
+            const uint32_t fn = (uint32_t)std::min<uint64_t>(regions_per, R - f0);
+            if (f0 == 0) {
+                if (req->seed_mode == WHF_SEEDS_LIST) {
+                    HIP_TRY(hipMemcpyAsync(c->d_seeds, req->seeds + s0, ns * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+                } else {
+                    Prof p(c, F_SEEDS);
+                    hipLaunchKernelGGL(k_seeds, dim3((ns + 255) / 256), dim3(256), 0, st, req->seed_mode, req->stream,
+                                       req->offset + s0, ns, c->d_seeds);
+                }
+            }
+            int rc = run_batch(c, req, layout, ns, f0, fn, W);
+            if (rc) return rc;
+        }
+    }
+    HIP_TRY(hipEventRecord(c->ev1, st));
+    unsigned long long cnt[CNT_N];
+    uint32_t ctl[CTL_N];
+    HIP_TRY(hipMemcpyAsync(cnt, c->d_cnt, sizeof cnt, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ctl, c->d_ctl, sizeof ctl, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const uint64_t hits_total = ctl[CTL_HITS];
+    std::vector<DevHit> host_hits(std::min<uint64_t>(hits_total, kHitCap));
+    if (!host_hits.empty())
+        HIP_TRY(hipMemcpy(host_hits.data(), c->d_hits, host_hits.size() * sizeof(DevHit), hipMemcpyDeviceToHost));
+    int rc = WHF_OK;
+    if (sink) {
+        sink->count = hits_total;
+        const uint64_t m = std::min<uint64_t>(sink->capacity, host_hits.size());
+        if (m && sink->hits) std::memcpy(sink->hits, host_hits.data(), m * sizeof(whf_hit));
+        if (hits_total > sink->capacity || host_hits.size() < hits_total) rc = WHF_EOVERFLOW;
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->seeds = req->n_seeds;
+        stats->checks = req->n_seeds * R;
+        uint64_t ero = 0;
+        for (int t = 0; t < 7; t++) { stats->tiers[t] = cnt[CNT_TIER0 + t]; ero += cnt[CNT_TIER0 + t]; }
+        stats->funnel[0] = ero;
+        for (int k = 1; k < WHF_KIND_N; k++) stats->funnel[k] = cnt[CNT_KIND0 + k];
+        stats->accepted = cnt[CNT_KIND0 + 10];
+        stats->reached_ladder = cnt[CNT_LADDER];
+        stats->samplers_built = cnt[CNT_BUILT];
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        stats->device_ms = ms;
+        if (c->profile) {
+            for (size_t i = 0; i + 1 < c->prof_ev.size(); i += 2) {
+                float m2 = 0.f;
+                (void)hipEventElapsedTime(&m2, c->prof_ev[i], c->prof_ev[i + 1]);
+                const int f = c->prof_fam[i / 2];
+                if (f >= 0 && f < 8) stats->kernel_ms[f] += m2;
+            }
+        }
+    }
+    return rc;
+}
+
+int whf_probe(whf_ctx *c, const int64_t *seeds, const int32_t *hut_x, const int32_t *hut_z, uint64_t n,
+              int max_height, int vflags, double *out) {
+    if (!c || (n && (!seeds || !hut_x || !hut_z || !out))) return WHF_EINVAL;
+    if (vflags < 0 || vflags >= kLayouts) return WHF_EINVAL;
+    if (n > kMaxSeeds) return WHF_EINVAL;
+    if (!n) return WHF_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t st = c->stream;
+    const uint32_t nn = (uint32_t)n;
+    int32_t *d_xz = nullptr;
+    double *d_out = nullptr;
+    HIP_TRY(hipMalloc(&d_xz, 2 * n * sizeof(int32_t)));
+    HIP_TRY(hipMalloc(&d_out, 38 * n * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(c->d_seeds, seeds, n * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_xz, hut_x, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_xz + n, hut_z, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    for (int s = 0; s < SET_COUNT; s++) {
+        const uint32_t jobs = nn * kSetSize[s];
+        hipLaunchKernelGGL(k_build, dim3((jobs + 63) / 64), dim3(64), 0, st, c->d_seeds, bd_of(c, vflags, s),
+                           kSetSize[s], (const uint32_t *)nullptr, (const uint32_t *)nullptr, nn, c->d_tab[s],
+                           c->d_par[s], c->d_cnt);
+    }
+    hipLaunchKernelGGL(k_probe, dim3((nn + 63) / 64), dim3(64), 0, st, c->d_gt + vflags, c->d_tab[SET_E],
+                       c->d_par[SET_E], c->d_tab[SET_C], c->d_par[SET_C], c->d_tab[SET_L], c->d_par[SET_L], d_xz,
+                       d_xz + n, nn, max_height, d_out);
+    int rc = check_launch();
+    if (rc == WHF_OK) {
+        HIP_TRY(hipMemcpyAsync(out, d_out, 38 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    (void)hipFree(d_xz);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,178 @@
+"""Python binding of the MI355X Seed Check library (libwhf_hip.so, C ABI in include/whf.h).
+
+Thin ctypes layer over the C ABI, used by the benchmark and the tests. It loads the in-tree library and
+raises if it is missing: there is no fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwhf_hip.so")
+PHASE1_SCAN = os.path.join(HERE, "bin", "phase1_scan")
+
+OK, EINVAL, ENOMEM, EDEVICE, EOVERFLOW, ENOTSUP = 0, -1, -2, -3, -4, -5
+SEEDS_LIST, SEEDS_SPLITMIX, SEEDS_COUNTER = 0, 1, 2
+V_118, V_262, V_LARGE, V_SINGLE = 1, 2, 4, 8
+PREC_FP64 = 0
+KIND_N = 11
+CFG_PROFILE = 1
+
+# Exported symbols of include/whf.h, checked by the tests.
+SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags", "whf_strerror",
+           "whf_device_name")
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("flags", ctypes.c_int), ("hbm_budget", ctypes.c_uint64)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("seed_mode", ctypes.c_int), ("precision", ctypes.c_int),
+                ("seeds", ctypes.POINTER(ctypes.c_int64)), ("n_seeds", ctypes.c_uint64),
+                ("stream", ctypes.c_uint64), ("offset", ctypes.c_uint64),
+                ("rx0", ctypes.c_int), ("rx1", ctypes.c_int), ("rz0", ctypes.c_int), ("rz1", ctypes.c_int),
+                ("max_height", ctypes.c_int), ("vflags", ctypes.c_int), ("salt", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
+
+
+class Hit(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int64), ("rx", ctypes.c_int32), ("rz", ctypes.c_int32),
+                ("cx", ctypes.c_int32), ("cz", ctypes.c_int32), ("floor_y", ctypes.c_int32),
+                ("flags", ctypes.c_int32)]
+
+
+class Sink(ctypes.Structure):
+    _fields_ = [("hits", ctypes.POINTER(Hit)), ("capacity", ctypes.c_uint64), ("count", ctypes.c_uint64)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("seeds", ctypes.c_uint64), ("checks", ctypes.c_uint64), ("accepted", ctypes.c_uint64),
+                ("reached_ladder", ctypes.c_uint64), ("funnel", ctypes.c_uint64 * KIND_N),
+                ("tiers", ctypes.c_uint64 * 8), ("samplers_built", ctypes.c_uint64), ("device_ms", ctypes.c_double),
+                ("kernel_ms", ctypes.c_double * 8)]
+
+
+KERNEL_FAMILIES = ("seeds", "build_erosion", "erosion_gate", "compact", "build_climate", "gate2", "build_ladder",
+                   "gate3")
+
+_lib = None
+
+
+def load():
+    """The loaded C library; raises if the in-tree build is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libwhf_hip.so is not built (run __graft_entry__.build()): %s" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.whf_open.restype = ctypes.c_int
+        L.whf_open.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_void_p)]
+        L.whf_close.restype = None
+        L.whf_close.argtypes = [ctypes.c_void_p]
+        L.whf_scan.restype = ctypes.c_int
+        L.whf_scan.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(Sink), ctypes.POINTER(Stats)]
+        L.whf_probe.restype = ctypes.c_int
+        L.whf_probe.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
+                                ctypes.POINTER(ctypes.c_int32), ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_double)]
+        L.whf_version_flags.restype = ctypes.c_int
+        L.whf_version_flags.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
+        L.whf_strerror.restype = ctypes.c_char_p
+        L.whf_strerror.argtypes = [ctypes.c_int]
+        L.whf_device_name.restype = ctypes.c_char_p
+        L.whf_device_name.argtypes = [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+class WhfError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__("%s: %s (%d)" % (what, load().whf_strerror(code).decode(), code))
+        self.code = code
+
+
+def version_flags(name):
+    v = ctypes.c_int(0)
+    rc = load().whf_version_flags(name.encode(), ctypes.byref(v))
+    if rc:
+        raise WhfError(rc, "version %r" % name)
+    return v.value
+
+
+class Context:
+    """One device context (whf_open / whf_close)."""
+
+    def __init__(self, device=0, profile=False):
+        self.L = load()
+        self.h = ctypes.c_void_p()
+        cfg = Config(device, CFG_PROFILE if profile else 0, 0)
+        rc = self.L.whf_open(ctypes.byref(cfg), ctypes.byref(self.h))
+        if rc:
+            raise WhfError(rc, "whf_open")
+
+    def close(self):
+        if self.h:
+            self.L.whf_close(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def device_name(self):
+        return self.L.whf_device_name(self.h).decode()
+
+    def scan(self, rect, seeds=None, splitmix=None, counter=None, max_height=-50, vflags=0, capacity=1 << 16,
+             salt=0):
+        """Phase-1 Seed Check. rect = (rx0, rx1, rz0, rz1); seeds = int64 array, or splitmix/counter =
+        (stream, count[, offset]). Returns (hits as list of tuples (seed, rx, rz, cx, cz), stats dict)."""
+        req = Request()
+        req.precision = PREC_FP64
+        keep = None
+        if seeds is not None:
+            keep = np.ascontiguousarray(seeds, dtype=np.int64)
+            req.seed_mode = SEEDS_LIST
+            req.seeds = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+            req.n_seeds = len(keep)
+        else:
+            gen = splitmix if splitmix is not None else counter
+            req.seed_mode = SEEDS_SPLITMIX if splitmix is not None else SEEDS_COUNTER
+            req.stream = gen[0] & 0xFFFFFFFFFFFFFFFF
+            req.n_seeds = gen[1]
+            req.offset = gen[2] if len(gen) > 2 else 0
+        req.rx0, req.rx1, req.rz0, req.rz1 = rect
+        req.max_height = max_height
+        req.vflags = vflags
+        req.salt = salt
+        buf = (Hit * capacity)()
+        sink = Sink(buf, capacity, 0)
+        st = Stats()
+        rc = self.L.whf_scan(self.h, ctypes.byref(req), ctypes.byref(sink), ctypes.byref(st))
+        if rc not in (OK, EOVERFLOW):
+            raise WhfError(rc, "whf_scan")
+        n = min(sink.count, capacity)
+        hits = sorted((buf[i].seed, buf[i].rx, buf[i].rz, buf[i].cx, buf[i].cz) for i in range(n))
+        stats = dict(seeds=st.seeds, checks=st.checks, accepted=st.accepted, reached_ladder=st.reached_ladder,
+                     funnel=list(st.funnel), tiers=list(st.tiers)[:7], samplers_built=st.samplers_built,
+                     device_ms=st.device_ms, kernel_ms=dict(zip(KERNEL_FAMILIES, list(st.kernel_ms))),
+                     hit_count=sink.count, overflow=rc == EOVERFLOW)
+        return hits, stats
+
+    def probe(self, seeds, xs, zs, max_height=-50, vflags=0):
+        """The 38 raw phase-1 values per (seed, hut_x, hut_z), as uint64 bit patterns (n x 38)."""
+        s = np.ascontiguousarray(seeds, dtype=np.int64)
+        x = np.ascontiguousarray(xs, dtype=np.int32)
+        z = np.ascontiguousarray(zs, dtype=np.int32)
+        out = np.zeros((len(s), 38), dtype=np.float64)
+        rc = self.L.whf_probe(self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                              x.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                              z.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(s), max_height, vflags,
+                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if rc:
+            raise WhfError(rc, "whf_probe")
+        return out.view(np.uint64)

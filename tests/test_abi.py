@@ -1,0 +1,66 @@
+"""CPU-side checks of the C ABI library and the phase1_scan host (no GPU compute)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from product import whf
+
+HEADER = os.path.join(whf.HERE, "..", "include", "whf.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(whf_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = whf.load()
+    names = declared_functions()
+    assert names, "no declarations parsed"
+    assert set(names) == set(whf.SYMBOLS)
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_version_flags_follow_lysh_names():
+    assert whf.version_flags("1.20") == 0
+    assert whf.version_flags("1.21") == 0
+    assert whf.version_flags("1.19.x~26.1") == 0
+    assert whf.version_flags("1.18.2") == whf.V_118
+    assert whf.version_flags("1.18") == whf.V_118
+    assert whf.version_flags("26.2") == whf.V_262
+    assert whf.version_flags("26.3") == whf.V_262
+    with pytest.raises(whf.WhfError):
+        whf.version_flags("1.17")
+
+
+def test_strerror_texts():
+    L = whf.load()
+    for code in (whf.OK, whf.EINVAL, whf.ENOMEM, whf.EOVERFLOW, whf.ENOTSUP):
+        assert L.whf_strerror(code)
+    assert L.whf_strerror(12345) == b"unknown error"
+
+
+def test_open_rejects_null_out():
+    L = whf.load()
+    assert L.whf_open(None, None) == whf.EINVAL
+
+
+def run_cli(*args, stdin=None):
+    return subprocess.run([whf.PHASE1_SCAN, *args], input=stdin, capture_output=True, text=True, timeout=60)
+
+
+@pytest.mark.skipif(not os.path.exists(whf.PHASE1_SCAN), reason="phase1_scan not built")
+def test_cli_argument_errors_exit_2_before_touching_the_gpu():
+    for args in (["--bogus"], ["--seed"], ["--min-region-x", "3", "--max-region-x", "3", "--seed", "1"],
+                 ["--self-test", "--seed", "1"], ["--quad-bases", "x.txt"], ["--version", "1.17", "--seed", "1"],
+                 ["--seed-gen", "1", "0"], ["--seed-gen", "5", "10", "--seed", "3"], ["--max-height"]):
+        r = run_cli(*args)
+        assert r.returncode == 2, (args, r.returncode, r.stderr)
+        assert "usage: phase1_scan" in r.stdout
+    r = run_cli()
+    assert r.returncode == 2
+    assert run_cli("--help").returncode == 0
