@@ -87,7 +87,9 @@ typedef struct {
     uint64_t reached_ladder;    /* regions past the climate gates (lysh scan's "reached cave ladder") */
     uint64_t funnel[WHF_KIND_N];/* per reject kind; [10] = accepted */
     uint64_t tiers[8];          /* erosion tier where the tiered gate fired (noise.c:362-377); [7] unused */
-    uint64_t samplers_built;    /* Perlin samplers constructed on the device */
+    uint64_t samplers_built[3]; /* Perlin samplers constructed on the device per set: erosion (8 per seed),
+                                   climate + Entrance (28 per seed past erosion), ladder tail (36 per seed past
+                                   Entrance@50/60) */
     double device_ms;           /* device time of the scan (HIP events around the kernels) */
     double kernel_ms[8];        /* per kernel family: seeds, erosion build, erosion gate, compact, late build, late gate */
 } whf_stats;

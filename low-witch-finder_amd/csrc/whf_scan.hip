@@ -44,7 +44,7 @@ enum Cnt {
     CNT_TIER0 = 0,                       // 0..6 tiers
     CNT_KIND0 = 8,                       // 8 + kind, kinds 1..10 (kind 0 = sum of tiers)
     CNT_LADDER = 20,
-    CNT_BUILT = 21,
+    CNT_BUILT = 21,                      // 21 + set
     CNT_N = 24
 };
 
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
                                               int set_n, const uint32_t *__restrict__ slot_seed,
                                               const uint32_t *__restrict__ n_slots_dev, uint32_t n_slots_host,
                                               uint8_t *__restrict__ out_tab, SPar *__restrict__ out_par,
-                                              unsigned long long *__restrict__ cnt) {
+                                              unsigned long long *__restrict__ built_cnt) {
     __shared__ uint32_t words[64 * 64];
     uint8_t *bytes = reinterpret_cast<uint8_t *>(words);
     const uint32_t lane = threadIdx.x;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
     // one atomic per wave
     unsigned long long tot = built;
     for (int o = 32; o > 0; o >>= 1) tot += __shfl_down(tot, o);
-    if (lane == 0 && tot) atomicAdd(&cnt[CNT_BUILT], tot);
+    if (lane == 0 && tot) atomicAdd(built_cnt, tot);
 }
 
 // ---------------------------------------------------------------------------------------------- erosion gate
@@ -772,7 +772,7 @@ int run_batch(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint6
         const uint32_t jobs = ns * 8;
         hipLaunchKernelGGL(k_build, dim3((jobs + 63) / 64), dim3(64), 0, st, c->d_seeds, bd_of(c, layout, SET_E), 8,
                            (const uint32_t *)nullptr, (const uint32_t *)nullptr, ns, c->d_tab[SET_E], c->d_par[SET_E],
-                           c->d_cnt);
+                           c->d_cnt + CNT_BUILT + SET_E);
     }
     {
         Prof p(c, F_EROSION);
@@ -805,7 +805,7 @@ int run_batch(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint6
         Prof p(c, F_BUILD_C);
         hipLaunchKernelGGL(k_build, dim3(std::min<uint32_t>(persist * 4, (ns * 28 + 63) / 64 + 1)), dim3(64), 0, st,
                            c->d_seeds, bd_of(c, layout, SET_C), 28, c->d_seedC, c->d_ctl + CTL_NC, 0u,
-                           c->d_tab[SET_C], c->d_par[SET_C], c->d_cnt);
+                           c->d_tab[SET_C], c->d_par[SET_C], c->d_cnt + CNT_BUILT + SET_C);
     }
     {
         Prof p(c, F_GATE2);
@@ -822,7 +822,7 @@ int run_batch(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint6
         Prof p(c, F_BUILD_L);
         hipLaunchKernelGGL(k_build, dim3(std::min<uint32_t>(persist, (ns * 36 + 63) / 64 + 1)), dim3(64), 0, st,
                            c->d_seeds, bd_of(c, layout, SET_L), 36, c->d_seedL, c->d_ctl + CTL_NL, 0u,
-                           c->d_tab[SET_L], c->d_par[SET_L], c->d_cnt);
+                           c->d_tab[SET_L], c->d_par[SET_L], c->d_cnt + CNT_BUILT + SET_L);
     }
     {
         Prof p(c, F_GATE3);
@@ -997,7 +997,7 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
         for (int k = 1; k < WHF_KIND_N; k++) stats->funnel[k] = cnt[CNT_KIND0 + k];
         stats->accepted = cnt[CNT_KIND0 + 10];
         stats->reached_ladder = cnt[CNT_LADDER];
-        stats->samplers_built = cnt[CNT_BUILT];
+        for (int s = 0; s < SET_COUNT; s++) stats->samplers_built[s] = cnt[CNT_BUILT + s];
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
         stats->device_ms = ms;
@@ -1033,7 +1033,7 @@ int whf_probe(whf_ctx *c, const int64_t *seeds, const int32_t *hut_x, const int3
         const uint32_t jobs = nn * kSetSize[s];
         hipLaunchKernelGGL(k_build, dim3((jobs + 63) / 64), dim3(64), 0, st, c->d_seeds, bd_of(c, vflags, s),
                            kSetSize[s], (const uint32_t *)nullptr, (const uint32_t *)nullptr, nn, c->d_tab[s],
-                           c->d_par[s], c->d_cnt);
+                           c->d_par[s], c->d_cnt + CNT_BUILT + s);
     }
     hipLaunchKernelGGL(k_probe, dim3((nn + 63) / 64), dim3(64), 0, st, c->d_gt + vflags, c->d_tab[SET_E],
                        c->d_par[SET_E], c->d_tab[SET_C], c->d_par[SET_C], c->d_tab[SET_L], c->d_par[SET_L], d_xz,

@@ -191,7 +191,8 @@ int run(whf_ctx *ctx, const Params &p) {
         for (int k = 0; k < WHF_KIND_N; k++) std::printf("%s%" PRIu64, k ? ", " : "", st.funnel[k]);
         std::printf("], \"tiers\": [");
         for (int k = 0; k < 7; k++) std::printf("%s%" PRIu64, k ? ", " : "", st.tiers[k]);
-        std::printf("], \"samplers_built\": %" PRIu64 ", \"device_ms\": %.3f}\n", st.samplers_built, st.device_ms);
+        std::printf("], \"samplers_built\": [%" PRIu64 ", %" PRIu64 ", %" PRIu64 "], \"device_ms\": %.3f}\n",
+                    st.samplers_built[0], st.samplers_built[1], st.samplers_built[2], st.device_ms);
     }
     if (!p.out.empty()) {
         FILE *f = std::fopen(p.out.c_str(), "w");

@@ -50,7 +50,7 @@ class Sink(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("seeds", ctypes.c_uint64), ("checks", ctypes.c_uint64), ("accepted", ctypes.c_uint64),
                 ("reached_ladder", ctypes.c_uint64), ("funnel", ctypes.c_uint64 * KIND_N),
-                ("tiers", ctypes.c_uint64 * 8), ("samplers_built", ctypes.c_uint64), ("device_ms", ctypes.c_double),
+                ("tiers", ctypes.c_uint64 * 8), ("samplers_built", ctypes.c_uint64 * 3), ("device_ms", ctypes.c_double),
                 ("kernel_ms", ctypes.c_double * 8)]
 
 
@@ -158,7 +158,7 @@ class Context:
         n = min(sink.count, capacity)
         hits = sorted((buf[i].seed, buf[i].rx, buf[i].rz, buf[i].cx, buf[i].cz) for i in range(n))
         stats = dict(seeds=st.seeds, checks=st.checks, accepted=st.accepted, reached_ladder=st.reached_ladder,
-                     funnel=list(st.funnel), tiers=list(st.tiers)[:7], samplers_built=st.samplers_built,
+                     funnel=list(st.funnel), tiers=list(st.tiers)[:7], samplers_built=list(st.samplers_built),
                      device_ms=st.device_ms, kernel_ms=dict(zip(KERNEL_FAMILIES, list(st.kernel_ms))),
                      hit_count=sink.count, overflow=rc == EOVERFLOW)
         return hits, stats
