@@ -203,6 +203,89 @@ __device__ __forceinline__ double perlin_3d(const Perm &P, const SPar &s, double
     return perlin_cell(P, (int)fi, (int)fj, (int)fk, d - fi, h, f - fk, fade(h));
 }
 
+// The same gradient dot product by selects: u = x or y, v = y, x or z, signs from the low two hash bits (Perlin's
+// improved-noise formulation of the same 16-entry table). Exactly the reference's three-term sum whenever the
+// result is non-zero, since adding the zero component or reordering two terms does not change a non-zero IEEE
+// sum; a zero result (only when both live coordinates are zero) is recomputed with the three-term form so that
+// its sign matches too.
+__device__ __forceinline__ double grad_sel(uint32_t h, double x, double y, double z) {
+    h &= 15u;
+    double u = h < 8u ? x : y;
+    double v = h < 4u ? y : ((h == 12u || h == 14u) ? x : z);
+    if (h & 1u) u = -u;
+    if (h & 2u) v = -v;
+    return u + v;
+}
+
+// y = 0 Perlin evaluation of two independent samples at once (two tables, two origins, two points), written as
+// lock-step lanes of a 2-array so the compiler interleaves the two dependency chains. Q(i) returns the byte pair
+// P(i & 255) | P((i + 1) & 255) << 8 of the permutation; every lookup and every arithmetic step is the one of
+// perlin_y0 above, so the two values are bit-identical to two perlin_y0 calls.
+template <typename Pairs>
+__device__ __forceinline__ void perlin_y0_x2(const Pairs *Q, const SPar *const *s, const double *x, const double *z,
+                                             double *out) {
+    double g[2], l[2], fx[2], fz[2];
+    int i[2], k[2];
+    uint32_t ab[2], cd[2], ef[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        double d = x[t] + s[t]->ox, f = z[t] + s[t]->oz;
+        double fi = floor(d), fk = floor(f);
+        i[t] = (int)fi;
+        k[t] = (int)fk;
+        g[t] = d - fi;
+        l[t] = f - fk;
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++) ab[t] = Q[t](i[t]);
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const int j = s[t]->jy;
+        cd[t] = Q[t]((int)(ab[t] & 255u) + j);
+        ef[t] = Q[t]((int)(ab[t] >> 8) + j);
+    }
+    uint32_t h0[2], h1[2], h2[2], h3[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        h0[t] = Q[t]((int)(cd[t] & 255u) + k[t]);   // P(c+k),  P(c+k+1)
+        h1[t] = Q[t]((int)(ef[t] & 255u) + k[t]);   // P(e+k),  P(e+k+1)
+        h2[t] = Q[t]((int)(cd[t] >> 8) + k[t]);     // P(d+k),  P(d+k+1)
+        h3[t] = Q[t]((int)(ef[t] >> 8) + k[t]);     // P(f+k),  P(f+k+1)
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const double h = s[t]->hy, hm = s[t]->hy1, gm = g[t] - 1, lm = l[t] - 1;
+        double c[8];
+        c[0] = grad_sel(h0[t], g[t], h, l[t]);
+        c[1] = grad_sel(h1[t], gm, h, l[t]);
+        c[2] = grad_sel(h2[t], g[t], hm, l[t]);
+        c[3] = grad_sel(h3[t], gm, hm, l[t]);
+        c[4] = grad_sel(h0[t] >> 8, g[t], h, lm);
+        c[5] = grad_sel(h1[t] >> 8, gm, h, lm);
+        c[6] = grad_sel(h2[t] >> 8, g[t], hm, lm);
+        c[7] = grad_sel(h3[t] >> 8, gm, hm, lm);
+        bool zero = false;
+#pragma unroll
+        for (int q = 0; q < 8; q++) zero |= c[q] == 0.0;
+        if (zero) {
+            c[0] = grad((int)h0[t], g[t], h, l[t]);
+            c[1] = grad((int)h1[t], gm, h, l[t]);
+            c[2] = grad((int)h2[t], g[t], hm, l[t]);
+            c[3] = grad((int)h3[t], gm, hm, l[t]);
+            c[4] = grad((int)(h0[t] >> 8), g[t], h, lm);
+            c[5] = grad((int)(h1[t] >> 8), gm, h, lm);
+            c[6] = grad((int)(h2[t] >> 8), g[t], hm, lm);
+            c[7] = grad((int)(h3[t] >> 8), gm, hm, lm);
+        }
+        fx[t] = fade(g[t]);
+        fz[t] = fade(l[t]);
+        const double fy = s[t]->fy;
+        const double lo = lerp(fy, lerp(fx[t], c[0], c[1]), lerp(fx[t], c[2], c[3]));
+        const double hi = lerp(fy, lerp(fx[t], c[4], c[5]), lerp(fx[t], c[6], c[7]));
+        out[t] = lerp(fz[t], lo, hi);
+    }
+}
+
 // OctavePerlinNoiseSampler.maintainPrecision (noise.c:28-34): identity inside +-1e7.
 __device__ __forceinline__ double keep_precision(double v) {
     if (v > -1.0e7 && v < 1.0e7) return v;

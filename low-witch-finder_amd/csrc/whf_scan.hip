@@ -160,15 +160,26 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
 }
 
 // ---------------------------------------------------------------------------------------------- erosion gate
+// Tiered erosion (noise.c:432-459) over (seed, region) items, level-synchronously: level L evaluates octave slot
+// L of both sub-samplers (evaluations 2L and 2L + 1) for every item still alive, as one interleaved pair, then
+// applies the tier checks that fall after those two evaluations in the reference's order. Items that survive are
+// compacted into the next level's list in LDS. The second evaluation of a pair is computed even when the check
+// after the first one rejects; it is then discarded, so the decisions and the tier histogram are the reference's.
 
-struct LPerm {
-    const uint8_t *p;
-    __device__ __forceinline__ int operator()(int i) const { return p[i & 255]; }
+struct LPair {
+    const uint16_t *p;
+    __device__ __forceinline__ uint32_t operator()(int i) const { return p[i & 255]; }
 };
 
 constexpr int kEroWG = 256;
-constexpr int kEroItems = 1024;         // items placed per pass (4 per lane)
-constexpr int kCzBias = 1 << 22;        // chunk z is stored biased so that 4 tile bits fit above it
+constexpr int kEroPass = 512;          // items per pass (2 per lane)
+constexpr int kCzBias = 1 << 22;       // chunk z is stored biased so that 4 tile bits fit above it
+
+__device__ __forceinline__ void unpack_item(int2 e, int &cx, int &cz, int &sl) {
+    cx = e.x;
+    sl = e.y >> 23;
+    cz = (e.y & ((1 << 23) - 1)) - kCzBias;
+}
 
 // ST seeds per workgroup. Items of a pass are (seed-in-tile, region) pairs laid out seed-major.
 template <int ST>
@@ -180,11 +191,12 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
                                                     uint4 *__restrict__ q1, uint32_t *__restrict__ ctl,
                                                     uint32_t *__restrict__ needC,
                                                     unsigned long long *__restrict__ cnt) {
-    __shared__ uint4 s_tab[ST * 8 * 16];
+    __shared__ uint16_t s_tab[ST * 8 * 256];     // byte-pair tables: P(i) | P(i + 1) << 8
     __shared__ SPar s_par[ST * 8];
-    __shared__ int2 s_item[kEroItems];
-    __shared__ double s_sc[8], s_w[8], s_thr[8];
-    __shared__ uint32_t s_head;
+    __shared__ int2 s_item[kEroPass];
+    __shared__ uint16_t s_li[2][kEroPass];       // alive lists: item index and running sum
+    __shared__ double s_ls[2][kEroPass];
+    __shared__ uint32_t s_n[4];                  // list sizes after levels 0, 1, 2
     __shared__ uint32_t s_cnt[8];
 
     const uint32_t tid = threadIdx.x;
@@ -195,113 +207,128 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
     const uint32_t rbase = ck * chunk;
     const uint32_t rn = min(chunk, flat_n - rbase);
 
-    {   // stage the tile's erosion tables and origins into LDS
-        const uint4 *src = reinterpret_cast<const uint4 *>(tabE + (size_t)seed0 * 8 * 256);
-        for (uint32_t w = tid; w < ns * 8 * 16; w += kEroWG) s_tab[w] = src[w];
-        for (uint32_t w = tid; w < ns * 8; w += kEroWG) s_par[w] = parE[(size_t)seed0 * 8 + w];
-        if (tid < 8) {
-            s_sc[tid] = gt->ero.sc[tid] * 1.0;
-            s_w[tid] = gt->ero.w[tid];
-            s_thr[tid] = tid >= 1 ? gt->ero.thr[tid - 1] : 0.0;
-            s_cnt[tid] = 0;
+    {   // stage the tile's erosion tables as byte pairs, and the origins
+        const uint8_t *src = tabE + (size_t)seed0 * 8 * 256;
+        for (uint32_t w = tid; w < ns * 8 * 256; w += kEroWG) {
+            const uint32_t t = w >> 8, i = w & 255;
+            s_tab[w] = (uint16_t)(src[w] | (uint32_t)src[(t << 8) | ((i + 1) & 255)] << 8);
         }
+        for (uint32_t w = tid; w < ns * 8; w += kEroWG) s_par[w] = parE[(size_t)seed0 * 8 + w];
+        if (tid < 8) s_cnt[tid] = 0;
+        if (tid < 4) s_n[tid] = 0;
     }
-    const double amplitude = gt->ero.amplitude;
-    // lacunarity of evaluation k: sc[k] is 1 or 1.0181268882175227, lac doubles per slot
-    __shared__ double s_lac[8];
-    if (tid < 8) s_lac[tid] = gt->ero.lac[tid];
+    const EroDesc &ed = gt->ero;
+    const double amplitude = ed.amplitude;
+    // first region of this chunk as (row, column); a region offset ro then needs one 32-bit division
+    const uint64_t fbase = flat0 + rbase;
+    const uint32_t row0 = (uint32_t)(fbase / W);
+    const uint32_t col0 = (uint32_t)(fbase - (uint64_t)row0 * W);
     __syncthreads();
 
-    const uint8_t *tabb = reinterpret_cast<const uint8_t *>(s_tab);
     const uint64_t total = (uint64_t)ns * rn;
-
-    for (uint64_t pass0 = 0; pass0 < total; pass0 += kEroItems) {
-        const uint32_t npass = (uint32_t)min((uint64_t)kEroItems, total - pass0);
-        // ---- placement of the pass's items (uniform work) ----
-        for (uint32_t i = tid; i < npass; i += kEroWG) {
-            const uint64_t it = pass0 + i;
-            const uint32_t sl = (uint32_t)(it / rn);
-            const uint32_t ro = (uint32_t)(it - (uint64_t)sl * rn);
-            const uint64_t flat = flat0 + rbase + ro;
-            const uint64_t row = flat / W;
-            const int rz = rz0 + (int)row;
-            const int rx = rx0 + (int)(flat - row * W);
-            int cx, cz;
-            hut_chunk(seeds[seed0 + sl], rx, rz, salt, cx, cz);
-            s_item[i] = make_int2(cx, (cz + kCzBias) | (int)(sl << 23));
-        }
-        if (tid == 0) s_head = kEroWG;
-        __syncthreads();
-
-        // ---- tiered erosion with a per-lane work queue: one Perlin evaluation per lane per step ----
-        uint32_t cur = tid;            // first item is static, later ones come from s_head
-        bool have = cur < npass;
-        int k = 0, sl = 0, cx = 0, cz = 0;
-        double x = 0.0, z = 0.0, sum = 0.0;
-        if (have) {
-            int2 e = s_item[cur];
-            cx = e.x;
-            sl = e.y >> 23;
-            cz = (e.y & ((1 << 23) - 1)) - kCzBias;
-            x = (double)(cx * 16 + 8) / 4.0;
-            z = (double)(cz * 16 + 8) / 4.0;
-        }
-        for (;;) {
-            bool pass_now = false;
-            if (have) {
+    for (uint64_t pass0 = 0; pass0 < total; pass0 += kEroPass) {
+        const uint32_t npass = (uint32_t)min((uint64_t)kEroPass, total - pass0);
+        // ---- level 0: placement, evaluations 0 and 1, tier 0 ----
+        for (uint32_t i0 = tid & ~63u; i0 < npass; i0 += kEroWG) {
+            const uint32_t i = i0 + lane_id();
+            bool alive = false;
+            double sum = 0.0;
+            if (i < npass) {
+                const uint32_t it = (uint32_t)(pass0 + i);
+                const uint32_t sl = it / rn;
+                const uint32_t c = col0 + (it - sl * rn);
+                const uint32_t drow = c / W;
+                int cx, cz;
+                hut_chunk(seeds[seed0 + sl], rx0 + (int)(c - drow * W), rz0 + (int)(row0 + drow), salt, cx, cz);
+                s_item[i] = make_int2(cx, (cz + kCzBias) | (int)(sl << 23));
                 if (single) {
-                    pass_now = true;
-                    have = false;
+                    alive = true;
                 } else {
-                    const double sc = s_sc[k], lac = s_lac[k];
-                    const double xs = keep_precision(x * sc * lac);
-                    const double zs = keep_precision(z * sc * lac);
-                    const uint32_t t = (uint32_t)sl * 8 + k;
-                    const double v = perlin_y0(LPerm{tabb + t * 256}, s_par[t], xs, zs);
-                    sum += s_w[k] * v;
-                    if (k >= 1) {
-                        if (sum * amplitude < s_thr[k]) {
-                            atomicAdd(&s_cnt[k - 1], 1u);
-                            have = false;
-                        } else if (k == 7) {
-                            pass_now = true;
-                            have = false;
+                    const double x = (double)(cx * 16 + 8) / 4.0, z = (double)(cz * 16 + 8) / 4.0;
+                    const LPair Q[2] = {LPair{s_tab + (sl * 8 + 0) * 256}, LPair{s_tab + (sl * 8 + 1) * 256}};
+                    const SPar *P[2] = {&s_par[sl * 8 + 0], &s_par[sl * 8 + 1]};
+                    const double xs[2] = {keep_precision(x * ed.sc[0] * ed.lac[0]), keep_precision(x * ed.sc[1] * ed.lac[1])};
+                    const double zs[2] = {keep_precision(z * ed.sc[0] * ed.lac[0]), keep_precision(z * ed.sc[1] * ed.lac[1])};
+                    double v[2];
+                    perlin_y0_x2(Q, P, xs, zs, v);
+                    sum += ed.w[0] * v[0];
+                    sum += ed.w[1] * v[1];
+                    if (sum * amplitude < ed.thr[0]) atomicAdd(&s_cnt[0], 1u);
+                    else alive = true;
+                }
+            }
+            if (single) {
+                const uint32_t pos = wave_append(alive, &ctl[CTL_Q1]);
+                if (alive) {
+                    const int2 e = s_item[i];
+                    int cx, cz, sl;
+                    unpack_item(e, cx, cz, sl);
+                    q1[pos] = make_uint4(seed0 + (uint32_t)sl, (uint32_t)cx, (uint32_t)cz, 0u);
+                    needC[seed0 + sl] = 1u;
+                }
+            } else {
+                const uint32_t pos = wave_append(alive, &s_n[0]);
+                if (alive) {
+                    s_li[0][pos] = (uint16_t)i;
+                    s_ls[0][pos] = sum;
+                }
+            }
+        }
+        __syncthreads();
+        if (!single) {
+            // ---- levels 1..3: evaluations 2L, 2L + 1 and tiers 2L - 1, 2L ----
+            for (int L = 1; L <= 3; L++) {
+                const int cur = (L - 1) & 1, nxt = L & 1;
+                const uint32_t n = s_n[L - 1];
+                const int k0 = 2 * L, k1 = 2 * L + 1;
+                for (uint32_t i0 = tid & ~63u; i0 < n; i0 += kEroWG) {
+                    const uint32_t q = i0 + lane_id();
+                    bool alive = false;
+                    uint32_t idx = 0;
+                    double sum = 0.0;
+                    int cx = 0, cz = 0, sl = 0;
+                    if (q < n) {
+                        idx = s_li[cur][q];
+                        sum = s_ls[cur][q];
+                        unpack_item(s_item[idx], cx, cz, sl);
+                        const double x = (double)(cx * 16 + 8) / 4.0, z = (double)(cz * 16 + 8) / 4.0;
+                        const LPair Q[2] = {LPair{s_tab + (sl * 8 + k0) * 256}, LPair{s_tab + (sl * 8 + k1) * 256}};
+                        const SPar *P[2] = {&s_par[sl * 8 + k0], &s_par[sl * 8 + k1]};
+                        const double xs[2] = {keep_precision(x * ed.sc[k0] * ed.lac[k0]),
+                                              keep_precision(x * ed.sc[k1] * ed.lac[k1])};
+                        const double zs[2] = {keep_precision(z * ed.sc[k0] * ed.lac[k0]),
+                                              keep_precision(z * ed.sc[k1] * ed.lac[k1])};
+                        double v[2];
+                        perlin_y0_x2(Q, P, xs, zs, v);
+                        sum += ed.w[k0] * v[0];
+                        if (sum * amplitude < ed.thr[k0 - 1]) {
+                            atomicAdd(&s_cnt[k0 - 1], 1u);
+                        } else {
+                            sum += ed.w[k1] * v[1];
+                            if (sum * amplitude < ed.thr[k1 - 1]) atomicAdd(&s_cnt[k1 - 1], 1u);
+                            else alive = true;
                         }
                     }
-                    k++;
-                }
-            }
-            // survivors of the erosion gate -> Q1
-            const uint32_t pos = wave_append(pass_now, &ctl[CTL_Q1]);
-            if (pass_now) {
-                q1[pos] = make_uint4(seed0 + (uint32_t)sl, (uint32_t)cx, (uint32_t)cz, 0u);
-                needC[seed0 + sl] = 1u;
-            }
-            // refill lanes whose item finished
-            const bool need = !have && cur < npass;
-            unsigned long long nmask = __ballot(need);
-            if (nmask) {
-                const int leader = __ffsll((long long)nmask) - 1;
-                uint32_t base = 0;
-                if ((int)lane_id() == leader) base = atomicAdd(&s_head, (uint32_t)__popcll(nmask));
-                base = __shfl(base, leader);
-                if (need) {
-                    cur = base + (uint32_t)__popcll(nmask & ((1ull << lane_id()) - 1ull));
-                    if (cur < npass) {
-                        const int2 e = s_item[cur];
-                        cx = e.x;
-                        sl = e.y >> 23;
-                        cz = (e.y & ((1 << 23) - 1)) - kCzBias;
-                        x = (double)(cx * 16 + 8) / 4.0;
-                        z = (double)(cz * 16 + 8) / 4.0;
-                        k = 0;
-                        sum = 0.0;
-                        have = true;
+                    if (L < 3) {
+                        const uint32_t pos = wave_append(alive, &s_n[L]);
+                        if (alive) {
+                            s_li[nxt][pos] = (uint16_t)idx;
+                            s_ls[nxt][pos] = sum;
+                        }
+                    } else {
+                        // survivors of the erosion gate -> Q1
+                        const uint32_t pos = wave_append(alive, &ctl[CTL_Q1]);
+                        if (alive) {
+                            q1[pos] = make_uint4(seed0 + (uint32_t)sl, (uint32_t)cx, (uint32_t)cz, 0u);
+                            needC[seed0 + sl] = 1u;
+                        }
                     }
                 }
+                __syncthreads();
             }
-            if (!__ballot(have)) break;
         }
+        // every thread has read the list sizes of this pass: clear them for the next one
+        if (tid < 4) s_n[tid] = 0;
         __syncthreads();
     }
     if (tid < 7 && s_cnt[tid]) atomicAdd(&cnt[CNT_TIER0 + tid], (unsigned long long)s_cnt[tid]);
@@ -781,7 +808,7 @@ int run_batch(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint6
         const GateTables *gt = c->d_gt + layout;
         // seeds per tile: narrow rects want many seeds per workgroup to fill the lanes
         const bool wide = fn >= 64;
-        const int ST = wide ? 4 : 16;
+        const int ST = wide ? 4 : 8;
         const uint32_t chunk = std::min<uint32_t>(fn, 1u << 16);
         const uint32_t n_chunks = (fn + chunk - 1) / chunk;
         const uint32_t tiles = (ns + ST - 1) / ST;
@@ -791,7 +818,7 @@ int run_batch(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint6
                                c->d_par[SET_E], gt, req->rx0, req->rz0, W, f0, fn, chunk, n_chunks, salt, single,
                                c->d_q1, c->d_ctl, c->d_needC, c->d_cnt);
         else
-            hipLaunchKernelGGL(k_erosion<16>, grid, dim3(kEroWG), 0, st, c->d_seeds, ns, c->d_tab[SET_E],
+            hipLaunchKernelGGL(k_erosion<8>, grid, dim3(kEroWG), 0, st, c->d_seeds, ns, c->d_tab[SET_E],
                                c->d_par[SET_E], gt, req->rx0, req->rz0, W, f0, fn, chunk, n_chunks, salt, single,
                                c->d_q1, c->d_ctl, c->d_needC, c->d_cnt);
     }
