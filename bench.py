@@ -34,6 +34,42 @@ PEAK_VALU_OPS = 256 * 4 * 16 * 2.4e9
 OPS_PER_SAMPLER = 271 * 21
 
 
+# Algorithmic FP64 operations of one y = 0 Perlin evaluation as the reference writes it (noise.c:292-318 with
+# its octave accumulation noise.c:441-449): 4 coordinate multiplies, 2 origin adds, 2 floors, 4 fraction
+# subtractions, 8 gradients x (3 multiplies + 2 adds), 2 fades x 7, 7 lerps x 3, and weight, sum, amplitude.
+OPS_PER_Y0_EVAL = 4 + 2 + 2 + 4 + 8 * 5 + 2 * 7 + 7 * 3 + 3
+KERNEL_OF = {"seeds": "k_seeds", "build_erosion": "k_build (set E)", "erosion_gate": "k_erosion<4>",
+             "compact": "k_compact", "build_climate": "k_build (set C)", "gate2": "k_gate2",
+             "build_ladder": "k_build (set L)", "gate3": "k_gate3"}
+WORK_UNIT = {"build_erosion": "Perlin sampler built = 271 Xoroshiro128++ steps x 21 int32 lane-ops",
+             "build_climate": "Perlin sampler built = 271 Xoroshiro128++ steps x 21 int32 lane-ops",
+             "build_ladder": "Perlin sampler built = 271 Xoroshiro128++ steps x 21 int32 lane-ops",
+             "erosion_gate": "y=0 Perlin evaluation = %d FP64 ops; evaluations per region from the tier histogram"
+                             % OPS_PER_Y0_EVAL}
+
+
+def algorithmic_ops(st):
+    """Algorithmic lane-operations per kernel family of one scan, from the scan's own counters."""
+    built = st["samplers_built"]
+    tiers, funnel = st["tiers"], st["funnel"]
+    # tier t fires after evaluation t + 1, i.e. after t + 2 evaluations; survivors take all 8
+    evals = sum((t + 2) * n for t, n in enumerate(tiers)) + 8 * (st["checks"] - funnel[0])
+    return {"build_erosion": built[0] * OPS_PER_SAMPLER, "build_climate": built[1] * OPS_PER_SAMPLER,
+            "build_ladder": built[2] * OPS_PER_SAMPLER, "erosion_gate": evals * OPS_PER_Y0_EVAL}
+
+
+def traffic_from_profiles(family):
+    """HBM bytes per launch of a kernel family from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE scaled by the copy-kernel calibration), or None when it was not measured."""
+    path = os.path.join(REPO, "profiles", "traffic_c2.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get(KERNEL_OF[family])
+
+
 def cpu_baseline(n_seeds_max, seconds):
     exe = os.path.join(REPO, "oracle", "_ref", "lysh_cpu_bench")
     if not os.path.exists(exe):
@@ -85,20 +121,23 @@ def main():
     with whf.Context(0, profile=True) as ctx:
         _, prof = ctx.scan(rect, **kw)
     fam = prof["kernel_ms"]
-    dominant = max(fam, key=fam.get)
     n_batches = -(-args.seeds // (1 << 17))
     built = prof["samplers_built"]
-    set_of = {"build_erosion": 0, "build_climate": 1, "build_ladder": 2}
-    if dominant in set_of:
-        ops = built[set_of[dominant]] * OPS_PER_SAMPLER
-        achieved = ops / (fam[dominant] / 1e3)
-        roofline = dict(bound="valu", achieved=round(achieved / 1e12, 3), peak=round(PEAK_VALU_OPS / 1e12, 2),
-                        unit="Tops/s", frac=round(achieved / PEAK_VALU_OPS, 4), traffic=None,
-                        kernel="k_build (%s)" % dominant, launch_ms=round(fam[dominant] / n_batches, 4),
-                        ops_per_launch=ops // n_batches)
-    else:
-        roofline = dict(bound="valu", achieved=None, peak=round(PEAK_VALU_OPS / 1e12, 2), unit="Tops/s", frac=None,
-                        traffic=None, kernel=dominant, launch_ms=round(fam[dominant] / n_batches, 4))
+    work = algorithmic_ops(prof)
+    rooflines = {}
+    for name, ms in fam.items():
+        ops = work.get(name)
+        if ops is None or ms <= 0:
+            continue
+        achieved = ops / (ms / 1e3)
+        rooflines[name] = dict(achieved=round(achieved / 1e12, 3), frac=round(achieved / PEAK_VALU_OPS, 4),
+                               launch_ms=round(ms / n_batches, 4), ops_per_launch=int(ops // n_batches))
+    dominant = max(fam, key=fam.get)
+    r = rooflines.get(dominant, {})
+    roofline = dict(bound="valu", achieved=r.get("achieved"), peak=round(PEAK_VALU_OPS / 1e12, 2), unit="Tops/s",
+                    frac=r.get("frac"), traffic=traffic_from_profiles(dominant), kernel=KERNEL_OF[dominant],
+                    launch_ms=round(fam[dominant] / n_batches, 4), ops_per_launch=r.get("ops_per_launch"),
+                    work_unit=WORK_UNIT.get(dominant))
 
     cpu = None if args.no_cpu else cpu_baseline(min(args.seeds, 1 << 20), args.cpu_seconds)
     line = {
@@ -110,7 +149,7 @@ def main():
                    "seeds": args.seeds, "regions_per_seed": 256, "version": "1.20", "precision": "fp64"},
         "roofline": roofline, "cpu_baseline": cpu,
         "funnel": last["funnel"], "tiers": last["tiers"], "accepted": last["accepted"],
-        "kernel_ms": {k: round(v, 3) for k, v in fam.items()}, "samplers_built": built,
+        "kernel_ms": {k: round(v, 3) for k, v in fam.items()}, "rooflines": rooflines, "samplers_built": built,
         "device_ms": round(last["device_ms"], 3),
     }
     print(json.dumps(line), flush=True)

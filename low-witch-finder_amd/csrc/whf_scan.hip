@@ -126,11 +126,14 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
 #pragma unroll 8
         for (int w = 0; w < 64; w++) words[w * 64 + lane] = 0x03020100u + (uint32_t)w * 0x04040404u;
         uint32_t ai = lane * 4;   // byte address of position i (i = 0)
+        // The draw of step i + 1 is computed between the reads and the writes of step i: it does not depend
+        // on the table, so it fills the LDS read latency.
+        uint32_t jj = xr_next_bounded(pr, 256u);
         for (int i = 0; i < 256; i++) {
-            const uint32_t jj = (uint32_t)i + xr_next_bounded(pr, 256u - (uint32_t)i);
             const uint32_t aj = ((jj >> 2) << 8) + lane * 4 + (jj & 3);
             const uint8_t vi = bytes[ai];
             const uint8_t vj = bytes[aj];
+            if (i < 255) jj = (uint32_t)(i + 1) + xr_next_bounded(pr, 255u - (uint32_t)i);
             bytes[ai] = vj;
             bytes[aj] = vi;
             ai += ((i & 3) == 3) ? 256 - 3 : 1;
