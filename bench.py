@@ -104,22 +104,29 @@ def main():
     rect = (-8, 8, -8, 8)
     vflags = whf.version_flags("1.20")
     kw = dict(splitmix=(2, args.seeds), max_height=-50, vflags=vflags, capacity=1 << 20)
+    timing = {}
+    results = {}
     with whf.Context(0) as ctx:
-        for _ in range(args.warmup):
-            ctx.scan(rect, **kw)
-        # whf_scan returns after synchronizing the library's stream, so the wall clock between calls brackets
-        # exactly the device work of the K steps
-        t0 = time.perf_counter()
-        last = None
-        for _ in range(args.steps):
-            hits, last = ctx.scan(rect, **kw)
-        t1 = time.perf_counter()
-    secs = (t1 - t0) / args.steps
+        for prec in (whf.PREC_FP64, whf.PREC_FP32_GATE):
+            for _ in range(args.warmup):
+                ctx.scan(rect, precision=prec, **kw)
+            # whf_scan returns after synchronizing the library's stream, so the wall clock between calls brackets
+            # exactly the device work of the K steps
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                hits, st = ctx.scan(rect, precision=prec, **kw)
+            timing[prec] = (time.perf_counter() - t0) / args.steps
+            results[prec] = (hits, st)
+        err = ctx.noise_error([int(x) for x in range(1, 257)], 4096)
+    h64, s64 = results[whf.PREC_FP64]
+    h32, last = results[whf.PREC_FP32_GATE]
+    identical = h64 == h32 and all(s64[k] == last[k] for k in ("funnel", "tiers", "accepted", "checks"))
+    secs = timing[whf.PREC_FP32_GATE]
     checks = last["checks"]
 
     # profiled step: per-kernel-family device time from HIP events recorded on the library's stream
     with whf.Context(0, profile=True) as ctx:
-        _, prof = ctx.scan(rect, **kw)
+        _, prof = ctx.scan(rect, precision=whf.PREC_FP32_GATE, **kw)
     fam = prof["kernel_ms"]
     n_batches = -(-args.seeds // (1 << 17))
     built = prof["samplers_built"]
@@ -143,10 +150,16 @@ def main():
     line = {
         "metric": METRIC, "value": round(checks / secs, 1), "unit": "checks/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(secs * 1e3, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f32-gate+f64", "data": "synthetic",
         "config": {"workload": "Seed Check at scale: %d SplitMix64(2) seeds x regions [-8,7]^2 (256 per seed), "
-                               "version 1.20, phase-1 maxHeight -50, plain FP64 kernel chain" % args.seeds,
-                   "seeds": args.seeds, "regions_per_seed": 256, "version": "1.20", "precision": "fp64"},
+                               "version 1.20, phase-1 maxHeight -50, FP32-gated erosion stage with FP64 "
+                               "re-evaluation (value) vs the plain FP64 chain (fp64_value)" % args.seeds,
+                   "seeds": args.seeds, "regions_per_seed": 256, "version": "1.20", "precision": "fp32-gate"},
+        "fp64_value": round(checks / timing[whf.PREC_FP64], 1), "fp64_ms_per_step": round(timing[whf.PREC_FP64] * 1e3, 3),
+        "fp32_gate_over_fp64": round(timing[whf.PREC_FP64] / secs, 3), "finds_identical_fp32_vs_fp64": identical,
+        "fp64_rechecks": last["fp64_rechecks"],
+        "fp32_error": {"stage": "erosion (8 octave samplers, y=0)", "max_abs_per_octave": err,
+                       "bound_used": whf.fp32_bound()},
         "roofline": roofline, "cpu_baseline": cpu,
         "funnel": last["funnel"], "tiers": last["tiers"], "accepted": last["accepted"],
         "kernel_ms": {k: round(v, 3) for k, v in fam.items()}, "rooflines": rooflines, "samplers_built": built,

@@ -39,8 +39,10 @@ extern "C" {
 #define WHF_V_LARGE    4   /* LARGE_BIOMES preset: the _large climate noises */
 #define WHF_V_SINGLE   8   /* SINGLE_BIOME preset: climate gates skipped */
 
-/* Arithmetic of the noise bulk. */
+/* Arithmetic of the noise bulk. Both give the same survivors and counters. */
 #define WHF_PREC_FP64      0   /* plain FP64 path, bit-exact with the reference CPU scanner */
+#define WHF_PREC_FP32_GATE 1   /* erosion stage: FP32 corner blend decides every tier check that is farther than
+                                  the stated error bound from its threshold; the rest is re-evaluated in FP64 */
 
 typedef struct whf_ctx whf_ctx;
 
@@ -91,7 +93,9 @@ typedef struct {
                                    climate + Entrance (28 per seed past erosion), ladder tail (36 per seed past
                                    Entrance@50/60) */
     double device_ms;           /* device time of the scan (HIP events around the kernels) */
-    double kernel_ms[8];        /* per kernel family: seeds, erosion build, erosion gate, compact, late build, late gate */
+    double kernel_ms[8];        /* with whf_config.flags & 1: device time per kernel family (seeds, erosion build,
+                                   erosion gate, compact, climate build, gate 2, ladder build, gate 3) */
+    uint64_t fp64_rechecks;     /* WHF_PREC_FP32_GATE: items the FP32 gate left to the FP64 re-evaluation */
 } whf_stats;
 
 /* Opens a context on a device and allocates its working buffers (replaces phase1_scan's allocateBuffers,
@@ -111,6 +115,15 @@ int whf_scan(whf_ctx *ctx, const whf_request *req, whf_sink *sink, whf_stats *st
  * (seeds[i], hut_x[i], hut_z[i]), computed by the device kernels; out has 38*n doubles. For parity checks. */
 int whf_probe(whf_ctx *ctx, const int64_t *seeds, const int32_t *hut_x, const int32_t *hut_z, uint64_t n,
               int max_height, int vflags, double *out);
+
+/* Measured error of the FP32-gate corner blend against FP64 for the 8 erosion samplers (the noise the FP32
+ * gate evaluates): max |v32 - v64| over points_per_seed points spread over the +-30M world for each seed.
+ * No counterpart in the reference; it reports the per-octave error beside whf_fp32_bound(). */
+int whf_noise_error(whf_ctx *ctx, const int64_t *seeds, uint64_t n_seeds, uint64_t points_per_seed, int vflags,
+                    double max_err[8]);
+
+/* The per-evaluation error bound the FP32 gate uses as its margin. */
+double whf_fp32_bound(void);
 
 /* Version name -> WHF_V_* flags, accepting lysh's version names (main.c:104-125, biome_trees.h:70499-70526). */
 int whf_version_flags(const char *name, int *vflags);

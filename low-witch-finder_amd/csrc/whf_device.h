@@ -286,6 +286,83 @@ __device__ __forceinline__ void perlin_y0_x2(const Pairs *Q, const SPar *const *
     }
 }
 
+// ---------------------------------------------------------------------------------------------- FP32 gate core
+// The same two evaluations with the lattice part (origin add, floor, fractions) in FP64 and the corner blend in
+// FP32: the fractions are rounded to float once, then gradient dots, fades and lerps run in single precision.
+// Worst-case error against the FP64 value, term by term (absolute):
+//   float inputs (fractions, hy, hy - 1, fade(hy)) are within 2^-25 of their doubles;
+//   a gradient dot (|.| <= 2): 2 x 2^-25 propagated + 2^-24 rounding                     <= 1.2e-7;
+//   fade(t) on [0, 1]: |f'| <= 1.875 on the input, six roundings at magnitudes <= 15    <= 1.6e-6;
+//   first lerp level: fade error x |b - a| <= 4, plus the dots' errors and roundings    <= 7.4e-6;
+//   second level (y fade, exact to 2^-25):                                              <= 8.0e-6;
+//   third level: 8.0e-6 + fade error x 4 + roundings                                     <= 1.5e-5.
+// The gate uses four times that, kFp32PerlinBound = 2^-14, as the margin around every threshold; the measured
+// maximum (whf_noise_error) is reported beside it.
+constexpr double kFp32PerlinBound = 6.103515625e-05;   // 2^-14
+
+__device__ __forceinline__ float grad_sel_f(uint32_t h, float x, float y, float z) {
+    h &= 15u;
+    float u = h < 8u ? x : y;
+    float v = h < 4u ? y : ((h == 12u || h == 14u) ? x : z);
+    if (h & 1u) u = -u;
+    if (h & 2u) v = -v;
+    return u + v;
+}
+__device__ __forceinline__ float fade_f(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
+__device__ __forceinline__ float lerp_f(float t, float a, float b) { return a + t * (b - a); }
+
+template <typename Pairs>
+__device__ __forceinline__ void perlin_y0_x2_f32(const Pairs *Q, const SPar *const *s, const double *x,
+                                                 const double *z, double *out) {
+    float g[2], l[2], gm[2], lm[2];
+    int i[2], k[2];
+    uint32_t ab[2], cd[2], ef[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const double d = x[t] + s[t]->ox, f = z[t] + s[t]->oz;
+        const double fi = floor(d), fk = floor(f);
+        i[t] = (int)fi;
+        k[t] = (int)fk;
+        const double gd = d - fi, ld = f - fk;
+        g[t] = (float)gd;
+        l[t] = (float)ld;
+        gm[t] = (float)(gd - 1);
+        lm[t] = (float)(ld - 1);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++) ab[t] = Q[t](i[t]);
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const int j = s[t]->jy;
+        cd[t] = Q[t]((int)(ab[t] & 255u) + j);
+        ef[t] = Q[t]((int)(ab[t] >> 8) + j);
+    }
+    uint32_t h0[2], h1[2], h2[2], h3[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        h0[t] = Q[t]((int)(cd[t] & 255u) + k[t]);
+        h1[t] = Q[t]((int)(ef[t] & 255u) + k[t]);
+        h2[t] = Q[t]((int)(cd[t] >> 8) + k[t]);
+        h3[t] = Q[t]((int)(ef[t] >> 8) + k[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const float h = (float)s[t]->hy, hm = (float)s[t]->hy1, fy = (float)s[t]->fy;
+        const float c0 = grad_sel_f(h0[t], g[t], h, l[t]);
+        const float c1 = grad_sel_f(h1[t], gm[t], h, l[t]);
+        const float c2 = grad_sel_f(h2[t], g[t], hm, l[t]);
+        const float c3 = grad_sel_f(h3[t], gm[t], hm, l[t]);
+        const float c4 = grad_sel_f(h0[t] >> 8, g[t], h, lm[t]);
+        const float c5 = grad_sel_f(h1[t] >> 8, gm[t], h, lm[t]);
+        const float c6 = grad_sel_f(h2[t] >> 8, g[t], hm, lm[t]);
+        const float c7 = grad_sel_f(h3[t] >> 8, gm[t], hm, lm[t]);
+        const float fx = fade_f(g[t]), fz = fade_f(l[t]);
+        const float lo = lerp_f(fy, lerp_f(fx, c0, c1), lerp_f(fx, c2, c3));
+        const float hi = lerp_f(fy, lerp_f(fx, c4, c5), lerp_f(fx, c6, c7));
+        out[t] = (double)lerp_f(fz, lo, hi);
+    }
+}
+
 // OctavePerlinNoiseSampler.maintainPrecision (noise.c:28-34): identity inside +-1e7.
 __device__ __forceinline__ double keep_precision(double v) {
     if (v > -1.0e7 && v < 1.0e7) return v;

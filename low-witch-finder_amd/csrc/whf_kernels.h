@@ -31,6 +31,7 @@ struct EroDesc {
     double sc[8], lac[8], w[8];
     double amplitude;
     double thr[7];
+    double margin[8];   // FP32 gate: bound on |estimate - FP64 value| of sum * amplitude after evaluation k
 };
 
 // Everything the gate kernels read about a request's noises.
@@ -49,6 +50,14 @@ struct SetView {
 struct GPerm {
     const uint8_t *p;
     __device__ __forceinline__ int operator()(int i) const { return p[i & 255]; }
+};
+
+// Byte pairs P(i) | P(i + 1) << 8 read from a 256-byte table in HBM.
+struct GPair {
+    const uint8_t *p;
+    __device__ __forceinline__ uint32_t operator()(int i) const {
+        return (uint32_t)p[i & 255] | (uint32_t)p[(i + 1) & 255] << 8;
+    }
 };
 
 // Double-Perlin 3D sample over tables in HBM (noise.c:263-289).

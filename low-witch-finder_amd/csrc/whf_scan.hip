@@ -37,7 +37,7 @@ namespace {
 constexpr int kWave = 64;
 
 // Device control words of a batch.
-enum Ctl { CTL_Q1 = 0, CTL_Q2, CTL_NC, CTL_NL, CTL_HITS, CTL_N };
+enum Ctl { CTL_Q1 = 0, CTL_Q2, CTL_NC, CTL_NL, CTL_QX, CTL_HITS, CTL_N };
 
 // Device counters of a scan (u64).
 enum Cnt {
@@ -45,7 +45,9 @@ enum Cnt {
     CNT_KIND0 = 8,                       // 8 + kind, kinds 1..10 (kind 0 = sum of tiers)
     CNT_LADDER = 20,
     CNT_BUILT = 21,                      // 21 + set
-    CNT_N = 24
+    CNT_QX = 24,                         // FP32 gate: items re-evaluated in FP64
+    CNT_QX_OVER = 25,                    // FP32 gate: re-evaluation queue overflow (the scan is redone in FP64)
+    CNT_N = 32
 };
 
 struct DevHit {
@@ -184,15 +186,38 @@ __device__ __forceinline__ void unpack_item(int2 e, int &cx, int &cz, int &sl) {
     cz = (e.y & ((1 << 23) - 1)) - kCzBias;
 }
 
-// ST seeds per workgroup. Items of a pass are (seed-in-tile, region) pairs laid out seed-major.
-template <int ST>
+constexpr uint32_t kQxCap = 1u << 22;   // FP32 gate re-evaluation queue
+
+// Tier decision of one check. Plain FP64: the reference's comparison. FP32 gate: decided only when the estimate
+// is farther than the margin from the threshold; otherwise the item is re-evaluated in FP64 (returns 2).
+template <bool F32>
+__device__ __forceinline__ int tier_decide(double val, double thr, double margin) {
+    if (!F32) return val < thr ? 0 : 1;
+    if (val < thr - margin) return 0;
+    if (val >= thr + margin) return 1;
+    return 2;
+}
+
+// Queue an undecided item for the FP64 re-evaluation (collective over the wave).
+__device__ __forceinline__ void push_recheck(bool pred, uint32_t slot, int cx, int cz, uint4 *qx, uint32_t *ctl,
+                                             unsigned long long *cnt) {
+    const uint32_t pos = wave_append(pred, &ctl[CTL_QX]);
+    if (pred) {
+        if (pos < kQxCap) qx[pos] = make_uint4(slot, (uint32_t)cx, (uint32_t)cz, 0u);
+        else atomicAdd(&cnt[CNT_QX_OVER], 1ull);
+    }
+}
+
+// ST seeds per workgroup. Items of a pass are (seed-in-tile, region) pairs laid out seed-major. F32 selects the
+// FP32-gated corner blend (perlin_y0_x2_f32) with FP64 re-evaluation of undecided items.
+template <int ST, bool F32>
 __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__ seeds, uint32_t n_seeds,
                                                     const uint8_t *__restrict__ tabE, const SPar *__restrict__ parE,
                                                     const GateTables *__restrict__ gt, int rx0, int rz0, uint32_t W,
                                                     uint64_t flat0, uint32_t flat_n, uint32_t chunk,
                                                     uint32_t n_chunks, int salt, int single,
                                                     uint4 *__restrict__ q1, uint32_t *__restrict__ ctl,
-                                                    uint32_t *__restrict__ needC,
+                                                    uint32_t *__restrict__ needC, uint4 *__restrict__ qx,
                                                     unsigned long long *__restrict__ cnt) {
     __shared__ uint16_t s_tab[ST * 8 * 256];     // byte-pair tables: P(i) | P(i + 1) << 8
     __shared__ SPar s_par[ST * 8];
@@ -234,14 +259,15 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
         // ---- level 0: placement, evaluations 0 and 1, tier 0 ----
         for (uint32_t i0 = tid & ~63u; i0 < npass; i0 += kEroWG) {
             const uint32_t i = i0 + lane_id();
-            bool alive = false;
+            bool alive = false, undecided = false;
             double sum = 0.0;
+            int cx = 0, cz = 0;
+            uint32_t sl = 0;
             if (i < npass) {
                 const uint32_t it = (uint32_t)(pass0 + i);
-                const uint32_t sl = it / rn;
+                sl = it / rn;
                 const uint32_t c = col0 + (it - sl * rn);
                 const uint32_t drow = c / W;
-                int cx, cz;
                 hut_chunk(seeds[seed0 + sl], rx0 + (int)(c - drow * W), rz0 + (int)(row0 + drow), salt, cx, cz);
                 s_item[i] = make_int2(cx, (cz + kCzBias) | (int)(sl << 23));
                 if (single) {
@@ -253,13 +279,17 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
                     const double xs[2] = {keep_precision(x * ed.sc[0] * ed.lac[0]), keep_precision(x * ed.sc[1] * ed.lac[1])};
                     const double zs[2] = {keep_precision(z * ed.sc[0] * ed.lac[0]), keep_precision(z * ed.sc[1] * ed.lac[1])};
                     double v[2];
-                    perlin_y0_x2(Q, P, xs, zs, v);
+                    if (F32) perlin_y0_x2_f32(Q, P, xs, zs, v);
+                    else perlin_y0_x2(Q, P, xs, zs, v);
                     sum += ed.w[0] * v[0];
                     sum += ed.w[1] * v[1];
-                    if (sum * amplitude < ed.thr[0]) atomicAdd(&s_cnt[0], 1u);
-                    else alive = true;
+                    const int dcs = tier_decide<F32>(sum * amplitude, ed.thr[0], ed.margin[1]);
+                    if (dcs == 0) atomicAdd(&s_cnt[0], 1u);
+                    else if (dcs == 1) alive = true;
+                    else undecided = true;
                 }
             }
+            if constexpr (F32) push_recheck(undecided, seed0 + sl, cx, cz, qx, ctl, cnt);
             if (single) {
                 const uint32_t pos = wave_append(alive, &ctl[CTL_Q1]);
                 if (alive) {
@@ -286,7 +316,7 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
                 const int k0 = 2 * L, k1 = 2 * L + 1;
                 for (uint32_t i0 = tid & ~63u; i0 < n; i0 += kEroWG) {
                     const uint32_t q = i0 + lane_id();
-                    bool alive = false;
+                    bool alive = false, undecided = false;
                     uint32_t idx = 0;
                     double sum = 0.0;
                     int cx = 0, cz = 0, sl = 0;
@@ -302,16 +332,23 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
                         const double zs[2] = {keep_precision(z * ed.sc[k0] * ed.lac[k0]),
                                               keep_precision(z * ed.sc[k1] * ed.lac[k1])};
                         double v[2];
-                        perlin_y0_x2(Q, P, xs, zs, v);
+                        if (F32) perlin_y0_x2_f32(Q, P, xs, zs, v);
+                        else perlin_y0_x2(Q, P, xs, zs, v);
                         sum += ed.w[k0] * v[0];
-                        if (sum * amplitude < ed.thr[k0 - 1]) {
+                        int dcs = tier_decide<F32>(sum * amplitude, ed.thr[k0 - 1], ed.margin[k0]);
+                        if (dcs == 0) {
                             atomicAdd(&s_cnt[k0 - 1], 1u);
-                        } else {
+                        } else if (dcs == 1) {
                             sum += ed.w[k1] * v[1];
-                            if (sum * amplitude < ed.thr[k1 - 1]) atomicAdd(&s_cnt[k1 - 1], 1u);
-                            else alive = true;
+                            dcs = tier_decide<F32>(sum * amplitude, ed.thr[k1 - 1], ed.margin[k1]);
+                            if (dcs == 0) atomicAdd(&s_cnt[k1 - 1], 1u);
+                            else if (dcs == 1) alive = true;
+                            else undecided = true;
+                        } else {
+                            undecided = true;
                         }
                     }
+                    if constexpr (F32) push_recheck(undecided, seed0 + (uint32_t)sl, cx, cz, qx, ctl, cnt);
                     if (L < 3) {
                         const uint32_t pos = wave_append(alive, &s_n[L]);
                         if (alive) {
@@ -335,6 +372,81 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
         __syncthreads();
     }
     if (tid < 7 && s_cnt[tid]) atomicAdd(&cnt[CNT_TIER0 + tid], (unsigned long long)s_cnt[tid]);
+}
+
+// FP64 re-evaluation of the items the FP32 gate left undecided: the reference's tiered gate from the first
+// evaluation on, with the set-E tables read from HBM (noise.c:432-459).
+__global__ __launch_bounds__(256) void k_erosion_exact(const uint8_t *__restrict__ tabE,
+                                                       const SPar *__restrict__ parE,
+                                                       const GateTables *__restrict__ gt,
+                                                       const uint4 *__restrict__ qx, uint4 *__restrict__ q1,
+                                                       uint32_t *__restrict__ ctl, uint32_t *__restrict__ needC,
+                                                       unsigned long long *__restrict__ cnt) {
+    __shared__ uint32_t s_cnt[8];
+    if (threadIdx.x < 8) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t n = min(ctl[CTL_QX], kQxCap);
+    const EroDesc &ed = gt->ero;
+    for (uint32_t q0 = blockIdx.x * 256 + (threadIdx.x & ~63u); q0 < n; q0 += gridDim.x * 256) {
+        const uint32_t q = q0 + lane_id();
+        bool alive = false;
+        uint4 e = make_uint4(0, 0, 0, 0);
+        if (q < n) {
+            e = qx[q];
+            const double x = (double)((int)e.y * 16 + 8) / 4.0, z = (double)((int)e.z * 16 + 8) / 4.0;
+            double sum = 0.0;
+            alive = true;
+            for (int k = 0; k < 8; k++) {
+                const size_t t = (size_t)e.x * 8 + k;
+                const double v = perlin_y0(GPerm{tabE + t * 256}, parE[t], keep_precision(x * ed.sc[k] * ed.lac[k]),
+                                           keep_precision(z * ed.sc[k] * ed.lac[k]));
+                sum += ed.w[k] * v;
+                if (k >= 1 && sum * ed.amplitude < ed.thr[k - 1]) {
+                    atomicAdd(&s_cnt[k - 1], 1u);
+                    alive = false;
+                    break;
+                }
+            }
+        }
+        const uint32_t pos = wave_append(alive, &ctl[CTL_Q1]);
+        if (alive) {
+            q1[pos] = e;
+            needC[e.x] = 1u;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 7 && s_cnt[threadIdx.x])
+        atomicAdd(&cnt[CNT_TIER0 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt[CNT_QX], (unsigned long long)ctl[CTL_QX]);
+}
+
+// Measured error of the FP32-gate corner blend against FP64, per set-E sampler (the erosion octaves): for
+// n_pts points per seed slot spread over the +-30M world, max |v32 - v64| as the bits of a non-negative double.
+__global__ void k_noise_error(const uint8_t *__restrict__ tabE, const SPar *__restrict__ parE,
+                              const GateTables *__restrict__ gt, uint32_t n_seeds, uint32_t n_pts,
+                              unsigned long long *__restrict__ maxbits) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n_seeds * n_pts) return;
+    const uint32_t slot = (uint32_t)(i / n_pts);
+    const uint64_t h = mix64(i * kGolden + 0x5EEDull);
+    const int hx = (int)(h % 60000001ull) - 30000000, hz = (int)((h >> 32) % 60000001ull) - 30000000;
+    const double x = (double)(hx + 8) / 4.0, z = (double)(hz + 8) / 4.0;
+    const EroDesc &ed = gt->ero;
+    for (int L = 0; L < 4; L++) {
+        const int k0 = 2 * L, k1 = k0 + 1;
+        const size_t t0 = (size_t)slot * 8 + k0, t1 = t0 + 1;
+        const GPair Q[2] = {GPair{tabE + t0 * 256}, GPair{tabE + t1 * 256}};
+        const SPar *P[2] = {&parE[t0], &parE[t1]};
+        const double xs[2] = {keep_precision(x * ed.sc[k0] * ed.lac[k0]), keep_precision(x * ed.sc[k1] * ed.lac[k1])};
+        const double zs[2] = {keep_precision(z * ed.sc[k0] * ed.lac[k0]), keep_precision(z * ed.sc[k1] * ed.lac[k1])};
+        double v64[2], v32[2];
+        perlin_y0_x2(Q, P, xs, zs, v64);
+        perlin_y0_x2_f32(Q, P, xs, zs, v32);
+        for (int t = 0; t < 2; t++) {
+            const double err = fabs(v32[t] - v64[t]);
+            atomicMax(&maxbits[k0 + t], (unsigned long long)__double_as_longlong(err));
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------- compaction
@@ -661,6 +773,13 @@ void build_layout(int vflags, Layout &L) {
         e.amplitude = d.amplitude;
         const double thr[7] = {0.05, 0.22, 0.42, 0.45, 0.49, 0.51, 0.55};
         for (int t = 0; t < 7; t++) e.thr[t] = thr[t];
+        // FP32 gate margin after evaluation k: amplitude x sum of |weights| so far x the per-evaluation bound,
+        // plus slack for the FP64 accumulation and the final product's rounding
+        double wsum = 0.0;
+        for (int k = 0; k < 8; k++) {
+            wsum += std::fabs(e.w[k]);
+            e.margin[k] = e.amplitude * wsum * kFp32PerlinBound + 1e-12;
+        }
     }
     // build descriptors
     for (int s = 0; s < SET_COUNT; s++)
@@ -688,7 +807,7 @@ struct whf_ctx {
     uint64_t *d_seeds = nullptr;
     uint8_t *d_tab[SET_COUNT] = {nullptr, nullptr, nullptr};
     SPar *d_par[SET_COUNT] = {nullptr, nullptr, nullptr};
-    uint4 *d_q1 = nullptr, *d_q2 = nullptr;
+    uint4 *d_q1 = nullptr, *d_q2 = nullptr, *d_qx = nullptr;
     uint32_t *d_needC = nullptr, *d_needL = nullptr, *d_slotC = nullptr, *d_slotL = nullptr;
     uint32_t *d_seedC = nullptr, *d_seedL = nullptr, *d_ctl = nullptr;
     unsigned long long *d_cnt = nullptr;
@@ -714,6 +833,7 @@ int ctx_alloc(whf_ctx *c) {
     }
     HIP_TRY(hipMalloc(&c->d_q1, (size_t)kMaxItems * sizeof(uint4)));
     HIP_TRY(hipMalloc(&c->d_q2, (size_t)kMaxItems * sizeof(uint4)));
+    HIP_TRY(hipMalloc(&c->d_qx, (size_t)kQxCap * sizeof(uint4)));
     HIP_TRY(hipMalloc(&c->d_needC, S * 4));
     HIP_TRY(hipMalloc(&c->d_needL, S * 4));
     HIP_TRY(hipMalloc(&c->d_slotC, S * 4));
@@ -816,14 +936,18 @@ int run_batch(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint6
         const uint32_t n_chunks = (fn + chunk - 1) / chunk;
         const uint32_t tiles = (ns + ST - 1) / ST;
         const dim3 grid(tiles * n_chunks);
-        if (wide)
-            hipLaunchKernelGGL(k_erosion<4>, grid, dim3(kEroWG), 0, st, c->d_seeds, ns, c->d_tab[SET_E],
-                               c->d_par[SET_E], gt, req->rx0, req->rz0, W, f0, fn, chunk, n_chunks, salt, single,
-                               c->d_q1, c->d_ctl, c->d_needC, c->d_cnt);
-        else
-            hipLaunchKernelGGL(k_erosion<8>, grid, dim3(kEroWG), 0, st, c->d_seeds, ns, c->d_tab[SET_E],
-                               c->d_par[SET_E], gt, req->rx0, req->rz0, W, f0, fn, chunk, n_chunks, salt, single,
-                               c->d_q1, c->d_ctl, c->d_needC, c->d_cnt);
+        const bool f32 = req->precision == WHF_PREC_FP32_GATE;
+#define WHF_EROSION_ARGS                                                                                          \
+    grid, dim3(kEroWG), 0, st, c->d_seeds, ns, c->d_tab[SET_E], c->d_par[SET_E], gt, req->rx0, req->rz0, W, f0, fn, \
+        chunk, n_chunks, salt, single, c->d_q1, c->d_ctl, c->d_needC, c->d_qx, c->d_cnt
+        if (wide && f32) hipLaunchKernelGGL((k_erosion<4, true>), WHF_EROSION_ARGS);
+        else if (wide) hipLaunchKernelGGL((k_erosion<4, false>), WHF_EROSION_ARGS);
+        else if (f32) hipLaunchKernelGGL((k_erosion<8, true>), WHF_EROSION_ARGS);
+        else hipLaunchKernelGGL((k_erosion<8, false>), WHF_EROSION_ARGS);
+#undef WHF_EROSION_ARGS
+        if (f32 && !single)
+            hipLaunchKernelGGL(k_erosion_exact, dim3((uint32_t)c->n_cu * 2), dim3(256), 0, st, c->d_tab[SET_E],
+                               c->d_par[SET_E], gt, c->d_qx, c->d_q1, c->d_ctl, c->d_needC, c->d_cnt);
     }
     {
         Prof p(c, F_COMPACT);
@@ -937,7 +1061,7 @@ void whf_close(whf_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_seeds, c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_par[0], c->d_par[1], c->d_par[2],
-                    c->d_q1, c->d_q2, c->d_needC, c->d_needL, c->d_slotC, c->d_slotL, c->d_seedC, c->d_seedL,
+                    c->d_q1, c->d_q2, c->d_qx, c->d_needC, c->d_needL, c->d_slotC, c->d_slotL, c->d_seedC, c->d_seedL,
                     c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -956,7 +1080,7 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
     if (req->rx1 <= req->rx0 || req->rz1 <= req->rz0) return WHF_EINVAL;
     if (req->seed_mode < WHF_SEEDS_LIST || req->seed_mode > WHF_SEEDS_COUNTER) return WHF_EINVAL;
     if (req->seed_mode == WHF_SEEDS_LIST && req->n_seeds && !req->seeds) return WHF_EINVAL;
-    if (req->precision != WHF_PREC_FP64) return WHF_ENOTSUP;
+    if (req->precision != WHF_PREC_FP64 && req->precision != WHF_PREC_FP32_GATE) return WHF_EINVAL;
     if (req->vflags < 0 || req->vflags >= kLayouts) return WHF_EINVAL;
     // chunk coordinates travel biased in 23 bits through the erosion kernel
     const int64_t lim = 65536;
@@ -1006,6 +1130,12 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
     HIP_TRY(hipMemcpyAsync(cnt, c->d_cnt, sizeof cnt, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(ctl, c->d_ctl, sizeof ctl, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (cnt[CNT_QX_OVER]) {
+        // the FP32 gate's re-evaluation queue overflowed in some batch: redo the whole request in FP64
+        whf_request r64 = *req;
+        r64.precision = WHF_PREC_FP64;
+        return whf_scan(c, &r64, sink, stats);
+    }
     const uint64_t hits_total = ctl[CTL_HITS];
     std::vector<DevHit> host_hits(std::min<uint64_t>(hits_total, kHitCap));
     if (!host_hits.empty())
@@ -1028,6 +1158,7 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
         stats->accepted = cnt[CNT_KIND0 + 10];
         stats->reached_ladder = cnt[CNT_LADDER];
         for (int s = 0; s < SET_COUNT; s++) stats->samplers_built[s] = cnt[CNT_BUILT + s];
+        stats->fp64_rechecks = cnt[CNT_QX];
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
         stats->device_ms = ms;
@@ -1077,5 +1208,37 @@ int whf_probe(whf_ctx *c, const int64_t *seeds, const int32_t *hut_x, const int3
     (void)hipFree(d_out);
     return rc;
 }
+
+int whf_noise_error(whf_ctx *c, const int64_t *seeds, uint64_t n_seeds, uint64_t points_per_seed, int vflags,
+                    double max_err[8]) {
+    if (!c || !max_err || (n_seeds && !seeds)) return WHF_EINVAL;
+    if (n_seeds > kMaxSeeds || vflags < 0 || vflags >= kLayouts || points_per_seed > (1u << 20)) return WHF_EINVAL;
+    for (int k = 0; k < 8; k++) max_err[k] = 0.0;
+    if (!n_seeds || !points_per_seed) return WHF_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t st = c->stream;
+    const uint32_t ns = (uint32_t)n_seeds, np = (uint32_t)points_per_seed;
+    unsigned long long *d_max = nullptr;
+    HIP_TRY(hipMalloc(&d_max, 8 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(d_max, 0, 8 * sizeof(unsigned long long), st));
+    HIP_TRY(hipMemcpyAsync(c->d_seeds, seeds, n_seeds * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_build, dim3((ns * 8 + 63) / 64), dim3(64), 0, st, c->d_seeds, bd_of(c, vflags, SET_E), 8,
+                       (const uint32_t *)nullptr, (const uint32_t *)nullptr, ns, c->d_tab[SET_E], c->d_par[SET_E],
+                       c->d_cnt + CNT_BUILT + SET_E);
+    const uint64_t total = (uint64_t)ns * np;
+    hipLaunchKernelGGL(k_noise_error, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, c->d_tab[SET_E],
+                       c->d_par[SET_E], c->d_gt + vflags, ns, np, d_max);
+    int rc = check_launch();
+    unsigned long long bits[8];
+    if (rc == WHF_OK) {
+        HIP_TRY(hipMemcpyAsync(bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int k = 0; k < 8; k++) std::memcpy(&max_err[k], &bits[k], 8);
+    }
+    (void)hipFree(d_max);
+    return rc;
+}
+
+double whf_fp32_bound(void) { return kFp32PerlinBound; }
 
 }  // extern "C"

@@ -15,13 +15,13 @@ PHASE1_SCAN = os.path.join(HERE, "bin", "phase1_scan")
 OK, EINVAL, ENOMEM, EDEVICE, EOVERFLOW, ENOTSUP = 0, -1, -2, -3, -4, -5
 SEEDS_LIST, SEEDS_SPLITMIX, SEEDS_COUNTER = 0, 1, 2
 V_118, V_262, V_LARGE, V_SINGLE = 1, 2, 4, 8
-PREC_FP64 = 0
+PREC_FP64, PREC_FP32_GATE = 0, 1
 KIND_N = 11
 CFG_PROFILE = 1
 
 # Exported symbols of include/whf.h, checked by the tests.
 SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags", "whf_strerror",
-           "whf_device_name")
+           "whf_device_name", "whf_noise_error", "whf_fp32_bound")
 
 
 class Config(ctypes.Structure):
@@ -51,7 +51,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("seeds", ctypes.c_uint64), ("checks", ctypes.c_uint64), ("accepted", ctypes.c_uint64),
                 ("reached_ladder", ctypes.c_uint64), ("funnel", ctypes.c_uint64 * KIND_N),
                 ("tiers", ctypes.c_uint64 * 8), ("samplers_built", ctypes.c_uint64 * 3), ("device_ms", ctypes.c_double),
-                ("kernel_ms", ctypes.c_double * 8)]
+                ("kernel_ms", ctypes.c_double * 8), ("fp64_rechecks", ctypes.c_uint64)]
 
 
 KERNEL_FAMILIES = ("seeds", "build_erosion", "erosion_gate", "compact", "build_climate", "gate2", "build_ladder",
@@ -83,6 +83,11 @@ def load():
         L.whf_strerror.argtypes = [ctypes.c_int]
         L.whf_device_name.restype = ctypes.c_char_p
         L.whf_device_name.argtypes = [ctypes.c_void_p]
+        L.whf_noise_error.restype = ctypes.c_int
+        L.whf_noise_error.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_uint64,
+                                      ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.whf_fp32_bound.restype = ctypes.c_double
+        L.whf_fp32_bound.argtypes = []
         _lib = L
     return _lib
 
@@ -128,11 +133,11 @@ class Context:
         return self.L.whf_device_name(self.h).decode()
 
     def scan(self, rect, seeds=None, splitmix=None, counter=None, max_height=-50, vflags=0, capacity=1 << 16,
-             salt=0):
+             salt=0, precision=PREC_FP64):
         """Phase-1 Seed Check. rect = (rx0, rx1, rz0, rz1); seeds = int64 array, or splitmix/counter =
         (stream, count[, offset]). Returns (hits as list of tuples (seed, rx, rz, cx, cz), stats dict)."""
         req = Request()
-        req.precision = PREC_FP64
+        req.precision = precision
         keep = None
         if seeds is not None:
             keep = np.ascontiguousarray(seeds, dtype=np.int64)
@@ -160,7 +165,7 @@ class Context:
         stats = dict(seeds=st.seeds, checks=st.checks, accepted=st.accepted, reached_ladder=st.reached_ladder,
                      funnel=list(st.funnel), tiers=list(st.tiers)[:7], samplers_built=list(st.samplers_built),
                      device_ms=st.device_ms, kernel_ms=dict(zip(KERNEL_FAMILIES, list(st.kernel_ms))),
-                     hit_count=sink.count, overflow=rc == EOVERFLOW)
+                     hit_count=sink.count, overflow=rc == EOVERFLOW, fp64_rechecks=st.fp64_rechecks)
         return hits, stats
 
     def probe(self, seeds, xs, zs, max_height=-50, vflags=0):
@@ -176,3 +181,17 @@ class Context:
         if rc:
             raise WhfError(rc, "whf_probe")
         return out.view(np.uint64)
+
+    def noise_error(self, seeds, points_per_seed, vflags=0):
+        """Max |FP32-gate - FP64| per erosion sampler (8 values) over points spread over the world."""
+        s = np.ascontiguousarray(seeds, dtype=np.int64)
+        out = (ctypes.c_double * 8)()
+        rc = self.L.whf_noise_error(self.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(s), points_per_seed,
+                                    vflags, out)
+        if rc:
+            raise WhfError(rc, "whf_noise_error")
+        return list(out)
+
+
+def fp32_bound():
+    return load().whf_fp32_bound()

@@ -13,7 +13,7 @@ HEADER = os.path.join(whf.HERE, "..", "include", "whf.h")
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(whf_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|double|const char \*)\s*\*?\s*(whf_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -184,7 +184,53 @@ def test_full_c2_properties(ctx, funnels):
     g = funnels["c2_prefix_v120"]
     _, sp = ctx.scan(rect, splitmix=(2, g["count"]))
     same_counts(sp, g)
+    # the FP32-gated chain gives the identical answer at full size
+    h32, s32 = ctx.scan(rect, splitmix=(2, n), capacity=1 << 20, precision=whf.PREC_FP32_GATE)
+    assert h32 == hits
+    for k in ("funnel", "tiers", "reached_ladder", "accepted", "checks"):
+        assert s32[k] == st[k], k
     # every survivor is a survivor of the oracle too (oracle re-checks each hit's seed over the whole rect)
     for seed in sorted({h[0] for h in hits})[:64]:
         ref = O.scan(np.array([seed], np.int64), rect, -50, 0, threads=1)
         assert [h for h in hits if h[0] == seed] == [tuple(x) for x in ref["hits"]]
+
+
+# ---------------------------------------------------------------- FP32 gate: same answers as FP64
+
+@pytest.mark.parametrize("case", ["c1_v120", "c1_v262", "c2_prefix_v120"])
+def test_fp32_gate_equals_reference_golden(ctx, funnels, case):
+    c = funnels[case]
+    hits, st = ctx.scan(tuple(c["rect"]), splitmix=(c["stream"], c["count"]), max_height=c["max_y"],
+                        vflags=whf.version_flags(c["version"]), precision=whf.PREC_FP32_GATE)
+    same_counts(st, c)
+
+
+@pytest.mark.parametrize("vflags", [0, whf.V_118, whf.V_LARGE, whf.V_SINGLE])
+def test_fp32_gate_equals_fp64(ctx, vflags):
+    rect = (-20, 20, -20, 20)
+    kw = dict(splitmix=(31, 4096), max_height=-50, vflags=vflags)
+    h64, s64 = ctx.scan(rect, **kw)
+    h32, s32 = ctx.scan(rect, precision=whf.PREC_FP32_GATE, **kw)
+    assert h32 == h64
+    for k in ("funnel", "tiers", "reached_ladder", "accepted", "checks"):
+        assert s32[k] == s64[k], k
+
+
+def test_fp32_gate_rechecks_near_thresholds(ctx):
+    """The known-answer block: FP32 gate output equals the oracle's, and some undecided items took the FP64
+    re-evaluation path (it is exercised, not dead code)."""
+    seeds = O.seeds_splitmix(12, 2048)
+    rect = (-16, 16, -16, 16)
+    hits, st = ctx.scan(rect, seeds=seeds, precision=whf.PREC_FP32_GATE)
+    ref = O.scan(seeds, rect, -50, 0)
+    same_counts(st, ref)
+    assert hits == [tuple(h) for h in ref["hits"]]
+    assert st["fp64_rechecks"] > 0
+
+
+def test_fp32_error_within_stated_bound(ctx):
+    seeds = O.seeds_splitmix(44, 2048)
+    err = ctx.noise_error(seeds, 2048)
+    bound = whf.fp32_bound()
+    print("per-octave max |fp32 - fp64|:", err, "bound", bound, flush=True)
+    assert all(0.0 < e < bound for e in err), (err, bound)
