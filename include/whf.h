@@ -59,7 +59,7 @@ typedef struct {
     uint64_t n_seeds;
     uint64_t stream, offset;    /* WHF_SEEDS_SPLITMIX / _COUNTER */
     int rx0, rx1, rz0, rz1;     /* hut regions, half-open [rx0,rx1) x [rz0,rz1) */
-    int max_height;             /* phase-1 maxHeight (lysh_phase1_check_ex's max_height, used as given) */
+    int max_height;             /* phase-1 maxHeight (lysh_phase1_check_ex's max_height, used as given); -140..100000 */
     int vflags;                 /* WHF_V_* */
     int salt;                   /* structure salt; 0 = swamp hut 14357620 */
     int reserved;

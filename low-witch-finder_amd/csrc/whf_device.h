@@ -204,10 +204,10 @@ __device__ __forceinline__ double perlin_3d(const Perm &P, const SPar &s, double
 }
 
 // The same gradient dot product by selects: u = x or y, v = y, x or z, signs from the low two hash bits (Perlin's
-// improved-noise formulation of the same 16-entry table). Exactly the reference's three-term sum whenever the
-// result is non-zero, since adding the zero component or reordering two terms does not change a non-zero IEEE
-// sum; a zero result (only when both live coordinates are zero) is recomputed with the three-term form so that
-// its sign matches too.
+// improved-noise formulation of the same 16-entry table). It equals the reference's three-term sum
+// (g0*x + g1*y) + g2*z bit for bit unless both live terms are zeros: adding the zero component to a non-zero
+// sum, or to the +0 of an exact cancellation, changes nothing, and reordering two terms is exact. Callers take
+// the three-term form whenever a coordinate that can reach a live term is zero.
 __device__ __forceinline__ double grad_sel(uint32_t h, double x, double y, double z) {
     h &= 15u;
     double u = h < 8u ? x : y;
@@ -264,10 +264,9 @@ __device__ __forceinline__ void perlin_y0_x2(const Pairs *Q, const SPar *const *
         c[5] = grad_sel(h1[t] >> 8, gm, h, lm);
         c[6] = grad_sel(h2[t] >> 8, g[t], hm, lm);
         c[7] = grad_sel(h3[t] >> 8, gm, hm, lm);
-        bool zero = false;
-#pragma unroll
-        for (int q = 0; q < 8; q++) zero |= c[q] == 0.0;
-        if (zero) {
+        // grad_sel differs from the three-term form only when both live terms of a corner are zero, which needs
+        // a zero coordinate: g, h or l (g - 1, h - 1, l - 1 are never zero since g, h, l lie in [0, 1))
+        if (g[t] == 0.0 || l[t] == 0.0 || h == 0.0) {
             c[0] = grad((int)h0[t], g[t], h, l[t]);
             c[1] = grad((int)h1[t], gm, h, l[t]);
             c[2] = grad((int)h2[t], g[t], hm, l[t]);

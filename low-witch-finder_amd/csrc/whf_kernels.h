@@ -32,7 +32,28 @@ struct EroDesc {
     double amplitude;
     double thr[7];
     double margin[8];   // FP32 gate: bound on |estimate - FP64 value| of sum * amplitude after evaluation k
+    double coord_lim;   // |x|, |z| below this keep every scaled coordinate inside maintainPrecision's identity range
 };
+
+// Division by a divisor fixed per launch, by multiply-high (Granlund and Montgomery, "Division by invariant
+// integers using multiplication", 1994, fig. 4.1): q = (t + ((n - t) >> 1)) >> (l - 1), t = mulhi(m, n).
+struct FastDiv {
+    uint32_t d, m, l;
+    __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const {
+        if (d == 1) return n;
+        const uint32_t t = (uint32_t)(((uint64_t)m * n) >> 32);
+        return (t + ((n - t) >> 1)) >> (l - 1);
+    }
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f{d, 0u, 0u};
+    if (d <= 1) { f.d = 1; return f; }
+    uint32_t l = 0;
+    while ((1ull << l) < d) l++;
+    f.l = l;
+    f.m = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
+    return f;
+}
 
 // Everything the gate kernels read about a request's noises.
 struct GateTables {

@@ -213,8 +213,8 @@ __device__ __forceinline__ void push_recheck(bool pred, uint32_t slot, int cx, i
 template <int ST, bool F32>
 __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__ seeds, uint32_t n_seeds,
                                                     const uint8_t *__restrict__ tabE, const SPar *__restrict__ parE,
-                                                    const GateTables *__restrict__ gt, int rx0, int rz0, uint32_t W,
-                                                    uint64_t flat0, uint32_t flat_n, uint32_t chunk,
+                                                    const GateTables *__restrict__ gt, int rx0, int rz0, FastDiv Wd,
+                                                    FastDiv Rd, uint64_t flat0, uint32_t flat_n, uint32_t chunk,
                                                     uint32_t n_chunks, int salt, int single,
                                                     uint4 *__restrict__ q1, uint32_t *__restrict__ ctl,
                                                     uint32_t *__restrict__ needC, uint4 *__restrict__ qx,
@@ -234,6 +234,9 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
     const uint32_t ns = min((uint32_t)ST, n_seeds - seed0);
     const uint32_t rbase = ck * chunk;
     const uint32_t rn = min(chunk, flat_n - rbase);
+    const uint32_t W = Wd.d;
+    // regions per seed in this workgroup: the full chunk (Rd) unless this is the last, partial chunk
+    const FastDiv Rn = rn == Rd.d ? Rd : FastDiv{rn, 0u, 0u};
 
     {   // stage the tile's erosion tables as byte pairs, and the origins
         const uint8_t *src = tabE + (size_t)seed0 * 8 * 256;
@@ -265,9 +268,9 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
             uint32_t sl = 0;
             if (i < npass) {
                 const uint32_t it = (uint32_t)(pass0 + i);
-                sl = it / rn;
+                sl = Rn.m ? Rn.div(it) : it / rn;
                 const uint32_t c = col0 + (it - sl * rn);
-                const uint32_t drow = c / W;
+                const uint32_t drow = Wd.div(c);
                 hut_chunk(seeds[seed0 + sl], rx0 + (int)(c - drow * W), rz0 + (int)(row0 + drow), salt, cx, cz);
                 s_item[i] = make_int2(cx, (cz + kCzBias) | (int)(sl << 23));
                 if (single) {
@@ -276,8 +279,12 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
                     const double x = (double)(cx * 16 + 8) / 4.0, z = (double)(cz * 16 + 8) / 4.0;
                     const LPair Q[2] = {LPair{s_tab + (sl * 8 + 0) * 256}, LPair{s_tab + (sl * 8 + 1) * 256}};
                     const SPar *P[2] = {&s_par[sl * 8 + 0], &s_par[sl * 8 + 1]};
-                    const double xs[2] = {keep_precision(x * ed.sc[0] * ed.lac[0]), keep_precision(x * ed.sc[1] * ed.lac[1])};
-                    const double zs[2] = {keep_precision(z * ed.sc[0] * ed.lac[0]), keep_precision(z * ed.sc[1] * ed.lac[1])};
+                    const bool raw = fabs(x) < ed.coord_lim && fabs(z) < ed.coord_lim;
+                    double xs[2] = {x * ed.sc[0] * ed.lac[0], x * ed.sc[1] * ed.lac[1]};
+                    double zs[2] = {z * ed.sc[0] * ed.lac[0], z * ed.sc[1] * ed.lac[1]};
+                    if (!raw) {
+                        for (int t = 0; t < 2; t++) { xs[t] = keep_precision(xs[t]); zs[t] = keep_precision(zs[t]); }
+                    }
                     double v[2];
                     if (F32) perlin_y0_x2_f32(Q, P, xs, zs, v);
                     else perlin_y0_x2(Q, P, xs, zs, v);
@@ -327,10 +334,12 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
                         const double x = (double)(cx * 16 + 8) / 4.0, z = (double)(cz * 16 + 8) / 4.0;
                         const LPair Q[2] = {LPair{s_tab + (sl * 8 + k0) * 256}, LPair{s_tab + (sl * 8 + k1) * 256}};
                         const SPar *P[2] = {&s_par[sl * 8 + k0], &s_par[sl * 8 + k1]};
-                        const double xs[2] = {keep_precision(x * ed.sc[k0] * ed.lac[k0]),
-                                              keep_precision(x * ed.sc[k1] * ed.lac[k1])};
-                        const double zs[2] = {keep_precision(z * ed.sc[k0] * ed.lac[k0]),
-                                              keep_precision(z * ed.sc[k1] * ed.lac[k1])};
+                        const bool raw = fabs(x) < ed.coord_lim && fabs(z) < ed.coord_lim;
+                        double xs[2] = {x * ed.sc[k0] * ed.lac[k0], x * ed.sc[k1] * ed.lac[k1]};
+                        double zs[2] = {z * ed.sc[k0] * ed.lac[k0], z * ed.sc[k1] * ed.lac[k1]};
+                        if (!raw) {
+                            for (int t = 0; t < 2; t++) { xs[t] = keep_precision(xs[t]); zs[t] = keep_precision(zs[t]); }
+                        }
                         double v[2];
                         if (F32) perlin_y0_x2_f32(Q, P, xs, zs, v);
                         else perlin_y0_x2(Q, P, xs, zs, v);
@@ -523,6 +532,23 @@ __global__ __launch_bounds__(kGateWG) void k_gate2(const uint64_t *__restrict__ 
     }
 }
 
+// Gate 3 per Q2 entry, in two stages per chunk of 256 entries. Stage A: cave@maxHeight, one lane per entry
+// (it decides 3 in 4 entries). Stage B: every remaining check of the survivors at once, one lane per (entry,
+// check): the negative ladder, the positive ladder, continentalness and the floodedness heights, in lysh's order
+// (phase1.c:151-196). Each check is a pure function of the entry, so the entry's reject kind is its first failing
+// check, as in the sequential scan, and the survivors are the entries with no failing check.
+constexpr int kG3Checks = 32;
+
+__device__ __forceinline__ int g3_layout(int max_h, bool single, int &n_neg, int &n_flood) {
+    int floor_y = max_h < -40 ? max_h : -40;
+    if (floor_y < -50) floor_y = -50;
+    n_neg = 0;
+    for (int y = 0; y >= floor_y; y -= 10)
+        if (max_h < y) n_neg++;
+    n_flood = max_h <= 60 ? (60 - max_h) / 10 + 1 : 0;
+    return n_neg + 4 + (single ? 0 : 1) + n_flood;
+}
+
 __global__ __launch_bounds__(kGateWG) void k_gate3(const uint64_t *__restrict__ seeds, const GateTables *__restrict__ gt,
                                                     const uint8_t *__restrict__ tabC, const SPar *__restrict__ parC,
                                                     const uint8_t *__restrict__ tabL, const SPar *__restrict__ parL,
@@ -532,55 +558,90 @@ __global__ __launch_bounds__(kGateWG) void k_gate3(const uint64_t *__restrict__ 
                                                     DevHit *__restrict__ hits, uint32_t hit_cap,
                                                     unsigned long long *__restrict__ cnt) {
     __shared__ uint32_t s_cnt[16];
+    __shared__ uint32_t s_list[kGateWG];
+    __shared__ uint32_t s_fail[kGateWG];
+    __shared__ uint32_t s_n;
     if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
-    __syncthreads();
     const uint32_t n = ctl[CTL_Q2];
     const bool single = (vflags & WHF_V_SINGLE) != 0;
-    for (uint32_t q0 = blockIdx.x * kGateWG + (threadIdx.x & ~63u); q0 < n; q0 += gridDim.x * kGateWG) {
-        const uint32_t q = q0 + lane_id();
-        bool accept = false;
-        uint4 e = make_uint4(0, 0, 0, 0);
-        if (q < n) {
-            e = q2[q];
-            const CaveCtx cv{gt, SetView{tabC, parC, kSetSize[SET_C]}, SetView{tabL, parL, kSetSize[SET_L]},
-                             slotC[e.x], slotL[e.x]};
-            const int hut_x = (int)e.y * 16, hut_z = (int)e.z * 16;
-            const int x = hut_x + 3, z = hut_z + 3;
-            int kind = -1;
-            if (cv.spaghetti(x, max_h, z) >= 0 && cv.cheese(x, max_h, z) >= 0) kind = 5;
-            if (kind < 0) {
-                int floor_y = max_h < -40 ? max_h : -40;
-                if (floor_y < -50) floor_y = -50;
-                for (int y = 0; y >= floor_y; y -= 10)
-                    if (max_h < y && cv.spaghetti(x, y, z) >= 0 && cv.cheese(x, y, z) >= 0) { kind = 6; break; }
+    int n_neg, n_flood;
+    const int n_checks = g3_layout(max_h, single, n_neg, n_flood);
+    const SetView C{tabC, parC, kSetSize[SET_C]}, L{tabL, parL, kSetSize[SET_L]};
+    for (uint32_t base = blockIdx.x * kGateWG; base < n; base += gridDim.x * kGateWG) {
+        if (threadIdx.x == 0) s_n = 0;
+        s_fail[threadIdx.x] = 0;
+        __syncthreads();
+        // ---- stage A: cave@maxHeight ----
+        {
+            const uint32_t q = base + threadIdx.x;
+            bool keep = false;
+            if (q < n) {
+                const uint4 e = q2[q];
+                const CaveCtx cv{gt, C, L, slotC[e.x], slotL[e.x]};
+                const int x = (int)e.y * 16 + 3, z = (int)e.z * 16 + 3;
+                if (cv.spaghetti(x, max_h, z) >= 0 && cv.cheese(x, max_h, z) >= 0) atomicAdd(&s_cnt[5], 1u);
+                else keep = true;
             }
-            if (kind < 0)
-                for (int y = 10; y <= 40; y += 10)
-                    if (cv.entrance_nonneg(x, y, z) && cv.cheese(x, y, z) >= 0) { kind = 7; break; }
-            if (kind < 0 && !single) {
+            const uint32_t pos = wave_append(keep, &s_n);
+            if (keep) s_list[pos] = q;
+        }
+        __syncthreads();
+        // ---- stage B: all remaining checks of the survivors, one lane per (entry, check) ----
+        const uint32_t ns = s_n;
+        for (uint32_t w = threadIdx.x; w < ns * (uint32_t)n_checks; w += kGateWG) {
+            const uint32_t li = w / (uint32_t)n_checks;
+            const int j = (int)(w - li * (uint32_t)n_checks);
+            const uint4 e = q2[s_list[li]];
+            const CaveCtx cv{gt, C, L, slotC[e.x], slotL[e.x]};
+            const int hut_x = (int)e.y * 16, hut_z = (int)e.z * 16, x = hut_x + 3, z = hut_z + 3;
+            bool fail;
+            if (j < n_neg) {
+                const int y = -10 * j;
+                fail = cv.spaghetti(x, y, z) >= 0 && cv.cheese(x, y, z) >= 0;
+            } else if (j < n_neg + 4) {
+                const int y = 10 * (j - n_neg + 1);
+                fail = cv.entrance_nonneg(x, y, z) && cv.cheese(x, y, z) >= 0;
+            } else if (!single && j == n_neg + 4) {
                 const NoiseDesc &d = gt->noise[G_CONTINENTALNESS];
-                const double c = noise_y0(d, cv.L, cv.slotL, (hut_x + 8) / 4.0, (hut_z + 8) / 4.0);
-                if (c < -0.11) kind = 8;
+                fail = noise_y0(d, cv.L, cv.slotL, (hut_x + 8) / 4.0, (hut_z + 8) / 4.0) < -0.11;
+            } else {
+                const int y = max_h + 10 * (j - n_neg - 4 - (single ? 0 : 1));
+                fail = cv.flooded(hut_x, hut_z, y) > 0.41;
             }
-            if (kind < 0)
-                for (int y = max_h; y <= 60; y += 10)
-                    if (cv.flooded(hut_x, hut_z, y) > 0.41) { kind = 9; break; }
-            if (kind < 0) kind = 10;
-            block_count(s_cnt, kind);
-            accept = kind == 10;
+            if (fail) atomicOr(&s_fail[li], 1u << j);
         }
-        const uint32_t pos = wave_append(accept, &ctl[CTL_HITS]);
-        if (accept && pos < hit_cap) {
-            DevHit h;
-            h.seed = (int64_t)seeds[e.x];
-            h.cx = (int)e.y;
-            h.cz = (int)e.z;
-            h.rx = h.cx >> 5;
-            h.rz = h.cz >> 5;
-            h.floor_y = INT32_MIN;
-            h.flags = 0;
-            hits[pos] = h;
+        __syncthreads();
+        // ---- verdicts: first failing check, in order ----
+        {
+            const uint32_t li = threadIdx.x;
+            bool accept = false;
+            uint4 e = make_uint4(0, 0, 0, 0);
+            if (li < ns) {
+                const uint32_t f = s_fail[li];
+                if (f == 0) {
+                    accept = true;
+                    atomicAdd(&s_cnt[10], 1u);
+                } else {
+                    const int j = __ffs((int)f) - 1;
+                    const int kind = j < n_neg ? 6 : (j < n_neg + 4 ? 7 : ((!single && j == n_neg + 4) ? 8 : 9));
+                    atomicAdd(&s_cnt[kind], 1u);
+                }
+                e = q2[s_list[li]];
+            }
+            const uint32_t pos = wave_append(accept, &ctl[CTL_HITS]);
+            if (accept && pos < hit_cap) {
+                DevHit h;
+                h.seed = (int64_t)seeds[e.x];
+                h.cx = (int)e.y;
+                h.cz = (int)e.z;
+                h.rx = h.cx >> 5;
+                h.rz = h.cz >> 5;
+                h.floor_y = INT32_MIN;
+                h.flags = 0;
+                hits[pos] = h;
+            }
         }
+        __syncthreads();
     }
     __syncthreads();
     if (threadIdx.x < 16 && threadIdx.x > 0 && s_cnt[threadIdx.x])
@@ -775,6 +836,11 @@ void build_layout(int vflags, Layout &L) {
         for (int t = 0; t < 7; t++) e.thr[t] = thr[t];
         // FP32 gate margin after evaluation k: amplitude x sum of |weights| so far x the per-evaluation bound,
         // plus slack for the FP64 accumulation and the final product's rounding
+        double smax = 0.0;
+        for (int k = 0; k < 8; k++) smax = std::max(smax, e.sc[k] * e.lac[k]);
+        // x * sc * lac stays below 1e7 in magnitude (maintainPrecision's identity range) when |x| < coord_lim;
+        // the 0.999 factor leaves room for the rounding of the two products
+        e.coord_lim = 1.0e7 / smax * 0.999;
         double wsum = 0.0;
         for (int k = 0; k < 8; k++) {
             wsum += std::fabs(e.w[k]);
@@ -937,8 +1003,9 @@ int run_batch(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint6
         const uint32_t tiles = (ns + ST - 1) / ST;
         const dim3 grid(tiles * n_chunks);
         const bool f32 = req->precision == WHF_PREC_FP32_GATE;
+        const FastDiv Wd = make_fastdiv(W), Rd = make_fastdiv(chunk);
 #define WHF_EROSION_ARGS                                                                                          \
-    grid, dim3(kEroWG), 0, st, c->d_seeds, ns, c->d_tab[SET_E], c->d_par[SET_E], gt, req->rx0, req->rz0, W, f0, fn, \
+    grid, dim3(kEroWG), 0, st, c->d_seeds, ns, c->d_tab[SET_E], c->d_par[SET_E], gt, req->rx0, req->rz0, Wd, Rd, f0, fn, \
         chunk, n_chunks, salt, single, c->d_q1, c->d_ctl, c->d_needC, c->d_qx, c->d_cnt
         if (wide && f32) hipLaunchKernelGGL((k_erosion<4, true>), WHF_EROSION_ARGS);
         else if (wide) hipLaunchKernelGGL((k_erosion<4, false>), WHF_EROSION_ARGS);
@@ -1082,6 +1149,8 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
     if (req->seed_mode == WHF_SEEDS_LIST && req->n_seeds && !req->seeds) return WHF_EINVAL;
     if (req->precision != WHF_PREC_FP64 && req->precision != WHF_PREC_FP32_GATE) return WHF_EINVAL;
     if (req->vflags < 0 || req->vflags >= kLayouts) return WHF_EINVAL;
+    // gate 3 keeps one bit per check: the floodedness heights max_height..60 must fit beside the ladders
+    if (req->max_height < -140 || req->max_height > 100000) return WHF_EINVAL;
     // chunk coordinates travel biased in 23 bits through the erosion kernel
     const int64_t lim = 65536;
     if (req->rx0 < -lim || req->rx1 > lim || req->rz0 < -lim || req->rz1 > lim) return WHF_EINVAL;

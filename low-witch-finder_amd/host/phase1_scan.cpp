@@ -8,7 +8,8 @@
 //   * --version V selects the game's phase-1 rules by lysh's version names (default 1.19.x~26.1, hunt.py's
 //     default; --v26-2 still selects the 26.2 ridge rule);
 //   * --splitmix S N [OFFSET] scans N seeds of the sequential SplitMix64 stream seeded with S;
-//   * --stats prints the funnel and erosion tier counters as one JSON line.
+//   * --stats prints the funnel and erosion tier counters as one JSON line;
+//   * --fp32-gate runs the erosion stage's corner blend in FP32 with FP64 re-evaluation near every threshold.
 // Options that only tuned the reference's own kernels (--block, --climate-gate, --no-climate-gate,
 // --no-density-prefilter) are accepted and have no effect on the answer. Options of searches this engine
 // does not run (--self-test, --quad-bases, --require-all-regions, --max-hut-dist) are errors.
@@ -36,7 +37,7 @@ struct Params {
     int minRX = -128, maxRX = 128, minRZ = -128, maxRZ = 128;
     int maxHeight = -40;
     int vflags = 0;
-    bool have_version = false, v262 = false, stats = false;
+    bool have_version = false, v262 = false, stats = false, fp32 = false;
     std::string out;
 };
 
@@ -80,6 +81,7 @@ void usage() {
         "  --v26-2               26.2 ridge rule\n"
         "  --out <file>          write candidates as CSV (seed,chunkX,chunkZ,blockX,blockZ,gpuEst)\n"
         "  --stats               print the funnel counters as JSON\n"
+        "  --fp32-gate           FP32-gated erosion stage with FP64 re-evaluation near thresholds (same answers)\n"
         "  --worker              read one request per stdin line; ===WORKER_READY===, ===SCAN_DONE rc=N===\n");
 }
 
@@ -126,6 +128,7 @@ bool parse(const std::vector<std::string> &t, Params &p) {
         else if (a == "--v26-2") p.v262 = true;
         else if (a == "--out") p.out = next("--out");
         else if (a == "--stats") p.stats = true;
+        else if (a == "--fp32-gate") p.fp32 = true;
         else if (a == "--no-density-prefilter" || a == "--climate-gate" || a == "--no-climate-gate") {}
         else if (a == "--block") next("--block");
         else if (a == "--worker") {}
@@ -150,7 +153,7 @@ int run(whf_ctx *ctx, const Params &p) {
     whf_request r;
     std::memset(&r, 0, sizeof r);
     r.seed_mode = p.seed_mode;
-    r.precision = WHF_PREC_FP64;
+    r.precision = p.fp32 ? WHF_PREC_FP32_GATE : WHF_PREC_FP64;
     if (p.seed_mode == WHF_SEEDS_LIST) {
         r.seeds = p.seeds.data();
         r.n_seeds = p.seeds.size();
