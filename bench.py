@@ -121,12 +121,14 @@ def main():
     h64, s64 = results[whf.PREC_FP64]
     h32, last = results[whf.PREC_FP32_GATE]
     identical = h64 == h32 and all(s64[k] == last[k] for k in ("funnel", "tiers", "accepted", "checks"))
-    secs = timing[whf.PREC_FP32_GATE]
+    # the headline is the faster of the two bit-identical paths; both are reported
+    best = min(timing, key=timing.get)
+    secs = timing[best]
     checks = last["checks"]
 
     # profiled step: per-kernel-family device time from HIP events recorded on the library's stream
     with whf.Context(0, profile=True) as ctx:
-        _, prof = ctx.scan(rect, precision=whf.PREC_FP32_GATE, **kw)
+        _, prof = ctx.scan(rect, precision=best, **kw)
     fam = prof["kernel_ms"]
     n_batches = -(-args.seeds // (1 << 17))
     built = prof["samplers_built"]
@@ -150,13 +152,17 @@ def main():
     line = {
         "metric": METRIC, "value": round(checks / secs, 1), "unit": "checks/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(secs * 1e3, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32-gate+f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64" if best == whf.PREC_FP64 else "f32-gate+f64", "data": "synthetic",
         "config": {"workload": "Seed Check at scale: %d SplitMix64(2) seeds x regions [-8,7]^2 (256 per seed), "
-                               "version 1.20, phase-1 maxHeight -50, FP32-gated erosion stage with FP64 "
-                               "re-evaluation (value) vs the plain FP64 chain (fp64_value)" % args.seeds,
-                   "seeds": args.seeds, "regions_per_seed": 256, "version": "1.20", "precision": "fp32-gate"},
+                               "version 1.20, phase-1 maxHeight -50; value = the faster of the plain FP64 chain and "
+                               "the FP32-gated erosion stage (identical finds)" % args.seeds,
+                   "seeds": args.seeds, "regions_per_seed": 256, "version": "1.20",
+                   "precision": "fp64" if best == whf.PREC_FP64 else "fp32-gate"},
         "fp64_value": round(checks / timing[whf.PREC_FP64], 1), "fp64_ms_per_step": round(timing[whf.PREC_FP64] * 1e3, 3),
-        "fp32_gate_over_fp64": round(timing[whf.PREC_FP64] / secs, 3), "finds_identical_fp32_vs_fp64": identical,
+        "fp32_gate_value": round(checks / timing[whf.PREC_FP32_GATE], 1),
+        "fp32_gate_ms_per_step": round(timing[whf.PREC_FP32_GATE] * 1e3, 3),
+        "fp32_gate_over_fp64": round(timing[whf.PREC_FP64] / timing[whf.PREC_FP32_GATE], 3),
+        "finds_identical_fp32_vs_fp64": identical,
         "fp64_rechecks": last["fp64_rechecks"],
         "fp32_error": {"stage": "erosion (8 octave samplers, y=0)", "max_abs_per_octave": err,
                        "bound_used": whf.fp32_bound()},
