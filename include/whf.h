@@ -66,12 +66,13 @@ typedef struct {
 } whf_request;
 
 /* One phase-1 survivor: a hut position that passed every phase-1 gate. */
+#define WHF_HIT_DEEPDIVE 1
 typedef struct {
     int64_t seed;
     int32_t rx, rz;             /* region */
     int32_t cx, cz;             /* hut chunk; the hut's block is (16*cx, 16*cz) */
-    int32_t floor_y;            /* not computed by phase 1: INT32_MIN */
-    int32_t flags;              /* reserved, 0 */
+    int32_t floor_y;            /* Seed Check: not computed by phase 1, INT32_MIN; Deep Dive: the room floor */
+    int32_t flags;              /* 0 for a Seed Check hit, WHF_HIT_DEEPDIVE for a Deep Dive find */
 } whf_hit;
 
 typedef struct {
@@ -124,6 +125,67 @@ int whf_noise_error(whf_ctx *ctx, const int64_t *seeds, uint64_t n_seeds, uint64
 
 /* The per-evaluation error bound the FP32 gate uses as its margin. */
 double whf_fp32_bound(void);
+
+/* ---- Deep Dive: mineshaft rooms under hut chunks, over 48-bit structure seeds ---- */
+
+typedef struct {
+    uint64_t s_first;           /* first structure seed; S_i = (s_first + i) mod 2^48, i < count */
+    uint64_t count;             /* structure seeds of the sweep; 0..2^48 */
+    int rx0, rx1, rz0, rz1;     /* hut regions, half-open */
+    int floor_max;              /* keep mineshaft starts whose room floor is <= floor_max (-62..63) */
+    int expand;                 /* 1: expand every kept start to its 2^16 world seeds (u << 48 | S) and run the
+                                   phase-1 swamp check on them at the hut's region; 0: sweep only */
+    int max_height;             /* phase-1 maxHeight of the expansion's check (as in whf_request) */
+    int vflags;                 /* WHF_V_* rules of the expansion's check */
+    int salt;                   /* structure salt; 0 = swamp hut 14357620 */
+    int reserved;
+} whf_dd_request;
+
+/* A mineshaft start in a hut chunk with a deep room: the solver's hit (S, cx, cz, X) plus the room floor. */
+typedef struct {
+    uint64_t s48;               /* structure seed (the world seed's low 48 bits) */
+    int32_t rx, rz, cx, cz;     /* hut region and chunk (the mineshaft starts in the same chunk) */
+    uint64_t carver;            /* the chunk's carver seed X */
+    int32_t room_floor;         /* the room's floor y after the move below sea level */
+    int32_t pieces;             /* pieces of the layout */
+} whf_dd_spot;
+
+typedef struct {
+    whf_dd_spot *spots;         /* caller-owned array */
+    uint64_t capacity;
+    uint64_t count;             /* out: spots found (may exceed capacity -> WHF_EOVERFLOW) */
+} whf_dd_sink;
+
+typedef struct {
+    uint64_t structure_seeds;   /* S values swept */
+    uint64_t tests;             /* hut-chunk start tests (structure seeds x regions) */
+    uint64_t starts;            /* mineshaft starts in hut chunks */
+    uint64_t walks[3];          /* layout walks: complete, stopped by a piece too low for floor_max, capped
+                                   (more than 512 pieces or a box beyond the int8 offsets the reference keeps) */
+    uint64_t spots;             /* starts with room floor <= floor_max */
+    uint64_t draws, pieces;     /* LCG steps and pieces over all walks */
+    uint64_t expanded_seeds;    /* world seeds checked by the expansion (spots x 65536) */
+    uint64_t accepted;          /* expansion survivors (finds) */
+    uint64_t funnel[WHF_KIND_N];/* expansion: per phase-1 reject kind, as in whf_stats */
+    uint64_t tiers[8];
+    uint64_t samplers_built[3];
+    double device_ms;
+    double stage_ms[3];         /* sweep (start tests), layout walks, expansion (device time per stage) */
+} whf_dd_stats;
+
+/* Deep Dive over a range of structure seeds: for every S and hut region, the hut chunk's carver seed X
+ * (room_core.h:55-58), the mineshaft start test nextDouble() < 0.004 (mstree.cpp:10-15) and, for starts, the
+ * layout walk that fixes the room floor (mineshaft_tree.h:529-565). Starts with a room floor <= floor_max go
+ * to `spots` (the hits of the reference solver, hut_lookup_v4_source.cu:1341 `S,cx,cz,X,1`, here with the
+ * floor); with expand = 1 each is expanded to its 2^16 world seeds and checked by the phase-1 chain, finds in
+ * `finds` (floor_y = the room floor, flags = 1). Replaces the solve + terrain stages hunt.py drives through
+ * SolveRun (hunt.py:947-957). Not re-entrant per context. */
+int whf_deepdive(whf_ctx *ctx, const whf_dd_request *req, whf_dd_sink *spots, whf_sink *finds, whf_dd_stats *stats);
+
+/* Layout walks of given carver seeds, each as if it were a start: status (0 complete, 2 capped), room floor
+ * and piece count per seed (mstree's emit, mstree.cpp:17-34, without the piece list). For parity checks. */
+int whf_room_floors(whf_ctx *ctx, const uint64_t *carvers, uint64_t n, int32_t *status, int32_t *room_floor,
+                    int32_t *pieces);
 
 /* Version name -> WHF_V_* flags, accepting lysh's version names (main.c:104-125, biome_trees.h:70499-70526). */
 int whf_version_flags(const char *name, int *vflags);

@@ -1,0 +1,500 @@
+// whf_deepdive.hip — the MI355X Deep Dive: mineshaft rooms under swamp-hut chunks, over 48-bit structure seeds.
+//
+// Per sweep batch of structure seeds S (one HIP stream):
+//   k_dd_sweep     one lane per S: Random(S) -> a, b; per hut region the hut chunk (structure.c:4-24), the chunk's
+//                  carver seed X = (cx*a ^ cz*b ^ S) mod 2^48 and the start test nextDouble() < 0.004
+//                  (room_core.h:55-72, mstree.cpp:10-15); starts -> queue.
+//   k_dd_floor     a group of G lanes per start walks the mineshaft layout (whf_mineshaft.h) with its pieces in
+//                  LDS; a room floor <= floor_max makes the start a spot. A piece below -12 - floor_max ends
+//                  the walk early: the room floor is r - minY - 12 with r >= 0 the final nextInt of the move
+//                  below sea level, so such a start can no longer qualify.
+// Expansion, 16 spots (2^20 world seeds u << 48 | S) per batch:
+//   k_dd_xseeds    the batch's world seeds.
+//   k_dd_erosion   the tiered erosion gate (noise.c:432-459) at the hut, one lane per world seed, building each
+//                  erosion octave sampler in LDS only when the seed is still alive at that evaluation (the
+//                  reference's staged kernels stageA0a..A2b, hut_lookup_v4_source.cu:523-619, do the same).
+//   then gates 2 and 3 of the Seed Check (whf_scan.hip) on the erosion survivors -> finds.
+#include <climits>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "whf_internal.h"
+#include "whf_mineshaft.h"
+
+using namespace whf;
+
+namespace {
+
+struct DdStart {
+    uint64_t s48, carver;
+    int32_t cx, cz;
+};
+
+constexpr uint32_t kDdStartCap = 1u << 21;      // starts per sweep batch
+constexpr uint32_t kDdSpotCap = 1u << 20;       // spots per sweep batch
+constexpr uint64_t kDdSweepMax = 1ull << 24;    // structure seeds per sweep batch
+constexpr uint32_t kDdSpotsPerExpand = kMaxBatchSeeds >> 16;
+constexpr int kDdGroup = 16;                    // lanes per layout walk
+
+// ---------------------------------------------------------------------------------------------- sweep
+
+__global__ __launch_bounds__(256) void k_dd_sweep(uint64_t s_first, uint32_t n, int rx0, int nrx, int rz0, int nrz,
+                                                   int salt, DdStart *__restrict__ starts, uint32_t cap,
+                                                   uint32_t *__restrict__ ctl) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const bool live = i < n;
+    const uint64_t s48 = (s_first + i) & kLcgM48;
+    uint64_t st = (s48 ^ kLcgMul) & kLcgM48;
+    const uint64_t a = lcg_next_long(st);
+    const uint64_t b = lcg_next_long(st);
+    for (int rz = 0; rz < nrz; rz++) {
+        for (int rx = 0; rx < nrx; rx++) {
+            bool start = false;
+            int cx = 0, cz = 0;
+            uint64_t X = 0;
+            if (live) {
+                hut_chunk(s48, rx0 + rx, rz0 + rz, salt, cx, cz);
+                X = carver_seed(s48, a, b, cx, cz);
+                start = ms_is_start(X);
+            }
+            const uint32_t pos = wave_append(start, &ctl[CTL_DD_STARTS]);
+            if (start && pos < cap) starts[pos] = DdStart{s48, X, cx, cz};
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------- layout walks
+
+template <int G>
+__global__ __launch_bounds__(64) void k_dd_floor(const DdStart *__restrict__ starts, const uint32_t *__restrict__ ctl_in,
+                                                  uint32_t cap, int cut_rel, int floor_max,
+                                                  whf_dd_spot *__restrict__ spots, uint32_t spot_cap,
+                                                  uint32_t *__restrict__ ctl, unsigned long long *__restrict__ cnt) {
+    constexpr int NG = 64 / G;
+    __shared__ uint2 s_box[NG][kMsMaxPieces];
+    __shared__ uint32_t s_frm[NG][kMsMaxFrames];
+    const int lane = (int)threadIdx.x, grp = lane / G, lig = lane % G;
+    const uint32_t n = min(ctl_in[CTL_DD_STARTS], cap);
+    unsigned long long c[5] = {0, 0, 0, 0, 0};   // done, cut, capped, draws, pieces
+    for (uint32_t q = blockIdx.x * NG + (uint32_t)grp; q < n; q += gridDim.x * NG) {
+        const DdStart s = starts[q];
+        const MsOut o = ms_room_floor<G>(s.carver, cut_rel, s_box[grp], s_frm[grp], lig, grp * G);
+        if (lig == 0) {
+            c[o.status]++;
+            c[3] += o.draws;
+            c[4] += (unsigned long long)o.pieces;
+            if (o.status == MS_DONE && o.floor <= floor_max) {
+                const uint32_t pos = atomicAdd(&ctl[CTL_DD_SPOTS], 1u);
+                if (pos < spot_cap) {
+                    whf_dd_spot sp;
+                    sp.s48 = s.s48;
+                    sp.rx = s.cx >> 5;
+                    sp.rz = s.cz >> 5;
+                    sp.cx = s.cx;
+                    sp.cz = s.cz;
+                    sp.carver = s.carver;
+                    sp.room_floor = o.floor;
+                    sp.pieces = o.pieces;
+                    spots[pos] = sp;
+                }
+            }
+        }
+    }
+    for (int k = 0; k < 5; k++) {
+        unsigned long long v = c[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+        if (lane == 0 && v) atomicAdd(&cnt[CNT_DD_DONE + k], v);
+    }
+}
+
+// Full walks of given carver seeds (no cut): status, room floor, pieces.
+template <int G>
+__global__ __launch_bounds__(64) void k_dd_walks(const uint64_t *__restrict__ carvers, uint32_t n,
+                                                  int4 *__restrict__ out) {
+    constexpr int NG = 64 / G;
+    __shared__ uint2 s_box[NG][kMsMaxPieces];
+    __shared__ uint32_t s_frm[NG][kMsMaxFrames];
+    const int lane = (int)threadIdx.x, grp = lane / G, lig = lane % G;
+    for (uint32_t q = blockIdx.x * NG + (uint32_t)grp; q < n; q += gridDim.x * NG) {
+        const MsOut o = ms_room_floor<G>(carvers[q], INT_MIN, s_box[grp], s_frm[grp], lig, grp * G);
+        if (lig == 0) out[q] = make_int4(o.status, o.floor, o.pieces, (int)o.draws);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------- expansion
+
+__global__ void k_dd_xseeds(const whf_dd_spot *__restrict__ spots, uint32_t ns, uint64_t *__restrict__ seeds) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    seeds[i] = ((uint64_t)(i & 0xFFFFu) << 48) | spots[i >> 16].s48;
+}
+
+// Byte i of this lane's 256-byte table in the lane-interleaved LDS layout (word w of lane l at w * 64 + l).
+struct LanePerm {
+    const uint8_t *bytes;
+    uint32_t lane4;
+    __device__ __forceinline__ int operator()(int i) const {
+        i &= 255;
+        return bytes[((uint32_t)(i >> 2) << 8) + lane4 + (uint32_t)(i & 3)];
+    }
+};
+
+constexpr int kXPass = 512;   // world seeds per pass of a workgroup (one wave)
+
+__global__ __launch_bounds__(64) void k_dd_erosion(const uint64_t *__restrict__ seeds, uint32_t ns,
+                                                    const whf_dd_spot *__restrict__ spots,
+                                                    const BuildDesc *__restrict__ bdE,
+                                                    const GateTables *__restrict__ gt, int single,
+                                                    uint4 *__restrict__ q1, uint32_t *__restrict__ ctl,
+                                                    uint32_t *__restrict__ needC,
+                                                    unsigned long long *__restrict__ cnt) {
+    __shared__ uint32_t words[64 * 64];
+    __shared__ uint16_t s_li[2][kXPass];
+    __shared__ double s_sum[kXPass];
+    __shared__ uint32_t s_cnt[8];
+    uint8_t *bytes = reinterpret_cast<uint8_t *>(words);
+    const uint32_t lane = threadIdx.x;
+    if (lane < 8) s_cnt[lane] = 0;
+    const EroDesc &ed = gt->ero;
+    const double amplitude = ed.amplitude;
+    unsigned long long built = 0;
+    __syncthreads();
+
+    // a pass never straddles two spots: 65536 is a multiple of kXPass
+    for (uint32_t p0 = blockIdx.x * kXPass; p0 < ns; p0 += gridDim.x * kXPass) {
+        const uint32_t np = min((uint32_t)kXPass, ns - p0);
+        const whf_dd_spot sp = spots[p0 >> 16];
+        if (single) {
+            // SINGLE_BIOME: the climate gates, erosion included, are skipped
+            for (uint32_t q0 = 0; q0 < np; q0 += 64) {
+                const bool alive = q0 + lane < np;
+                const uint32_t pos = wave_append(alive, &ctl[CTL_Q1]);
+                if (alive) {
+                    q1[pos] = make_uint4(p0 + q0 + lane, (uint32_t)sp.cx, (uint32_t)sp.cz, 0u);
+                    needC[p0 + q0 + lane] = 1u;
+                }
+            }
+            continue;
+        }
+        const double x = (double)(sp.cx * 16 + 8) / 4.0, z = (double)(sp.cz * 16 + 8) / 4.0;
+        const bool raw = fabs(x) < ed.coord_lim && fabs(z) < ed.coord_lim;
+        uint32_t n_alive = np;
+        for (int k = 0; k < 8; k++) {
+            const int cur = (k - 1) & 1, nxt = k & 1;
+            const BuildDesc d = bdE[k];
+            double xs = x * ed.sc[k] * ed.lac[k], zs = z * ed.sc[k] * ed.lac[k];
+            if (!raw) { xs = keep_precision(xs); zs = keep_precision(zs); }
+            uint32_t n_next = 0;
+            for (uint32_t q0 = 0; q0 < n_alive; q0 += 64) {
+                const uint32_t q = q0 + lane;
+                bool alive = false;
+                uint32_t idx = 0;
+                if (q < n_alive) {
+                    idx = k == 0 ? q : s_li[cur][q];
+                    double sum = k == 0 ? 0.0 : s_sum[idx];
+                    // worldSeed -> deriver -> createRandom(erosion) -> octave deriver -> createRandom("octave_<n>")
+                    // (noise.c:80-143, rng.c:157-170), then PerlinNoiseSampler(random) (noise.c:61-78)
+                    const uint64_t seed = seeds[p0 + idx];
+                    Xr r = xr_seed(seed);
+                    Xr der = xr_fork(r);
+                    Xr nr = xr_make(d.na ^ der.lo, d.nb ^ der.hi);
+                    if (d.sub) { xr_next(nr); xr_next(nr); }
+                    Xr od = xr_fork(nr);
+                    Xr pr = xr_make(d.oa ^ od.lo, d.ob ^ od.hi);
+                    SPar s;
+                    s.ox = xr_next_double(pr) * 256.0;
+                    s.oy = xr_next_double(pr) * 256.0;
+                    s.oz = xr_next_double(pr) * 256.0;
+#pragma unroll 8
+                    for (int w = 0; w < 64; w++) words[w * 64 + lane] = 0x03020100u + (uint32_t)w * 0x04040404u;
+                    uint32_t ai = lane * 4;
+                    uint32_t jj = xr_next_bounded(pr, 256u);
+                    for (int i = 0; i < 256; i++) {
+                        const uint32_t aj = ((jj >> 2) << 8) + lane * 4 + (jj & 3);
+                        const uint8_t vi = bytes[ai];
+                        const uint8_t vj = bytes[aj];
+                        if (i < 255) jj = (uint32_t)(i + 1) + xr_next_bounded(pr, 255u - (uint32_t)i);
+                        bytes[ai] = vj;
+                        bytes[aj] = vi;
+                        ai += ((i & 3) == 3) ? 256 - 3 : 1;
+                    }
+                    const double fj = floor(s.oy);
+                    s.jy = (int)fj;
+                    s.hy = s.oy - fj;
+                    s.hy1 = s.hy - 1.0;
+                    s.fy = fade(s.hy);
+                    s.pad = 0;
+                    built++;
+                    const double v = perlin_y0(LanePerm{bytes, lane * 4}, s, xs, zs);
+                    sum += ed.w[k] * v;
+                    if (k >= 1 && sum * amplitude < ed.thr[k - 1]) {
+                        atomicAdd(&s_cnt[k - 1], 1u);
+                    } else {
+                        alive = true;
+                        s_sum[idx] = sum;
+                    }
+                }
+                if (k < 7) {
+                    const unsigned long long m = __ballot(alive);
+                    if (alive) s_li[nxt][n_next + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)idx;
+                    n_next += (uint32_t)__popcll(m);
+                } else {
+                    const uint32_t pos = wave_append(alive, &ctl[CTL_Q1]);
+                    if (alive) {
+                        q1[pos] = make_uint4(p0 + idx, (uint32_t)sp.cx, (uint32_t)sp.cz, 0u);
+                        needC[p0 + idx] = 1u;
+                    }
+                }
+            }
+            __syncthreads();
+            n_alive = n_next;
+        }
+    }
+    __syncthreads();
+    if (lane < 7 && s_cnt[lane]) atomicAdd(&cnt[CNT_TIER0 + lane], (unsigned long long)s_cnt[lane]);
+    for (int o = 32; o > 0; o >>= 1) built += __shfl_down(built, o);
+    if (lane == 0 && built) atomicAdd(&cnt[CNT_BUILT + SET_E], built);
+}
+
+// ---------------------------------------------------------------------------------------------- host
+
+struct DdBufs {
+    DdStart *starts;
+    whf_dd_spot *spots;
+};
+
+int dd_alloc(whf_ctx *c, DdBufs &b) {
+    const size_t bytes = (size_t)kDdStartCap * sizeof(DdStart) + (size_t)kDdSpotCap * sizeof(whf_dd_spot);
+    if (!c->d_dd) {
+        HIP_TRY(hipMalloc(&c->d_dd, bytes));
+        c->dd_bytes = bytes;
+    }
+    b.starts = reinterpret_cast<DdStart *>(c->d_dd);
+    b.spots = reinterpret_cast<whf_dd_spot *>(reinterpret_cast<uint8_t *>(c->d_dd) +
+                                              (size_t)kDdStartCap * sizeof(DdStart));
+    return WHF_OK;
+}
+
+// One sweep batch: start tests and layout walks of S in [s_first, s_first + n) (mod 2^48); the batch's spots are
+// left at the front of b.spots.
+int dd_sweep(whf_ctx *c, const whf_dd_request *req, const DdBufs &b, uint64_t s_first, uint32_t n,
+             uint32_t *n_spots) {
+    const hipStream_t st = c->stream;
+    const int nrx = req->rx1 - req->rx0, nrz = req->rz1 - req->rz0;
+    const int salt = req->salt ? req->salt : 14357620;
+    HIP_TRY(hipMemsetAsync(c->d_ctl + CTL_DD_STARTS, 0, 2 * sizeof(uint32_t), st));
+    {
+        Prof p(c, F_DD_SWEEP);
+        hipLaunchKernelGGL(k_dd_sweep, dim3((n + 255) / 256), dim3(256), 0, st, s_first, n, req->rx0, nrx, req->rz0,
+                           nrz, salt, b.starts, kDdStartCap, c->d_ctl);
+    }
+    uint32_t n_starts = 0;
+    HIP_TRY(hipMemcpyAsync(&n_starts, c->d_ctl + CTL_DD_STARTS, sizeof n_starts, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (n_starts > kDdStartCap) {
+        // batches are sized for half the queue on average (the start count is binomial, p = 0.004)
+        std::snprintf(g_last_error, sizeof g_last_error, "start queue overflow (%u starts in one batch)", n_starts);
+        return WHF_EOVERFLOW;
+    }
+    {
+        Prof p(c, F_DD_FLOOR);
+        const int cut_rel = -62 - req->floor_max;
+        const uint32_t groups = 64 / kDdGroup;
+        const uint32_t grid = std::max(1u, std::min((n_starts + groups - 1) / groups, (uint32_t)c->n_cu * 16));
+        hipLaunchKernelGGL(k_dd_floor<kDdGroup>, dim3(grid), dim3(64), 0, st, b.starts, c->d_ctl, kDdStartCap, cut_rel,
+                           req->floor_max, b.spots, kDdSpotCap, c->d_ctl, c->d_cnt);
+    }
+    HIP_TRY(hipMemcpyAsync(n_spots, c->d_ctl + CTL_DD_SPOTS, sizeof *n_spots, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return check_launch();
+}
+
+int dd_expand(whf_ctx *c, const whf_dd_request *req, const whf_dd_spot *d_spots, uint32_t n_spots) {
+    const hipStream_t st = c->stream;
+    const int layout = req->vflags;
+    for (uint32_t s0 = 0; s0 < n_spots; s0 += kDdSpotsPerExpand) {
+        const uint32_t k = std::min(kDdSpotsPerExpand, n_spots - s0);
+        const uint32_t ns = k << 16;
+        launch_reset(c, ns);
+        {
+            Prof p(c, F_DD_EXPAND);
+            hipLaunchKernelGGL(k_dd_xseeds, dim3((ns + 255) / 256), dim3(256), 0, st, d_spots + s0, ns, c->d_seeds);
+            const uint32_t grid = std::min((ns + kXPass - 1) / kXPass, (uint32_t)c->n_cu * 8);
+            hipLaunchKernelGGL(k_dd_erosion, dim3(grid), dim3(64), 0, st, c->d_seeds, ns, d_spots + s0,
+                               bd_of(c, layout, SET_E), c->d_gt + layout, (req->vflags & WHF_V_SINGLE) ? 1 : 0,
+                               c->d_q1, c->d_ctl, c->d_needC, c->d_cnt);
+        }
+        int rc = run_tail(c, layout, ns, req->vflags, req->max_height);
+        if (rc) return rc;
+    }
+    return WHF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int whf_deepdive(whf_ctx *c, const whf_dd_request *req, whf_dd_sink *spots, whf_sink *finds, whf_dd_stats *stats) {
+    if (!c || !req) return WHF_EINVAL;
+    g_last_error[0] = 0;
+    if (req->rx1 <= req->rx0 || req->rz1 <= req->rz0) return WHF_EINVAL;
+    const int64_t lim = 65536;
+    if (req->rx0 < -lim || req->rx1 > lim || req->rz0 < -lim || req->rz1 > lim) return WHF_EINVAL;
+    if ((uint64_t)(req->rx1 - req->rx0) * (uint64_t)(req->rz1 - req->rz0) > 4096) return WHF_EINVAL;
+    if (req->count > (1ull << 48)) return WHF_EINVAL;
+    if (req->floor_max < -1000 || req->floor_max > 1000) return WHF_EINVAL;
+    if (req->expand != 0 && req->expand != 1) return WHF_EINVAL;
+    if (req->vflags < 0 || req->vflags >= kLayouts) return WHF_EINVAL;
+    if (req->expand && (req->max_height < -140 || req->max_height > 100000)) return WHF_EINVAL;
+    if (spots && spots->capacity && !spots->spots) return WHF_EINVAL;
+    if (finds && finds->capacity && !finds->hits) return WHF_EINVAL;
+    HIP_TRY(hipSetDevice(c->device));
+    DdBufs b;
+    int rc = dd_alloc(c, b);
+    if (rc) return rc;
+    const hipStream_t st = c->stream;
+    HIP_TRY(hipMemsetAsync(c->d_cnt, 0, CNT_N * sizeof(unsigned long long), st));
+    HIP_TRY(hipMemsetAsync(c->d_ctl, 0, CTL_N * sizeof(uint32_t), st));
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    c->prof_ev.clear();
+    c->prof_fam.clear();
+    HIP_TRY(hipEventRecord(c->ev0, st));
+
+    // structure seeds per batch: about kDdStartCap / 2 expected starts (start probability 0.004 = 1/250)
+    const uint64_t regions = (uint64_t)(req->rx1 - req->rx0) * (uint64_t)(req->rz1 - req->rz0);
+    const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(kDdSweepMax, (uint64_t)(kDdStartCap / 2) * 250 / regions));
+    uint64_t spots_total = 0;
+    std::map<std::tuple<uint64_t, int, int>, int> floor_of;   // (S, cx, cz) -> room floor, to label the finds
+    std::vector<whf_dd_spot> host_spots;
+    for (uint64_t done = 0; done < req->count; done += per) {
+        const uint32_t n = (uint32_t)std::min<uint64_t>(per, req->count - done);
+        uint32_t ns = 0;
+        rc = dd_sweep(c, req, b, (req->s_first + done) & kLcgM48, n, &ns);
+        if (rc) return rc;
+        if (ns > kDdSpotCap) {
+            std::snprintf(g_last_error, sizeof g_last_error, "spot buffer overflow (%u spots in one batch)", ns);
+            return WHF_EOVERFLOW;
+        }
+        if (ns) {
+            host_spots.resize(ns);
+            HIP_TRY(hipMemcpyAsync(host_spots.data(), b.spots, ns * sizeof(whf_dd_spot), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            for (const whf_dd_spot &s : host_spots) {
+                if (spots && spots_total < spots->capacity) spots->spots[spots_total] = s;
+                floor_of[std::make_tuple(s.s48, s.cx, s.cz)] = s.room_floor;
+                spots_total++;
+            }
+            if (req->expand) {
+                rc = dd_expand(c, req, b.spots, ns);
+                if (rc) return rc;
+            }
+        }
+    }
+    HIP_TRY(hipEventRecord(c->ev1, st));
+    unsigned long long cnt[CNT_N];
+    uint32_t ctl[CTL_N];
+    HIP_TRY(hipMemcpyAsync(cnt, c->d_cnt, sizeof cnt, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ctl, c->d_ctl, sizeof ctl, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (cnt[CNT_SLOT_OVER]) {
+        std::snprintf(g_last_error, sizeof g_last_error, "%llu gate survivors beyond the sampler-table slots",
+                      (unsigned long long)cnt[CNT_SLOT_OVER]);
+        return WHF_EOVERFLOW;
+    }
+    rc = WHF_OK;
+    if (spots) {
+        spots->count = spots_total;
+        if (spots_total > spots->capacity) rc = WHF_EOVERFLOW;
+    }
+    const uint64_t hits_total = req->expand ? ctl[CTL_HITS] : 0;
+    if (finds) {
+        std::vector<DevHit> hh(std::min<uint64_t>(hits_total, kHitCap));
+        if (!hh.empty()) HIP_TRY(hipMemcpy(hh.data(), c->d_hits, hh.size() * sizeof(DevHit), hipMemcpyDeviceToHost));
+        finds->count = hits_total;
+        const uint64_t m = std::min<uint64_t>(finds->capacity, hh.size());
+        for (uint64_t i = 0; i < m; i++) {
+            whf_hit h;
+            std::memcpy(&h, &hh[i], sizeof h);
+            auto it = floor_of.find(std::make_tuple((uint64_t)h.seed & kLcgM48, h.cx, h.cz));
+            h.floor_y = it == floor_of.end() ? INT32_MIN : it->second;
+            h.flags = WHF_HIT_DEEPDIVE;
+            finds->hits[i] = h;
+        }
+        if (hits_total > finds->capacity || hh.size() < hits_total) rc = WHF_EOVERFLOW;
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->structure_seeds = req->count;
+        stats->tests = req->count * regions;
+        stats->walks[0] = cnt[CNT_DD_DONE];
+        stats->walks[1] = cnt[CNT_DD_CUT];
+        stats->walks[2] = cnt[CNT_DD_CAPPED];
+        stats->starts = stats->walks[0] + stats->walks[1] + stats->walks[2];
+        stats->spots = spots_total;
+        stats->draws = cnt[CNT_DD_DRAWS];
+        stats->pieces = cnt[CNT_DD_PIECES];
+        if (req->expand) {
+            stats->expanded_seeds = spots_total << 16;
+            uint64_t ero = 0;
+            for (int t = 0; t < 7; t++) { stats->tiers[t] = cnt[CNT_TIER0 + t]; ero += cnt[CNT_TIER0 + t]; }
+            stats->funnel[0] = ero;
+            for (int k = 1; k < WHF_KIND_N; k++) stats->funnel[k] = cnt[CNT_KIND0 + k];
+            stats->accepted = cnt[CNT_KIND0 + 10];
+            for (int s = 0; s < SET_COUNT; s++) stats->samplers_built[s] = cnt[CNT_BUILT + s];
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        stats->device_ms = ms;
+        if (c->profile) {
+            for (size_t i = 0; i + 1 < c->prof_ev.size(); i += 2) {
+                float m2 = 0.f;
+                (void)hipEventElapsedTime(&m2, c->prof_ev[i], c->prof_ev[i + 1]);
+                const int f = c->prof_fam[i / 2];
+                if (f == F_DD_SWEEP) stats->stage_ms[0] += m2;
+                else if (f == F_DD_FLOOR) stats->stage_ms[1] += m2;
+                else stats->stage_ms[2] += m2;   // the expansion and the Seed Check gates it runs
+            }
+        }
+    }
+    return rc;
+}
+
+int whf_room_floors(whf_ctx *c, const uint64_t *carvers, uint64_t n, int32_t *status, int32_t *room_floor,
+                    int32_t *pieces) {
+    if (!c || (n && (!carvers || !status || !room_floor || !pieces))) return WHF_EINVAL;
+    if (n > (1u << 24)) return WHF_EINVAL;
+    if (!n) return WHF_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t st = c->stream;
+    uint64_t *d_in = nullptr;
+    int4 *d_out = nullptr;
+    HIP_TRY(hipMalloc(&d_in, n * sizeof(uint64_t)));
+    if (hipMalloc(&d_out, n * sizeof(int4)) != hipSuccess) {
+        (void)hipFree(d_in);
+        return WHF_ENOMEM;
+    }
+    std::vector<int4> out(n);
+    int rc = WHF_OK;
+    if (hipMemcpyAsync(d_in, carvers, n * sizeof(uint64_t), hipMemcpyHostToDevice, st) != hipSuccess) rc = WHF_EDEVICE;
+    if (rc == WHF_OK) {
+        const uint32_t groups = 64 / kDdGroup;
+        const uint32_t grid = std::min<uint32_t>(((uint32_t)n + groups - 1) / groups, (uint32_t)c->n_cu * 16);
+        hipLaunchKernelGGL(k_dd_walks<kDdGroup>, dim3(grid), dim3(64), 0, st, d_in, (uint32_t)n, d_out);
+        rc = check_launch();
+    }
+    if (rc == WHF_OK && (hipMemcpyAsync(out.data(), d_out, n * sizeof(int4), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                         hipStreamSynchronize(st) != hipSuccess))
+        rc = WHF_EDEVICE;
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    if (rc) return rc;
+    for (uint64_t i = 0; i < n; i++) {
+        status[i] = out[i].x;
+        room_floor[i] = out[i].y;
+        pieces[i] = out[i].z;
+    }
+    return WHF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+// whf_internal.h — what the library's two translation units share: the context, the device control words and
+// counters, the wave-aggregated queue append, error and profiling helpers, and the Seed Check stages that the
+// Deep Dive expansion reuses (whf_scan.hip defines them, whf_deepdive.hip calls them).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <vector>
+
+#include "../../include/whf.h"
+#include "whf_kernels.h"
+#include "whf_tables.h"
+
+namespace whf {
+
+constexpr int kWave = 64;
+
+// Device control words of a batch.
+enum Ctl { CTL_Q1 = 0, CTL_Q2, CTL_NC, CTL_NL, CTL_QX, CTL_HITS, CTL_DD_STARTS, CTL_DD_SPOTS, CTL_N };
+
+// Device counters of a scan (u64).
+enum Cnt {
+    CNT_TIER0 = 0,                       // 0..6 tiers
+    CNT_KIND0 = 8,                       // 8 + kind, kinds 1..10 (kind 0 = sum of tiers)
+    CNT_LADDER = 20,
+    CNT_BUILT = 21,                      // 21 + set
+    CNT_QX = 24,                         // FP32 gate: items re-evaluated in FP64
+    CNT_QX_OVER = 25,                    // FP32 gate: re-evaluation queue overflow (the scan is redone in FP64)
+    CNT_SLOT_OVER = 26,                  // more gate survivors than sampler-table slots (the batch is redone smaller)
+    CNT_DD_DONE = 27,                    // Deep Dive layout walks: complete, cut below the line, capped
+    CNT_DD_CUT = 28,
+    CNT_DD_CAPPED = 29,
+    CNT_DD_DRAWS = 30,
+    CNT_DD_PIECES = 31,
+    CNT_N = 32
+};
+
+struct DevHit {
+    int64_t seed;
+    int32_t rx, rz, cx, cz, floor_y, flags;
+};
+static_assert(sizeof(DevHit) == sizeof(whf_hit), "hit layout");
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-aggregated append of `pred` lanes to a queue: returns this lane's slot (valid where pred).
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter) {
+    unsigned long long mask = __ballot(pred);
+    uint32_t base = 0;
+    if (mask) {
+        int leader = __ffsll((long long)mask) - 1;
+        if ((int)lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+    }
+    unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
+    return base + (uint32_t)__popcll(below);
+}
+
+constexpr uint32_t kMaxSeeds = 1u << 17;      // sampler-table slots per set (seeds with tables at once)
+constexpr uint32_t kMaxBatchSeeds = 1u << 20; // seeds per batch (the Deep Dive expansion runs 16 spots at once)
+constexpr uint32_t kMaxItems = 1u << 25;      // (seed, region) items per batch
+constexpr uint32_t kHitCap = 1u << 20;
+constexpr uint32_t kQxCap = 1u << 22;         // FP32 gate re-evaluation queue
+constexpr int kLayouts = 16;                  // every combination of the four rule flags
+constexpr int kBdPerLayout = 8 + 28 + 36;
+
+extern thread_local char g_last_error[512];
+
+#define HIP_TRY(expr)                                                                                       \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) {                                                                             \
+            std::snprintf(::whf::g_last_error, sizeof ::whf::g_last_error, "%s: %s", #expr,                \
+                          hipGetErrorString(e_));                                                           \
+            return WHF_EDEVICE;                                                                             \
+        }                                                                                                   \
+    } while (0)
+
+// Kernel families for the profile breakdown (Seed Check 0..7, Deep Dive 8..10).
+enum Fam {
+    F_SEEDS = 0, F_BUILD_E, F_EROSION, F_COMPACT, F_BUILD_C, F_GATE2, F_BUILD_L, F_GATE3,
+    F_DD_SWEEP, F_DD_FLOOR, F_DD_EXPAND, F_COUNT
+};
+
+}  // namespace whf
+
+struct whf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char name[256] = {0};
+    bool profile = false;
+    uint64_t *d_seeds = nullptr;     // [kMaxBatchSeeds]
+    uint8_t *d_tab[whf::SET_COUNT] = {nullptr, nullptr, nullptr};
+    whf::SPar *d_par[whf::SET_COUNT] = {nullptr, nullptr, nullptr};
+    uint4 *d_q1 = nullptr, *d_q2 = nullptr, *d_qx = nullptr;
+    uint32_t *d_needC = nullptr, *d_needL = nullptr, *d_slotC = nullptr, *d_slotL = nullptr;   // [kMaxBatchSeeds]
+    uint32_t *d_seedC = nullptr, *d_seedL = nullptr, *d_ctl = nullptr;
+    unsigned long long *d_cnt = nullptr;
+    whf::DevHit *d_hits = nullptr;
+    whf::GateTables *d_gt = nullptr;      // [kLayouts]
+    whf::BuildDesc *d_bd = nullptr;       // [kLayouts][8 + 28 + 36]
+    void *d_dd = nullptr;                 // Deep Dive buffers (whf_deepdive.hip), allocated on first use
+    size_t dd_bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> prof_ev;  // pairs per kernel launch when profiling
+    std::vector<int> prof_fam;
+    int n_cu = 256;
+};
+
+namespace whf {
+
+// Brackets the launches of one kernel family with HIP events on the library's stream when profiling.
+struct Prof {
+    whf_ctx *c;
+    int fam;
+    Prof(whf_ctx *c_, int f) : c(c_), fam(f) {
+        if (c->profile) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            (void)hipEventRecord(e, c->stream);
+            c->prof_ev.push_back(e);
+            c->prof_fam.push_back(fam);
+        }
+    }
+    ~Prof() {
+        if (c->profile) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            (void)hipEventRecord(e, c->stream);
+            c->prof_ev.push_back(e);
+        }
+    }
+};
+
+int check_launch();
+const BuildDesc *bd_of(whf_ctx *c, int layout, int set);
+// Zero the batch's queue counts and the gate flags of its first ns seeds.
+void launch_reset(whf_ctx *c, uint32_t ns);
+// Gates 2 and 3 of a batch whose erosion survivors are already in Q1 (with needC set): compact, build set C,
+// gate 2, compact, build set L, gate 3 -> hits.
+int run_tail(whf_ctx *c, int layout, uint32_t ns, int vflags, int max_height);
+
+}  // namespace whf

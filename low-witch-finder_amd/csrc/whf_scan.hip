@@ -24,52 +24,13 @@
 #include <string>
 #include <vector>
 
-#include "../../include/whf.h"
-#include "whf_kernels.h"
-#include "whf_tables.h"
+#include "whf_internal.h"
 
 using namespace whf;
 
 // ================================================================================================ device
 
 namespace {
-
-constexpr int kWave = 64;
-
-// Device control words of a batch.
-enum Ctl { CTL_Q1 = 0, CTL_Q2, CTL_NC, CTL_NL, CTL_QX, CTL_HITS, CTL_N };
-
-// Device counters of a scan (u64).
-enum Cnt {
-    CNT_TIER0 = 0,                       // 0..6 tiers
-    CNT_KIND0 = 8,                       // 8 + kind, kinds 1..10 (kind 0 = sum of tiers)
-    CNT_LADDER = 20,
-    CNT_BUILT = 21,                      // 21 + set
-    CNT_QX = 24,                         // FP32 gate: items re-evaluated in FP64
-    CNT_QX_OVER = 25,                    // FP32 gate: re-evaluation queue overflow (the scan is redone in FP64)
-    CNT_N = 32
-};
-
-struct DevHit {
-    int64_t seed;
-    int32_t rx, rz, cx, cz, floor_y, flags;
-};
-static_assert(sizeof(DevHit) == sizeof(whf_hit), "hit layout");
-
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-// Wave-aggregated append of `pred` lanes to a queue: returns this lane's slot (valid where pred).
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter) {
-    unsigned long long mask = __ballot(pred);
-    uint32_t base = 0;
-    if (mask) {
-        int leader = __ffsll((long long)mask) - 1;
-        if ((int)lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-        base = __shfl(base, leader);
-    }
-    unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
-    return base + (uint32_t)__popcll(below);
-}
 
 // ---------------------------------------------------------------------------------------------- seeds
 
@@ -101,7 +62,8 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
     __shared__ uint32_t words[64 * 64];
     uint8_t *bytes = reinterpret_cast<uint8_t *>(words);
     const uint32_t lane = threadIdx.x;
-    const uint32_t n_slots = n_slots_dev ? *n_slots_dev : n_slots_host;
+    // with a device count, n_slots_host caps it at the table capacity
+    const uint32_t n_slots = n_slots_dev ? min(*n_slots_dev, n_slots_host) : n_slots_host;
     const uint64_t n_jobs = (uint64_t)n_slots * (uint32_t)set_n;
     uint32_t built = 0;
     for (uint64_t j0 = (uint64_t)blockIdx.x * 64; j0 < n_jobs; j0 += (uint64_t)gridDim.x * 64) {
@@ -185,8 +147,6 @@ __device__ __forceinline__ void unpack_item(int2 e, int &cx, int &cz, int &sl) {
     sl = e.y >> 23;
     cz = (e.y & ((1 << 23) - 1)) - kCzBias;
 }
-
-constexpr uint32_t kQxCap = 1u << 22;   // FP32 gate re-evaluation queue
 
 // Tier decision of one check. Plain FP64: the reference's comparison. FP32 gate: decided only when the estimate
 // is farther than the margin from the threshold; otherwise the item is re-evaluated in FP64 (returns 2).
@@ -461,13 +421,13 @@ __global__ void k_noise_error(const uint8_t *__restrict__ tabE, const SPar *__re
 // ---------------------------------------------------------------------------------------------- compaction
 
 __global__ void k_compact(const uint32_t *__restrict__ need, uint32_t n, uint32_t *__restrict__ slot_of,
-                          uint32_t *__restrict__ seed_of, uint32_t *__restrict__ count) {
+                          uint32_t *__restrict__ seed_of, uint32_t *__restrict__ count, uint32_t cap) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool f = i < n && need[i];
     uint32_t pos = wave_append(f, count);
     if (f) {
-        slot_of[i] = pos;
-        seed_of[pos] = i;
+        slot_of[i] = pos;   // a slot >= cap has no tables: the gates count it as overflow
+        if (pos < cap) seed_of[pos] = i;
     }
 }
 
@@ -498,7 +458,10 @@ __global__ __launch_bounds__(kGateWG) void k_gate2(const uint64_t *__restrict__ 
             const uint32_t slot = slotC[e.x];
             const int hut_x = (int)e.y * 16, hut_z = (int)e.z * 16;
             int kind = -1;
-            if (!single) {
+            if (slot >= kMaxSeeds) {
+                atomicAdd(&cnt[CNT_SLOT_OVER], 1ull);
+                kind = 0;   // no tables: neither counted nor kept
+            } else if (!single) {
                 const double qx = (hut_x + 8) / 4.0, qz = (hut_z + 8) / 4.0;
                 const double t = noise_y0(gt->noise[G_TEMPERATURE], C, slot, qx, qz);
                 const bool t_bad = (vflags & WHF_V_118) ? (t < -0.45) : (t > 0.2 || t < -0.45);
@@ -517,7 +480,7 @@ __global__ __launch_bounds__(kGateWG) void k_gate2(const uint64_t *__restrict__ 
                 else if (cv.entrance_nonneg(x, 60, z)) kind = 4;
             }
             if (kind > 0) block_count(s_cnt, kind);
-            else survive = true;
+            else if (kind < 0) survive = true;
         }
         const uint32_t pos = wave_append(survive, &ctl[CTL_Q2]);
         if (survive) {
@@ -579,7 +542,8 @@ __global__ __launch_bounds__(kGateWG) void k_gate3(const uint64_t *__restrict__ 
                 const uint4 e = q2[q];
                 const CaveCtx cv{gt, C, L, slotC[e.x], slotL[e.x]};
                 const int x = (int)e.y * 16 + 3, z = (int)e.z * 16 + 3;
-                if (cv.spaghetti(x, max_h, z) >= 0 && cv.cheese(x, max_h, z) >= 0) atomicAdd(&s_cnt[5], 1u);
+                if (cv.slotC >= kMaxSeeds || cv.slotL >= kMaxSeeds) atomicAdd(&cnt[CNT_SLOT_OVER], 1ull);
+                else if (cv.spaghetti(x, max_h, z) >= 0 && cv.cheese(x, max_h, z) >= 0) atomicAdd(&s_cnt[5], 1u);
                 else keep = true;
             }
             const uint32_t pos = wave_append(keep, &s_n);
@@ -725,16 +689,6 @@ struct Md5 {
     }
 };
 
-#define HIP_TRY(expr)                                                                                       \
-    do {                                                                                                    \
-        hipError_t e_ = (expr);                                                                             \
-        if (e_ != hipSuccess) {                                                                             \
-            std::snprintf(g_last_error, sizeof g_last_error, "%s: %s", #expr, hipGetErrorString(e_));      \
-            return WHF_EDEVICE;                                                                             \
-        }                                                                                                   \
-    } while (0)
-
-thread_local char g_last_error[512];
 
 // Host description of one sampler of a set.
 struct SamplerRef {
@@ -858,41 +812,18 @@ void build_layout(int vflags, Layout &L) {
         }
 }
 
-constexpr uint32_t kMaxSeeds = 1u << 17;     // seed slots per batch
-constexpr uint32_t kMaxItems = 1u << 25;     // (seed, region) items per batch
-constexpr uint32_t kHitCap = 1u << 20;
-constexpr int kLayouts = 16;                 // every combination of the four rule flags
 
 }  // namespace
 
-struct whf_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    char name[256] = {0};
-    bool profile = false;
-    uint64_t *d_seeds = nullptr;
-    uint8_t *d_tab[SET_COUNT] = {nullptr, nullptr, nullptr};
-    SPar *d_par[SET_COUNT] = {nullptr, nullptr, nullptr};
-    uint4 *d_q1 = nullptr, *d_q2 = nullptr, *d_qx = nullptr;
-    uint32_t *d_needC = nullptr, *d_needL = nullptr, *d_slotC = nullptr, *d_slotL = nullptr;
-    uint32_t *d_seedC = nullptr, *d_seedL = nullptr, *d_ctl = nullptr;
-    unsigned long long *d_cnt = nullptr;
-    DevHit *d_hits = nullptr;
-    GateTables *d_gt = nullptr;      // [kLayouts]
-    BuildDesc *d_bd = nullptr;       // [kLayouts][8 + 28 + 36]
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<hipEvent_t> prof_ev;  // pairs per kernel launch when profiling
-    std::vector<int> prof_fam;
-    int n_cu = 256;
-};
+namespace whf {
+thread_local char g_last_error[512];
+}
 
 namespace {
 
-constexpr int kBdPerLayout = 8 + 28 + 36;
-
 int ctx_alloc(whf_ctx *c) {
-    const size_t S = kMaxSeeds;
-    HIP_TRY(hipMalloc(&c->d_seeds, S * sizeof(uint64_t)));
+    const size_t S = kMaxSeeds, B = kMaxBatchSeeds;
+    HIP_TRY(hipMalloc(&c->d_seeds, B * sizeof(uint64_t)));
     for (int s = 0; s < SET_COUNT; s++) {
         HIP_TRY(hipMalloc(&c->d_tab[s], S * kSetSize[s] * 256));
         HIP_TRY(hipMalloc(&c->d_par[s], S * kSetSize[s] * sizeof(SPar)));
@@ -900,13 +831,14 @@ int ctx_alloc(whf_ctx *c) {
     HIP_TRY(hipMalloc(&c->d_q1, (size_t)kMaxItems * sizeof(uint4)));
     HIP_TRY(hipMalloc(&c->d_q2, (size_t)kMaxItems * sizeof(uint4)));
     HIP_TRY(hipMalloc(&c->d_qx, (size_t)kQxCap * sizeof(uint4)));
-    HIP_TRY(hipMalloc(&c->d_needC, S * 4));
-    HIP_TRY(hipMalloc(&c->d_needL, S * 4));
-    HIP_TRY(hipMalloc(&c->d_slotC, S * 4));
-    HIP_TRY(hipMalloc(&c->d_slotL, S * 4));
+    HIP_TRY(hipMalloc(&c->d_needC, B * 4));
+    HIP_TRY(hipMalloc(&c->d_needL, B * 4));
+    HIP_TRY(hipMalloc(&c->d_slotC, B * 4));
+    HIP_TRY(hipMalloc(&c->d_slotL, B * 4));
     HIP_TRY(hipMalloc(&c->d_seedC, S * 4));
     HIP_TRY(hipMalloc(&c->d_seedL, S * 4));
     HIP_TRY(hipMalloc(&c->d_ctl, 64));
+    static_assert(CTL_N * sizeof(uint32_t) <= 64, "control words");
     HIP_TRY(hipMalloc(&c->d_cnt, CNT_N * sizeof(unsigned long long)));
     HIP_TRY(hipMalloc(&c->d_hits, (size_t)kHitCap * sizeof(DevHit)));
     HIP_TRY(hipMalloc(&c->d_gt, kLayouts * sizeof(GateTables)));
@@ -935,36 +867,15 @@ int ctx_alloc(whf_ctx *c) {
     return WHF_OK;
 }
 
+}  // namespace
+
+namespace whf {
+
 const BuildDesc *bd_of(whf_ctx *c, int layout, int set) {
     size_t o = (size_t)layout * kBdPerLayout;
     for (int s = 0; s < set; s++) o += kSetSize[s];
     return c->d_bd + o;
 }
-
-// Kernel families for the profile breakdown.
-enum Fam { F_SEEDS = 0, F_BUILD_E, F_EROSION, F_COMPACT, F_BUILD_C, F_GATE2, F_BUILD_L, F_GATE3 };
-
-struct Prof {
-    whf_ctx *c;
-    int fam;
-    Prof(whf_ctx *c_, int f) : c(c_), fam(f) {
-        if (c->profile) {
-            hipEvent_t e;
-            (void)hipEventCreate(&e);
-            (void)hipEventRecord(e, c->stream);
-            c->prof_ev.push_back(e);
-            c->prof_fam.push_back(fam);
-        }
-    }
-    ~Prof() {
-        if (c->profile) {
-            hipEvent_t e;
-            (void)hipEventCreate(&e);
-            (void)hipEventRecord(e, c->stream);
-            c->prof_ev.push_back(e);
-        }
-    }
-};
 
 int check_launch() {
     hipError_t e = hipGetLastError();
@@ -975,14 +886,63 @@ int check_launch() {
     return WHF_OK;
 }
 
+void launch_reset(whf_ctx *c, uint32_t ns) {
+    Prof p(c, F_SEEDS);
+    hipLaunchKernelGGL(k_reset, dim3(std::max(1u, (ns + 255) / 256)), dim3(256), 0, c->stream, c->d_ctl, c->d_needC,
+                       c->d_needL, ns);
+}
+
+int run_tail(whf_ctx *c, int layout, uint32_t ns, int vflags, int max_height) {
+    const hipStream_t st = c->stream;
+    const uint32_t blocks_n = std::max(1u, (ns + 255) / 256);
+    const uint32_t persist = (uint32_t)c->n_cu * 8;
+    // at most kMaxSeeds seeds of a batch get tables; the rest are counted as slot overflow by the gates
+    const uint32_t nb = std::min(ns, kMaxSeeds);
+    {
+        Prof p(c, F_COMPACT);
+        hipLaunchKernelGGL(k_compact, dim3(blocks_n), dim3(256), 0, st, c->d_needC, ns, c->d_slotC, c->d_seedC,
+                           c->d_ctl + CTL_NC, kMaxSeeds);
+    }
+    {
+        Prof p(c, F_BUILD_C);
+        hipLaunchKernelGGL(k_build, dim3(std::min<uint32_t>(persist * 4, (nb * 28 + 63) / 64 + 1)), dim3(64), 0, st,
+                           c->d_seeds, bd_of(c, layout, SET_C), 28, c->d_seedC, c->d_ctl + CTL_NC, kMaxSeeds,
+                           c->d_tab[SET_C], c->d_par[SET_C], c->d_cnt + CNT_BUILT + SET_C);
+    }
+    {
+        Prof p(c, F_GATE2);
+        hipLaunchKernelGGL(k_gate2, dim3(persist), dim3(kGateWG), 0, st, c->d_seeds, c->d_gt + layout,
+                           c->d_tab[SET_C], c->d_par[SET_C], c->d_slotC, c->d_q1, c->d_q2, c->d_ctl, c->d_needL,
+                           vflags, c->d_cnt);
+    }
+    {
+        Prof p(c, F_COMPACT);
+        hipLaunchKernelGGL(k_compact, dim3(blocks_n), dim3(256), 0, st, c->d_needL, ns, c->d_slotL, c->d_seedL,
+                           c->d_ctl + CTL_NL, kMaxSeeds);
+    }
+    {
+        Prof p(c, F_BUILD_L);
+        hipLaunchKernelGGL(k_build, dim3(std::min<uint32_t>(persist, (nb * 36 + 63) / 64 + 1)), dim3(64), 0, st,
+                           c->d_seeds, bd_of(c, layout, SET_L), 36, c->d_seedL, c->d_ctl + CTL_NL, kMaxSeeds,
+                           c->d_tab[SET_L], c->d_par[SET_L], c->d_cnt + CNT_BUILT + SET_L);
+    }
+    {
+        Prof p(c, F_GATE3);
+        hipLaunchKernelGGL(k_gate3, dim3(persist), dim3(kGateWG), 0, st, c->d_seeds, c->d_gt + layout,
+                           c->d_tab[SET_C], c->d_par[SET_C], c->d_tab[SET_L], c->d_par[SET_L], c->d_slotC,
+                           c->d_slotL, c->d_q2, c->d_ctl, vflags, max_height, c->d_hits, kHitCap, c->d_cnt);
+    }
+    return check_launch();
+}
+
+}  // namespace whf
+
+namespace {
+
 // One batch: seeds [s0, s0 + ns) of the request, flat region range [f0, f0 + fn).
 int run_batch(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_t f0, uint32_t fn, uint32_t W) {
     const hipStream_t st = c->stream;
-    const uint32_t blocks_n = (ns + 255) / 256;
-    {
-        Prof p(c, F_SEEDS);
-        hipLaunchKernelGGL(k_reset, dim3(std::max(1u, blocks_n)), dim3(256), 0, st, c->d_ctl, c->d_needC, c->d_needL, ns);
-    }
+    launch_reset(c, ns);
     {
         Prof p(c, F_BUILD_E);
         const uint32_t jobs = ns * 8;
@@ -1016,43 +976,7 @@ int run_batch(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint6
             hipLaunchKernelGGL(k_erosion_exact, dim3((uint32_t)c->n_cu * 2), dim3(256), 0, st, c->d_tab[SET_E],
                                c->d_par[SET_E], gt, c->d_qx, c->d_q1, c->d_ctl, c->d_needC, c->d_cnt);
     }
-    {
-        Prof p(c, F_COMPACT);
-        hipLaunchKernelGGL(k_compact, dim3(std::max(1u, blocks_n)), dim3(256), 0, st, c->d_needC, ns, c->d_slotC,
-                           c->d_seedC, c->d_ctl + CTL_NC);
-    }
-    const uint32_t persist = (uint32_t)c->n_cu * 8;
-    {
-        Prof p(c, F_BUILD_C);
-        hipLaunchKernelGGL(k_build, dim3(std::min<uint32_t>(persist * 4, (ns * 28 + 63) / 64 + 1)), dim3(64), 0, st,
-                           c->d_seeds, bd_of(c, layout, SET_C), 28, c->d_seedC, c->d_ctl + CTL_NC, 0u,
-                           c->d_tab[SET_C], c->d_par[SET_C], c->d_cnt + CNT_BUILT + SET_C);
-    }
-    {
-        Prof p(c, F_GATE2);
-        hipLaunchKernelGGL(k_gate2, dim3(persist), dim3(kGateWG), 0, st, c->d_seeds, c->d_gt + layout,
-                           c->d_tab[SET_C], c->d_par[SET_C], c->d_slotC, c->d_q1, c->d_q2, c->d_ctl, c->d_needL,
-                           req->vflags, c->d_cnt);
-    }
-    {
-        Prof p(c, F_COMPACT);
-        hipLaunchKernelGGL(k_compact, dim3(std::max(1u, blocks_n)), dim3(256), 0, st, c->d_needL, ns, c->d_slotL,
-                           c->d_seedL, c->d_ctl + CTL_NL);
-    }
-    {
-        Prof p(c, F_BUILD_L);
-        hipLaunchKernelGGL(k_build, dim3(std::min<uint32_t>(persist, (ns * 36 + 63) / 64 + 1)), dim3(64), 0, st,
-                           c->d_seeds, bd_of(c, layout, SET_L), 36, c->d_seedL, c->d_ctl + CTL_NL, 0u,
-                           c->d_tab[SET_L], c->d_par[SET_L], c->d_cnt + CNT_BUILT + SET_L);
-    }
-    {
-        Prof p(c, F_GATE3);
-        hipLaunchKernelGGL(k_gate3, dim3(persist), dim3(kGateWG), 0, st, c->d_seeds, c->d_gt + layout,
-                           c->d_tab[SET_C], c->d_par[SET_C], c->d_tab[SET_L], c->d_par[SET_L], c->d_slotC,
-                           c->d_slotL, c->d_q2, c->d_ctl, req->vflags, req->max_height, c->d_hits, kHitCap,
-                           c->d_cnt);
-    }
-    return check_launch();
+    return run_tail(c, layout, ns, req->vflags, req->max_height);
 }
 
 }  // namespace
@@ -1129,7 +1053,7 @@ void whf_close(whf_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_seeds, c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_par[0], c->d_par[1], c->d_par[2],
                     c->d_q1, c->d_q2, c->d_qx, c->d_needC, c->d_needL, c->d_slotC, c->d_slotL, c->d_seedC, c->d_seedL,
-                    c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd};
+                    c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd, c->d_dd};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
@@ -1161,7 +1085,7 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
     const int layout = req->vflags;
     const hipStream_t st = c->stream;
     HIP_TRY(hipMemsetAsync(c->d_cnt, 0, CNT_N * sizeof(unsigned long long), st));
-    HIP_TRY(hipMemsetAsync(c->d_ctl, 0, 64, st));
+    HIP_TRY(hipMemsetAsync(c->d_ctl, 0, CTL_N * sizeof(uint32_t), st));
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     c->prof_ev.clear();
     c->prof_fam.clear();

@@ -21,7 +21,8 @@ CFG_PROFILE = 1
 
 # Exported symbols of include/whf.h, checked by the tests.
 SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags", "whf_strerror",
-           "whf_device_name", "whf_noise_error", "whf_fp32_bound")
+           "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_room_floors")
+HIT_DEEPDIVE = 1
 
 
 class Config(ctypes.Structure):
@@ -52,6 +53,32 @@ class Stats(ctypes.Structure):
                 ("reached_ladder", ctypes.c_uint64), ("funnel", ctypes.c_uint64 * KIND_N),
                 ("tiers", ctypes.c_uint64 * 8), ("samplers_built", ctypes.c_uint64 * 3), ("device_ms", ctypes.c_double),
                 ("kernel_ms", ctypes.c_double * 8), ("fp64_rechecks", ctypes.c_uint64)]
+
+
+class DdRequest(ctypes.Structure):
+    _fields_ = [("s_first", ctypes.c_uint64), ("count", ctypes.c_uint64), ("rx0", ctypes.c_int), ("rx1", ctypes.c_int),
+                ("rz0", ctypes.c_int), ("rz1", ctypes.c_int), ("floor_max", ctypes.c_int), ("expand", ctypes.c_int),
+                ("max_height", ctypes.c_int), ("vflags", ctypes.c_int), ("salt", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
+
+
+class DdSpot(ctypes.Structure):
+    _fields_ = [("s48", ctypes.c_uint64), ("rx", ctypes.c_int32), ("rz", ctypes.c_int32), ("cx", ctypes.c_int32),
+                ("cz", ctypes.c_int32), ("carver", ctypes.c_uint64), ("room_floor", ctypes.c_int32),
+                ("pieces", ctypes.c_int32)]
+
+
+class DdSink(ctypes.Structure):
+    _fields_ = [("spots", ctypes.POINTER(DdSpot)), ("capacity", ctypes.c_uint64), ("count", ctypes.c_uint64)]
+
+
+class DdStats(ctypes.Structure):
+    _fields_ = [("structure_seeds", ctypes.c_uint64), ("tests", ctypes.c_uint64), ("starts", ctypes.c_uint64),
+                ("walks", ctypes.c_uint64 * 3), ("spots", ctypes.c_uint64), ("draws", ctypes.c_uint64),
+                ("pieces", ctypes.c_uint64), ("expanded_seeds", ctypes.c_uint64), ("accepted", ctypes.c_uint64),
+                ("funnel", ctypes.c_uint64 * KIND_N), ("tiers", ctypes.c_uint64 * 8),
+                ("samplers_built", ctypes.c_uint64 * 3), ("device_ms", ctypes.c_double),
+                ("stage_ms", ctypes.c_double * 3)]
 
 
 KERNEL_FAMILIES = ("seeds", "build_erosion", "erosion_gate", "compact", "build_climate", "gate2", "build_ladder",
@@ -88,6 +115,13 @@ def load():
                                       ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.whf_fp32_bound.restype = ctypes.c_double
         L.whf_fp32_bound.argtypes = []
+        L.whf_deepdive.restype = ctypes.c_int
+        L.whf_deepdive.argtypes = [ctypes.c_void_p, ctypes.POINTER(DdRequest), ctypes.POINTER(DdSink),
+                                   ctypes.POINTER(Sink), ctypes.POINTER(DdStats)]
+        L.whf_room_floors.restype = ctypes.c_int
+        L.whf_room_floors.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
+                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
@@ -191,6 +225,46 @@ class Context:
         if rc:
             raise WhfError(rc, "whf_noise_error")
         return list(out)
+
+
+    def deepdive(self, s_first, count, rect, floor_max=-55, expand=False, max_height=-50, vflags=0, salt=0,
+                 spot_capacity=1 << 16, find_capacity=1 << 16):
+        """Deep Dive over structure seeds S = s_first + i (mod 2^48), i < count, hut regions rect. Returns
+        (spots as tuples (s48, rx, rz, cx, cz, carver, room_floor, pieces), finds as tuples
+        (seed, rx, rz, cx, cz, room_floor), stats dict)."""
+        req = DdRequest(s_first & 0xFFFFFFFFFFFFFFFF, count, rect[0], rect[1], rect[2], rect[3], floor_max,
+                        1 if expand else 0, max_height, vflags, salt, 0)
+        sbuf = (DdSpot * spot_capacity)()
+        ssink = DdSink(sbuf, spot_capacity, 0)
+        fbuf = (Hit * find_capacity)()
+        fsink = Sink(fbuf, find_capacity, 0)
+        st = DdStats()
+        rc = self.L.whf_deepdive(self.h, ctypes.byref(req), ctypes.byref(ssink), ctypes.byref(fsink), ctypes.byref(st))
+        if rc not in (OK, EOVERFLOW):
+            raise WhfError(rc, "whf_deepdive")
+        spots = sorted((p.s48, p.rx, p.rz, p.cx, p.cz, p.carver, p.room_floor, p.pieces)
+                       for p in sbuf[:min(ssink.count, spot_capacity)])
+        finds = sorted((h.seed, h.rx, h.rz, h.cx, h.cz, h.floor_y) for h in fbuf[:min(fsink.count, find_capacity)])
+        stats = dict(structure_seeds=st.structure_seeds, tests=st.tests, starts=st.starts, walks=list(st.walks),
+                     spots=st.spots, draws=st.draws, pieces=st.pieces, expanded_seeds=st.expanded_seeds,
+                     accepted=st.accepted, funnel=list(st.funnel), tiers=list(st.tiers)[:7],
+                     samplers_built=list(st.samplers_built), device_ms=st.device_ms, stage_ms=list(st.stage_ms),
+                     overflow=rc == EOVERFLOW, find_count=fsink.count)
+        return spots, finds, stats
+
+    def room_floors(self, carvers):
+        """Full layout walks of the given carver seeds: arrays (status, room_floor, pieces)."""
+        c = np.ascontiguousarray(carvers, dtype=np.uint64)
+        n = len(c)
+        st = np.zeros(n, np.int32)
+        fl = np.zeros(n, np.int32)
+        pc = np.zeros(n, np.int32)
+        p32 = ctypes.POINTER(ctypes.c_int32)
+        rc = self.L.whf_room_floors(self.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n,
+                                    st.ctypes.data_as(p32), fl.ctypes.data_as(p32), pc.ctypes.data_as(p32))
+        if rc:
+            raise WhfError(rc, "whf_room_floors")
+        return st, fl, pc
 
 
 def fp32_bound():

@@ -118,6 +118,52 @@ uint64_t wo_perlin_fingerprint(const wo_perlin *p);
 /* Fingerprint of (noise id, sub, slot) for a world seed; 0 when the slot is empty. */
 uint64_t wo_sampler_fingerprint(uint64_t world_seed, int noise_id, int sub, int slot);
 
+/* ---- Deep Dive mineshaft stage (whf_oracle_ms.c) ---- */
+enum { WO_MS_ROOM = 0, WO_MS_CORRIDOR = 1, WO_MS_CROSSING = 2, WO_MS_STAIRS = 3 };
+#define WO_MS_MAXP 512   /* piece cap of the reference generator (mineshaft_tree.h:11-13) */
+
+typedef struct {
+    int32_t x0, y0, z0, x1, y1, z1;
+    int8_t type, facing, depth, rails;   /* facing: 0 N, 1 S, 2 W, 3 E, -1 for the room */
+    int16_t parent;                      /* index of the piece whose placement tried this one; -1 for the room */
+} wo_ms_piece;
+
+typedef struct {
+    int status;        /* 0 complete; 1 a piece went below the abort cut; 2 piece cap or int8 box range exceeded */
+    int pieces, span_y, min_y, max_y, room_floor;   /* min_y/max_y before the move; room_floor after it */
+    uint64_t draws;          /* LCG steps taken, the two start-test draws included */
+    uint64_t overlap_tests;  /* box-collision queries */
+} wo_ms_result;
+
+uint64_t wo_ms_carver_seed(uint64_t s48, int32_t cx, int32_t cz);   /* room_core.h:55-58 */
+double   wo_ms_start_double(uint64_t carver);                        /* new Random(X).nextDouble() */
+int      wo_ms_is_start(uint64_t carver);                            /* mstree.cpp:10-15 */
+/* Layout of the mineshaft started by `carver` in chunk (cx, cz) (mineshaft_tree.h:529-565). Boxes are
+ * absolute block coordinates after the move below sea level (status 0). Pieces stop at `abort_below`
+ * (INT_MIN: never), as the reference's abortBelow cut. Returns the status. */
+int wo_ms_layout(uint64_t carver, int32_t cx, int32_t cz, int abort_below, wo_ms_piece *pieces, int cap,
+                 wo_ms_result *out);
+int wo_ms_room_floor(uint64_t carver, int abort_below, wo_ms_result *out);
+
+typedef struct {
+    uint64_t structure_seeds, tests, starts, survivors;
+    uint64_t status[3];      /* per start: complete, cut below the abort line, cap/range */
+    uint64_t draws, pieces;  /* summed over starts */
+} wo_dd_stats;
+
+typedef struct {
+    uint64_t s48;
+    int32_t rx, rz, cx, cz;
+    uint64_t carver;
+    int32_t room_floor, pieces;
+} wo_dd_hit;
+
+/* Forward sweep: S = (s_first + i) mod 2^48 for i < count, hut regions [rx0,rx1) x [rz0,rz1); a hit is a
+ * mineshaft start in the hut chunk whose room floor is <= floor_max. */
+int wo_deepdive_sweep(uint64_t s_first, uint64_t count, int rx0, int rx1, int rz0, int rz1, int floor_max,
+                      int abort_below, int threads, wo_dd_stats *out, wo_dd_hit *hits, int64_t cap,
+                      int64_t *n_hits);
+
 /* Heap helpers for bindings: a phase-1 noise set or a single noise for a world seed (free with wo_probe_free). */
 void  *wo_probe_alloc(uint64_t world_seed, int vflags);
 void  *wo_noise_alloc(uint64_t world_seed, int noise_id);

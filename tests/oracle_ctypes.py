@@ -30,6 +30,26 @@ class Hit(ctypes.Structure):
                 ("cx", ctypes.c_int32), ("cz", ctypes.c_int32)]
 
 
+class MsPiece(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("x0", "y0", "z0", "x1", "y1", "z1")] + \
+               [(n, ctypes.c_int8) for n in ("type", "facing", "depth", "rails")] + [("parent", ctypes.c_int16)]
+
+
+class MsResult(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("status", "pieces", "span_y", "min_y", "max_y", "room_floor")] + \
+               [("draws", ctypes.c_uint64), ("overlap_tests", ctypes.c_uint64)]
+
+
+class DdStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("structure_seeds", "tests", "starts", "survivors")] + \
+               [("status", ctypes.c_uint64 * 3), ("draws", ctypes.c_uint64), ("pieces", ctypes.c_uint64)]
+
+
+class DdHit(ctypes.Structure):
+    _fields_ = [("s48", ctypes.c_uint64)] + [(n, ctypes.c_int32) for n in ("rx", "rz", "cx", "cz")] + \
+               [("carver", ctypes.c_uint64), ("room_floor", ctypes.c_int32), ("pieces", ctypes.c_int32)]
+
+
 class XR(ctypes.Structure):
     _fields_ = [("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64)]
 
@@ -85,6 +105,20 @@ def lib():
         L.wo_erosion_tiered.restype = i32
         L.wo_erosion_tiered.argtypes = [ctypes.c_void_p, dbl, dbl, ctypes.POINTER(dbl), ctypes.POINTER(i32)]
         L.wo_noise_perm.argtypes = [ctypes.c_void_p, i32, i32, ctypes.c_char_p]
+        L.wo_ms_carver_seed.restype = u64
+        L.wo_ms_carver_seed.argtypes = [u64, ctypes.c_int32, ctypes.c_int32]
+        L.wo_ms_start_double.restype = dbl
+        L.wo_ms_start_double.argtypes = [u64]
+        L.wo_ms_is_start.restype = i32
+        L.wo_ms_is_start.argtypes = [u64]
+        L.wo_ms_layout.restype = i32
+        L.wo_ms_layout.argtypes = [u64, ctypes.c_int32, ctypes.c_int32, i32, ctypes.POINTER(MsPiece), i32,
+                                   ctypes.POINTER(MsResult)]
+        L.wo_ms_room_floor.restype = i32
+        L.wo_ms_room_floor.argtypes = [u64, i32, ctypes.POINTER(MsResult)]
+        L.wo_deepdive_sweep.restype = i32
+        L.wo_deepdive_sweep.argtypes = [u64, u64, i32, i32, i32, i32, i32, i32, i32, ctypes.POINTER(DdStats),
+                                        ctypes.POINTER(DdHit), i64, ctypes.POINTER(i64)]
         _lib = L
     return _lib
 
@@ -183,3 +217,54 @@ def md5(data):
     out = ctypes.create_string_buffer(16)
     lib().wo_md5(data, len(data), out)
     return out.raw.hex()
+
+
+# ---- Deep Dive (whf_oracle_ms.c) ----
+NO_CUT = -(1 << 31)
+MS_KIND = {0: "R", 1: "C", 3: "S"}
+
+
+def carver_seed(s48, cx, cz):
+    return lib().wo_ms_carver_seed(u64(s48), cx, cz)
+
+
+def is_start(carver):
+    return bool(lib().wo_ms_is_start(u64(carver)))
+
+
+def start_double(carver):
+    return lib().wo_ms_start_double(u64(carver))
+
+
+def ms_layout(carver, cx, cz, cut=NO_CUT):
+    """(status, room_floor, pieces as mstree lines, result struct) for the mineshaft started by `carver` in
+    chunk (cx, cz); the lines follow mstree's emit format (kind x0 y0 z0 x1 y1 z1 facing parent rails)."""
+    buf = (MsPiece * 512)()
+    r = MsResult()
+    st = lib().wo_ms_layout(u64(carver), cx, cz, cut, buf, 512, ctypes.byref(r))
+    lines = []
+    for i in range(r.pieces):
+        p = buf[i]
+        kind = ("Y" if p.y1 - p.y0 + 1 > 3 else "X") if p.type == 2 else MS_KIND[p.type]
+        lines.append("%s %d %d %d %d %d %d %d %d %d" % (kind, p.x0, p.y0, p.z0, p.x1, p.y1, p.z1, p.facing,
+                                                        p.parent, p.rails))
+    return st, r.room_floor, lines, r
+
+
+def room_floor(carver, cut=NO_CUT):
+    r = MsResult()
+    st = lib().wo_ms_room_floor(u64(carver), cut, ctypes.byref(r))
+    return st, r.room_floor, r.pieces, r
+
+
+def deepdive_sweep(s_first, count, rect, floor_max=-55, cut=NO_CUT, threads=8, cap=1 << 16):
+    st = DdStats()
+    hits = (DdHit * cap)()
+    n = ctypes.c_int64(0)
+    rc = lib().wo_deepdive_sweep(u64(s_first), count, rect[0], rect[1], rect[2], rect[3], floor_max, cut, threads,
+                                 ctypes.byref(st), hits, cap, ctypes.byref(n))
+    assert rc == 0
+    spots = sorted((h.s48, h.rx, h.rz, h.cx, h.cz, h.carver, h.room_floor, h.pieces)
+                   for h in hits[:min(n.value, cap)])
+    return dict(structure_seeds=st.structure_seeds, tests=st.tests, starts=st.starts, survivors=st.survivors,
+                status=list(st.status), draws=st.draws, pieces=st.pieces, spots=spots)
