@@ -86,6 +86,171 @@ def cpu_baseline(n_seeds_max, seconds):
                 funnel=r["funnel"], tiers=r["tiers"], seeds=r["seeds"])
 
 
+DD_RECT = (-2, 2, -2, 2)
+DD_FLOOR = -55
+DD_TOTAL = 1 << 32   # structure seeds of the full Deep Dive workload
+
+
+def splitmix_first(stream):
+    """First output of sequential SplitMix64 seeded with `stream` (Vigna's splitmix64.c)."""
+    m = (1 << 64) - 1
+    z = (stream + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def deep_dive_cpu(s0, seconds):
+    exe = os.path.join(REPO, "oracle", "_ref", "mstree_cpu_bench")
+    if not os.path.exists(exe):
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    out = subprocess.run([exe, "--s-first", str(s0), "--count", str(1 << 26), "--rx0", str(DD_RECT[0]), "--rx1",
+                          str(DD_RECT[1]), "--rz0", str(DD_RECT[2]), "--rz1", str(DD_RECT[3]), "--floor", str(DD_FLOOR),
+                          "--threads", str(threads), "--time-limit", str(seconds)], check=True, capture_output=True,
+                         text=True).stdout
+    r = json.loads(out)
+    return dict(value=round(r["seeds_per_s"], 1), unit="structure seeds/s (start tests + room-floor walks)",
+                cores=threads, kind="reference",
+                sample="first %d structure seeds of the same stream, regions [-2,1]^2, the reference's mineshaft "
+                       "generator (mineshaft_tree_iter.h) with its abortBelow cut, time-bounded %.0f s"
+                       % (r["structure_seeds"], seconds), starts=r["starts"], spots=r["spots"])
+
+
+def deep_dive(whf, sweep_n, expand_n, cpu_seconds):
+    """Deep Dive: room-floor sweep over sweep_n structure seeds, and the 2^16 expansion of the spots of the first
+    expand_n; both from the first SplitMix64(3) output, hut regions [-2,1]^2, room floor <= -55, version 1.20."""
+    s0 = splitmix_first(3) & ((1 << 48) - 1)
+    with whf.Context(0) as ctx:
+        ctx.deepdive(s0, 1 << 20, DD_RECT, DD_FLOOR)   # warm-up
+        t0 = time.perf_counter()
+        spots, _, st = ctx.deepdive(s0, sweep_n, DD_RECT, DD_FLOOR, spot_capacity=1 << 21)
+        t_sweep = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        xspots, finds, xs = ctx.deepdive(s0, expand_n, DD_RECT, DD_FLOOR, expand=True, max_height=-50,
+                                         spot_capacity=1 << 21)
+        t_exp_total = time.perf_counter() - t0
+    with whf.Context(0, profile=True) as ctx:
+        _, _, ps = ctx.deepdive(s0, min(sweep_n, 1 << 26), DD_RECT, DD_FLOOR, spot_capacity=1 << 21)
+        _, _, px = ctx.deepdive(s0, expand_n, DD_RECT, DD_FLOOR, expand=True, max_height=-50, spot_capacity=1 << 21)
+    sweep_rate = sweep_n / t_sweep
+    # expansion rate from the profiled stage time (the sweep part of that call is excluded)
+    x_sec = px["stage_ms"][2] / 1e3
+    x_rate = xs["expanded_seeds"] / x_sec if x_sec > 0 else None
+    spots_per_seed = st["spots"] / sweep_n
+    total_s = DD_TOTAL / sweep_rate + (DD_TOTAL * spots_per_seed * 65536 / x_rate if x_rate else 0)
+    built = px["samplers_built"]
+    x_ops = sum(built) * OPS_PER_SAMPLER
+    return dict(
+        workload="Deep Dive: structure seeds S = first SplitMix64(3) output + i (mod 2^48), hut regions [-2,1]^2, "
+                 "mineshaft room floor <= -55, each spot expanded to its 2^16 world seeds with the phase-1 check "
+                 "(version 1.20, maxHeight -50)",
+        sweep=dict(structure_seeds=sweep_n, seconds=round(t_sweep, 4), structure_seeds_per_s=round(sweep_rate, 1),
+                   starts=st["starts"], walks=st["walks"], spots=st["spots"], lcg_draws=st["draws"],
+                   stage_ms=[round(v, 3) for v in ps["stage_ms"][:2]], stage_ms_structure_seeds=min(sweep_n, 1 << 26)),
+        expansion=dict(structure_seeds=expand_n, spots=xs["spots"], world_seeds=xs["expanded_seeds"],
+                       seconds_total=round(t_exp_total, 4), stage_seconds=round(x_sec, 4),
+                       world_seeds_per_s=round(x_rate, 1) if x_rate else None, finds=len(finds),
+                       funnel=xs["funnel"], tiers=xs["tiers"], samplers_built=built,
+                       roofline=dict(bound="valu", unit="Tops/s", peak=round(PEAK_VALU_OPS / 1e12, 2),
+                                     achieved=round(x_ops / x_sec / 1e12, 3) if x_sec > 0 else None,
+                                     frac=round(x_ops / x_sec / PEAK_VALU_OPS, 4) if x_sec > 0 else None,
+                                     work_unit="Perlin sampler built (all sets) = 271 Xoroshiro128++ steps x 21 "
+                                               "int32 lane-ops; evaluations not counted")),
+        full_workload=dict(structure_seeds=DD_TOTAL, projected_seconds=round(total_s, 1),
+                           structure_seeds_per_s=round(DD_TOTAL / total_s, 1) if total_s else None),
+        cpu_baseline=None if cpu_seconds <= 0 else deep_dive_cpu(s0, cpu_seconds),
+    )
+
+
+def serving_stream(n, seed):
+    """The serving workload: n requests, sizes round(2^U(0,20)) seeds from SplitMix64(request index), radius
+    uniform over {512, 1024, 2048, 4096} blocks -> regions +-(r // 512 + 1), threshold uniform over -50..-54 with
+    'or lower' / 'exactly' at 50/50 (-54 is always 'or lower'), version 1.20. Phase 1 runs at maxHeight -50 (the
+    threshold is the finisher's)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(2.0 ** rng.uniform(0, 20, n)).astype(np.int64)
+    radius = rng.choice([512, 1024, 2048, 4096], n)
+    thr = rng.integers(-54, -49, n)
+    exactly = (rng.random(n) < 0.5) & (thr != -54)
+    reqs = []
+    for i in range(n):
+        r = int(radius[i]) // 512 + 1
+        reqs.append(dict(rect=(-r, r, -r, r), splitmix=(i, int(sizes[i])), threshold=int(thr[i]),
+                         exactly=bool(exactly[i]), checks=int(sizes[i]) * (2 * r) ** 2))
+    return reqs
+
+
+def serve(whf, reqs, rate, slots, chunk_checks):
+    """Submits reqs at Poisson arrivals of `rate` per second (rate None: all at once); returns per-request events
+    and the wall time from the first submit to the last completion."""
+    import threading
+    import numpy as np
+    vflags = whf.version_flags("1.20")
+    rng = np.random.default_rng(12345)
+    gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
+    events = {}
+    with whf.Server(0, slots=slots, chunk_checks=chunk_checks) as srv:
+        lock = threading.Lock()
+        ids = []
+
+        def feeder():
+            t0 = time.perf_counter()
+            t = 0.0
+            for q, g in zip(reqs, gaps):
+                t += g
+                d = t0 + t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                with lock:
+                    ids.append(srv.submit(q["rect"], capacity=1024, splitmix=q["splitmix"], max_height=-50,
+                                          vflags=vflags))
+
+        th = threading.Thread(target=feeder)
+        t0 = time.perf_counter()
+        th.start()
+        while len(events) < len(reqs):
+            with lock:
+                evs = srv.poll(timeout_ms=0)
+            for e in evs:
+                events[e["id"]] = e
+            if not evs:
+                time.sleep(0.0005)
+        wall = time.perf_counter() - t0
+        th.join()
+    return [events[i] for i in ids], wall
+
+
+def serving(whf, n, load, slots, chunk_checks):
+    import numpy as np
+    reqs = serving_stream(n, 4)
+    mean_checks = float(np.mean([q["checks"] for q in reqs]))
+    # capacity: a prefix of the same stream submitted at once (saturation)
+    sat = reqs[: max(50, n // 20)]
+    ev, wall = serve(whf, sat, None, slots, chunk_checks)
+    capacity = sum(e["stats"]["checks"] for e in ev) / wall
+    rate = load * capacity / mean_checks
+    ev, wall = serve(whf, reqs, rate, slots, chunk_checks)
+    lat = np.array([e["done_s"] - e["submit_s"] for e in ev]) * 1e3
+    big = np.array([q["splitmix"][1] >= 1 << 19 for q in reqs])
+    checks = sum(e["stats"]["checks"] for e in ev)
+    assert all(e["status"] == whf.OK for e in ev)
+    return dict(
+        workload="serving: %d Seed Check requests, Poisson arrivals at %.0f%% of measured capacity, sizes "
+                 "round(2^U(0,20)) SplitMix64(request index) seeds, radius {512,1024,2048,4096} blocks, "
+                 "threshold -50..-54 or-lower/exactly, version 1.20" % (n, 100 * load),
+        slots=slots, chunk_checks=chunk_checks, capacity_checks_per_s=round(capacity, 1),
+        mean_request_checks=round(mean_checks, 1), arrival_rate_per_s=round(rate, 2),
+        offered_checks_per_s=round(rate * mean_checks, 1), throughput_checks_per_s=round(checks / wall, 1),
+        wall_s=round(wall, 3), requests=n, finds=sum(len(e["hits"]) for e in ev),
+        latency_ms=dict(p50=round(float(np.percentile(lat, 50)), 3), p99=round(float(np.percentile(lat, 99)), 3),
+                        max=round(float(lat.max()), 3),
+                        p99_large=round(float(np.percentile(lat[big], 99)), 3) if big.any() else None,
+                        large_requests=int(big.sum()), large="requests of >= 2^19 seeds"),
+    )
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +259,13 @@ def main():
     ap.add_argument("--seeds", type=int, default=1 << 26)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dd-seeds", type=int, default=1 << 28, help="structure seeds of the Deep Dive sweep")
+    ap.add_argument("--dd-expand", type=int, default=1 << 22, help="structure seeds whose spots are expanded")
+    ap.add_argument("--no-deep-dive", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-load", type=float, default=0.8)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-chunk", type=int, default=1 << 24)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would scan its own seed "
@@ -149,6 +321,10 @@ def main():
                     work_unit=WORK_UNIT.get(dominant))
 
     cpu = None if args.no_cpu else cpu_baseline(min(args.seeds, 1 << 20), args.cpu_seconds)
+    srv = None if args.serve_requests <= 0 else serving(whf, args.serve_requests, args.serve_load, args.serve_slots,
+                                                        args.serve_chunk)
+    dd = None if args.no_deep_dive else deep_dive(whf, args.dd_seeds, args.dd_expand,
+                                                  0 if args.no_cpu else min(10.0, args.cpu_seconds))
     line = {
         "metric": METRIC, "value": round(checks / secs, 1), "unit": "checks/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(secs * 1e3, 3), "higher_is_better": True, "scaling": "weak",
@@ -170,6 +346,9 @@ def main():
         "funnel": last["funnel"], "tiers": last["tiers"], "accepted": last["accepted"],
         "kernel_ms": {k: round(v, 3) for k, v in fam.items()}, "rooflines": rooflines, "samplers_built": built,
         "device_ms": round(last["device_ms"], 3),
+        "latency_ms": srv["latency_ms"] if srv else None,
+        "serving": srv,
+        "deep_dive": dd,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -869,6 +869,14 @@ int ctx_alloc(whf_ctx *c) {
 
 }  // namespace
 
+__attribute__((visibility("hidden"))) uint64_t whf_ctx_device_bytes(const whf_ctx *c) {
+    const uint64_t S = kMaxSeeds, B = kMaxBatchSeeds;
+    uint64_t b = B * 8 + 4 * B * 4 + 2 * S * 4 + 2ull * kMaxItems * sizeof(uint4) + (uint64_t)kQxCap * sizeof(uint4) +
+                 (uint64_t)kHitCap * sizeof(DevHit) + kLayouts * (sizeof(GateTables) + kBdPerLayout * sizeof(BuildDesc));
+    for (int s = 0; s < SET_COUNT; s++) b += S * kSetSize[s] * (256 + sizeof(SPar));
+    return b + (c ? c->dd_bytes : 0);
+}
+
 namespace whf {
 
 const BuildDesc *bd_of(whf_ctx *c, int layout, int set) {

@@ -1,0 +1,229 @@
+// whf_server.cpp — continuous batching of Seed Check requests over several device slots (C ABI in whf.h).
+//
+// A slot is a whf_ctx (its own HIP stream and working buffers) driven by its own host thread. Requests are cut
+// into chunks of seeds (about chunk_checks (seed, region) checks each); an idle slot takes the next chunk of the
+// request with the least work left (the oldest on ties), runs it with whf_scan and folds the chunk's hits and
+// counters into the request. A request admitted while others run therefore starts when the first slot frees up, and the chunks of
+// several requests run at once on the device. Each chunk is a plain whf_scan over a sub-range of the request's
+// seeds, and counters add over chunks, so a request's hits and counters are those of whf_scan on it alone.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/whf.h"
+
+// Device bytes one context holds (whf_scan.hip).
+__attribute__((visibility("hidden"))) uint64_t whf_ctx_device_bytes(const whf_ctx *c);
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Job {
+    uint64_t id = 0;
+    whf_request req{};
+    whf_sink *sink = nullptr;
+    uint64_t per_chunk = 1, next = 0, done_seeds = 0;   // seeds
+    uint64_t regions = 1;
+    int in_flight = 0;
+    int status = WHF_OK;
+    whf_stats stats{};
+    std::vector<whf_hit> hits;
+    double submit_s = 0, start_s = -1;
+    uint64_t remaining_checks() const { return (req.n_seeds - next) * regions; }
+};
+
+void add_stats(whf_stats &a, const whf_stats &b) {
+    a.seeds += b.seeds;
+    a.checks += b.checks;
+    a.accepted += b.accepted;
+    a.reached_ladder += b.reached_ladder;
+    for (int k = 0; k < WHF_KIND_N; k++) a.funnel[k] += b.funnel[k];
+    for (int k = 0; k < 8; k++) a.tiers[k] += b.tiers[k];
+    for (int k = 0; k < 3; k++) a.samplers_built[k] += b.samplers_built[k];
+    a.device_ms += b.device_ms;
+    for (int k = 0; k < 8; k++) a.kernel_ms[k] += b.kernel_ms[k];
+    a.fp64_rechecks += b.fp64_rechecks;
+}
+
+bool hit_less(const whf_hit &x, const whf_hit &y) {
+    if (x.seed != y.seed) return x.seed < y.seed;
+    if (x.rx != y.rx) return x.rx < y.rx;
+    return x.rz < y.rz;
+}
+
+constexpr uint64_t kChunkHits = 1u << 20;
+
+}  // namespace
+
+struct whf_server {
+    std::vector<whf_ctx *> ctx;
+    std::vector<std::thread> threads;
+    std::mutex mu;
+    std::condition_variable work_cv, event_cv;
+    std::deque<std::shared_ptr<Job>> jobs;   // admitted requests with chunks left, in arrival order
+    std::deque<whf_event> events;
+    uint64_t chunk_checks = 1ull << 24;
+    uint64_t next_id = 1;
+    int in_flight_jobs = 0;
+    bool stop = false;
+    Clock::time_point epoch = Clock::now();
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - epoch).count(); }
+
+    void finish(const std::shared_ptr<Job> &j) {   // mu held
+        std::sort(j->hits.begin(), j->hits.end(), hit_less);
+        int status = j->status;
+        if (j->sink) {
+            j->sink->count = j->hits.size();
+            const uint64_t m = std::min<uint64_t>(j->sink->capacity, j->hits.size());
+            if (m && j->sink->hits) std::memcpy(j->sink->hits, j->hits.data(), m * sizeof(whf_hit));
+            if (j->hits.size() > j->sink->capacity && status == WHF_OK) status = WHF_EOVERFLOW;
+        }
+        whf_event e{};
+        e.id = j->id;
+        e.status = status;
+        e.stats = j->stats;
+        e.stats.seeds = j->req.n_seeds;
+        e.submit_s = j->submit_s;
+        e.start_s = j->start_s;
+        e.done_s = now();
+        events.push_back(e);
+        in_flight_jobs--;
+        event_cv.notify_all();
+    }
+
+    void worker(int slot) {
+        whf_ctx *c = ctx[slot];
+        std::vector<whf_hit> buf(kChunkHits);
+        for (;;) {
+            std::shared_ptr<Job> j;
+            uint64_t s0 = 0, n = 0;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                work_cv.wait(lk, [&] { return stop || !jobs.empty(); });
+                if (jobs.empty()) return;   // stop with nothing queued
+                // shortest remaining work first (ties: the older request), so small requests overtake big ones
+                auto best = jobs.begin();
+                for (auto it = jobs.begin(); it != jobs.end(); ++it)
+                    if ((*it)->remaining_checks() < (*best)->remaining_checks()) best = it;
+                j = *best;
+                s0 = j->next;
+                n = std::min(j->per_chunk, j->req.n_seeds - s0);
+                j->next += n;
+                j->in_flight++;
+                if (j->start_s < 0) j->start_s = now();
+                if (j->next >= j->req.n_seeds) jobs.erase(best);
+            }
+            whf_request sub = j->req;
+            sub.n_seeds = n;
+            if (sub.seed_mode == WHF_SEEDS_LIST) sub.seeds = j->req.seeds + s0;
+            else sub.offset = j->req.offset + s0;
+            whf_sink sink{buf.data(), kChunkHits, 0};
+            whf_stats st{};
+            const int rc = n ? whf_scan(c, &sub, &sink, &st) : WHF_OK;
+            std::lock_guard<std::mutex> lk(mu);
+            if (rc == WHF_OK || rc == WHF_EOVERFLOW) {
+                add_stats(j->stats, st);
+                const uint64_t m = std::min<uint64_t>(sink.count, kChunkHits);
+                j->hits.insert(j->hits.end(), buf.begin(), buf.begin() + m);
+                if (rc == WHF_EOVERFLOW && j->status == WHF_OK) j->status = WHF_EOVERFLOW;
+            } else if (j->status == WHF_OK || j->status == WHF_EOVERFLOW) {
+                j->status = rc;
+            }
+            j->done_seeds += n;
+            j->in_flight--;
+            if (j->done_seeds >= j->req.n_seeds && j->in_flight == 0) finish(j);
+        }
+    }
+};
+
+extern "C" {
+
+int whf_server_open(const whf_server_config *cfg, whf_server **out) {
+    if (!out) return WHF_EINVAL;
+    *out = nullptr;
+    int slots = cfg && cfg->slots > 0 ? cfg->slots : 4;
+    if (slots > 16) return WHF_EINVAL;
+    whf_server *s = new (std::nothrow) whf_server();
+    if (!s) return WHF_ENOMEM;
+    if (cfg && cfg->chunk_checks) s->chunk_checks = cfg->chunk_checks;
+    const uint64_t budget = cfg ? cfg->hbm_budget : 0;
+    whf_config cc{cfg ? cfg->device : 0, 0, 0};
+    for (int i = 0; i < slots; i++) {
+        whf_ctx *c = nullptr;
+        int rc = whf_open(&cc, &c);
+        if (rc) {
+            for (whf_ctx *x : s->ctx) whf_close(x);
+            delete s;
+            return rc;
+        }
+        s->ctx.push_back(c);
+        // admission by memory: stop adding slots once the next one would exceed the budget
+        if (budget && (uint64_t)(i + 2) * whf_ctx_device_bytes(c) > budget) break;
+    }
+    for (int i = 0; i < (int)s->ctx.size(); i++) s->threads.emplace_back(&whf_server::worker, s, i);
+    *out = s;
+    return WHF_OK;
+}
+
+void whf_server_close(whf_server *s) {
+    if (!s) return;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->work_cv.notify_all();
+    for (auto &t : s->threads) t.join();
+    for (whf_ctx *c : s->ctx) whf_close(c);
+    delete s;
+}
+
+int whf_submit(whf_server *s, const whf_request *req, whf_sink *sink, uint64_t *id) {
+    if (!s || !req) return WHF_EINVAL;
+    if (req->rx1 <= req->rx0 || req->rz1 <= req->rz0) return WHF_EINVAL;
+    if (req->seed_mode == WHF_SEEDS_LIST && req->n_seeds && !req->seeds) return WHF_EINVAL;
+    if (sink && sink->capacity && !sink->hits) return WHF_EINVAL;
+    auto j = std::make_shared<Job>();
+    j->req = *req;
+    j->sink = sink;
+    const uint64_t R = (uint64_t)(req->rx1 - req->rx0) * (uint64_t)(req->rz1 - req->rz0);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return WHF_EINVAL;
+    j->id = s->next_id++;
+    j->regions = R;
+    j->per_chunk = std::max<uint64_t>(1, s->chunk_checks / R);
+    j->submit_s = s->now();
+    if (id) *id = j->id;
+    s->in_flight_jobs++;
+    if (req->n_seeds == 0) {
+        s->finish(j);
+    } else {
+        s->jobs.push_back(j);
+        s->work_cv.notify_all();
+    }
+    return WHF_OK;
+}
+
+int whf_poll(whf_server *s, whf_event *ev, int max_events, int timeout_ms) {
+    if (!s || (max_events > 0 && !ev) || max_events < 0) return WHF_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto ready = [&] { return !s->events.empty(); };
+    if (timeout_ms < 0) s->event_cv.wait(lk, ready);
+    else s->event_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+    int n = 0;
+    while (n < max_events && !s->events.empty()) {
+        ev[n++] = s->events.front();
+        s->events.pop_front();
+    }
+    return n;
+}
+
+}  // extern "C"

@@ -21,7 +21,8 @@ CFG_PROFILE = 1
 
 # Exported symbols of include/whf.h, checked by the tests.
 SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags", "whf_strerror",
-           "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_room_floors")
+           "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_room_floors",
+           "whf_server_open", "whf_server_close", "whf_submit", "whf_poll")
 HIT_DEEPDIVE = 1
 
 
@@ -81,6 +82,16 @@ class DdStats(ctypes.Structure):
                 ("stage_ms", ctypes.c_double * 3)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("chunk_checks", ctypes.c_uint64),
+                ("hbm_budget", ctypes.c_uint64), ("flags", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
+class Event(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int), ("reserved", ctypes.c_int), ("stats", Stats),
+                ("submit_s", ctypes.c_double), ("start_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
 KERNEL_FAMILIES = ("seeds", "build_erosion", "erosion_gate", "compact", "build_climate", "gate2", "build_ladder",
                    "gate3")
 
@@ -115,6 +126,15 @@ def load():
                                       ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.whf_fp32_bound.restype = ctypes.c_double
         L.whf_fp32_bound.argtypes = []
+        L.whf_server_open.restype = ctypes.c_int
+        L.whf_server_open.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(ctypes.c_void_p)]
+        L.whf_server_close.restype = None
+        L.whf_server_close.argtypes = [ctypes.c_void_p]
+        L.whf_submit.restype = ctypes.c_int
+        L.whf_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(Sink),
+                                 ctypes.POINTER(ctypes.c_uint64)]
+        L.whf_poll.restype = ctypes.c_int
+        L.whf_poll.argtypes = [ctypes.c_void_p, ctypes.POINTER(Event), ctypes.c_int, ctypes.c_int]
         L.whf_deepdive.restype = ctypes.c_int
         L.whf_deepdive.argtypes = [ctypes.c_void_p, ctypes.POINTER(DdRequest), ctypes.POINTER(DdSink),
                                    ctypes.POINTER(Sink), ctypes.POINTER(DdStats)]
@@ -265,6 +285,87 @@ class Context:
         if rc:
             raise WhfError(rc, "whf_room_floors")
         return st, fl, pc
+
+
+def make_request(rect, seeds=None, splitmix=None, counter=None, max_height=-50, vflags=0, salt=0,
+                 precision=PREC_FP64):
+    """A whf_request and the array it points to (keep both alive while the request runs)."""
+    req = Request()
+    req.precision = precision
+    keep = None
+    if seeds is not None:
+        keep = np.ascontiguousarray(seeds, dtype=np.int64)
+        req.seed_mode = SEEDS_LIST
+        req.seeds = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        req.n_seeds = len(keep)
+    else:
+        gen = splitmix if splitmix is not None else counter
+        req.seed_mode = SEEDS_SPLITMIX if splitmix is not None else SEEDS_COUNTER
+        req.stream = gen[0] & 0xFFFFFFFFFFFFFFFF
+        req.n_seeds = gen[1]
+        req.offset = gen[2] if len(gen) > 2 else 0
+    req.rx0, req.rx1, req.rz0, req.rz1 = rect
+    req.max_height = max_height
+    req.vflags = vflags
+    req.salt = salt
+    return req, keep
+
+
+def stats_dict(st):
+    return dict(seeds=st.seeds, checks=st.checks, accepted=st.accepted, reached_ladder=st.reached_ladder,
+                funnel=list(st.funnel), tiers=list(st.tiers)[:7], samplers_built=list(st.samplers_built),
+                device_ms=st.device_ms, fp64_rechecks=st.fp64_rechecks)
+
+
+class Server:
+    """Continuous batching (whf_server_open / whf_submit / whf_poll / whf_server_close)."""
+
+    def __init__(self, device=0, slots=4, chunk_checks=0, hbm_budget=0):
+        self.L = load()
+        self.h = ctypes.c_void_p()
+        cfg = ServerConfig(device, slots, chunk_checks, hbm_budget, 0, 0)
+        rc = self.L.whf_server_open(ctypes.byref(cfg), ctypes.byref(self.h))
+        if rc:
+            raise WhfError(rc, "whf_server_open")
+        self.pending = {}
+
+    def submit(self, rect, capacity=4096, **kw):
+        """Queues a Seed Check request (arguments as Context.scan); returns its id."""
+        req, keep = make_request(rect, **kw)
+        buf = (Hit * capacity)()
+        sink = Sink(buf, capacity, 0)
+        rid = ctypes.c_uint64(0)
+        rc = self.L.whf_submit(self.h, ctypes.byref(req), ctypes.byref(sink), ctypes.byref(rid))
+        if rc:
+            raise WhfError(rc, "whf_submit")
+        self.pending[rid.value] = (req, keep, buf, sink)
+        return rid.value
+
+    def poll(self, timeout_ms=-1, max_events=256):
+        """Completed requests: list of dicts (id, status, hits, stats, submit_s, start_s, done_s)."""
+        ev = (Event * max_events)()
+        n = self.L.whf_poll(self.h, ev, max_events, timeout_ms)
+        if n < 0:
+            raise WhfError(n, "whf_poll")
+        out = []
+        for e in ev[:n]:
+            req, keep, buf, sink = self.pending.pop(e.id)
+            m = min(sink.count, sink.capacity)
+            hits = sorted((buf[i].seed, buf[i].rx, buf[i].rz, buf[i].cx, buf[i].cz) for i in range(m))
+            out.append(dict(id=e.id, status=e.status, hits=hits, stats=stats_dict(e.stats), submit_s=e.submit_s,
+                            start_s=e.start_s, done_s=e.done_s))
+        return out
+
+    def close(self):
+        if self.h:
+            self.L.whf_server_close(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def fp32_bound():

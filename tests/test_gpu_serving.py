@@ -1,0 +1,105 @@
+"""Continuous batching (whf_server_* through the C ABI): every request's hits and counters in the mix equal those of
+the same request scanned alone with whf_scan, whatever the chunking, arrival order or overlap with other requests.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import oracle_ctypes as O
+from product import whf
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("checks", "accepted", "reached_ladder", "funnel", "tiers")
+
+
+def request_mix(n, seed, max_seeds_log2=14):
+    """Requests of log-uniform size, mixed rects, versions, seed sources and precisions."""
+    rng = np.random.default_rng(seed)
+    versions = ["1.18", "1.20", "1.21", "26.2"]
+    out = []
+    for i in range(n):
+        size = int(round(2 ** rng.uniform(0, max_seeds_log2)))
+        r = int(rng.choice([2, 3, 5]))
+        rect = (-r, r, -r, r)
+        kw = dict(vflags=whf.version_flags(versions[i % 4]), max_height=-50,
+                  precision=whf.PREC_FP32_GATE if i % 5 == 0 else whf.PREC_FP64)
+        mode = i % 3
+        if mode == 0:
+            kw["splitmix"] = (1000 + i, size)
+        elif mode == 1:
+            kw["counter"] = (77 * i, size, 5)
+        else:
+            kw["seeds"] = O.seeds_splitmix(5000 + i, size)
+        out.append((rect, kw))
+    return out
+
+
+def alone(reqs):
+    res = []
+    with whf.Context(0) as ctx:
+        for rect, kw in reqs:
+            hits, st = ctx.scan(rect, capacity=1 << 16, **kw)
+            res.append((hits, st))
+    return res
+
+
+def check_same(ev, ref):
+    hits, st = ref
+    assert ev["status"] == whf.OK
+    assert ev["hits"] == hits
+    for k in KEYS:
+        assert ev["stats"][k] == st[k], k
+    assert ev["submit_s"] <= ev["start_s"] <= ev["done_s"]
+
+
+@pytest.mark.timeout(600)
+def test_mix_equals_alone_all_at_once():
+    reqs = request_mix(48, 1)
+    print("scanning %d requests alone" % len(reqs), flush=True)
+    ref = alone(reqs)
+    print("serving them together", flush=True)
+    with whf.Server(0, slots=4, chunk_checks=1 << 20) as srv:
+        ids = [srv.submit(rect, capacity=1 << 16, **kw) for rect, kw in reqs]
+        got = {}
+        while len(got) < len(ids):
+            for e in srv.poll(timeout_ms=60000):
+                got[e["id"]] = e
+    for rid, r in zip(ids, ref):
+        check_same(got[rid], r)
+
+
+@pytest.mark.timeout(600)
+def test_mix_equals_alone_staggered_arrivals():
+    """Requests arrive while others run (one big request first, then a stream of small ones)."""
+    reqs = [((-9, 9, -9, 9), dict(splitmix=(31337, 1 << 20), max_height=-50))] + request_mix(24, 2, 10)
+    ref = alone(reqs)
+    with whf.Server(0, slots=3, chunk_checks=1 << 22) as srv:
+        ids = []
+        for rect, kw in reqs:
+            ids.append(srv.submit(rect, capacity=1 << 16, **kw))
+            time.sleep(0.002)
+        got = {}
+        while len(got) < len(ids):
+            for e in srv.poll(timeout_ms=60000):
+                got[e["id"]] = e
+    for rid, r in zip(ids, ref):
+        check_same(got[rid], r)
+    # the small requests did not wait for the big one to finish
+    big_done = got[ids[0]]["done_s"]
+    assert sum(got[i]["done_s"] < big_done for i in ids[1:]) > 0
+
+
+def test_empty_request_and_sink_overflow():
+    with whf.Server(0, slots=2) as srv:
+        a = srv.submit((0, 1, 0, 1), splitmix=(1, 0))
+        b = srv.submit((-79, 79, -79, 79), capacity=0, seeds=np.array([1473884015873122059], dtype=np.int64),
+                       max_height=-50)
+        got = {}
+        while len(got) < 2:
+            for e in srv.poll(timeout_ms=60000):
+                got[e["id"]] = e
+    assert got[a]["status"] == whf.OK and got[a]["stats"]["checks"] == 0
+    assert got[b]["status"] == whf.EOVERFLOW and got[b]["stats"]["accepted"] >= 1
+    assert got[b]["stats"]["checks"] == 158 * 158

@@ -187,3 +187,20 @@ def test_sweep_counts_and_spots():
     assert out["starts"] == starts
     assert out["spots"] == sorted(want)
     assert 0.0025 < starts / out["tests"] < 0.0055
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(HERE, "..", "oracle", "_ref", "mstree_cpu_bench")),
+                    reason="reference Deep Dive harness not built (needs the reference sources)")
+def test_reference_generator_sweep_agrees():
+    """The reference's own generator (mineshaft_tree_iter.h with abortBelow, in oracle/_ref/mstree_cpu_bench) and
+    the oracle count the same starts, walk outcomes and spots."""
+    import subprocess
+    exe = os.path.join(HERE, "..", "oracle", "_ref", "mstree_cpu_bench")
+    s0 = O.splitmix(3, 0) & M48
+    for floor_max, rect in ((-55, (-2, 2, -2, 2)), (-50, (-3, 1, 0, 2))):
+        out = json.loads(subprocess.run([exe, "--s-first", str(s0), "--count", "16384", "--rx0", str(rect[0]), "--rx1",
+                                         str(rect[1]), "--rz0", str(rect[2]), "--rz1", str(rect[3]), "--floor",
+                                         str(floor_max), "--threads", "2"], check=True, capture_output=True,
+                                        text=True).stdout)
+        ref = O.deepdive_sweep(s0, 16384, rect, floor_max=floor_max, cut=-12 - floor_max, threads=4)
+        assert (out["starts"], out["walks"], out["spots"]) == (ref["starts"], ref["status"], len(ref["spots"]))
