@@ -10,9 +10,10 @@
 //                  below sea level, so such a start can no longer qualify.
 // Expansion, 16 spots (2^20 world seeds u << 48 | S) per batch:
 //   k_dd_xseeds    the batch's world seeds.
-//   k_dd_erosion   the tiered erosion gate (noise.c:432-459) at the hut, one lane per world seed, building each
-//                  erosion octave sampler in LDS only when the seed is still alive at that evaluation (the
-//                  reference's staged kernels stageA0a..A2b, hut_lookup_v4_source.cu:523-619, do the same).
+//   k_dd_erosion   the tiered erosion gate (noise.c:432-459) at the hut, one lane per world seed at a time,
+//                  building each erosion octave sampler in LDS only when the seed is still alive at that
+//                  evaluation (the reference's staged kernels stageA0a..A2b, hut_lookup_v4_source.cu:523-619,
+//                  also build octave by octave).
 //   then gates 2 and 3 of the Seed Check (whf_scan.hip) on the erosion survivors -> finds.
 #include <climits>
 #include <cstring>
@@ -140,8 +141,11 @@ struct LanePerm {
     }
 };
 
-constexpr int kXPass = 512;   // world seeds per pass of a workgroup (one wave)
-
+// One lane carries one world seed through the tiered gate: at level k it builds erosion sampler k (set-E order A0 B0
+// A1 B1 A3 B3 A4 B4) in its LDS column, evaluates it at the hut, adds w[k] * v and, from k = 1 on, applies tier
+// k - 1; a seed that fails a tier, or passes all eight, hands its lane the next seed from a batch-wide counter
+// (fetched 64 at a time per wave). Every lane runs the same build loop each iteration, whatever level its seed is
+// at, so no lane waits on another seed's level and no per-level lists are kept.
 __global__ __launch_bounds__(64) void k_dd_erosion(const uint64_t *__restrict__ seeds, uint32_t ns,
                                                     const whf_dd_spot *__restrict__ spots,
                                                     const BuildDesc *__restrict__ bdE,
@@ -150,105 +154,119 @@ __global__ __launch_bounds__(64) void k_dd_erosion(const uint64_t *__restrict__ 
                                                     uint32_t *__restrict__ needC,
                                                     unsigned long long *__restrict__ cnt) {
     __shared__ uint32_t words[64 * 64];
-    __shared__ uint16_t s_li[2][kXPass];
-    __shared__ double s_sum[kXPass];
     __shared__ uint32_t s_cnt[8];
     uint8_t *bytes = reinterpret_cast<uint8_t *>(words);
     const uint32_t lane = threadIdx.x;
     if (lane < 8) s_cnt[lane] = 0;
+    __syncthreads();
     const EroDesc &ed = gt->ero;
     const double amplitude = ed.amplitude;
     unsigned long long built = 0;
-    __syncthreads();
 
-    // a pass never straddles two spots: 65536 is a multiple of kXPass
-    for (uint32_t p0 = blockIdx.x * kXPass; p0 < ns; p0 += gridDim.x * kXPass) {
-        const uint32_t np = min((uint32_t)kXPass, ns - p0);
-        const whf_dd_spot sp = spots[p0 >> 16];
-        if (single) {
-            // SINGLE_BIOME: the climate gates, erosion included, are skipped
-            for (uint32_t q0 = 0; q0 < np; q0 += 64) {
-                const bool alive = q0 + lane < np;
-                const uint32_t pos = wave_append(alive, &ctl[CTL_Q1]);
-                if (alive) {
-                    q1[pos] = make_uint4(p0 + q0 + lane, (uint32_t)sp.cx, (uint32_t)sp.cz, 0u);
-                    needC[p0 + q0 + lane] = 1u;
-                }
-            }
-            continue;
+    // fetch: every lane whose seed is finished takes the next index of the batch
+    auto fetch = [&](bool need) -> uint32_t {
+        const unsigned long long m = __ballot(need);
+        uint32_t base = 0;
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            if ((int)lane == leader) base = atomicAdd(&ctl[CTL_DD_WORK], (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
         }
-        const double x = (double)(sp.cx * 16 + 8) / 4.0, z = (double)(sp.cz * 16 + 8) / 4.0;
-        const bool raw = fabs(x) < ed.coord_lim && fabs(z) < ed.coord_lim;
-        uint32_t n_alive = np;
-        for (int k = 0; k < 8; k++) {
-            const int cur = (k - 1) & 1, nxt = k & 1;
+        return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    };
+
+    if (single) {
+        // SINGLE_BIOME: the climate gates, erosion included, are skipped
+        for (uint32_t i = blockIdx.x * 64 + lane; i < ns; i += gridDim.x * 64) {
+            const whf_dd_spot sp = spots[i >> 16];
+            const uint32_t pos = atomicAdd(&ctl[CTL_Q1], 1u);
+            q1[pos] = make_uint4(i, (uint32_t)sp.cx, (uint32_t)sp.cz, 0u);
+            needC[i] = 1u;
+        }
+        return;
+    }
+
+    uint32_t idx = fetch(true);
+    int k = 0;
+    double sum = 0.0, x = 0.0, z = 0.0;
+    bool raw = true;
+    int cx = 0, cz = 0;
+    if (idx < ns) {
+        const whf_dd_spot sp = spots[idx >> 16];
+        cx = sp.cx;
+        cz = sp.cz;
+        x = (double)(cx * 16 + 8) / 4.0;
+        z = (double)(cz * 16 + 8) / 4.0;
+        raw = fabs(x) < ed.coord_lim && fabs(z) < ed.coord_lim;
+    }
+    while (__ballot(idx < ns)) {
+        bool done = false, pass = false;
+        if (idx < ns) {
+            // worldSeed -> deriver -> createRandom(erosion) -> octave deriver -> createRandom("octave_<n>")
+            // (noise.c:80-143, rng.c:157-170), then PerlinNoiseSampler(random) (noise.c:61-78)
             const BuildDesc d = bdE[k];
+            Xr r = xr_seed(seeds[idx]);
+            Xr der = xr_fork(r);
+            Xr nr = xr_make(d.na ^ der.lo, d.nb ^ der.hi);
+            if (d.sub) { xr_next(nr); xr_next(nr); }
+            Xr od = xr_fork(nr);
+            Xr pr = xr_make(d.oa ^ od.lo, d.ob ^ od.hi);
+            SPar s;
+            s.ox = xr_next_double(pr) * 256.0;
+            s.oy = xr_next_double(pr) * 256.0;
+            s.oz = xr_next_double(pr) * 256.0;
+#pragma unroll 8
+            for (int w = 0; w < 64; w++) words[w * 64 + lane] = 0x03020100u + (uint32_t)w * 0x04040404u;
+            uint32_t ai = lane * 4;
+            uint32_t jj = xr_next_bounded(pr, 256u);
+            for (int i = 0; i < 256; i++) {
+                const uint32_t aj = ((jj >> 2) << 8) + lane * 4 + (jj & 3);
+                const uint8_t vi = bytes[ai];
+                const uint8_t vj = bytes[aj];
+                if (i < 255) jj = (uint32_t)(i + 1) + xr_next_bounded(pr, 255u - (uint32_t)i);
+                bytes[ai] = vj;
+                bytes[aj] = vi;
+                ai += ((i & 3) == 3) ? 256 - 3 : 1;
+            }
+            const double fj = floor(s.oy);
+            s.jy = (int)fj;
+            s.hy = s.oy - fj;
+            s.hy1 = s.hy - 1.0;
+            s.fy = fade(s.hy);
+            s.pad = 0;
+            built++;
             double xs = x * ed.sc[k] * ed.lac[k], zs = z * ed.sc[k] * ed.lac[k];
             if (!raw) { xs = keep_precision(xs); zs = keep_precision(zs); }
-            uint32_t n_next = 0;
-            for (uint32_t q0 = 0; q0 < n_alive; q0 += 64) {
-                const uint32_t q = q0 + lane;
-                bool alive = false;
-                uint32_t idx = 0;
-                if (q < n_alive) {
-                    idx = k == 0 ? q : s_li[cur][q];
-                    double sum = k == 0 ? 0.0 : s_sum[idx];
-                    // worldSeed -> deriver -> createRandom(erosion) -> octave deriver -> createRandom("octave_<n>")
-                    // (noise.c:80-143, rng.c:157-170), then PerlinNoiseSampler(random) (noise.c:61-78)
-                    const uint64_t seed = seeds[p0 + idx];
-                    Xr r = xr_seed(seed);
-                    Xr der = xr_fork(r);
-                    Xr nr = xr_make(d.na ^ der.lo, d.nb ^ der.hi);
-                    if (d.sub) { xr_next(nr); xr_next(nr); }
-                    Xr od = xr_fork(nr);
-                    Xr pr = xr_make(d.oa ^ od.lo, d.ob ^ od.hi);
-                    SPar s;
-                    s.ox = xr_next_double(pr) * 256.0;
-                    s.oy = xr_next_double(pr) * 256.0;
-                    s.oz = xr_next_double(pr) * 256.0;
-#pragma unroll 8
-                    for (int w = 0; w < 64; w++) words[w * 64 + lane] = 0x03020100u + (uint32_t)w * 0x04040404u;
-                    uint32_t ai = lane * 4;
-                    uint32_t jj = xr_next_bounded(pr, 256u);
-                    for (int i = 0; i < 256; i++) {
-                        const uint32_t aj = ((jj >> 2) << 8) + lane * 4 + (jj & 3);
-                        const uint8_t vi = bytes[ai];
-                        const uint8_t vj = bytes[aj];
-                        if (i < 255) jj = (uint32_t)(i + 1) + xr_next_bounded(pr, 255u - (uint32_t)i);
-                        bytes[ai] = vj;
-                        bytes[aj] = vi;
-                        ai += ((i & 3) == 3) ? 256 - 3 : 1;
-                    }
-                    const double fj = floor(s.oy);
-                    s.jy = (int)fj;
-                    s.hy = s.oy - fj;
-                    s.hy1 = s.hy - 1.0;
-                    s.fy = fade(s.hy);
-                    s.pad = 0;
-                    built++;
-                    const double v = perlin_y0(LanePerm{bytes, lane * 4}, s, xs, zs);
-                    sum += ed.w[k] * v;
-                    if (k >= 1 && sum * amplitude < ed.thr[k - 1]) {
-                        atomicAdd(&s_cnt[k - 1], 1u);
-                    } else {
-                        alive = true;
-                        s_sum[idx] = sum;
-                    }
-                }
-                if (k < 7) {
-                    const unsigned long long m = __ballot(alive);
-                    if (alive) s_li[nxt][n_next + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)idx;
-                    n_next += (uint32_t)__popcll(m);
-                } else {
-                    const uint32_t pos = wave_append(alive, &ctl[CTL_Q1]);
-                    if (alive) {
-                        q1[pos] = make_uint4(p0 + idx, (uint32_t)sp.cx, (uint32_t)sp.cz, 0u);
-                        needC[p0 + idx] = 1u;
-                    }
-                }
+            const double v = perlin_y0(LanePerm{bytes, lane * 4}, s, xs, zs);
+            sum += ed.w[k] * v;
+            if (k >= 1 && sum * amplitude < ed.thr[k - 1]) {
+                atomicAdd(&s_cnt[k - 1], 1u);
+                done = true;
+            } else if (k == 7) {
+                pass = true;
+                done = true;
+            } else {
+                k++;
             }
-            __syncthreads();
-            n_alive = n_next;
+        }
+        const uint32_t pos = wave_append(pass, &ctl[CTL_Q1]);
+        if (pass) {
+            q1[pos] = make_uint4(idx, (uint32_t)cx, (uint32_t)cz, 0u);
+            needC[idx] = 1u;
+        }
+        const uint32_t nidx = fetch(done);
+        if (done) {
+            idx = nidx;
+            k = 0;
+            sum = 0.0;
+            if (idx < ns) {
+                const whf_dd_spot sp = spots[idx >> 16];
+                cx = sp.cx;
+                cz = sp.cz;
+                x = (double)(cx * 16 + 8) / 4.0;
+                z = (double)(cz * 16 + 8) / 4.0;
+                raw = fabs(x) < ed.coord_lim && fabs(z) < ed.coord_lim;
+            }
         }
     }
     __syncthreads();
@@ -320,7 +338,9 @@ int dd_expand(whf_ctx *c, const whf_dd_request *req, const whf_dd_spot *d_spots,
         {
             Prof p(c, F_DD_EXPAND);
             hipLaunchKernelGGL(k_dd_xseeds, dim3((ns + 255) / 256), dim3(256), 0, st, d_spots + s0, ns, c->d_seeds);
-            const uint32_t grid = std::min((ns + kXPass - 1) / kXPass, (uint32_t)c->n_cu * 8);
+            HIP_TRY(hipMemsetAsync(c->d_ctl + CTL_DD_WORK, 0, sizeof(uint32_t), st));
+            // persistent lanes: about one workgroup per LDS slot of the device (16 KB each)
+            const uint32_t grid = std::min((ns + 63) / 64, (uint32_t)c->n_cu * 10);
             hipLaunchKernelGGL(k_dd_erosion, dim3(grid), dim3(64), 0, st, c->d_seeds, ns, d_spots + s0,
                                bd_of(c, layout, SET_E), c->d_gt + layout, (req->vflags & WHF_V_SINGLE) ? 1 : 0,
                                c->d_q1, c->d_ctl, c->d_needC, c->d_cnt);
