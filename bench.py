@@ -163,7 +163,7 @@ def deep_dive(whf, sweep_n, expand_n, cpu_seconds):
     )
 
 
-def serving_stream(n, seed):
+def serving_stream(n, seed, versions=("1.20",)):
     """The serving workload: n requests, sizes round(2^U(0,20)) seeds from SplitMix64(request index), radius
     uniform over {512, 1024, 2048, 4096} blocks -> regions +-(r // 512 + 1), threshold uniform over -50..-54 with
     'or lower' / 'exactly' at 50/50 (-54 is always 'or lower'), version 1.20. Phase 1 runs at maxHeight -50 (the
@@ -178,22 +178,28 @@ def serving_stream(n, seed):
     for i in range(n):
         r = int(radius[i]) // 512 + 1
         reqs.append(dict(rect=(-r, r, -r, r), splitmix=(i, int(sizes[i])), threshold=int(thr[i]),
-                         exactly=bool(exactly[i]), checks=int(sizes[i]) * (2 * r) ** 2))
+                         exactly=bool(exactly[i]), checks=int(sizes[i]) * (2 * r) ** 2,
+                         version=versions[i % len(versions)]))
     return reqs
 
 
-def serve(whf, reqs, rate, slots, chunk_checks):
-    """Submits reqs at Poisson arrivals of `rate` per second (rate None: all at once); returns per-request events
-    and the wall time from the first submit to the last completion."""
+def serve(whf, reqs, rate, slots, chunk_checks, background=None):
+    """Submits reqs at Poisson arrivals of `rate` per second (rate None: all at once), after an optional background
+    Deep Dive (s_first, count); returns per-request events, the wall time from the first submit to the last
+    request's completion, and the Deep Dive's event."""
     import threading
     import numpy as np
-    vflags = whf.version_flags("1.20")
     rng = np.random.default_rng(12345)
     gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
     events = {}
+    dd_event = None
     with whf.Server(0, slots=slots, chunk_checks=chunk_checks) as srv:
         lock = threading.Lock()
         ids = []
+        dd_id = None
+        if background:
+            dd_id = srv.submit_deepdive(background[0], background[1], DD_RECT, floor_max=DD_FLOOR, expand=True,
+                                        max_height=-50)
 
         def feeder():
             t0 = time.perf_counter()
@@ -205,41 +211,47 @@ def serve(whf, reqs, rate, slots, chunk_checks):
                     time.sleep(d)
                 with lock:
                     ids.append(srv.submit(q["rect"], capacity=1024, splitmix=q["splitmix"], max_height=-50,
-                                          vflags=vflags))
+                                          vflags=whf.version_flags(q["version"])))
 
         th = threading.Thread(target=feeder)
         t0 = time.perf_counter()
         th.start()
-        while len(events) < len(reqs):
+        while len(events) < len(reqs) or (dd_id is not None and dd_event is None):
             with lock:
                 evs = srv.poll(timeout_ms=0)
             for e in evs:
-                events[e["id"]] = e
+                if e["id"] == dd_id:
+                    dd_event = e
+                else:
+                    events[e["id"]] = e
+                    if len(events) == len(reqs):
+                        wall = time.perf_counter() - t0
             if not evs:
                 time.sleep(0.0005)
-        wall = time.perf_counter() - t0
         th.join()
-    return [events[i] for i in ids], wall
+    return [events[i] for i in ids], wall, dd_event
 
 
-def serving(whf, n, load, slots, chunk_checks):
+def serving(whf, n, load, slots, chunk_checks, versions=("1.20",), background=None):
     import numpy as np
-    reqs = serving_stream(n, 4)
+    reqs = serving_stream(n, 4, versions)
     mean_checks = float(np.mean([q["checks"] for q in reqs]))
     # capacity: a prefix of the same stream submitted at once (saturation)
     sat = reqs[: max(50, n // 20)]
-    ev, wall = serve(whf, sat, None, slots, chunk_checks)
+    ev, wall, _ = serve(whf, sat, None, slots, chunk_checks)
     capacity = sum(e["stats"]["checks"] for e in ev) / wall
     rate = load * capacity / mean_checks
-    ev, wall = serve(whf, reqs, rate, slots, chunk_checks)
+    ev, wall, dd = serve(whf, reqs, rate, slots, chunk_checks, background)
     lat = np.array([e["done_s"] - e["submit_s"] for e in ev]) * 1e3
     big = np.array([q["splitmix"][1] >= 1 << 19 for q in reqs])
     checks = sum(e["stats"]["checks"] for e in ev)
     assert all(e["status"] == whf.OK for e in ev)
-    return dict(
+    out = dict(
         workload="serving: %d Seed Check requests, Poisson arrivals at %.0f%% of measured capacity, sizes "
                  "round(2^U(0,20)) SplitMix64(request index) seeds, radius {512,1024,2048,4096} blocks, "
-                 "threshold -50..-54 or-lower/exactly, version 1.20" % (n, 100 * load),
+                 "threshold -50..-54 or-lower/exactly, versions %s%s" % (
+                     n, 100 * load, "/".join(versions),
+                     ", with a background Deep Dive of %d structure seeds" % background[1] if background else ""),
         slots=slots, chunk_checks=chunk_checks, capacity_checks_per_s=round(capacity, 1),
         mean_request_checks=round(mean_checks, 1), arrival_rate_per_s=round(rate, 2),
         offered_checks_per_s=round(rate * mean_checks, 1), throughput_checks_per_s=round(checks / wall, 1),
@@ -249,6 +261,13 @@ def serving(whf, n, load, slots, chunk_checks):
                         p99_large=round(float(np.percentile(lat[big], 99)), 3) if big.any() else None,
                         large_requests=int(big.sum()), large="requests of >= 2^19 seeds"),
     )
+    if background:
+        assert dd["status"] == whf.OK
+        dd_s = dd["done_s"] - dd["start_s"]
+        out["deep_dive"] = dict(structure_seeds=background[1], seconds=round(dd_s, 3),
+                                structure_seeds_per_s=round(background[1] / dd_s, 1), spots=len(dd["spots"]),
+                                expanded_seeds=dd["dd_stats"]["expanded_seeds"], finds=len(dd["finds"]))
+    return out
 
 
 def main():
@@ -266,6 +285,8 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-chunk", type=int, default=1 << 24)
+    ap.add_argument("--mixed-dd-seeds", type=int, default=1 << 26,
+                    help="structure seeds of the background Deep Dive in the mixed serving run (0: no mixed run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would scan its own seed "
@@ -323,6 +344,11 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(min(args.seeds, 1 << 20), args.cpu_seconds)
     srv = None if args.serve_requests <= 0 else serving(whf, args.serve_requests, args.serve_load, args.serve_slots,
                                                         args.serve_chunk)
+    mixed = None
+    if args.serve_requests > 0 and args.mixed_dd_seeds > 0:
+        mixed = serving(whf, args.serve_requests, args.serve_load, args.serve_slots, args.serve_chunk,
+                        versions=("1.18", "1.20", "1.21"),
+                        background=(splitmix_first(3) & ((1 << 48) - 1), args.mixed_dd_seeds))
     dd = None if args.no_deep_dive else deep_dive(whf, args.dd_seeds, args.dd_expand,
                                                   0 if args.no_cpu else min(10.0, args.cpu_seconds))
     line = {
@@ -348,6 +374,7 @@ def main():
         "device_ms": round(last["device_ms"], 3),
         "latency_ms": srv["latency_ms"] if srv else None,
         "serving": srv,
+        "serving_mixed": mixed,
         "deep_dive": dd,
     }
     print(json.dumps(line), flush=True)

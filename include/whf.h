@@ -126,47 +126,6 @@ int whf_noise_error(whf_ctx *ctx, const int64_t *seeds, uint64_t n_seeds, uint64
 /* The per-evaluation error bound the FP32 gate uses as its margin. */
 double whf_fp32_bound(void);
 
-/* ---- Serving: continuous batching of Seed Check requests ---- */
-
-typedef struct whf_server whf_server;
-
-typedef struct {
-    int device;
-    int slots;                  /* concurrent device slots (a context, stream and buffers each); 0 = 4 */
-    uint64_t chunk_checks;      /* (seed, region) checks per scheduled chunk; 0 = 2^24 */
-    uint64_t hbm_budget;        /* device bytes all slots may hold; 0 = no limit (slots are cut to fit) */
-    int flags;                  /* reserved, 0 */
-    int reserved;
-} whf_server_config;
-
-/* Completion of one request. Times are seconds on the server's steady clock (0 = whf_server_open). */
-typedef struct {
-    uint64_t id;                /* as returned by whf_submit */
-    int status;                 /* WHF_OK, WHF_EOVERFLOW (more hits than the request's sink) or an error */
-    int reserved;
-    whf_stats stats;            /* the request's own counters: the same as a whf_scan of it alone */
-    double submit_s, start_s, done_s;
-} whf_event;
-
-/* Opens a server: `slots` contexts on the device, each driven by its own host thread. Requests are cut into
- * chunks of about chunk_checks checks; an idle slot takes the next chunk of the request with the least work left
- * (the oldest on ties), so a request that arrives while others run starts as soon as a slot finishes its chunk,
- * small requests overtake big ones, and the work of several requests overlaps on the device (one HIP stream per
- * slot). Replaces the one-request-at-a-time `--worker` loop
- * (phase1_scan.cu:1346-1387) that hunt.py's GpuScanWorker drives (hunt.py:1276-1332). */
-int whf_server_open(const whf_server_config *cfg, whf_server **out);
-
-/* Finishes the requests in flight, then releases the slots. Events not polled are dropped. */
-void whf_server_close(whf_server *srv);
-
-/* Queues a request (copied; for WHF_SEEDS_LIST the seeds array must stay valid until its event). The sink is
- * filled when the request completes, its hits sorted by (seed, rx, rz). Thread-safe. */
-int whf_submit(whf_server *srv, const whf_request *req, whf_sink *sink, uint64_t *id);
-
-/* Waits up to timeout_ms (-1: no limit) for completions; returns the number of events written (0..max_events)
- * or a negative error code. Thread-safe. */
-int whf_poll(whf_server *srv, whf_event *events, int max_events, int timeout_ms);
-
 /* ---- Deep Dive: mineshaft rooms under hut chunks, over 48-bit structure seeds ---- */
 
 typedef struct {
@@ -227,6 +186,55 @@ int whf_deepdive(whf_ctx *ctx, const whf_dd_request *req, whf_dd_sink *spots, wh
  * and piece count per seed (mstree's emit, mstree.cpp:17-34, without the piece list). For parity checks. */
 int whf_room_floors(whf_ctx *ctx, const uint64_t *carvers, uint64_t n, int32_t *status, int32_t *room_floor,
                     int32_t *pieces);
+
+/* ---- Serving: continuous batching of Seed Check requests ---- */
+
+typedef struct whf_server whf_server;
+
+typedef struct {
+    int device;
+    int slots;                  /* concurrent device slots (a context, stream and buffers each); 0 = 4 */
+    uint64_t chunk_checks;      /* (seed, region) checks per scheduled chunk; 0 = 2^24 */
+    uint64_t hbm_budget;        /* device bytes all slots may hold; 0 = no limit (slots are cut to fit) */
+    int flags;                  /* reserved, 0 */
+    int reserved;
+} whf_server_config;
+
+/* Completion of one request. Times are seconds on the server's steady clock (0 = whf_server_open). */
+typedef struct {
+    uint64_t id;                /* as returned by whf_submit */
+    int status;                 /* WHF_OK, WHF_EOVERFLOW (more hits than the request's sink) or an error */
+    int reserved;
+    whf_stats stats;            /* the request's own counters: the same as a whf_scan of it alone */
+    double submit_s, start_s, done_s;
+} whf_event;
+
+/* Opens a server: `slots` contexts on the device, each driven by its own host thread. Requests are cut into
+ * chunks of about chunk_checks checks; an idle slot takes the next chunk of the request with the least work left
+ * (the oldest on ties), so a request that arrives while others run starts as soon as a slot finishes its chunk,
+ * small requests overtake big ones, and the work of several requests overlaps on the device (one HIP stream per
+ * slot). Replaces the one-request-at-a-time `--worker` loop
+ * (phase1_scan.cu:1346-1387) that hunt.py's GpuScanWorker drives (hunt.py:1276-1332). */
+int whf_server_open(const whf_server_config *cfg, whf_server **out);
+
+/* Finishes the requests in flight, then releases the slots. Events not polled are dropped. */
+void whf_server_close(whf_server *srv);
+
+/* Queues a request (copied; for WHF_SEEDS_LIST the seeds array must stay valid until its event). The sink is
+ * filled when the request completes, its hits sorted by (seed, rx, rz). Thread-safe. */
+int whf_submit(whf_server *srv, const whf_request *req, whf_sink *sink, uint64_t *id);
+
+/* Queues a background Deep Dive (request copied; see whf_deepdive). Slots run its chunks of 2^18 structure seeds
+ * only when no Seed Check chunk is waiting, and at most half the slots at once, so interactive requests keep
+ * their latency while the Deep Dive uses the rest of the device. The sinks and *stats are filled when it
+ * completes; its completion comes through whf_poll like a request's (with zero whf_stats). Thread-safe. No
+ * counterpart in the reference, which runs Seed Check and Deep Dive as separate programs (hunt.py:58-63). */
+int whf_submit_deepdive(whf_server *srv, const whf_dd_request *req, whf_dd_sink *spots, whf_sink *finds,
+                        whf_dd_stats *stats, uint64_t *id);
+
+/* Waits up to timeout_ms (-1: no limit) for completions; returns the number of events written (0..max_events)
+ * or a negative error code. Thread-safe. */
+int whf_poll(whf_server *srv, whf_event *events, int max_events, int timeout_ms);
 
 /* Version name -> WHF_V_* flags, accepting lysh's version names (main.c:104-125, biome_trees.h:70499-70526). */
 int whf_version_flags(const char *name, int *vflags);

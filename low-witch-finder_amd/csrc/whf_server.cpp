@@ -1,5 +1,9 @@
 // whf_server.cpp — continuous batching of Seed Check requests over several device slots (C ABI in whf.h).
 //
+// Background Deep Dive jobs share the slots: a slot takes a Deep Dive chunk (2^18 structure seeds: sweep, walks and
+// the expansion of its spots) only when no Seed Check chunk is waiting, and at most half the slots run Deep Dive
+// chunks at once, so interactive requests keep at least half the slots and wait at most one chunk for the rest.
+//
 // A slot is a whf_ctx (its own HIP stream and working buffers) driven by its own host thread. Requests are cut
 // into chunks of seeds (about chunk_checks (seed, region) checks each); an idle slot takes the next chunk of the
 // request with the least work left (the oldest on ties), runs it with whf_scan and folds the chunk's hits and
@@ -40,6 +44,48 @@ struct Job {
     uint64_t remaining_checks() const { return (req.n_seeds - next) * regions; }
 };
 
+struct DdJob {
+    uint64_t id = 0;
+    whf_dd_request req{};
+    whf_dd_sink *spots = nullptr;
+    whf_sink *finds = nullptr;
+    whf_dd_stats *out = nullptr;
+    uint64_t next = 0, done = 0;   // structure seeds
+    int in_flight = 0;
+    int status = WHF_OK;
+    whf_dd_stats stats{};
+    std::vector<whf_dd_spot> spot_list;
+    std::vector<whf_hit> find_list;
+    double submit_s = 0, start_s = -1;
+};
+
+constexpr uint64_t kDdChunk = 1ull << 18;   // structure seeds per Deep Dive chunk (about 20 ms of device time)
+constexpr uint64_t kDdChunkSpots = 1ull << 16;
+constexpr uint64_t kDdChunkFinds = 1ull << 14;
+
+void add_dd_stats(whf_dd_stats &a, const whf_dd_stats &b) {
+    a.structure_seeds += b.structure_seeds;
+    a.tests += b.tests;
+    a.starts += b.starts;
+    for (int k = 0; k < 3; k++) a.walks[k] += b.walks[k];
+    a.spots += b.spots;
+    a.draws += b.draws;
+    a.pieces += b.pieces;
+    a.expanded_seeds += b.expanded_seeds;
+    a.accepted += b.accepted;
+    for (int k = 0; k < WHF_KIND_N; k++) a.funnel[k] += b.funnel[k];
+    for (int k = 0; k < 8; k++) a.tiers[k] += b.tiers[k];
+    for (int k = 0; k < 3; k++) a.samplers_built[k] += b.samplers_built[k];
+    a.device_ms += b.device_ms;
+    for (int k = 0; k < 3; k++) a.stage_ms[k] += b.stage_ms[k];
+}
+
+bool spot_less(const whf_dd_spot &x, const whf_dd_spot &y) {
+    if (x.s48 != y.s48) return x.s48 < y.s48;
+    if (x.rx != y.rx) return x.rx < y.rx;
+    return x.rz < y.rz;
+}
+
 void add_stats(whf_stats &a, const whf_stats &b) {
     a.seeds += b.seeds;
     a.checks += b.checks;
@@ -69,6 +115,8 @@ struct whf_server {
     std::mutex mu;
     std::condition_variable work_cv, event_cv;
     std::deque<std::shared_ptr<Job>> jobs;   // admitted requests with chunks left, in arrival order
+    std::deque<std::shared_ptr<DdJob>> dd_jobs;   // background Deep Dive jobs with chunks left
+    int dd_running = 0;                             // slots running a Deep Dive chunk now
     std::deque<whf_event> events;
     uint64_t chunk_checks = 1ull << 24;
     uint64_t next_id = 1;
@@ -100,16 +148,96 @@ struct whf_server {
         event_cv.notify_all();
     }
 
+    void finish_dd(const std::shared_ptr<DdJob> &j) {   // mu held
+        std::sort(j->spot_list.begin(), j->spot_list.end(), spot_less);
+        std::sort(j->find_list.begin(), j->find_list.end(), hit_less);
+        int status = j->status;
+        if (j->spots) {
+            j->spots->count = j->spot_list.size();
+            const uint64_t m = std::min<uint64_t>(j->spots->capacity, j->spot_list.size());
+            if (m && j->spots->spots) std::memcpy(j->spots->spots, j->spot_list.data(), m * sizeof(whf_dd_spot));
+            if (j->spot_list.size() > j->spots->capacity && status == WHF_OK) status = WHF_EOVERFLOW;
+        }
+        if (j->finds) {
+            j->finds->count = j->find_list.size();
+            const uint64_t m = std::min<uint64_t>(j->finds->capacity, j->find_list.size());
+            if (m && j->finds->hits) std::memcpy(j->finds->hits, j->find_list.data(), m * sizeof(whf_hit));
+            if (j->find_list.size() > j->finds->capacity && status == WHF_OK) status = WHF_EOVERFLOW;
+        }
+        if (j->out) *j->out = j->stats;
+        whf_event e{};
+        e.id = j->id;
+        e.status = status;
+        e.submit_s = j->submit_s;
+        e.start_s = j->start_s;
+        e.done_s = now();
+        events.push_back(e);
+        in_flight_jobs--;
+        event_cv.notify_all();
+    }
+
+    // Runs one Deep Dive chunk of job j on context c (mu not held).
+    void run_dd_chunk(whf_ctx *c, const std::shared_ptr<DdJob> &j, uint64_t s0, uint64_t n,
+                      std::vector<whf_dd_spot> &sbuf, std::vector<whf_hit> &fbuf) {
+        whf_dd_request sub = j->req;
+        sub.s_first = j->req.s_first + s0;
+        sub.count = n;
+        whf_dd_sink ss{sbuf.data(), sbuf.size(), 0};
+        whf_sink fs{fbuf.data(), fbuf.size(), 0};
+        whf_dd_stats st{};
+        const int rc = whf_deepdive(c, &sub, &ss, &fs, &st);
+        std::lock_guard<std::mutex> lk(mu);
+        if (rc == WHF_OK || rc == WHF_EOVERFLOW) {
+            add_dd_stats(j->stats, st);
+            const uint64_t ms = std::min<uint64_t>(ss.count, sbuf.size()), mf = std::min<uint64_t>(fs.count, fbuf.size());
+            j->spot_list.insert(j->spot_list.end(), sbuf.begin(), sbuf.begin() + ms);
+            j->find_list.insert(j->find_list.end(), fbuf.begin(), fbuf.begin() + mf);
+            if (rc == WHF_EOVERFLOW && j->status == WHF_OK) j->status = WHF_EOVERFLOW;
+        } else if (j->status == WHF_OK || j->status == WHF_EOVERFLOW) {
+            j->status = rc;
+        }
+        j->done += n;
+        j->in_flight--;
+        dd_running--;
+        if (j->done >= j->req.count && j->in_flight == 0) finish_dd(j);
+        work_cv.notify_all();
+    }
+
     void worker(int slot) {
         whf_ctx *c = ctx[slot];
         std::vector<whf_hit> buf(kChunkHits);
+        std::vector<whf_dd_spot> dd_spots;
+        std::vector<whf_hit> dd_finds;
+        const int dd_limit = std::max(1, (int)ctx.size() / 2);
         for (;;) {
             std::shared_ptr<Job> j;
             uint64_t s0 = 0, n = 0;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                work_cv.wait(lk, [&] { return stop || !jobs.empty(); });
-                if (jobs.empty()) return;   // stop with nothing queued
+                work_cv.wait(lk, [&] {
+                    return stop || !jobs.empty() || (!dd_jobs.empty() && dd_running < dd_limit);
+                });
+                if (jobs.empty()) {
+                    if (!dd_jobs.empty() && dd_running < dd_limit) {
+                        // no interactive work waiting: one chunk of the oldest background Deep Dive
+                        std::shared_ptr<DdJob> d = dd_jobs.front();
+                        const uint64_t d0 = d->next, dn = std::min(kDdChunk, d->req.count - d->next);
+                        d->next += dn;
+                        d->in_flight++;
+                        dd_running++;
+                        if (d->start_s < 0) d->start_s = now();
+                        if (d->next >= d->req.count) dd_jobs.pop_front();
+                        lk.unlock();
+                        if (dd_spots.empty()) {
+                            dd_spots.resize(kDdChunkSpots);
+                            dd_finds.resize(kDdChunkFinds);
+                        }
+                        run_dd_chunk(c, d, d0, dn, dd_spots, dd_finds);
+                        continue;
+                    }
+                    if (stop && dd_jobs.empty()) return;   // stop with nothing queued
+                    continue;
+                }
                 // shortest remaining work first (ties: the older request), so small requests overtake big ones
                 auto best = jobs.begin();
                 for (auto it = jobs.begin(); it != jobs.end(); ++it)
@@ -141,6 +269,7 @@ struct whf_server {
             j->done_seeds += n;
             j->in_flight--;
             if (j->done_seeds >= j->req.n_seeds && j->in_flight == 0) finish(j);
+            work_cv.notify_all();
         }
     }
 };
@@ -207,6 +336,32 @@ int whf_submit(whf_server *s, const whf_request *req, whf_sink *sink, uint64_t *
         s->finish(j);
     } else {
         s->jobs.push_back(j);
+        s->work_cv.notify_all();
+    }
+    return WHF_OK;
+}
+
+int whf_submit_deepdive(whf_server *s, const whf_dd_request *req, whf_dd_sink *spots, whf_sink *finds,
+                        whf_dd_stats *stats, uint64_t *id) {
+    if (!s || !req) return WHF_EINVAL;
+    if (req->rx1 <= req->rx0 || req->rz1 <= req->rz0 || req->count > (1ull << 48)) return WHF_EINVAL;
+    if (spots && spots->capacity && !spots->spots) return WHF_EINVAL;
+    if (finds && finds->capacity && !finds->hits) return WHF_EINVAL;
+    auto j = std::make_shared<DdJob>();
+    j->req = *req;
+    j->spots = spots;
+    j->finds = finds;
+    j->out = stats;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return WHF_EINVAL;
+    j->id = s->next_id++;
+    j->submit_s = s->now();
+    if (id) *id = j->id;
+    s->in_flight_jobs++;
+    if (req->count == 0) {
+        s->finish_dd(j);
+    } else {
+        s->dd_jobs.push_back(j);
         s->work_cv.notify_all();
     }
     return WHF_OK;

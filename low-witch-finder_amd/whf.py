@@ -22,7 +22,7 @@ CFG_PROFILE = 1
 # Exported symbols of include/whf.h, checked by the tests.
 SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags", "whf_strerror",
            "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_room_floors",
-           "whf_server_open", "whf_server_close", "whf_submit", "whf_poll")
+           "whf_server_open", "whf_server_close", "whf_submit", "whf_submit_deepdive", "whf_poll")
 HIT_DEEPDIVE = 1
 
 
@@ -133,6 +133,9 @@ def load():
         L.whf_submit.restype = ctypes.c_int
         L.whf_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(Sink),
                                  ctypes.POINTER(ctypes.c_uint64)]
+        L.whf_submit_deepdive.restype = ctypes.c_int
+        L.whf_submit_deepdive.argtypes = [ctypes.c_void_p, ctypes.POINTER(DdRequest), ctypes.POINTER(DdSink),
+                                          ctypes.POINTER(Sink), ctypes.POINTER(DdStats), ctypes.POINTER(ctypes.c_uint64)]
         L.whf_poll.restype = ctypes.c_int
         L.whf_poll.argtypes = [ctypes.c_void_p, ctypes.POINTER(Event), ctypes.c_int, ctypes.c_int]
         L.whf_deepdive.restype = ctypes.c_int
@@ -341,15 +344,46 @@ class Server:
         self.pending[rid.value] = (req, keep, buf, sink)
         return rid.value
 
+    def submit_deepdive(self, s_first, count, rect, floor_max=-55, expand=True, max_height=-50, vflags=0, salt=0,
+                        spot_capacity=1 << 20, find_capacity=1 << 16):
+        """Queues a background Deep Dive; its event (from poll) carries spots, finds and dd_stats."""
+        req = DdRequest(s_first & 0xFFFFFFFFFFFFFFFF, count, rect[0], rect[1], rect[2], rect[3], floor_max,
+                        1 if expand else 0, max_height, vflags, salt, 0)
+        sbuf = (DdSpot * spot_capacity)()
+        ssink = DdSink(sbuf, spot_capacity, 0)
+        fbuf = (Hit * find_capacity)()
+        fsink = Sink(fbuf, find_capacity, 0)
+        st = DdStats()
+        rid = ctypes.c_uint64(0)
+        rc = self.L.whf_submit_deepdive(self.h, ctypes.byref(req), ctypes.byref(ssink), ctypes.byref(fsink),
+                                        ctypes.byref(st), ctypes.byref(rid))
+        if rc:
+            raise WhfError(rc, "whf_submit_deepdive")
+        self.pending[rid.value] = ("dd", req, sbuf, ssink, fbuf, fsink, st)
+        return rid.value
+
     def poll(self, timeout_ms=-1, max_events=256):
-        """Completed requests: list of dicts (id, status, hits, stats, submit_s, start_s, done_s)."""
+        """Completed requests: list of dicts (id, status, hits, stats, submit_s, start_s, done_s); a Deep Dive's
+        has spots, finds and dd_stats instead of hits and stats."""
         ev = (Event * max_events)()
         n = self.L.whf_poll(self.h, ev, max_events, timeout_ms)
         if n < 0:
             raise WhfError(n, "whf_poll")
         out = []
         for e in ev[:n]:
-            req, keep, buf, sink = self.pending.pop(e.id)
+            p = self.pending.pop(e.id)
+            if p[0] == "dd":
+                _, req, sbuf, ssink, fbuf, fsink, st = p
+                spots = sorted((q.s48, q.rx, q.rz, q.cx, q.cz, q.carver, q.room_floor, q.pieces)
+                               for q in sbuf[:min(ssink.count, ssink.capacity)])
+                finds = sorted((h.seed, h.rx, h.rz, h.cx, h.cz, h.floor_y) for h in fbuf[:min(fsink.count, fsink.capacity)])
+                out.append(dict(id=e.id, status=e.status, deepdive=True, spots=spots, finds=finds,
+                                dd_stats=dict(structure_seeds=st.structure_seeds, starts=st.starts, walks=list(st.walks),
+                                              spots=st.spots, expanded_seeds=st.expanded_seeds, accepted=st.accepted,
+                                              funnel=list(st.funnel), tiers=list(st.tiers)[:7]),
+                                submit_s=e.submit_s, start_s=e.start_s, done_s=e.done_s))
+                continue
+            req, keep, buf, sink = p
             m = min(sink.count, sink.capacity)
             hits = sorted((buf[i].seed, buf[i].rx, buf[i].rz, buf[i].cx, buf[i].cz) for i in range(m))
             out.append(dict(id=e.id, status=e.status, hits=hits, stats=stats_dict(e.stats), submit_s=e.submit_s,

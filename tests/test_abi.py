@@ -64,3 +64,15 @@ def test_cli_argument_errors_exit_2_before_touching_the_gpu():
     r = run_cli()
     assert r.returncode == 2
     assert run_cli("--help").returncode == 0
+
+
+def test_header_compiles_as_c99():
+    """include/whf.h is plain C: it compiles on its own with a C compiler and no warnings."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "t.c")
+        with open(src, "w") as f:
+            f.write('#include "whf.h"\nint main(void) { whf_hit h; (void)h; return 0; }\n')
+        r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", os.path.dirname(HEADER), "-c",
+                            src, "-o", os.path.join(d, "t.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr

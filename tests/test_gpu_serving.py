@@ -103,3 +103,40 @@ def test_empty_request_and_sink_overflow():
     assert got[a]["status"] == whf.OK and got[a]["stats"]["checks"] == 0
     assert got[b]["status"] == whf.EOVERFLOW and got[b]["stats"]["accepted"] >= 1
     assert got[b]["stats"]["checks"] == 158 * 158
+
+
+@pytest.mark.timeout(600)
+def test_background_deepdive_and_requests_equal_alone():
+    """A background Deep Dive (4 chunks, the last partial) shares the slots with Seed Check requests of versions
+    1.18, 1.20 and 1.21: every result equals its run alone."""
+    s0 = O.splitmix(3, 0) & ((1 << 48) - 1)
+    count = 3 * (1 << 20) + 12345
+    rect = (-2, 2, -2, 2)
+    reqs = []
+    for i, v in enumerate(["1.18", "1.20", "1.21"] * 8):
+        reqs.append(((-3, 3, -3, 3), dict(splitmix=(900 + i, 1 + 97 * i), max_height=-50,
+                                          vflags=whf.version_flags(v))))
+    ref = alone(reqs)
+    with whf.Context(0) as ctx:
+        dd_spots, dd_finds, dd_st = ctx.deepdive(s0, count, rect, floor_max=-55, expand=True, max_height=-50,
+                                                 spot_capacity=1 << 16)
+    with whf.Server(0, slots=3) as srv:
+        did = srv.submit_deepdive(s0, count, rect, floor_max=-55, expand=True, max_height=-50)
+        ids = []
+        for rect_i, kw in reqs:
+            ids.append(srv.submit(rect_i, capacity=1 << 16, **kw))
+            time.sleep(0.003)
+        got = {}
+        while len(got) < len(ids) + 1:
+            for e in srv.poll(timeout_ms=60000):
+                got[e["id"]] = e
+    for rid, r in zip(ids, ref):
+        check_same(got[rid], r)
+    d = got[did]
+    assert d["status"] == whf.OK
+    assert d["spots"] == dd_spots
+    assert d["finds"] == dd_finds
+    for k in ("structure_seeds", "starts", "walks", "spots", "expanded_seeds", "accepted", "funnel", "tiers"):
+        assert d["dd_stats"][k] == dd_st[k], k
+    # the interactive requests were not held behind the whole Deep Dive
+    assert min(got[i]["done_s"] for i in ids) < d["done_s"]
