@@ -76,3 +76,15 @@ def test_header_compiles_as_c99():
         r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", os.path.dirname(HEADER), "-c",
                             src, "-o", os.path.join(d, "t.o")], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(whf.DEEPDIVE_SCAN), reason="deepdive_scan not built")
+def test_deepdive_cli_argument_errors_exit_2_before_touching_the_gpu():
+    base = ["--s-first", "1", "--count", "4", "--min-region-x", "0", "--max-region-x", "1", "--min-region-z", "0",
+            "--max-region-z", "1"]
+    for args in (["--bogus"], base[:4], base[:-2], base[:6] + ["0"] + base[7:], base + ["--version", "1.17"],
+                 base + ["--floor"], ["--count", "x"] + base[2:]):
+        r = subprocess.run([whf.DEEPDIVE_SCAN, *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2, (args, r.returncode, r.stderr)
+        assert "usage: deepdive_scan" in r.stdout
+    assert subprocess.run([whf.DEEPDIVE_SCAN, "--help"], capture_output=True, timeout=60).returncode == 0

@@ -152,3 +152,22 @@ def test_invalid_requests_are_rejected(ctx):
         args.update(kw)
         with pytest.raises(whf.WhfError):
             ctx.deepdive(args.pop("s_first"), args.pop("count"), args.pop("rect"), **args)
+
+
+def test_deepdive_cli_writes_solver_hits_and_finds(tmp_path, mst):
+    """bin/deepdive_scan on a find's structure seed: the hits file has the solver's "S,cx,cz,X,1" row and the
+    finds file the world seed, chunk, carver seed and room floor."""
+    import subprocess
+    f = mst["finds"][7]
+    rx, rz = f["cx"] >> 5, f["cz"] >> 5
+    hits, out = tmp_path / "hits.csv", tmp_path / "finds.csv"
+    r = subprocess.run([whf.DEEPDIVE_SCAN, "--s-first", str(f["seed"] & M48), "--count", "1", "--min-region-x", str(rx),
+                        "--max-region-x", str(rx + 1), "--min-region-z", str(rz), "--max-region-z", str(rz + 1),
+                        "--floor", "-55", "--expand", "--max-height", "-50", "--dump-hits", str(hits), "--out", str(out),
+                        "--stats"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    X = O.carver_seed(f["seed"] & M48, f["cx"], f["cz"])
+    assert hits.read_text() == "%d,%d,%d,%d,1\n" % (f["seed"] & M48, f["cx"], f["cz"], X)
+    lines = out.read_text().split("\n")
+    assert lines[0] == "seed,chunkX,chunkZ,layoutX,roomFloor"
+    assert "%d,%d,%d,%d,%d" % (f["seed"], f["cx"], f["cz"], X, f["room_floor"]) in lines
