@@ -36,7 +36,7 @@ constexpr uint32_t kDdStartCap = 1u << 21;      // starts per sweep batch
 constexpr uint32_t kDdSpotCap = 1u << 20;       // spots per sweep batch
 constexpr uint64_t kDdSweepMax = 1ull << 24;    // structure seeds per sweep batch
 constexpr uint32_t kDdSpotsPerExpand = kMaxBatchSeeds >> 16;
-constexpr int kDdGroup = 16;                    // lanes per layout walk
+constexpr int kDdGroup = 32;                    // lanes per layout walk (measured: 16 -> 1.35x slower, 64 -> 1.2x)
 
 // ---------------------------------------------------------------------------------------------- sweep
 
