@@ -187,6 +187,43 @@ int whf_deepdive(whf_ctx *ctx, const whf_dd_request *req, whf_dd_sink *spots, wh
 int whf_room_floors(whf_ctx *ctx, const uint64_t *carvers, uint64_t n, int32_t *status, int32_t *room_floor,
                     int32_t *pieces);
 
+/* ---- Biome stage: the swamp-hut biome check of the multi-noise biome source ---- */
+
+#define WHF_BIOME_SWAMP 6     /* the trees' biome id of minecraft:swamp (biome.h:77) */
+#define WHF_BIOME_QY0   (-17) /* quart heights an occupied height can fall in: -17 .. 79 (surface -64 .. 320) */
+#define WHF_BIOME_QY_N  97
+
+typedef struct {
+    int64_t seed;               /* world seed */
+    int32_t hut_x, hut_z;       /* a block of the hut's chunk (16*cx, 16*cz); the check samples the chunk's middle */
+    int32_t occ_y;              /* first occupied height at the chunk middle: its WORLD_SURFACE_WG height - 1 */
+    int32_t reserved;
+} whf_biome_query;
+
+typedef struct {
+    int64_t target[6];          /* quantized temperature, humidity, continentalness, erosion, depth, weirdness
+                                   (x 10000, Climate.quantizeCoord) at the query's occupied height */
+    int32_t qx, qy, qz;         /* quart position of the check */
+    int32_t occ_y;
+    int32_t winner;             /* biome id of the tree's nearest leaf at the query's height */
+    int32_t ok;                 /* winner == WHF_BIOME_SWAMP */
+    uint64_t swamp_qy[2];       /* bit i (word i / 64): the winner at quart height WHF_BIOME_QY0 + i is swamp */
+} whf_biome_result;
+
+/* The biome tree a game version uses (biome_trees.h:70501-70526; the rule-set aliases of the command line, such
+ * as "1.19.x~26.1", take their rule set's default tree, eval.c:104-106). Returns WHF_EINVAL for unknown names. */
+int whf_biome_tree(const char *version, int *tree);
+
+/* The swamp-hut biome check of lysh_swamp_hut_biome_ok (biome.c:94-156) for each query, with the height callback's
+ * answer given as occ_y: the six climate parameters at the chunk middle's quart point (terrain.c:22-44 shifted
+ * coordinates, column_top.c:124-130 overworld offset for depth), quantized, and the nearest leaf of biome tree
+ * `tree` (0..7, Climate.RTree.SubTree.search, biome.c:34-47). Besides the answer at occ_y, each result carries the
+ * swamp verdict at every quart height the surface can have; `winners` (may be NULL) receives the winner at each
+ * of them, WHF_BIOME_QY_N bytes per query. vflags: WHF_V_LARGE selects the LARGE_BIOMES climate noises; other
+ * bits are ignored. Not re-entrant per context. */
+int whf_biome(whf_ctx *ctx, const whf_biome_query *queries, uint64_t n, int tree, int vflags, whf_biome_result *out,
+              uint8_t *winners);
+
 /* ---- Serving: continuous batching of Seed Check requests ---- */
 
 typedef struct whf_server whf_server;

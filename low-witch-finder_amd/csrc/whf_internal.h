@@ -33,7 +33,9 @@ enum Cnt {
     CNT_DD_CAPPED = 29,
     CNT_DD_DRAWS = 30,
     CNT_DD_PIECES = 31,
-    CNT_N = 32
+    CNT_BIO_BUILT = 32,                  // biome stage: Perlin samplers built
+    CNT_BIO_CHECKS = 33,                 // biome stage: (query, quart height) tree searches
+    CNT_N = 40
 };
 
 struct DevHit {
@@ -102,6 +104,8 @@ struct whf_ctx {
     whf::BuildDesc *d_bd = nullptr;       // [kLayouts][8 + 28 + 36]
     void *d_dd = nullptr;                 // Deep Dive buffers (whf_deepdive.hip), allocated on first use
     size_t dd_bytes = 0;
+    void *d_bio = nullptr;                // biome stage buffers and tables (whf_biome.hip), allocated on first use
+    size_t bio_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> prof_ev;  // pairs per kernel launch when profiling
     std::vector<int> prof_fam;
@@ -133,8 +137,20 @@ struct Prof {
     }
 };
 
+// One Perlin sampler of a set: noise parameter id, sub-sampler (0/1) and octave slot.
+struct SamplerRef {
+    int noise, sub, slot;
+};
+// The MD5-keyed derivation descriptor of a sampler, and the descriptor of a double-Perlin noise over a set
+// (its table indices, lacunarities, amplitudes, persistences and output amplitude, noise.c:80-156).
+BuildDesc make_build_desc(const SamplerRef &r);
+void fill_noise_desc(NoiseDesc &d, int pid, const std::vector<SamplerRef> &set, int set_id);
+
 int check_launch();
 const BuildDesc *bd_of(whf_ctx *c, int layout, int set);
+// Perlin sampler tables of n_slots seeds (seeds[slot]) for the set_n samplers of bd, on the context's stream.
+void launch_build(whf_ctx *c, const uint64_t *seeds, const BuildDesc *bd, int set_n, uint32_t n_slots,
+                  uint8_t *tab, SPar *par, unsigned long long *built);
 // Zero the batch's queue counts and the gate flags of its first ns seeds.
 void launch_reset(whf_ctx *c, uint32_t ns);
 // Gates 2 and 3 of a batch whose erosion survivors are already in Q1 (with needC set): compact, build set C,

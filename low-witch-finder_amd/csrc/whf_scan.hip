@@ -690,10 +690,43 @@ struct Md5 {
 };
 
 
-// Host description of one sampler of a set.
-struct SamplerRef {
-    int noise, sub, slot;
-};
+
+}  // namespace
+
+namespace whf {
+
+BuildDesc make_build_desc(const SamplerRef &r) {
+    BuildDesc b{};
+    Md5::digest(kNoise[r.noise].name, b.na, b.nb);
+    Md5::digest("octave_" + std::to_string(kNoise[r.noise].first_octave + r.slot), b.oa, b.ob);
+    b.sub = r.sub;
+    return b;
+}
+
+void fill_noise_desc(NoiseDesc &d, int pid, const std::vector<SamplerRef> &set, int set_id) {
+    const NoiseParam &np = kNoise[pid];
+    d.set = set_id;
+    int p = 0, first = -1, last = -1;
+    const double per0 = std::ldexp(1.0, np.n - 1) / (std::ldexp(1.0, np.n) - 1.0);
+    for (int i = 0; i < np.n; i++) {
+        if (np.amps[i] == 0.0) continue;
+        if (first < 0) first = i;
+        last = i;
+        for (int s = 0; s < 2; s++)
+            for (size_t t = 0; t < set.size(); t++)
+                if (set[t].noise == pid && set[t].sub == s && set[t].slot == i) d.tab[s][p] = (int)t;
+        d.lac[p] = std::ldexp(1.0, np.first_octave + i);
+        d.amp[p] = np.amps[i];
+        d.per[p] = std::ldexp(per0, -i);   // persistence halves per slot (noise.c:277-278), exactly
+        p++;
+    }
+    d.np = p;
+    d.amplitude = 0.16666666666666666 / (0.1 * (1.0 + 1.0 / ((double)(last - first) + 1.0)));
+}
+
+}  // namespace whf
+
+namespace {
 
 // Which noise stands in each gate role for these rule flags (LARGE_BIOMES swaps three climate noises).
 int gate_noise_param(int g, int vflags) {
@@ -752,29 +785,8 @@ void build_layout(int vflags, Layout &L) {
                 if (np.amps[i] != 0.0) L.set[gate_set(g)].push_back({pid, s, i});
     }
     // noise descriptors
-    for (int g = 0; g < G_COUNT; g++) {
-        const int pid = gate_noise_param(g, vflags);
-        const NoiseParam &np = kNoise[pid];
-        NoiseDesc &d = L.gt.noise[g];
-        d.set = gate_set(g);
-        const std::vector<SamplerRef> &set = L.set[d.set];
-        int p = 0, first = -1, last = -1;
-        const double per0 = std::ldexp(1.0, np.n - 1) / (std::ldexp(1.0, np.n) - 1.0);
-        for (int i = 0; i < np.n; i++) {
-            if (np.amps[i] == 0.0) continue;
-            if (first < 0) first = i;
-            last = i;
-            for (int s = 0; s < 2; s++)
-                for (size_t t = 0; t < set.size(); t++)
-                    if (set[t].noise == pid && set[t].sub == s && set[t].slot == i) d.tab[s][p] = (int)t;
-            d.lac[p] = std::ldexp(1.0, np.first_octave + i);
-            d.amp[p] = np.amps[i];
-            d.per[p] = std::ldexp(per0, -i);   // persistence halves per slot (noise.c:277-278), exactly
-            p++;
-        }
-        d.np = p;
-        d.amplitude = 0.16666666666666666 / (0.1 * (1.0 + 1.0 / ((double)(last - first) + 1.0)));
-    }
+    for (int g = 0; g < G_COUNT; g++)
+        fill_noise_desc(L.gt.noise[g], gate_noise_param(g, vflags), L.set[gate_set(g)], gate_set(g));
     // tiered erosion
     {
         const NoiseDesc &d = L.gt.noise[G_EROSION];
@@ -803,13 +815,7 @@ void build_layout(int vflags, Layout &L) {
     }
     // build descriptors
     for (int s = 0; s < SET_COUNT; s++)
-        for (const SamplerRef &r : L.set[s]) {
-            BuildDesc b{};
-            Md5::digest(kNoise[r.noise].name, b.na, b.nb);
-            Md5::digest("octave_" + std::to_string(kNoise[r.noise].first_octave + r.slot), b.oa, b.ob);
-            b.sub = r.sub;
-            L.bd[s].push_back(b);
-        }
+        for (const SamplerRef &r : L.set[s]) L.bd[s].push_back(make_build_desc(r));
 }
 
 
@@ -874,7 +880,7 @@ __attribute__((visibility("hidden"))) uint64_t whf_ctx_device_bytes(const whf_ct
     uint64_t b = B * 8 + 4 * B * 4 + 2 * S * 4 + 2ull * kMaxItems * sizeof(uint4) + (uint64_t)kQxCap * sizeof(uint4) +
                  (uint64_t)kHitCap * sizeof(DevHit) + kLayouts * (sizeof(GateTables) + kBdPerLayout * sizeof(BuildDesc));
     for (int s = 0; s < SET_COUNT; s++) b += S * kSetSize[s] * (256 + sizeof(SPar));
-    return b + (c ? c->dd_bytes : 0);
+    return b + (c ? c->dd_bytes + c->bio_bytes : 0);
 }
 
 namespace whf {
@@ -892,6 +898,13 @@ int check_launch() {
         return WHF_EDEVICE;
     }
     return WHF_OK;
+}
+
+void launch_build(whf_ctx *c, const uint64_t *seeds, const BuildDesc *bd, int set_n, uint32_t n_slots,
+                  uint8_t *tab, SPar *par, unsigned long long *built) {
+    const uint64_t jobs = (uint64_t)n_slots * (uint32_t)set_n;
+    hipLaunchKernelGGL(k_build, dim3((uint32_t)std::max<uint64_t>(1, (jobs + 63) / 64)), dim3(64), 0, c->stream, seeds,
+                       bd, set_n, (const uint32_t *)nullptr, (const uint32_t *)nullptr, n_slots, tab, par, built);
 }
 
 void launch_reset(whf_ctx *c, uint32_t ns) {
@@ -1061,7 +1074,7 @@ void whf_close(whf_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_seeds, c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_par[0], c->d_par[1], c->d_par[2],
                     c->d_q1, c->d_q2, c->d_qx, c->d_needC, c->d_needL, c->d_slotC, c->d_slotL, c->d_seedC, c->d_seedL,
-                    c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd, c->d_dd};
+                    c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd, c->d_dd, c->d_bio};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);

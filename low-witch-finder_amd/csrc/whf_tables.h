@@ -1,7 +1,8 @@
 // whf_tables.h — game data the phase-1 Seed Check needs, and the sampler-set layouts of the HIP path.
 //
 // The noise parameters are the game's NormalNoise settings: identifier, first octave, amplitudes. They are
-// data and must equal the reference's table (engine/tools/lysh/src/noise_table.h:7-67); the identifier
+// data and must equal the reference's table (engine/tools/lysh/src/noise_table.h:7-67, and :77, :81, :85 for
+// the biome stage's vegetation and offset noises); the identifier
 // strings are hashed with MD5 in the seed derivation and must be spelled exactly.
 #pragma once
 
@@ -17,7 +18,8 @@ struct NoiseParam {
 enum NoiseId {
     N_TEMPERATURE = 0, N_CONTINENTALNESS, N_EROSION, N_TEMPERATURE_LARGE, N_CONTINENTALNESS_LARGE,
     N_EROSION_LARGE, N_RIDGE, N_FLOODEDNESS, N_CAVE_ENTRANCE, N_SPAG3D_RARITY, N_SPAG3D_THICKNESS,
-    N_SPAG3D_1, N_SPAG3D_2, N_SPAG_ROUGH_MOD, N_SPAG_ROUGH, N_CAVE_LAYER, N_CAVE_CHEESE, N_PARAM_COUNT
+    N_SPAG3D_1, N_SPAG3D_2, N_SPAG_ROUGH_MOD, N_SPAG_ROUGH, N_CAVE_LAYER, N_CAVE_CHEESE, N_VEGETATION,
+    N_VEGETATION_LARGE, N_OFFSET, N_PARAM_COUNT
 };
 
 static const NoiseParam kNoise[N_PARAM_COUNT] = {
@@ -38,6 +40,9 @@ static const NoiseParam kNoise[N_PARAM_COUNT] = {
     {"minecraft:spaghetti_roughness", -5, 1, {1.0}},
     {"minecraft:cave_layer", -8, 1, {1.0}},
     {"minecraft:cave_cheese", -8, 9, {0.5, 1.0, 2.0, 1.0, 2.0, 1.0, 0.0, 2.0, 0.0}},
+    {"minecraft:vegetation", -8, 6, {1.0, 1.0, 0.0, 0.0, 0.0, 0.0}},
+    {"minecraft:vegetation_large", -10, 6, {1.0, 1.0, 0.0, 0.0, 0.0, 0.0}},
+    {"minecraft:offset", -3, 4, {1.0, 1.0, 1.0, 0.0}},
 };
 
 // Sampler sets. A set is a list of (noise, sub-sampler, octave slot) Perlin samplers built together for a

@@ -23,8 +23,10 @@ CFG_PROFILE = 1
 # Exported symbols of include/whf.h, checked by the tests.
 SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags", "whf_strerror",
            "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_room_floors",
-           "whf_server_open", "whf_server_close", "whf_submit", "whf_submit_deepdive", "whf_poll")
+           "whf_server_open", "whf_server_close", "whf_submit", "whf_submit_deepdive", "whf_poll",
+           "whf_biome_tree", "whf_biome")
 HIT_DEEPDIVE = 1
+BIOME_SWAMP, BIOME_QY0, BIOME_QY_N = 6, -17, 97
 
 
 class Config(ctypes.Structure):
@@ -81,6 +83,17 @@ class DdStats(ctypes.Structure):
                 ("funnel", ctypes.c_uint64 * KIND_N), ("tiers", ctypes.c_uint64 * 8),
                 ("samplers_built", ctypes.c_uint64 * 3), ("device_ms", ctypes.c_double),
                 ("stage_ms", ctypes.c_double * 3)]
+
+
+class BiomeQuery(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int64), ("hut_x", ctypes.c_int32), ("hut_z", ctypes.c_int32),
+                ("occ_y", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class BiomeResult(ctypes.Structure):
+    _fields_ = [("target", ctypes.c_int64 * 6)] + \
+               [(n, ctypes.c_int32) for n in ("qx", "qy", "qz", "occ_y", "winner", "ok")] + \
+               [("swamp_qy", ctypes.c_uint64 * 2)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -146,6 +159,11 @@ def load():
         L.whf_room_floors.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                       ctypes.POINTER(ctypes.c_int32)]
+        L.whf_biome_tree.restype = ctypes.c_int
+        L.whf_biome_tree.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
+        L.whf_biome.restype = ctypes.c_int
+        L.whf_biome.argtypes = [ctypes.c_void_p, ctypes.POINTER(BiomeQuery), ctypes.c_uint64, ctypes.c_int,
+                                ctypes.c_int, ctypes.POINTER(BiomeResult), ctypes.POINTER(ctypes.c_uint8)]
         _lib = L
     return _lib
 
@@ -289,6 +307,39 @@ class Context:
         if rc:
             raise WhfError(rc, "whf_room_floors")
         return st, fl, pc
+
+
+    def biome(self, queries, tree, vflags=0, winners=False):
+        """Swamp-hut biome checks of (seed, hut_x, hut_z, occ_y) queries against biome tree `tree`: a list of result
+        dicts (targets, quart position, winner, ok, swamp quart heights as a 97-bit int), and with winners=True
+        also the winner at every quart height, an (n, 97) uint8 array."""
+        n = len(queries)
+        q = (BiomeQuery * max(n, 1))()
+        for i, (seed, hx, hz, occ) in enumerate(queries):
+            q[i].seed = _s64(seed)
+            q[i].hut_x, q[i].hut_z, q[i].occ_y = hx, hz, occ
+        res = (BiomeResult * max(n, 1))()
+        win = np.zeros((max(n, 1), BIOME_QY_N), np.uint8) if winners else None
+        rc = self.L.whf_biome(self.h, q, n, tree, vflags, res,
+                              win.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)) if winners else None)
+        if rc:
+            raise WhfError(rc, "whf_biome")
+        out = [dict(target=list(r.target), qx=r.qx, qy=r.qy, qz=r.qz, occ_y=r.occ_y, winner=r.winner, ok=r.ok,
+                    swamp_qy=r.swamp_qy[0] | r.swamp_qy[1] << 64) for r in res[:n]]
+        return (out, win[:n]) if winners else out
+
+
+def biome_tree(version):
+    t = ctypes.c_int(0)
+    rc = load().whf_biome_tree(version.encode(), ctypes.byref(t))
+    if rc:
+        raise WhfError(rc, "whf_biome_tree")
+    return t.value
+
+
+def _s64(v):
+    v &= 0xFFFFFFFFFFFFFFFF
+    return v - (1 << 64) if v >= 1 << 63 else v
 
 
 def make_request(rect, seeds=None, splitmix=None, counter=None, max_height=-50, vflags=0, salt=0,

@@ -164,6 +164,48 @@ int wo_deepdive_sweep(uint64_t s_first, uint64_t count, int rx0, int rx1, int rz
                       int abort_below, int threads, wo_dd_stats *out, wo_dd_hit *hits, int64_t cap,
                       int64_t *n_hits);
 
+/* ---- biome stage (whf_oracle_biome.c) ---- */
+typedef struct {
+    wo_noise offset, continentalness, erosion, ridge, temperature, vegetation;
+} wo_biome;
+
+typedef struct {
+    double shifted_x, shifted_z;
+    double continentalness, weirdness, erosion;
+} wo_terrain_info;
+
+typedef struct {
+    int64_t t[6];          /* quantized temperature, humidity, continentalness, erosion, depth, weirdness */
+    int qx, qy, qz, occ_y;
+    int winner, ok;
+} wo_biome_result;
+
+/* Quart heights an occupied height can fall in: the surface height is -64..320 (phase2.c:318-334), the
+ * occupied height one less, qy = occ_y >> 2 in [-17, 79]. */
+#define WO_BIOME_QY0  (-17)
+#define WO_BIOME_QY_N 97
+
+void    wo_biome_init(wo_biome *b, uint64_t world_seed, int vflags);
+void    wo_terrain_info_at(const wo_biome *b, int qx, int qz, wo_terrain_info *out);   /* terrain.c:22-44 */
+float   wo_spline_apply(int node, const float p[4]);                                   /* spline.c:28-62 */
+float   wo_normalized_weirdness(float w);                                              /* spline.c:63-70 */
+void    wo_terrain_spline(float c, float e, float w, float out[3]);                     /* spline.c:83-111 */
+double  wo_overworld_offset(const wo_biome *b, int block_x, int block_z);               /* column_top.c:124-130 */
+double  wo_depth_gradient(double y);                                                   /* biome.c:77-80 */
+int64_t wo_tree_dist(int tree, int node, const int64_t np[6]);                         /* biome.c:23-31 */
+int     wo_tree_leaf(int tree, const int64_t np[6]);                                   /* biome.c:34-47 */
+int     wo_tree_biome(int tree, const int64_t np[6]);                                  /* biome.c:49-53 */
+int64_t wo_tree_min_dist(int tree, const int64_t np[6], int *n_min);                   /* brute force */
+/* The swamp-hut biome check at the hut chunk's middle for an occupied height (biome.c:94-156). */
+int     wo_biome_check(const wo_biome *b, int tree, int hut_x, int hut_z, int occ_y, wo_biome_result *out);
+/* The tree's winner at every quart height qy = WO_BIOME_QY0 + i. */
+void    wo_biome_heights(const wo_biome *b, int tree, int hut_x, int hut_z, uint8_t winners[WO_BIOME_QY_N]);
+/* Version name -> biome tree (main.c:104-125, eval.c:104-106). Returns 0 when known. */
+int     wo_biome_tree(const char *version, int *tree);
+/* shifted x, z, continentalness, weirdness, erosion, temperature, humidity, overworld offset at a quart point */
+void    wo_biome_point(const wo_biome *b, int qx, int qz, double out[8]);
+void   *wo_biome_alloc(uint64_t world_seed, int vflags);   /* free with wo_probe_free */
+
 /* Heap helpers for bindings: a phase-1 noise set or a single noise for a world seed (free with wo_probe_free). */
 void  *wo_probe_alloc(uint64_t world_seed, int vflags);
 void  *wo_noise_alloc(uint64_t world_seed, int noise_id);

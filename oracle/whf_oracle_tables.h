@@ -1,10 +1,12 @@
-/* whf_oracle_tables.h — the game's noise parameters used by the phase-1 Seed Check (oracle copy).
+/* whf_oracle_tables.h — the game's noise parameters used by the phase-1 Seed Check and the biome stage
+ * (oracle copy).
  *
  * TEST INFRASTRUCTURE: part of the CPU oracle; never linked into the HIP product path.
  *
  * Each entry is Minecraft's NormalNoise parameter set for one identifier: the first octave and the
  * amplitude list. They are game data, not code, so the values must be identical to the reference's
- * table (engine/tools/lysh/src/noise_table.h:7-67, rows "minecraft:temperature" .. "minecraft:cave_cheese";
+ * table (engine/tools/lysh/src/noise_table.h:7-67, rows "minecraft:temperature" .. "minecraft:cave_cheese", and
+ * "minecraft:offset" at noise_table.h:85 with its amplitudes LYSH_AMPS_9 at :16;
  * the identifier strings feed the MD5-keyed seed derivation and must be spelled exactly).
  */
 #ifndef WHF_ORACLE_TABLES_H
@@ -37,6 +39,7 @@ enum {
     WO_N_SPAG_ROUGH,
     WO_N_CAVE_LAYER,
     WO_N_CAVE_CHEESE,
+    WO_N_OFFSET,
     WO_N_COUNT
 };
 
@@ -60,6 +63,7 @@ static const wo_noise_param WO_NOISE[WO_N_COUNT] = {
     { "minecraft:spaghetti_roughness",             -5, 1, { 1.0 } },
     { "minecraft:cave_layer",                      -8, 1, { 1.0 } },
     { "minecraft:cave_cheese",                     -8, 9, { 0.5, 1.0, 2.0, 1.0, 2.0, 1.0, 0.0, 2.0, 0.0 } },
+    { "minecraft:offset",                          -3, 4, { 1.0, 1.0, 1.0, 0.0 } },
 };
 
 #endif /* WHF_ORACLE_TABLES_H */

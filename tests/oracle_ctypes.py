@@ -17,7 +17,8 @@ V_118, V_262, V_LARGE, V_SINGLE = 1, 2, 4, 8
 
 (N_TEMPERATURE, N_VEGETATION, N_CONTINENTALNESS, N_EROSION, N_TEMPERATURE_LARGE, N_VEGETATION_LARGE,
  N_CONTINENTALNESS_LARGE, N_EROSION_LARGE, N_RIDGE, N_FLOODEDNESS, N_CAVE_ENTRANCE, N_SPAG3D_RARITY,
- N_SPAG3D_THICKNESS, N_SPAG3D_1, N_SPAG3D_2, N_SPAG_ROUGH_MOD, N_SPAG_ROUGH, N_CAVE_LAYER, N_CAVE_CHEESE) = range(19)
+ N_SPAG3D_THICKNESS, N_SPAG3D_1, N_SPAG3D_2, N_SPAG_ROUGH_MOD, N_SPAG_ROUGH, N_CAVE_LAYER, N_CAVE_CHEESE,
+ N_OFFSET) = range(20)
 
 
 class Funnel(ctypes.Structure):
@@ -48,6 +49,10 @@ class DdStats(ctypes.Structure):
 class DdHit(ctypes.Structure):
     _fields_ = [("s48", ctypes.c_uint64)] + [(n, ctypes.c_int32) for n in ("rx", "rz", "cx", "cz")] + \
                [("carver", ctypes.c_uint64), ("room_floor", ctypes.c_int32), ("pieces", ctypes.c_int32)]
+
+
+class BiomeResult(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_int64 * 6)] + [(n, ctypes.c_int) for n in ("qx", "qy", "qz", "occ_y", "winner", "ok")]
 
 
 class XR(ctypes.Structure):
@@ -119,6 +124,26 @@ def lib():
         L.wo_deepdive_sweep.restype = i32
         L.wo_deepdive_sweep.argtypes = [u64, u64, i32, i32, i32, i32, i32, i32, i32, ctypes.POINTER(DdStats),
                                         ctypes.POINTER(DdHit), i64, ctypes.POINTER(i64)]
+        L.wo_biome_alloc.restype = ctypes.c_void_p
+        L.wo_biome_alloc.argtypes = [u64, i32]
+        L.wo_biome_point.argtypes = [ctypes.c_void_p, i32, i32, ctypes.POINTER(dbl)]
+        L.wo_biome_check.restype = i32
+        L.wo_biome_check.argtypes = [ctypes.c_void_p, i32, i32, i32, i32, ctypes.POINTER(BiomeResult)]
+        L.wo_biome_heights.argtypes = [ctypes.c_void_p, i32, i32, i32, ctypes.c_char_p]
+        L.wo_biome_tree.restype = i32
+        L.wo_biome_tree.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]
+        L.wo_terrain_spline.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]
+        L.wo_depth_gradient.restype = dbl
+        L.wo_depth_gradient.argtypes = [dbl]
+        i64a = ctypes.POINTER(i64)
+        L.wo_tree_dist.restype = i64
+        L.wo_tree_dist.argtypes = [i32, i32, i64a]
+        L.wo_tree_leaf.restype = i32
+        L.wo_tree_leaf.argtypes = [i32, i64a]
+        L.wo_tree_biome.restype = i32
+        L.wo_tree_biome.argtypes = [i32, i64a]
+        L.wo_tree_min_dist.restype = i64
+        L.wo_tree_min_dist.argtypes = [i32, i64a, ctypes.POINTER(i32)]
         _lib = L
     return _lib
 
@@ -268,3 +293,73 @@ def deepdive_sweep(s_first, count, rect, floor_max=-55, cut=NO_CUT, threads=8, c
                    for h in hits[:min(n.value, cap)])
     return dict(structure_seeds=st.structure_seeds, tests=st.tests, starts=st.starts, survivors=st.survivors,
                 status=list(st.status), draws=st.draws, pieces=st.pieces, spots=spots)
+
+
+# ---- biome stage (whf_oracle_biome.c) ----
+BIOME_QY0, BIOME_QY_N = -17, 97
+SWAMP = 6
+
+
+def _np6(t):
+    return (ctypes.c_int64 * 6)(*[int(v) for v in t])
+
+
+class Biome:
+    """The six climate noises of a world seed, for the biome check."""
+
+    def __init__(self, seed, vflags=0):
+        self.h = lib().wo_biome_alloc(u64(seed), vflags)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().wo_probe_free(self.h)
+
+    def point(self, qx, qz):
+        out = (ctypes.c_double * 8)()
+        lib().wo_biome_point(self.h, qx, qz, out)
+        return list(out)
+
+    def check(self, tree, hut_x, hut_z, occ_y):
+        r = BiomeResult()
+        lib().wo_biome_check(self.h, tree, hut_x, hut_z, occ_y, ctypes.byref(r))
+        return dict(t=list(r.t), qx=r.qx, qy=r.qy, qz=r.qz, occ_y=r.occ_y, winner=r.winner, ok=r.ok)
+
+    def heights(self, tree, hut_x, hut_z):
+        buf = ctypes.create_string_buffer(BIOME_QY_N)
+        lib().wo_biome_heights(self.h, tree, hut_x, hut_z, buf)
+        return list(buf.raw)
+
+
+def biome_tree(version):
+    t = ctypes.c_int(0)
+    if lib().wo_biome_tree(version.encode(), ctypes.byref(t)):
+        raise ValueError(version)
+    return t.value
+
+
+def terrain_spline(c, e, w):
+    out = (ctypes.c_float * 3)()
+    lib().wo_terrain_spline(c, e, w, out)
+    return list(out)
+
+
+def depth_gradient(y):
+    return lib().wo_depth_gradient(y)
+
+
+def tree_leaf(tree, t):
+    return lib().wo_tree_leaf(tree, _np6(t))
+
+
+def tree_biome(tree, t):
+    return lib().wo_tree_biome(tree, _np6(t))
+
+
+def tree_dist(tree, node, t):
+    return lib().wo_tree_dist(tree, node, _np6(t))
+
+
+def tree_min_dist(tree, t):
+    n = ctypes.c_int(0)
+    d = lib().wo_tree_min_dist(tree, _np6(t), ctypes.byref(n))
+    return d, n.value
