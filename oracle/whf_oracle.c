@@ -6,6 +6,7 @@
  * -ffp-contract=off, every expression keeps the reference's association order, x/s stays a division.
  */
 #include "whf_oracle.h"
+#include "whf_oracle_int.h"
 
 #include <math.h>
 #include <pthread.h>
@@ -94,7 +95,7 @@ static uint64_t stafford13(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-static void xr_set(wo_xr *s, uint64_t lo, uint64_t hi) {
+void wo_xr_set(wo_xr *s, uint64_t lo, uint64_t hi) {
     /* two-long constructor; an all-zero state is replaced by the golden-ratio pair (rng.c:104-111) */
     if ((lo | hi) == 0) { lo = UINT64_C(0x9E3779B97F4A7C15); hi = UINT64_C(0x6A09E667F3BCC909); }
     s->lo = lo;
@@ -140,17 +141,17 @@ static uint64_t be64(const uint8_t *p) {
 }
 
 /* createRandom(name) of a deriver: MD5(name) split into two big-endian longs XORed into the state (rng.c:164-170) */
-static void xr_fork_named(wo_xr *out, const wo_xr *deriver, const char *name) {
+void wo_xr_fork_named(wo_xr *out, const wo_xr *deriver, const char *name) {
     uint8_t h[16];
     wo_md5((const uint8_t *)name, strlen(name), h);
-    xr_set(out, be64(h) ^ deriver->lo, be64(h + 8) ^ deriver->hi);
+    wo_xr_set(out, be64(h) ^ deriver->lo, be64(h + 8) ^ deriver->hi);
 }
 
 /* createRandomDeriver(): two longs of the parent become the new state (rng.c:157-161) */
-static void xr_fork(wo_xr *out, wo_xr *parent) {
+void wo_xr_fork(wo_xr *out, wo_xr *parent) {
     uint64_t a = wo_xr_next(parent);
     uint64_t b = wo_xr_next(parent);
-    xr_set(out, a, b);
+    wo_xr_set(out, a, b);
 }
 
 /* ======================================================================== java.util.Random (rng.c:187-227) */
@@ -199,7 +200,7 @@ static const int GRAD[16][3] = {
     {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1}, {1, 1, 0}, {0, -1, 1}, {-1, 1, 0}, {0, -1, -1},
 };
 
-static void perlin_build(wo_perlin *p, wo_xr *r) {
+void wo_perlin_build(wo_perlin *p, wo_xr *r) {
     p->ox = wo_xr_next_double(r) * 256.0;
     p->oy = wo_xr_next_double(r) * 256.0;
     p->oz = wo_xr_next_double(r) * 256.0;
@@ -220,7 +221,7 @@ static inline double dotg(int h, double x, double y, double z) {
 }
 
 /* Improved-noise corner blend with the y fade supplied separately (noise.c:224-244). */
-static double perlin_core(const wo_perlin *p, int i, int j, int k, double x, double y, double z, double ty) {
+double wo_perlin_core(const wo_perlin *p, int i, int j, int k, double x, double y, double z, double ty) {
     const uint8_t *P = p->perm;
     int a = P[i & 255], b = P[(i + 1) & 255];
     int aa = P[(a + j) & 255], ab = P[(a + j + 1) & 255];
@@ -244,11 +245,11 @@ static double perlin_sample3(const wo_perlin *p, double x, double y, double z) {
     double d = x + p->ox, e = y + p->oy, f = z + p->oz;
     int i = (int)floor(d), j = (int)floor(e), k = (int)floor(f);
     double h = e - j;
-    return perlin_core(p, i, j, k, d - i, h, f - k, h);
+    return wo_perlin_core(p, i, j, k, d - i, h, f - k, h);
 }
 
 /* OctavePerlinNoiseSampler.maintainPrecision (noise.c:28-34) */
-static double keep_precision(double v) {
+double wo_keep_precision(double v) {
     return v - (double)(long)floor(v / 33554432.0 + 0.5) * 33554432.0;
 }
 
@@ -263,15 +264,15 @@ static void octave_build(wo_octave *o, wo_xr *noise_rng, const wo_noise_param *n
     o->lac0 = ldexp(1.0, np->first_octave);
     o->per0 = ldexp(1.0, np->n - 1) / (ldexp(1.0, np->n) - 1.0);
     wo_xr der;
-    xr_fork(&der, noise_rng);
+    wo_xr_fork(&der, noise_rng);
     for (int i = 0; i < np->n; i++) {
         o->amp[i] = np->amps[i];
         if (np->amps[i] == 0.0) continue;
         char name[24];
         snprintf(name, sizeof name, "octave_%d", np->first_octave + i);
         wo_xr r;
-        xr_fork_named(&r, &der, name);
-        perlin_build(&o->oct[i], &r);
+        wo_xr_fork_named(&r, &der, name);
+        wo_perlin_build(&o->oct[i], &r);
         o->present[i] = 1;
     }
 }
@@ -283,8 +284,8 @@ void wo_noise_init(wo_noise *n, uint64_t world_seed, int noise_id) {
     const wo_noise_param *np = &WO_NOISE[noise_id];
     wo_xr r, der, nr;
     wo_xr_seed(&r, world_seed);
-    xr_fork(&der, &r);
-    xr_fork_named(&nr, &der, np->name);
+    wo_xr_fork(&der, &r);
+    wo_xr_fork_named(&nr, &der, np->name);
     octave_build(&n->sub[0], &nr, np);
     octave_build(&n->sub[1], &nr, np);
     int first = -1, last = -1;
@@ -302,9 +303,9 @@ static double octave_sample3(const wo_octave *o, double x, double y, double z) {
     for (int i = 0; i < o->n; i++) {
         if (o->present[i]) {
             double xs = x * lac, ys = y * lac, zs = z * lac;
-            if (!(xs > -1.0e7 && xs < 1.0e7)) xs = keep_precision(xs);
-            if (!(ys > -1.0e7 && ys < 1.0e7)) ys = keep_precision(ys);
-            if (!(zs > -1.0e7 && zs < 1.0e7)) zs = keep_precision(zs);
+            if (!(xs > -1.0e7 && xs < 1.0e7)) xs = wo_keep_precision(xs);
+            if (!(ys > -1.0e7 && ys < 1.0e7)) ys = wo_keep_precision(ys);
+            if (!(zs > -1.0e7 && zs < 1.0e7)) zs = wo_keep_precision(zs);
             v += (o->amp[i] * perlin_sample3(&o->oct[i], xs, ys, zs)) * per;
         }
         lac *= 2.0;
@@ -343,8 +344,8 @@ int wo_erosion_tiered(const wo_noise *n, double x, double z, double *value, int 
             const wo_octave *o = s ? B : A;
             double sc = s ? WO_SECOND_SCALE : 1.0;
             double xs = x * sc * lac, zs = z * sc * lac;
-            if (!(xs > -1.0e7 && xs < 1.0e7)) xs = keep_precision(xs);
-            if (!(zs > -1.0e7 && zs < 1.0e7)) zs = keep_precision(zs);
+            if (!(xs > -1.0e7 && xs < 1.0e7)) xs = wo_keep_precision(xs);
+            if (!(zs > -1.0e7 && zs < 1.0e7)) zs = wo_keep_precision(zs);
             double w = o->amp[i] * per;
             sum += w * perlin_sample3(&o->oct[i], xs, 0.0, zs);
             if (k >= 1 && t < 7) {

@@ -206,6 +206,33 @@ int     wo_biome_tree(const char *version, int *tree);
 void    wo_biome_point(const wo_biome *b, int qx, int qz, double out[8]);
 void   *wo_biome_alloc(uint64_t world_seed, int vflags);   /* free with wo_probe_free */
 
+/* ---- terrain stage (whf_oracle_p2.c) ---- */
+#define WO_P2_NOISES 25
+typedef struct {
+    wo_biome bio;                 /* offset, continentalness, erosion, ridge (+ temperature, vegetation) */
+    wo_noise n[WO_P2_NOISES];     /* density, noodle and aquifer noises, in the order wo_p2_init lists them */
+    wo_perlin lower[16], upper[16], main[8];   /* blended noise stacks, by octave (0 = the coarsest) */
+    uint64_t aq_lo, aq_hi;        /* the aquifer's positional random factory */
+} wo_phase2;
+
+void   wo_p2_init(wo_phase2 *p, uint64_t seed, int vflags);
+void  *wo_p2_alloc(uint64_t seed, int vflags);                            /* free with wo_probe_free */
+/* offset, factor, peaks of the terrain point at a quart position, 26.1.2 semantics (terrain.c:33-39) */
+void   wo_p2_terrain_point(const wo_phase2 *p, int qx, int qz, float out[3]);
+double wo_p2_blended(const wo_phase2 *p, int x, int y, int z);            /* interp_noise.c:91-139 */
+double wo_p2_noise_column(const wo_phase2 *p, int x, int y, int z, const float tp[3], double base); /* density.c:228-281 */
+double wo_p2_noodle_child(const wo_phase2 *p, int which, int x, int y, int z);   /* density.c:186-209 */
+/* the 49 corner samples of main density and the three interpolated noodle children (phase2.c:71-106) */
+void   wo_p2_corner_column(const wo_phase2 *p, int cell_x, int cell_z, double main[49], double noodle[3][49]);
+double wo_p2_density(const wo_phase2 *p, int x, int y, int z);            /* phase2.c:161-194 */
+int    wo_p2_prelim_surface(const wo_phase2 *p, int x, int z);            /* column_top.c:85-122 */
+int    wo_p2_skip_above(const wo_phase2 *p, int chunk_x, int chunk_z);    /* aquifer.c:160-196 */
+void   wo_p2_fluid(const wo_phase2 *p, int x, int y, int z, int *level, int *type);   /* aquifer.c:286-331 */
+/* block state 0 solid, 1 air, 2 water, 3 lava for a block of the density (aquifer.c:416-563) */
+int    wo_p2_substance(const wo_phase2 *p, int x, int y, int z, double density);
+/* pre-carver surface height (phase2.c:318-334); *top_state = state of the block below it */
+int    wo_p2_height(const wo_phase2 *p, int x, int z, int *top_state);
+
 /* Heap helpers for bindings: a phase-1 noise set or a single noise for a world seed (free with wo_probe_free). */
 void  *wo_probe_alloc(uint64_t world_seed, int vflags);
 void  *wo_noise_alloc(uint64_t world_seed, int noise_id);

@@ -6,7 +6,8 @@
  * Each entry is Minecraft's NormalNoise parameter set for one identifier: the first octave and the
  * amplitude list. They are game data, not code, so the values must be identical to the reference's
  * table (engine/tools/lysh/src/noise_table.h:7-67, rows "minecraft:temperature" .. "minecraft:cave_cheese", and
- * "minecraft:offset" at noise_table.h:85 with its amplitudes LYSH_AMPS_9 at :16;
+ * "minecraft:offset" at noise_table.h:85 with its amplitudes LYSH_AMPS_9 at :16, and the terrain stage's noises at
+ * :86-114 (jagged, spaghetti_2d*, pillar*, noodle*, aquifer_barrier/_fluid_level_spread/_lava);
  * the identifier strings feed the MD5-keyed seed derivation and must be spelled exactly).
  */
 #ifndef WHF_ORACLE_TABLES_H
@@ -16,7 +17,7 @@ typedef struct {
     const char *name;
     int first_octave;
     int n;
-    double amps[9];
+    double amps[16];
 } wo_noise_param;
 
 enum {
@@ -40,6 +41,21 @@ enum {
     WO_N_CAVE_LAYER,
     WO_N_CAVE_CHEESE,
     WO_N_OFFSET,
+    WO_N_JAGGED,
+    WO_N_SPAG2D,
+    WO_N_SPAG2D_ELEVATION,
+    WO_N_SPAG2D_MODULATOR,
+    WO_N_SPAG2D_THICKNESS,
+    WO_N_PILLAR,
+    WO_N_PILLAR_RARENESS,
+    WO_N_PILLAR_THICKNESS,
+    WO_N_NOODLE,
+    WO_N_NOODLE_THICKNESS,
+    WO_N_NOODLE_RIDGE_A,
+    WO_N_NOODLE_RIDGE_B,
+    WO_N_AQUIFER_BARRIER,
+    WO_N_AQUIFER_SPREAD,
+    WO_N_AQUIFER_LAVA,
     WO_N_COUNT
 };
 
@@ -64,6 +80,22 @@ static const wo_noise_param WO_NOISE[WO_N_COUNT] = {
     { "minecraft:cave_layer",                      -8, 1, { 1.0 } },
     { "minecraft:cave_cheese",                     -8, 9, { 0.5, 1.0, 2.0, 1.0, 2.0, 1.0, 0.0, 2.0, 0.0 } },
     { "minecraft:offset",                          -3, 4, { 1.0, 1.0, 1.0, 0.0 } },
+    { "minecraft:jagged",                         -16, 16, { 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0,
+                                                             1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0 } },
+    { "minecraft:spaghetti_2d",                    -7, 1, { 1.0 } },
+    { "minecraft:spaghetti_2d_elevation",          -8, 1, { 1.0 } },
+    { "minecraft:spaghetti_2d_modulator",         -11, 1, { 1.0 } },
+    { "minecraft:spaghetti_2d_thickness",         -11, 1, { 1.0 } },
+    { "minecraft:pillar",                          -7, 2, { 1.0, 1.0 } },
+    { "minecraft:pillar_rareness",                 -8, 1, { 1.0 } },
+    { "minecraft:pillar_thickness",                -8, 1, { 1.0 } },
+    { "minecraft:noodle",                          -8, 1, { 1.0 } },
+    { "minecraft:noodle_thickness",                -8, 1, { 1.0 } },
+    { "minecraft:noodle_ridge_a",                  -7, 1, { 1.0 } },
+    { "minecraft:noodle_ridge_b",                  -7, 1, { 1.0 } },
+    { "minecraft:aquifer_barrier",                 -3, 1, { 1.0 } },
+    { "minecraft:aquifer_fluid_level_spread",      -5, 1, { 1.0 } },
+    { "minecraft:aquifer_lava",                    -1, 1, { 1.0 } },
 };
 
 #endif /* WHF_ORACLE_TABLES_H */

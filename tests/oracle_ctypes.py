@@ -144,6 +144,25 @@ def lib():
         L.wo_tree_biome.argtypes = [i32, i64a]
         L.wo_tree_min_dist.restype = i64
         L.wo_tree_min_dist.argtypes = [i32, i64a, ctypes.POINTER(i32)]
+        L.wo_p2_alloc.restype = ctypes.c_void_p
+        L.wo_p2_alloc.argtypes = [u64, i32]
+        L.wo_p2_terrain_point.argtypes = [ctypes.c_void_p, i32, i32, ctypes.POINTER(ctypes.c_float)]
+        L.wo_p2_blended.restype = dbl
+        L.wo_p2_blended.argtypes = [ctypes.c_void_p, i32, i32, i32]
+        L.wo_p2_noodle_child.restype = dbl
+        L.wo_p2_noodle_child.argtypes = [ctypes.c_void_p, i32, i32, i32, i32]
+        L.wo_p2_corner_column.argtypes = [ctypes.c_void_p, i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+        L.wo_p2_density.restype = dbl
+        L.wo_p2_density.argtypes = [ctypes.c_void_p, i32, i32, i32]
+        L.wo_p2_prelim_surface.restype = i32
+        L.wo_p2_prelim_surface.argtypes = [ctypes.c_void_p, i32, i32]
+        L.wo_p2_skip_above.restype = i32
+        L.wo_p2_skip_above.argtypes = [ctypes.c_void_p, i32, i32]
+        L.wo_p2_fluid.argtypes = [ctypes.c_void_p, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.wo_p2_substance.restype = i32
+        L.wo_p2_substance.argtypes = [ctypes.c_void_p, i32, i32, i32, dbl]
+        L.wo_p2_height.restype = i32
+        L.wo_p2_height.argtypes = [ctypes.c_void_p, i32, i32, ctypes.POINTER(i32)]
         _lib = L
     return _lib
 
@@ -363,3 +382,57 @@ def tree_min_dist(tree, t):
     n = ctypes.c_int(0)
     d = lib().wo_tree_min_dist(tree, _np6(t), ctypes.byref(n))
     return d, n.value
+
+
+# ---- terrain stage (whf_oracle_p2.c) ----
+SOLID, AIR, WATER, LAVA = 0, 1, 2, 3
+
+
+class Terrain:
+    """The phase-2 noises of a world seed: final density, preliminary surface, aquifer, surface height."""
+
+    def __init__(self, seed, vflags=0):
+        self.h = lib().wo_p2_alloc(u64(seed), vflags)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().wo_probe_free(self.h)
+
+    def point(self, qx, qz):
+        out = (ctypes.c_float * 3)()
+        lib().wo_p2_terrain_point(self.h, qx, qz, out)
+        return list(out)
+
+    def blended(self, x, y, z):
+        return lib().wo_p2_blended(self.h, x, y, z)
+
+    def noodle(self, which, x, y, z):
+        return lib().wo_p2_noodle_child(self.h, which, x, y, z)
+
+    def corner_column(self, cx, cz):
+        main = (ctypes.c_double * 49)()
+        nood = (ctypes.c_double * 147)()
+        lib().wo_p2_corner_column(self.h, cx, cz, main, nood)
+        return list(main), [list(nood[49 * c:49 * c + 49]) for c in range(3)]
+
+    def density(self, x, y, z):
+        return lib().wo_p2_density(self.h, x, y, z)
+
+    def prelim_surface(self, x, z):
+        return lib().wo_p2_prelim_surface(self.h, x, z)
+
+    def skip_above(self, cx, cz):
+        return lib().wo_p2_skip_above(self.h, cx, cz)
+
+    def fluid(self, x, y, z):
+        lv, ty = ctypes.c_int(0), ctypes.c_int(0)
+        lib().wo_p2_fluid(self.h, x, y, z, ctypes.byref(lv), ctypes.byref(ty))
+        return lv.value, ty.value
+
+    def substance(self, x, y, z, density):
+        return lib().wo_p2_substance(self.h, x, y, z, density)
+
+    def height(self, x, z):
+        st = ctypes.c_int(0)
+        h = lib().wo_p2_height(self.h, x, z, ctypes.byref(st))
+        return h, st.value
