@@ -192,11 +192,13 @@ int whf_room_floors(whf_ctx *ctx, const uint64_t *carvers, uint64_t n, int32_t *
 #define WHF_BIOME_SWAMP 6     /* the trees' biome id of minecraft:swamp (biome.h:77) */
 #define WHF_BIOME_QY0   (-17) /* quart heights an occupied height can fall in: -17 .. 79 (surface -64 .. 320) */
 #define WHF_BIOME_QY_N  97
+#define WHF_OCC_SURFACE (-2147483647 - 1)   /* whf_biome_query.occ_y: use the pre-carver surface at the chunk middle */
 
 typedef struct {
     int64_t seed;               /* world seed */
     int32_t hut_x, hut_z;       /* a block of the hut's chunk (16*cx, 16*cz); the check samples the chunk's middle */
-    int32_t occ_y;              /* first occupied height at the chunk middle: its WORLD_SURFACE_WG height - 1 */
+    int32_t occ_y;              /* first occupied height at the chunk middle: its WORLD_SURFACE_WG height - 1;
+                                   WHF_OCC_SURFACE: computed here from the pre-carver surface (whf_surface_heights) */
     int32_t reserved;
 } whf_biome_query;
 
@@ -223,6 +225,28 @@ int whf_biome_tree(const char *version, int *tree);
  * bits are ignored. Not re-entrant per context. */
 int whf_biome(whf_ctx *ctx, const whf_biome_query *queries, uint64_t n, int tree, int vflags, whf_biome_result *out,
               uint8_t *winners);
+
+/* ---- Terrain stage: final density and the pre-carver surface height ---- */
+
+/* Block states of a column's top block. */
+#define WHF_BLOCK_SOLID 0
+#define WHF_BLOCK_AIR   1
+#define WHF_BLOCK_WATER 2
+#define WHF_BLOCK_LAVA  3
+
+typedef struct {
+    int64_t seed;               /* world seed */
+    int32_t x, z;               /* block column */
+} whf_column;
+
+/* Pre-carver surface height of each column: one above the first block from y = 319 down that is not air in the
+ * overworld's final density and aquifer (lysh_phase2_height_pre_carver, phase2.c:318-334: the interpolated final
+ * density of phase2.c:161-194 and computeSubstance of aquifer.c:416-563), -64 for a column of air; this is
+ * WORLD_SURFACE_WG before carvers, the height the biome check uses (eval.c:62-67). `state` (may be NULL) receives
+ * the WHF_BLOCK_* state of the top block (water for a sea surface). vflags: WHF_V_LARGE selects the LARGE_BIOMES
+ * noises. Not re-entrant per context. */
+int whf_surface_heights(whf_ctx *ctx, const whf_column *cols, uint64_t n, int vflags, int32_t *height,
+                        int32_t *state);
 
 /* ---- Serving: continuous batching of Seed Check requests ---- */
 

@@ -18,7 +18,7 @@ struct SplineTab {
     whf_gd_spline_node node[128];
     float f[640];
     int32_t child[384];
-    int offset_root;
+    int root[3];            // offset, factor, peaks
 };
 
 __device__ __forceinline__ int64_t quantize(float f) { return (int64_t)(f * 10000.0f); }   // biome.c:71-73

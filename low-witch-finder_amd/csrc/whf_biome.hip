@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64) void k_bio_point(const BioTables *__restrict__ 
     const double hum = noise_y0(bt->noise[B_VEGETATION], sv, slot, x, z);
     // overworld offset at block (4 qx, 4 qz): its terrain point is this quart point's (column_top.c:80-83, 124-130)
     const float cf = (float)cont, ef = (float)ero, wf = (float)weird;
-    const double raw = (double)spline_at<kSplineMaxDepth>(S, S->offset_root, cf, ef, normalized_weirdness(wf), wf);
+    const double raw = (double)spline_at<kSplineMaxDepth>(S, S->root[0], cf, ef, normalized_weirdness(wf), wf);
     const double alpha = 1.0, blend = 0.0;   // blend alpha and offset of an unblended chunk (column_top.c:71-76)
     BioPoint p;
     p.offset = blend * (1.0 + -1.0 * alpha) + (-0.5037500262260437 + raw) * alpha;
@@ -195,7 +195,7 @@ int bio_init(whf_ctx *c, BioBufs &b) {
     std::memcpy(sp.node, whf_gd_spline_nodes, whf_gd_n_spline_nodes * sizeof(whf_gd_spline_node));
     std::memcpy(sp.f, whf_gd_spline_floats, whf_gd_n_spline_floats * sizeof(float));
     std::memcpy(sp.child, whf_gd_spline_child, whf_gd_n_spline_child * sizeof(int32_t));
-    sp.offset_root = whf_gd_spline_root[0];
+    for (int r = 0; r < 3; r++) sp.root[r] = whf_gd_spline_root[r];
     HIP_TRY(hipMemcpyAsync(b.spline, &sp, sizeof sp, hipMemcpyHostToDevice, st));
     BioTables bt[2];
     std::vector<BuildDesc> bd[2];
@@ -245,13 +245,33 @@ int whf_biome(whf_ctx *c, const whf_biome_query *queries, uint64_t n, int tree, 
     std::vector<uint64_t> seeds;
     std::vector<uint32_t> slot;
     std::unordered_map<uint64_t, uint32_t> seen;
+    std::vector<whf_biome_query> qb;
+    std::vector<whf_column> cols;
+    std::vector<int32_t> heights;
+    std::vector<uint32_t> need;
     for (uint64_t q0 = 0; q0 < n; q0 += kBioBatch) {
         const uint32_t nq = (uint32_t)std::min<uint64_t>(kBioBatch, n - q0);
+        // queries that ask for the surface: the pre-carver height at the chunk middle (biome.c:108-114 with the
+        // height callback of eval.c:62-67), occupied height = surface - 1
+        qb.assign(queries + q0, queries + q0 + nq);
+        cols.clear();
+        need.clear();
+        for (uint32_t i = 0; i < nq; i++)
+            if (qb[i].occ_y == WHF_OCC_SURFACE) {
+                cols.push_back(whf_column{qb[i].seed, (qb[i].hut_x >> 4) * 16 + 8, (qb[i].hut_z >> 4) * 16 + 8});
+                need.push_back(i);
+            }
+        if (!cols.empty()) {
+            heights.resize(cols.size());
+            rc = terrain_heights(c, cols.data(), cols.size(), vflags, heights.data(), nullptr);
+            if (rc != WHF_OK) return rc;
+            for (size_t k = 0; k < need.size(); k++) qb[need[k]].occ_y = heights[k] - 1;
+        }
         seeds.clear();
         slot.resize(nq);
         seen.clear();
         for (uint32_t i = 0; i < nq; i++) {
-            const uint64_t s = (uint64_t)queries[q0 + i].seed;
+            const uint64_t s = (uint64_t)qb[i].seed;
             auto it = seen.find(s);
             if (it == seen.end()) {
                 it = seen.emplace(s, (uint32_t)seeds.size()).first;
@@ -261,7 +281,7 @@ int whf_biome(whf_ctx *c, const whf_biome_query *queries, uint64_t n, int tree, 
         }
         const uint32_t ns = (uint32_t)seeds.size();
         HIP_TRY(hipMemcpyAsync(b.seeds, seeds.data(), ns * sizeof(uint64_t), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(b.q, queries + q0, nq * sizeof(whf_biome_query), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(b.q, qb.data(), nq * sizeof(whf_biome_query), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(b.slot, slot.data(), nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
         launch_build(c, b.seeds, b.bd + large * kBioSamplers, kBioSamplers, ns, b.tab, b.par, c->d_cnt + CNT_BIO_BUILT);
         hipLaunchKernelGGL(k_bio_point, dim3((nq + 63) / 64), dim3(64), 0, st, b.tabs + large, b.spline, b.tab, b.par,

@@ -146,6 +146,48 @@ __device__ __forceinline__ double grad(int hash, double x, double y, double z) {
 __device__ __forceinline__ double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
 __device__ __forceinline__ double lerp(double t, double a, double b) { return a + t * (b - a); }
 
+// One Perlin sampler from random pr (PerlinNoiseSampler(random), noise.c:61-78): the origin, then Fisher-Yates over
+// 0..255 with nextInt(256 - i). The permutation is shuffled in the calling lane's column of a 64-lane LDS area laid
+// out lane-interleaved (word w of lane l at words[w * 64 + l]), so the 64 lanes' byte accesses hit 64 distinct
+// banks whatever the swap index; the table then goes to out_tab (256 bytes) and the y constants to out.
+__device__ __forceinline__ void build_sampler(Xr &pr, uint32_t *words, uint32_t lane, uint8_t *out_tab, SPar &out) {
+    uint8_t *bytes = reinterpret_cast<uint8_t *>(words);
+    const double ox = xr_next_double(pr) * 256.0;
+    const double oy = xr_next_double(pr) * 256.0;
+    const double oz = xr_next_double(pr) * 256.0;
+#pragma unroll 8
+    for (int w = 0; w < 64; w++) words[w * 64 + lane] = 0x03020100u + (uint32_t)w * 0x04040404u;
+    uint32_t ai = lane * 4;   // byte address of position i (i = 0)
+    // The draw of step i + 1 is computed between the reads and the writes of step i: it does not depend on the
+    // table, so it fills the LDS read latency.
+    uint32_t jj = xr_next_bounded(pr, 256u);
+    for (int i = 0; i < 256; i++) {
+        const uint32_t aj = ((jj >> 2) << 8) + lane * 4 + (jj & 3);
+        const uint8_t vi = bytes[ai];
+        const uint8_t vj = bytes[aj];
+        if (i < 255) jj = (uint32_t)(i + 1) + xr_next_bounded(pr, 255u - (uint32_t)i);
+        bytes[ai] = vj;
+        bytes[aj] = vi;
+        ai += ((i & 3) == 3) ? 256 - 3 : 1;
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(out_tab);
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        dst[q] = make_uint4(words[(4 * q) * 64 + lane], words[(4 * q + 1) * 64 + lane], words[(4 * q + 2) * 64 + lane],
+                            words[(4 * q + 3) * 64 + lane]);
+    SPar sp;
+    sp.ox = ox;
+    sp.oy = oy;
+    sp.oz = oz;
+    const double fj = floor(oy);
+    sp.jy = (int)fj;
+    sp.hy = oy - fj;
+    sp.hy1 = sp.hy - 1.0;
+    sp.fy = fade(sp.hy);
+    sp.pad = 0;
+    out = sp;
+}
+
 // Corner blend shared by the y = 0 and 3D samplers (noise.c:224-244): P is the permutation, (i, j, k) the
 // lattice cell, (x, y, z) the fractional position and ty the y value whose fade is used.
 template <typename Perm>

@@ -106,6 +106,8 @@ struct whf_ctx {
     size_t dd_bytes = 0;
     void *d_bio = nullptr;                // biome stage buffers and tables (whf_biome.hip), allocated on first use
     size_t bio_bytes = 0;
+    void *d_ter = nullptr;                // terrain stage buffers and tables (whf_terrain.hip), allocated on first use
+    size_t ter_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> prof_ev;  // pairs per kernel launch when profiling
     std::vector<int> prof_fam;
@@ -144,6 +146,8 @@ struct SamplerRef {
 // The MD5-keyed derivation descriptor of a sampler, and the descriptor of a double-Perlin noise over a set
 // (its table indices, lacunarities, amplitudes, persistences and output amplitude, noise.c:80-156).
 BuildDesc make_build_desc(const SamplerRef &r);
+// MD5 of an identifier as two big-endian longs (the createRandom(name) key, rng.c:164-170).
+void md5_longs(const char *name, uint64_t &hi, uint64_t &lo);
 void fill_noise_desc(NoiseDesc &d, int pid, const std::vector<SamplerRef> &set, int set_id);
 
 int check_launch();
@@ -153,6 +157,9 @@ void launch_build(whf_ctx *c, const uint64_t *seeds, const BuildDesc *bd, int se
                   uint8_t *tab, SPar *par, unsigned long long *built);
 // Zero the batch's queue counts and the gate flags of its first ns seeds.
 void launch_reset(whf_ctx *c, uint32_t ns);
+// Pre-carver surface heights of n columns (seed, x, z) and the state of each column's top block (0 solid,
+// 2 water, 3 lava; 1 for a column of air); whf_terrain.hip.
+int terrain_heights(whf_ctx *c, const whf_column *cols, uint64_t n, int vflags, int32_t *height, int32_t *state);
 // Gates 2 and 3 of a batch whose erosion survivors are already in Q1 (with needC set): compact, build set C,
 // gate 2, compact, build set L, gate 3 -> hits.
 int run_tail(whf_ctx *c, int layout, uint32_t ns, int vflags, int max_height);

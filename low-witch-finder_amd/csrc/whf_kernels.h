@@ -20,9 +20,9 @@ struct BuildDesc {
 struct NoiseDesc {
     int set;                 // SET_E / SET_C / SET_L
     int np;                  // non-empty octaves per sub-sampler
-    int tab[2][9];
+    int tab[2][16];
     double amplitude;        // (1/6) / (0.1 * (1 + 1/(span+1)))  (noise.c:116-130)
-    double lac[9], amp[9], per[9];
+    double lac[16], amp[16], per[16];
 };
 
 // Tiered erosion: evaluation k uses set-E table k with coordinate scale sc[k] * lac[k] and weight w[k]

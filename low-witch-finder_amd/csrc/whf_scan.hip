@@ -60,7 +60,6 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
                                               uint8_t *__restrict__ out_tab, SPar *__restrict__ out_par,
                                               unsigned long long *__restrict__ built_cnt) {
     __shared__ uint32_t words[64 * 64];
-    uint8_t *bytes = reinterpret_cast<uint8_t *>(words);
     const uint32_t lane = threadIdx.x;
     // with a device count, n_slots_host caps it at the table capacity
     const uint32_t n_slots = n_slots_dev ? min(*n_slots_dev, n_slots_host) : n_slots_host;
@@ -84,40 +83,7 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
         Xr pr = xr_make(d.oa ^ od.lo, d.ob ^ od.hi);
 
         // PerlinNoiseSampler(random): origin, then Fisher-Yates over 0..255 (noise.c:61-78)
-        const double ox = xr_next_double(pr) * 256.0;
-        const double oy = xr_next_double(pr) * 256.0;
-        const double oz = xr_next_double(pr) * 256.0;
-#pragma unroll 8
-        for (int w = 0; w < 64; w++) words[w * 64 + lane] = 0x03020100u + (uint32_t)w * 0x04040404u;
-        uint32_t ai = lane * 4;   // byte address of position i (i = 0)
-        // The draw of step i + 1 is computed between the reads and the writes of step i: it does not depend
-        // on the table, so it fills the LDS read latency.
-        uint32_t jj = xr_next_bounded(pr, 256u);
-        for (int i = 0; i < 256; i++) {
-            const uint32_t aj = ((jj >> 2) << 8) + lane * 4 + (jj & 3);
-            const uint8_t vi = bytes[ai];
-            const uint8_t vj = bytes[aj];
-            if (i < 255) jj = (uint32_t)(i + 1) + xr_next_bounded(pr, 255u - (uint32_t)i);
-            bytes[ai] = vj;
-            bytes[aj] = vi;
-            ai += ((i & 3) == 3) ? 256 - 3 : 1;
-        }
-        uint4 *dst = reinterpret_cast<uint4 *>(out_tab + j * 256);
-#pragma unroll
-        for (int q = 0; q < 16; q++)
-            dst[q] = make_uint4(words[(4 * q) * 64 + lane], words[(4 * q + 1) * 64 + lane],
-                                words[(4 * q + 2) * 64 + lane], words[(4 * q + 3) * 64 + lane]);
-        SPar sp;
-        sp.ox = ox;
-        sp.oy = oy;
-        sp.oz = oz;
-        const double fj = floor(oy);
-        sp.jy = (int)fj;
-        sp.hy = oy - fj;
-        sp.hy1 = sp.hy - 1.0;
-        sp.fy = fade(sp.hy);
-        sp.pad = 0;
-        out_par[j] = sp;
+        build_sampler(pr, words, lane, out_tab + j * 256, out_par[j]);
         built++;
     }
     // one atomic per wave
@@ -695,6 +661,8 @@ struct Md5 {
 
 namespace whf {
 
+void md5_longs(const char *name, uint64_t &hi, uint64_t &lo) { Md5::digest(name, hi, lo); }
+
 BuildDesc make_build_desc(const SamplerRef &r) {
     BuildDesc b{};
     Md5::digest(kNoise[r.noise].name, b.na, b.nb);
@@ -880,7 +848,7 @@ __attribute__((visibility("hidden"))) uint64_t whf_ctx_device_bytes(const whf_ct
     uint64_t b = B * 8 + 4 * B * 4 + 2 * S * 4 + 2ull * kMaxItems * sizeof(uint4) + (uint64_t)kQxCap * sizeof(uint4) +
                  (uint64_t)kHitCap * sizeof(DevHit) + kLayouts * (sizeof(GateTables) + kBdPerLayout * sizeof(BuildDesc));
     for (int s = 0; s < SET_COUNT; s++) b += S * kSetSize[s] * (256 + sizeof(SPar));
-    return b + (c ? c->dd_bytes + c->bio_bytes : 0);
+    return b + (c ? c->dd_bytes + c->bio_bytes + c->ter_bytes : 0);
 }
 
 namespace whf {
@@ -1074,7 +1042,7 @@ void whf_close(whf_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_seeds, c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_par[0], c->d_par[1], c->d_par[2],
                     c->d_q1, c->d_q2, c->d_qx, c->d_needC, c->d_needL, c->d_slotC, c->d_slotL, c->d_seedC, c->d_seedL,
-                    c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd, c->d_dd, c->d_bio};
+                    c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd, c->d_dd, c->d_bio, c->d_ter};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);

@@ -1,8 +1,8 @@
 // whf_tables.h — game data the phase-1 Seed Check needs, and the sampler-set layouts of the HIP path.
 //
 // The noise parameters are the game's NormalNoise settings: identifier, first octave, amplitudes. They are
-// data and must equal the reference's table (engine/tools/lysh/src/noise_table.h:7-67, and :77, :81, :85 for
-// the biome stage's vegetation and offset noises); the identifier
+// data and must equal the reference's table (engine/tools/lysh/src/noise_table.h:7-67, :77, :81, :85 for the
+// biome stage's vegetation and offset noises, :86-114 for the terrain stage's); the identifier
 // strings are hashed with MD5 in the seed derivation and must be spelled exactly.
 #pragma once
 
@@ -12,14 +12,16 @@ struct NoiseParam {
     const char *name;
     int first_octave;
     int n;
-    double amps[9];
+    double amps[16];
 };
 
 enum NoiseId {
     N_TEMPERATURE = 0, N_CONTINENTALNESS, N_EROSION, N_TEMPERATURE_LARGE, N_CONTINENTALNESS_LARGE,
     N_EROSION_LARGE, N_RIDGE, N_FLOODEDNESS, N_CAVE_ENTRANCE, N_SPAG3D_RARITY, N_SPAG3D_THICKNESS,
     N_SPAG3D_1, N_SPAG3D_2, N_SPAG_ROUGH_MOD, N_SPAG_ROUGH, N_CAVE_LAYER, N_CAVE_CHEESE, N_VEGETATION,
-    N_VEGETATION_LARGE, N_OFFSET, N_PARAM_COUNT
+    N_VEGETATION_LARGE, N_OFFSET, N_JAGGED, N_SPAG2D, N_SPAG2D_ELEVATION, N_SPAG2D_MODULATOR, N_SPAG2D_THICKNESS,
+    N_PILLAR, N_PILLAR_RARENESS, N_PILLAR_THICKNESS, N_NOODLE, N_NOODLE_THICKNESS, N_NOODLE_RIDGE_A, N_NOODLE_RIDGE_B,
+    N_AQUIFER_BARRIER, N_AQUIFER_SPREAD, N_AQUIFER_LAVA, N_PARAM_COUNT
 };
 
 static const NoiseParam kNoise[N_PARAM_COUNT] = {
@@ -43,6 +45,21 @@ static const NoiseParam kNoise[N_PARAM_COUNT] = {
     {"minecraft:vegetation", -8, 6, {1.0, 1.0, 0.0, 0.0, 0.0, 0.0}},
     {"minecraft:vegetation_large", -10, 6, {1.0, 1.0, 0.0, 0.0, 0.0, 0.0}},
     {"minecraft:offset", -3, 4, {1.0, 1.0, 1.0, 0.0}},
+    {"minecraft:jagged", -16, 16, {1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0}},
+    {"minecraft:spaghetti_2d", -7, 1, {1.0}},
+    {"minecraft:spaghetti_2d_elevation", -8, 1, {1.0}},
+    {"minecraft:spaghetti_2d_modulator", -11, 1, {1.0}},
+    {"minecraft:spaghetti_2d_thickness", -11, 1, {1.0}},
+    {"minecraft:pillar", -7, 2, {1.0, 1.0}},
+    {"minecraft:pillar_rareness", -8, 1, {1.0}},
+    {"minecraft:pillar_thickness", -8, 1, {1.0}},
+    {"minecraft:noodle", -8, 1, {1.0}},
+    {"minecraft:noodle_thickness", -8, 1, {1.0}},
+    {"minecraft:noodle_ridge_a", -7, 1, {1.0}},
+    {"minecraft:noodle_ridge_b", -7, 1, {1.0}},
+    {"minecraft:aquifer_barrier", -3, 1, {1.0}},
+    {"minecraft:aquifer_fluid_level_spread", -5, 1, {1.0}},
+    {"minecraft:aquifer_lava", -1, 1, {1.0}},
 };
 
 // Sampler sets. A set is a list of (noise, sub-sampler, octave slot) Perlin samplers built together for a

@@ -24,9 +24,11 @@ CFG_PROFILE = 1
 SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags", "whf_strerror",
            "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_room_floors",
            "whf_server_open", "whf_server_close", "whf_submit", "whf_submit_deepdive", "whf_poll",
-           "whf_biome_tree", "whf_biome")
+           "whf_biome_tree", "whf_biome", "whf_surface_heights")
 HIT_DEEPDIVE = 1
 BIOME_SWAMP, BIOME_QY0, BIOME_QY_N = 6, -17, 97
+OCC_SURFACE = -(1 << 31)
+BLOCK_SOLID, BLOCK_AIR, BLOCK_WATER, BLOCK_LAVA = 0, 1, 2, 3
 
 
 class Config(ctypes.Structure):
@@ -96,6 +98,10 @@ class BiomeResult(ctypes.Structure):
                [("swamp_qy", ctypes.c_uint64 * 2)]
 
 
+class Column(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int64), ("x", ctypes.c_int32), ("z", ctypes.c_int32)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("chunk_checks", ctypes.c_uint64),
                 ("hbm_budget", ctypes.c_uint64), ("flags", ctypes.c_int), ("reserved", ctypes.c_int)]
@@ -159,6 +165,9 @@ def load():
         L.whf_room_floors.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                       ctypes.POINTER(ctypes.c_int32)]
+        L.whf_surface_heights.restype = ctypes.c_int
+        L.whf_surface_heights.argtypes = [ctypes.c_void_p, ctypes.POINTER(Column), ctypes.c_uint64, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.whf_biome_tree.restype = ctypes.c_int
         L.whf_biome_tree.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         L.whf_biome.restype = ctypes.c_int
@@ -327,6 +336,22 @@ class Context:
         out = [dict(target=list(r.target), qx=r.qx, qy=r.qy, qz=r.qz, occ_y=r.occ_y, winner=r.winner, ok=r.ok,
                     swamp_qy=r.swamp_qy[0] | r.swamp_qy[1] << 64) for r in res[:n]]
         return (out, win[:n]) if winners else out
+
+
+    def surface_heights(self, cols, vflags=0):
+        """Pre-carver surface heights of (seed, x, z) columns: arrays (height, top block state)."""
+        n = len(cols)
+        q = (Column * max(n, 1))()
+        for i, (seed, x, z) in enumerate(cols):
+            q[i].seed = _s64(seed)
+            q[i].x, q[i].z = x, z
+        h = np.zeros(max(n, 1), np.int32)
+        st = np.zeros(max(n, 1), np.int32)
+        p32 = ctypes.POINTER(ctypes.c_int32)
+        rc = self.L.whf_surface_heights(self.h, q, n, vflags, h.ctypes.data_as(p32), st.ctypes.data_as(p32))
+        if rc:
+            raise WhfError(rc, "whf_surface_heights")
+        return h[:n], st[:n]
 
 
 def biome_tree(version):
