@@ -1,10 +1,11 @@
-/* whf_game_data.h — game data of the biome stage: Minecraft's overworld biome search trees (one per game
- * version that changed them) and the overworld terrain spline.
+/* whf_game_data.h — game data of the biome and terrain stages: Minecraft's overworld biome search trees (one per
+ * game version that changed them), the overworld terrain spline and the game's sine table.
  *
  * Both are data the game defines; the values must be the reference's node for node, so they are not retyped
  * here. gen_game_data.py reads them at build time from the reference's generated tables
  * (engine/tools/lysh/src/biome_trees.h: 8 trees of {int16 range[12], kids, biome, end} plus the version table
- * :70499-70526; engine/tools/lysh/src/terrain_table.h: 115 spline nodes and their float/child pools) and writes
+ * :70499-70526; engine/tools/lysh/src/terrain_table.h: 115 spline nodes and their float/child pools;
+ * engine/tools/lysh/src/sin_table.h: 65536 floats) and writes
  * gen/whf_game_data.c in the layout below. The same object links into the HIP library and into the CPU oracle.
  */
 #ifndef WHF_GAME_DATA_H
@@ -58,6 +59,10 @@ extern const int32_t whf_gd_spline_child[];
 extern const int whf_gd_n_spline_child;
 extern const int whf_gd_spline_root[3];   /* offset, factor, peaks */
 extern const int whf_gd_spline_depth;     /* nesting depth of the deepest root (1 = a constant) */
+
+/* Mth.SIN as the game's JVM computed it (sin_table.h): sin(a) = SIN[(long)(a * 10430.378350470453) & 65535],
+ * cos(a) the same at + 16384.0. The carvers' only trigonometry. */
+extern const float whf_gd_mc_sin[65536];
 
 #ifdef __cplusplus
 }

@@ -1,9 +1,10 @@
-"""Writes csrc/gen/whf_game_data.c: the biome search trees and the terrain spline, read from the reference's
-generated tables (engine/tools/lysh/src/biome_trees.h, terrain_table.h) in the layout csrc/whf_game_data.h
-declares.
+"""Writes csrc/gen/whf_game_data.c: the biome search trees, the terrain spline and the game's sine table, read from
+the reference's generated tables (engine/tools/lysh/src/biome_trees.h, terrain_table.h, sin_table.h) in the layout
+csrc/whf_game_data.h declares.
 
-The data are the game's (dumped from each Minecraft version by the reference's tools/accuracy/dump_trees.py and
-its TerrainTable export); they have to be identical node for node, so they are converted, not retyped. The
+The data are the game's (dumped from each Minecraft version by the reference's tools/accuracy/dump_trees.py, its
+TerrainTable export and its JVM dump of Mth.SIN); they have to be identical entry for entry, so they are converted,
+not retyped. The
 script checks the structure it reads (subtree ends, child counts, spline pool bounds) and fails loudly when
 the input does not parse.
 
@@ -133,13 +134,27 @@ def parse_spline(src):
     return floats, child, nodes, roots, d
 
 
+def parse_sin(src):
+    m = re.search(r"LYSH_MC_SIN\[65536\]\s*=\s*\{(.*?)\};", src, re.S)
+    if not m:
+        raise SystemExit("sin table does not parse")
+    vals = [float.fromhex(t.rstrip("fF")) for t in re.findall(r"-?0x[0-9a-fA-F.]+p[-+]?\d+f", m.group(1))]
+    if len(vals) != 65536:
+        raise SystemExit("sin table has %d entries" % len(vals))
+    vals = [struct.unpack("<f", struct.pack("<f", v))[0] for v in vals]
+    # spot checks of the table's meaning: sin(0) = 0 and sin at a quarter turn = 1
+    if vals[0] != 0.0 or vals[16384] != 1.0:
+        raise SystemExit("sin table: unexpected values at 0 or a quarter turn")
+    return vals
+
+
 def c_float(v):
     if v == 0.0:
         return "-0.0f" if struct.pack("<f", v) == struct.pack("<f", -0.0) else "0.0f"
     return float.hex(v) + "f"
 
 
-def emit(trees, versions, spline, out):
+def emit(trees, versions, spline, sin_table, out):
     floats, child, nodes, roots, depth = spline
     lines = ["/* Generated by gen_game_data.py from the reference's biome_trees.h and terrain_table.h; do not edit. */",
              '#include "../whf_game_data.h"', ""]
@@ -171,6 +186,10 @@ def emit(trees, versions, spline, out):
     lines.append("const int whf_gd_n_spline_child = %d;" % len(child))
     lines.append("const int whf_gd_spline_root[3] = {%s};" % ",".join(map(str, roots)))
     lines.append("const int whf_gd_spline_depth = %d;" % depth)
+    lines.append("const float whf_gd_mc_sin[65536] = {")
+    for i in range(0, 65536, 16):
+        lines.append(",".join(c_float(v) for v in sin_table[i:i + 16]) + ",")
+    lines.append("};")
     os.makedirs(os.path.dirname(out), exist_ok=True)
     tmp = out + ".tmp"
     with open(tmp, "w") as f:
@@ -188,7 +207,9 @@ def main(argv=None):
         trees, versions = parse_trees(f.read())
     with open(os.path.join(src_dir, "terrain_table.h")) as f:
         spline = parse_spline(f.read())
-    emit(trees, versions, spline, a.out)
+    with open(os.path.join(src_dir, "sin_table.h")) as f:
+        sin_table = parse_sin(f.read())
+    emit(trees, versions, spline, sin_table, a.out)
     print("game data: %d trees (%d nodes), %d versions, %d spline nodes (depth %d) -> %s"
           % (len(trees), sum(len(t[1]) for t in trees), len(versions), len(spline[2]), spline[4], a.out))
 

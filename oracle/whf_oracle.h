@@ -233,6 +233,24 @@ int    wo_p2_substance(const wo_phase2 *p, int x, int y, int z, double density);
 /* pre-carver surface height (phase2.c:318-334); *top_state = state of the block below it */
 int    wo_p2_height(const wo_phase2 *p, int x, int z, int *top_state);
 
+/* ---- carvers and hut evaluation (whf_oracle_hut.c) ---- */
+enum { WO_HUT_OK = 0, WO_HUT_REJECT_Y = 1, WO_HUT_REJECT_FLOOD = 2, WO_HUT_REJECT_BIOME = 3 };   /* eval.h:88-93 */
+typedef struct {
+    int dir, size_x, size_z;       /* orientation (0 N, 1 E, 2 S, 3 W) and footprint */
+    int biome_ok, winner, occ_y;   /* the biome check at the chunk middle's pre-carver surface */
+    int wet, flooded;              /* footprint columns with water above y = 62; all 63 */
+    int carve_skipped;             /* flooded with every pre-carver top block at or below 63: no carving */
+    long long sum_h;
+    int avg_y;                     /* sum_h / 63, truncated */
+    int ok, reject;
+} wo_hut_result;
+
+int wo_hut_orientation(uint64_t seed, int hut_x, int hut_z);                               /* eval.c:41-50 */
+/* post-carver surface height of a column (phase2.c:438-456, carvers of carver.c:498-523) */
+int wo_post_carver_height(const wo_phase2 *p, uint64_t seed, int x, int z);
+/* lysh_eval_hut (eval.c:121-226) with biome tree `tree` */
+int wo_hut_eval(const wo_phase2 *p, int tree, uint64_t seed, int hut_x, int hut_z, int max_y, wo_hut_result *out);
+
 /* Heap helpers for bindings: a phase-1 noise set or a single noise for a world seed (free with wo_probe_free). */
 void  *wo_probe_alloc(uint64_t world_seed, int vflags);
 void  *wo_noise_alloc(uint64_t world_seed, int noise_id);

@@ -356,19 +356,25 @@ static void corners_for(const wo_phase2 *p, int x, int z, corner_set *cs) {
         wo_p2_corner_column(p, cs->cell_x + (c & 1), cs->cell_z + (c >> 1), cs->main[c], cs->noodle[c]);
 }
 
-static double density_in(const wo_phase2 *p, const corner_set *cs, int x, int y, int z) {
-    /* phase2.c:161-194 for -64 <= y < 320 */
+static double density_core(const wo_phase2 *p, const double *const m[4], const double (*const nd[4])[49], int x, int y,
+                           int z) {
+    /* phase2.c:161-194 for -64 <= y < 320, over the corner columns (x0,z0), (x1,z0), (x0,z1), (x1,z1) */
     const int yo = y + 64;
     const int cy = yo / 8;
     const double dx = (double)(x & 3) / 4.0, dy = (double)(yo - cy * 8) / 8.0, dz = (double)(z & 3) / 4.0;
-    const double v = lerp3c(dx, dy, dz, cs->main[0], cs->main[1], cs->main[2], cs->main[3], cy);
+    const double v = lerp3c(dx, dy, dz, m[0], m[1], m[2], m[3], cy);
     const double a = j_clamp(v * 0.64, -1.0, 1.0);
     const double squeezed = a / 2.0 - a * a * a / 24.0;   /* density.c:172-175 */
     double ch[3];
-    for (int c = 0; c < 3; c++)
-        ch[c] = lerp3c(dx, dy, dz, cs->noodle[0][c], cs->noodle[1][c], cs->noodle[2][c], cs->noodle[3][c], cy);
+    for (int c = 0; c < 3; c++) ch[c] = lerp3c(dx, dy, dz, nd[0][c], nd[1][c], nd[2][c], nd[3][c], cy);
     const double noodle = noodle_combine(wo_p2_noodle_child(p, 0, x, y, z), ch[0], ch[1], ch[2]);
     return j_min(squeezed, noodle);
+}
+
+static double density_in(const wo_phase2 *p, const corner_set *cs, int x, int y, int z) {
+    const double *const m[4] = {cs->main[0], cs->main[1], cs->main[2], cs->main[3]};
+    const double (*const nd[4])[49] = {cs->noodle[0], cs->noodle[1], cs->noodle[2], cs->noodle[3]};
+    return density_core(p, m, nd, x, y, z);
 }
 
 double wo_p2_density(const wo_phase2 *p, int x, int y, int z) {
@@ -522,21 +528,15 @@ int wo_p2_skip_above(const wo_phase2 *p, int chunk_x, int chunk_z) {
     return g * 12 + 11 - 1;
 }
 
-typedef struct {
-    int valid, level, type;
-    int gx, gy, gz;
-} status_slot;
-
-typedef struct {
-    int skip_above;
-    status_slot st[64];    /* fluid statuses of anchors met in one column, by open addressing on the cell */
-} aq_column;
+typedef wo_aq_memo aq_column;
 
 static void status_of(const wo_phase2 *p, aq_column *ac, int gx, int gy, int gz, int ax, int ay, int az, int *level,
                       int *type) {
-    unsigned h = (unsigned)(gx * 73856093 ^ gy * 19349663 ^ gz * 83492791) & 63u;
-    for (int probe = 0; probe < 64; probe++) {
-        status_slot *s = &ac->st[(h + probe) & 63u];
+    /* fluid statuses of the anchors met, by open addressing on the grid cell */
+    const unsigned mask = WO_AQ_SLOTS - 1u;
+    unsigned h = (unsigned)(gx * 73856093 ^ gy * 19349663 ^ gz * 83492791) & mask;
+    for (int probe = 0; probe < WO_AQ_SLOTS; probe++) {
+        wo_aq_slot *s = &ac->st[(h + probe) & mask];
         if (s->valid && s->gx == gx && s->gy == gy && s->gz == gz) { *level = s->level; *type = s->type; return; }
         if (!s->valid) {
             compute_fluid(p, ax, ay, az, &s->level, &s->type);
@@ -662,4 +662,47 @@ int wo_p2_height(const wo_phase2 *p, int x, int z, int *top_state) {
     free(ac);
     if (top_state) *top_state = st;
     return h;
+}
+
+/* ---------------------------------------------------------------- a whole chunk (for the carvers) */
+
+void wo_p2_chunk_init(wo_p2_chunk *ch, const wo_phase2 *p, int cx, int cz) {
+    /* the chunk's 5 x 5 corner columns, computed on first use (phase2.c:108-130), and its aquifer grid */
+    memset(ch, 0, sizeof *ch);
+    ch->p = p;
+    ch->cx = cx;
+    ch->cz = cz;
+    ch->aq.skip_above = wo_p2_skip_above(p, cx, cz);
+}
+
+static void chunk_corner(wo_p2_chunk *ch, int gx, int gz) {
+    const int idx = gx * 5 + gz;
+    if (ch->have & (1u << idx)) return;
+    wo_p2_corner_column(ch->p, ch->cx * 4 + gx, ch->cz * 4 + gz, ch->main[idx], ch->noodle[idx]);
+    ch->have |= 1u << idx;
+}
+
+double wo_p2_chunk_density(wo_p2_chunk *ch, int x, int y, int z) {
+    const int gx = (x >> 2) - ch->cx * 4, gz = (z >> 2) - ch->cz * 4;
+    for (int c = 0; c < 4; c++) chunk_corner(ch, gx + (c & 1), gz + (c >> 1));
+    const int i00 = gx * 5 + gz, i10 = (gx + 1) * 5 + gz, i01 = gx * 5 + gz + 1, i11 = (gx + 1) * 5 + gz + 1;
+    const double *const m[4] = {ch->main[i00], ch->main[i10], ch->main[i01], ch->main[i11]};
+    const double (*const nd[4])[49] = {ch->noodle[i00], ch->noodle[i10], ch->noodle[i01], ch->noodle[i11]};
+    return density_core(ch->p, m, nd, x, y, z);
+}
+
+int wo_p2_chunk_substance(wo_p2_chunk *ch, int x, int y, int z, double density) {
+    return substance(ch->p, &ch->aq, x, y, z, density);
+}
+
+int wo_p2_chunk_state(wo_p2_chunk *ch, int x, int y, int z) {
+    /* lysh_phase2_block_at (phase2.c:311-316): the aquifer's verdict on the interpolated density */
+    return substance(ch->p, &ch->aq, x, y, z, wo_p2_chunk_density(ch, x, y, z));
+}
+
+int wo_p2_chunk_height(wo_p2_chunk *ch, int x, int z) {
+    /* phase2.c:318-334 within the chunk */
+    for (int y = 319; y >= -64; y--)
+        if (wo_p2_chunk_state(ch, x, y, z) != B_AIR) return y + 1;
+    return -64;
 }

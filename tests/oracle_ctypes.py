@@ -55,6 +55,12 @@ class BiomeResult(ctypes.Structure):
     _fields_ = [("t", ctypes.c_int64 * 6)] + [(n, ctypes.c_int) for n in ("qx", "qy", "qz", "occ_y", "winner", "ok")]
 
 
+class HutResult(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("dir", "size_x", "size_z", "biome_ok", "winner", "occ_y", "wet", "flooded",
+                                             "carve_skipped")] + \
+               [("sum_h", ctypes.c_longlong)] + [(n, ctypes.c_int) for n in ("avg_y", "ok", "reject")]
+
+
 class XR(ctypes.Structure):
     _fields_ = [("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64)]
 
@@ -163,6 +169,12 @@ def lib():
         L.wo_p2_substance.argtypes = [ctypes.c_void_p, i32, i32, i32, dbl]
         L.wo_p2_height.restype = i32
         L.wo_p2_height.argtypes = [ctypes.c_void_p, i32, i32, ctypes.POINTER(i32)]
+        L.wo_post_carver_height.restype = i32
+        L.wo_post_carver_height.argtypes = [ctypes.c_void_p, u64, i32, i32]
+        L.wo_hut_orientation.restype = i32
+        L.wo_hut_orientation.argtypes = [u64, i32, i32]
+        L.wo_hut_eval.restype = i32
+        L.wo_hut_eval.argtypes = [ctypes.c_void_p, i32, u64, i32, i32, i32, ctypes.POINTER(HutResult)]
         _lib = L
     return _lib
 
@@ -392,6 +404,7 @@ class Terrain:
     """The phase-2 noises of a world seed: final density, preliminary surface, aquifer, surface height."""
 
     def __init__(self, seed, vflags=0):
+        self.seed = seed
         self.h = lib().wo_p2_alloc(u64(seed), vflags)
 
     def __del__(self):
@@ -436,3 +449,15 @@ class Terrain:
         st = ctypes.c_int(0)
         h = lib().wo_p2_height(self.h, x, z, ctypes.byref(st))
         return h, st.value
+
+    def post_carver_height(self, x, z):
+        return lib().wo_post_carver_height(self.h, u64(self.seed), x, z)
+
+    def hut_eval(self, tree, hut_x, hut_z, max_y=-50):
+        r = HutResult()
+        lib().wo_hut_eval(self.h, tree, u64(self.seed), hut_x, hut_z, max_y, ctypes.byref(r))
+        return {n: getattr(r, n) for n, _ in HutResult._fields_}
+
+
+def hut_orientation(seed, hut_x, hut_z):
+    return lib().wo_hut_orientation(u64(seed), hut_x, hut_z)
