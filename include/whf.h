@@ -248,6 +248,44 @@ typedef struct {
 int whf_surface_heights(whf_ctx *ctx, const whf_column *cols, uint64_t n, int vflags, int32_t *height,
                         int32_t *state);
 
+/* Post-carver surface height of each column: the pre-carver surface after the cave and canyon carvers of the 17 x 17
+ * chunks around the column's chunk (lysh_phase2_height, phase2.c:438-456, carvers carver.c:498-523); `pre` (may be
+ * NULL) receives the pre-carver height. This is the height the hut's floor averages (eval.c:170-219). */
+int whf_post_carver_heights(whf_ctx *ctx, const whf_column *cols, uint64_t n, int vflags, int32_t *post,
+                            int32_t *pre);
+
+/* ---- Hut evaluation ---- */
+
+#define WHF_HUT_OK           0
+#define WHF_HUT_REJECT_Y     1   /* avg_y > max_y */
+#define WHF_HUT_REJECT_FLOOD 2   /* all 63 footprint columns hold water above y = 62 */
+#define WHF_HUT_REJECT_BIOME 3   /* the chunk middle's biome is not swamp */
+
+typedef struct {
+    int64_t seed;
+    int32_t cx, cz;             /* the hut's chunk; the hut stands at (16 cx, 16 cz) */
+} whf_hut_query;
+
+typedef struct {
+    int32_t dir;                /* orientation: nextInt(4) after setLargeFeatureSeed, 0 N 1 E 2 S 3 W */
+    int32_t size_x, size_z;     /* footprint, 7 x 9 or 9 x 7 */
+    int32_t biome_ok, winner, occ_y;   /* the biome check at the chunk middle's pre-carver surface */
+    int32_t wet;                /* footprint columns with water above y = 62 */
+    int32_t flooded;            /* wet == 63 */
+    int32_t carve_skipped;      /* flooded with all 63 pre-carver top blocks at or below 63: pre-carver sum used */
+    int32_t avg_y;              /* sum_h / 63, truncated toward zero: the hut's floor */
+    int64_t sum_h;              /* the 63 footprint heights summed */
+    int32_t ok, reject;         /* WHF_HUT_OK or the WHF_HUT_REJECT_* reason, in lysh's order */
+} whf_hut_result;
+
+/* The reference CPU scanner's verdict on each hut (lysh_eval_hut, eval.c:121-226) with biome tree `tree` and
+ * floor threshold max_y: orientation, the biome at the pre-carver surface, the flood test, the average post-carver
+ * height of the 63 footprint columns, and the verdict biome_ok && !flooded && avg_y <= max_y. Replaces the `lysh
+ * hut` processes hunt.py runs per candidate (hunt.py:1390-1392). vflags: WHF_V_LARGE for the LARGE_BIOMES noises.
+ * Not re-entrant per context. */
+int whf_hut_eval(whf_ctx *ctx, const whf_hut_query *huts, uint64_t n, int tree, int vflags, int max_y,
+                 whf_hut_result *out);
+
 /* ---- Serving: continuous batching of Seed Check requests ---- */
 
 typedef struct whf_server whf_server;

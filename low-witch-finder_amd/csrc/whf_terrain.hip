@@ -11,13 +11,18 @@
 //                      C  the final density at every y (trilinear over the corners, squeezed, min with the noodle);
 //                         the top solid block y_d is its highest positive density;
 //                      D  above y_d, the aquifer decides each block up to skipSamplingAboveY, the sea above it; the
-//                         height is one above the highest block that is not air.
+//                         height is one above the highest block that is not air;
+//                      E  (hut evaluation) below that surface, the cells the carvers claim turn into the carver's
+//                         air or fluid where their block allows it; the post-carver height is one above the highest
+//                         block that is then not air; and whether water stands above y = 62 in the column.
+//   k_claims         one 256-lane block per target chunk: the cells its 17 x 17 x 3 carvers claim (whf_carver.h)
 // This is work per phase-1 survivor (one in 1e5..1e7 regions), so it is laid out for clarity of the reference's
 // order of operations, not for throughput: every lane calls the same pointwise functions the reference calls.
 #include <cstring>
 #include <unordered_map>
 #include <vector>
 
+#include "whf_carver.h"
 #include "whf_internal.h"
 #include "whf_terrain.h"
 
@@ -29,6 +34,15 @@ constexpr uint32_t kTerBatch = 1024;   // columns per device round (distinct see
 constexpr int kTerSamplers = 144;      // named-noise samplers per seed
 constexpr int kCornerJobs = 4 * 49;    // main corner samples
 constexpr int kNoodleJobs = 4 * 3 * 49;
+constexpr uint32_t kHutBatch = 16;     // huts per device round (16 x 63 columns fit one round of columns)
+constexpr int kClaimWords = 256 * 6;   // a chunk's claimed-cell bits
+
+// One column of a round: its seed slot, position, and the target chunk whose carver claims apply (-1: none).
+struct ColJob {
+    uint32_t slot;
+    int32_t x, z;
+    int32_t chunk;
+};
 
 struct TerBufs {
     TerrainTables *tt;        // [2]: NORMAL, LARGE_BIOMES
@@ -40,9 +54,11 @@ struct TerBufs {
     SPar *lpar;
     ulonglong2 *aq;           // [kTerBatch]
     uint64_t *seeds;          // [kTerBatch]
-    whf_column *q;            // [kTerBatch]
-    uint32_t *slot;           // [kTerBatch]
-    int2 *out;                // [kTerBatch]: height, top state
+    ColJob *jobs;             // [kTerBatch]
+    int4 *out;                // [kTerBatch]: pre-carver height, its top state, post-carver height, water above 62
+    float *sin_tab;           // Mth.SIN
+    int4 *chunks;             // [kHutBatch]: seed slot, chunk x, chunk z
+    unsigned long long *claims;   // [kHutBatch][kClaimWords]
 };
 
 __global__ __launch_bounds__(64) void k_build_blended(const uint64_t *__restrict__ seeds, uint32_t ns, uint64_t t_hi,
@@ -73,20 +89,21 @@ __global__ __launch_bounds__(64) void k_build_blended(const uint64_t *__restrict
     aq[s] = f;
 }
 
+template <bool kPost>
 __global__ __launch_bounds__(256) void k_surface(const TerrainTables *__restrict__ tt, const SplineTab *__restrict__ S,
                                                  const uint8_t *__restrict__ tab, const SPar *__restrict__ par,
                                                  const uint8_t *__restrict__ ltab, const SPar *__restrict__ lpar,
-                                                 const ulonglong2 *__restrict__ aq, const whf_column *__restrict__ q,
-                                                 const uint32_t *__restrict__ slot_of, int2 *__restrict__ out) {
+                                                 const ulonglong2 *__restrict__ aq, const ColJob *__restrict__ jobs,
+                                                 const unsigned long long *__restrict__ claims, int4 *__restrict__ out) {
     __shared__ double s_main[4][49];
     __shared__ double s_nood[4][3][49];
     __shared__ double s_dens[384];
     __shared__ float s_tp[4][3];
-    __shared__ int s_psl, s_top_solid, s_best, s_state;
+    __shared__ int s_psl, s_top_solid, s_best, s_state, s_wet;
     const uint32_t qi = blockIdx.x;
     const int t = threadIdx.x;
-    const whf_column col = q[qi];
-    const uint32_t slot = slot_of[qi];
+    const ColJob col = jobs[qi];
+    const uint32_t slot = col.slot;
     const ulonglong2 f = aq[slot];
     const Terr T{tt, SetView{tab, par, kTerSamplers}, slot, ltab + (size_t)slot * kLegacyPerlins * 256,
                  lpar + (size_t)slot * kLegacyPerlins, S, f.x, f.y};
@@ -96,6 +113,7 @@ __global__ __launch_bounds__(256) void k_surface(const TerrainTables *__restrict
         s_top_solid = -65;
         s_best = -0x7FFFFFFF - 1;
         s_state = BS_AIR;
+        s_wet = 0;
     }
     __syncthreads();
 
@@ -107,6 +125,11 @@ __global__ __launch_bounds__(256) void k_surface(const TerrainTables *__restrict
     } else if (t >= 128 && t < 132) {
         const int c = t - 128;
         T.point261((x >> 2) + (c & 1), (z >> 2) + (c >> 1), s_tp[c]);
+    } else if (kPost && t == 192) {
+        // the hut's flood test for this column: water above y = 62 (eval.c:151-168, computeFluid aquifer.c:286-331)
+        int lv, ty;
+        T.fluid(x, 62, z, lv, ty);
+        s_wet = (ty == BS_WATER && 62 < lv) ? 1 : 0;
     }
     __syncthreads();
     const int skip_above = (floor_div((s_psl + 8) + 12, 12) + 1) * 12 + 11 - 1;
@@ -165,15 +188,68 @@ __global__ __launch_bounds__(256) void k_surface(const TerrainTables *__restrict
         }
     }
     __syncthreads();
+    int h = y_d, st = BS_SOLID;   // uniform: every lane reads the same shared values
+    if (s_best != -0x7FFFFFFF - 1 && s_best > h) { h = s_best; st = s_state; }
+    if (skip_above < 62 && 62 > h) { h = 62; st = BS_WATER; }   // the sea above the aquifer's reach
+    const int pre = h >= -64 ? h + 1 : -64;
+    int post = pre;
+
+    if constexpr (kPost) {
+        // E: post-carver surface (phase2.c:337-345, 438-456): below the pre-carver surface, a claimed cell whose
+        // block is solid or water becomes lava at y <= -56 and elsewhere the aquifer's substance at density 0,
+        // unless that is solid (carver.c:145-175); other cells keep their block state. Top-down rounds of 256.
+        const unsigned long long *cw =
+            col.chunk >= 0 ? claims + (size_t)col.chunk * kClaimWords + (size_t)(((z & 15) * 16 + (x & 15)) * 6) : nullptr;
+        __syncthreads();
+        if (t == 0) s_best = -0x7FFFFFFF - 1;
+        __syncthreads();
+        int found = -0x7FFFFFFF - 1;
+        for (int y_top = pre - 1; y_top >= -64; y_top -= 256) {
+            const int y = y_top - t;
+            if (y >= -64) {
+                const double d = s_dens[y + 64];
+                int bs = d > 0.0 ? BS_SOLID : T.substance(x, y, z, d, skip_above);
+                const int b = y + 64;
+                if (cw && ((cw[b >> 6] >> (b & 63)) & 1ull) && (bs == BS_SOLID || bs == BS_WATER)) {
+                    const int cs = y <= -56 ? BS_LAVA : T.substance(x, y, z, 0.0, skip_above);
+                    if (cs != BS_SOLID) bs = cs;
+                }
+                if (bs != BS_AIR) atomicMax(&s_best, y);
+            }
+            __syncthreads();
+            found = s_best;
+            __syncthreads();
+            if (found != -0x7FFFFFFF - 1) break;
+        }
+        post = found != -0x7FFFFFFF - 1 ? found + 1 : -64;
+    }
     if (t == 0) {
-        int h = y_d, st = BS_SOLID;
-        if (s_best != -0x7FFFFFFF - 1 && s_best > h) { h = s_best; st = s_state; }
-        if (skip_above < 62 && 62 > h) { h = 62; st = BS_WATER; }   // the sea above the aquifer's reach
-        int2 o;
-        o.x = h >= -64 ? h + 1 : -64;
+        int4 o;
+        o.x = pre;
         o.y = h >= -64 ? st : BS_AIR;
+        o.z = post;
+        o.w = s_wet;
         out[qi] = o;
     }
+}
+
+__global__ __launch_bounds__(256) void k_claims(const float *__restrict__ sin_tab, const uint64_t *__restrict__ seeds,
+                                                const int4 *__restrict__ chunks,
+                                                unsigned long long *__restrict__ claims) {
+    // the cells of one target chunk that its 17 x 17 origin chunks' three carvers claim (carver.c:498-523)
+    __shared__ unsigned long long bits[kClaimWords];
+    const int4 ch = chunks[blockIdx.x];
+    for (int i = threadIdx.x; i < kClaimWords; i += 256) bits[i] = 0ull;
+    __syncthreads();
+    const CarveClaims cc{sin_tab, bits, ch.y * 16, ch.z * 16, (double)(ch.y * 16 + 8), (double)(ch.z * 16 + 8)};
+    const uint64_t world = seeds[ch.x];
+    for (int job = threadIdx.x; job < 17 * 17 * 3; job += 256) {
+        const int o = job / 3, ci = job - o * 3;
+        cc.carver(world, ch.y + o / 17 - 8, ch.z + o % 17 - 8, ci);
+    }
+    __syncthreads();
+    unsigned long long *dst = claims + (size_t)blockIdx.x * kClaimWords;
+    for (int i = threadIdx.x; i < kClaimWords; i += 256) dst[i] = bits[i];
 }
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -190,9 +266,11 @@ size_t ter_carve(uint8_t *base, TerBufs &b) {
     b.lpar = (SPar *)take((size_t)kTerBatch * kLegacyPerlins * sizeof(SPar));
     b.aq = (ulonglong2 *)take(kTerBatch * sizeof(ulonglong2));
     b.seeds = (uint64_t *)take(kTerBatch * sizeof(uint64_t));
-    b.q = (whf_column *)take(kTerBatch * sizeof(whf_column));
-    b.slot = (uint32_t *)take(kTerBatch * sizeof(uint32_t));
-    b.out = (int2 *)take(kTerBatch * sizeof(int2));
+    b.jobs = (ColJob *)take(kTerBatch * sizeof(ColJob));
+    b.out = (int4 *)take(kTerBatch * sizeof(int4));
+    b.sin_tab = (float *)take(65536 * sizeof(float));
+    b.chunks = (int4 *)take(kHutBatch * sizeof(int4));
+    b.claims = (unsigned long long *)take((size_t)kHutBatch * kClaimWords * sizeof(unsigned long long));
     return o;
 }
 
@@ -256,8 +334,33 @@ int ter_init(whf_ctx *c, TerBufs &b) {
     std::memcpy(sp.child, whf_gd_spline_child, whf_gd_n_spline_child * sizeof(int32_t));
     for (int r = 0; r < 3; r++) sp.root[r] = whf_gd_spline_root[r];
     HIP_TRY(hipMemcpyAsync(b.spline, &sp, sizeof sp, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b.sin_tab, whf_gd_mc_sin, 65536 * sizeof(float), hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
     return WHF_OK;
+}
+
+// Sampler tables of a round's distinct seeds (slots 0..ns-1 of the round).
+int build_round(whf_ctx *c, TerBufs &b, const std::vector<uint64_t> &seeds, int large) {
+    const hipStream_t st = c->stream;
+    const uint32_t ns = (uint32_t)seeds.size();
+    uint64_t t_hi, t_lo, a_hi, a_lo;
+    md5_longs("minecraft:terrain", t_hi, t_lo);
+    md5_longs("minecraft:aquifer", a_hi, a_lo);
+    HIP_TRY(hipMemcpyAsync(b.seeds, seeds.data(), ns * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    launch_build(c, b.seeds, b.bd + large * kTerSamplers, kTerSamplers, ns, b.tab, b.par, c->d_cnt + CNT_BIO_BUILT);
+    hipLaunchKernelGGL(k_build_blended, dim3((ns + 63) / 64), dim3(64), 0, st, b.seeds, ns, t_hi, t_lo, a_hi, a_lo,
+                       b.ltab, b.lpar, b.aq);
+    return check_launch();
+}
+
+// Slot of a seed in the round, adding it when new.
+uint32_t slot_for(uint64_t seed, std::vector<uint64_t> &seeds, std::unordered_map<uint64_t, uint32_t> &seen) {
+    auto it = seen.find(seed);
+    if (it == seen.end()) {
+        it = seen.emplace(seed, (uint32_t)seeds.size()).first;
+        seeds.push_back(seed);
+    }
+    return it->second;
 }
 
 }  // namespace
@@ -271,45 +374,49 @@ int terrain_heights(whf_ctx *c, const whf_column *cols, uint64_t n, int vflags, 
     if (rc != WHF_OK) return rc;
     const hipStream_t st = c->stream;
     const int large = (vflags & WHF_V_LARGE) ? 1 : 0;
-    uint64_t t_hi, t_lo, a_hi, a_lo;
-    md5_longs("minecraft:terrain", t_hi, t_lo);
-    md5_longs("minecraft:aquifer", a_hi, a_lo);
     std::vector<uint64_t> seeds;
-    std::vector<uint32_t> slot;
-    std::vector<int2> res(kTerBatch);
+    std::vector<ColJob> jobs;
+    std::vector<int4> res(kTerBatch);
     std::unordered_map<uint64_t, uint32_t> seen;
     for (uint64_t q0 = 0; q0 < n; q0 += kTerBatch) {
         const uint32_t nq = (uint32_t)std::min<uint64_t>(kTerBatch, n - q0);
         seeds.clear();
         seen.clear();
-        slot.resize(nq);
-        for (uint32_t i = 0; i < nq; i++) {
-            const uint64_t s = (uint64_t)cols[q0 + i].seed;
-            auto it = seen.find(s);
-            if (it == seen.end()) {
-                it = seen.emplace(s, (uint32_t)seeds.size()).first;
-                seeds.push_back(s);
-            }
-            slot[i] = it->second;
-        }
-        const uint32_t ns = (uint32_t)seeds.size();
-        HIP_TRY(hipMemcpyAsync(b.seeds, seeds.data(), ns * sizeof(uint64_t), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(b.q, cols + q0, nq * sizeof(whf_column), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(b.slot, slot.data(), nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-        launch_build(c, b.seeds, b.bd + large * kTerSamplers, kTerSamplers, ns, b.tab, b.par, c->d_cnt + CNT_BIO_BUILT);
-        hipLaunchKernelGGL(k_build_blended, dim3((ns + 63) / 64), dim3(64), 0, st, b.seeds, ns, t_hi, t_lo, a_hi, a_lo,
-                           b.ltab, b.lpar, b.aq);
-        hipLaunchKernelGGL(k_surface, dim3(nq), dim3(256), 0, st, b.tt + large, b.spline, b.tab, b.par, b.ltab, b.lpar,
-                           b.aq, b.q, b.slot, b.out);
-        rc = check_launch();
-        if (rc != WHF_OK) return rc;
-        HIP_TRY(hipMemcpyAsync(res.data(), b.out, nq * sizeof(int2), hipMemcpyDeviceToHost, st));
+        jobs.resize(nq);
+        for (uint32_t i = 0; i < nq; i++)
+            jobs[i] = ColJob{slot_for((uint64_t)cols[q0 + i].seed, seeds, seen), cols[q0 + i].x, cols[q0 + i].z, -1};
+        if ((rc = build_round(c, b, seeds, large)) != WHF_OK) return rc;
+        HIP_TRY(hipMemcpyAsync(b.jobs, jobs.data(), nq * sizeof(ColJob), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_surface<false>, dim3(nq), dim3(256), 0, st, b.tt + large, b.spline, b.tab, b.par, b.ltab,
+                           b.lpar, b.aq, b.jobs, (const unsigned long long *)nullptr, b.out);
+        if ((rc = check_launch()) != WHF_OK) return rc;
+        HIP_TRY(hipMemcpyAsync(res.data(), b.out, nq * sizeof(int4), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         for (uint32_t i = 0; i < nq; i++) {
             height[q0 + i] = res[i].x;
             if (state) state[q0 + i] = res[i].y;
         }
     }
+    return WHF_OK;
+}
+
+// Pre- and post-carver heights and the water-above-62 flag of columns grouped by target chunk: chunk k of the round
+// is (chunks[k].seed, chunk x, chunk z); the claims of each chunk are gathered once.
+int carved_columns(whf_ctx *c, TerBufs &b, int large, const std::vector<uint64_t> &seeds,
+                   const std::vector<int4> &chunks, const std::vector<ColJob> &jobs, std::vector<int4> &res) {
+    const hipStream_t st = c->stream;
+    int rc = build_round(c, b, seeds, large);
+    if (rc != WHF_OK) return rc;
+    HIP_TRY(hipMemcpyAsync(b.chunks, chunks.data(), chunks.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b.jobs, jobs.data(), jobs.size() * sizeof(ColJob), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_claims, dim3((uint32_t)chunks.size()), dim3(256), 0, st, b.sin_tab, b.seeds, b.chunks,
+                       b.claims);
+    hipLaunchKernelGGL(k_surface<true>, dim3((uint32_t)jobs.size()), dim3(256), 0, st, b.tt + large, b.spline, b.tab,
+                       b.par, b.ltab, b.lpar, b.aq, b.jobs, (const unsigned long long *)b.claims, b.out);
+    if ((rc = check_launch()) != WHF_OK) return rc;
+    res.resize(jobs.size());
+    HIP_TRY(hipMemcpyAsync(res.data(), b.out, jobs.size() * sizeof(int4), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     return WHF_OK;
 }
 
@@ -322,4 +429,134 @@ extern "C" int whf_surface_heights(whf_ctx *c, const whf_column *cols, uint64_t 
     if (n == 0) return WHF_OK;
     HIP_TRY(hipSetDevice(c->device));
     return terrain_heights(c, cols, n, vflags, height, state);
+}
+
+namespace {
+
+// Rounds of columns grouped by target chunk: at most kHutBatch chunks and kTerBatch columns per round.
+struct CarveRound {
+    std::vector<uint64_t> seeds;
+    std::unordered_map<uint64_t, uint32_t> seen;
+    std::vector<int4> chunks;
+    std::vector<ColJob> jobs;
+    void clear() { seeds.clear(); seen.clear(); chunks.clear(); jobs.clear(); }
+};
+
+}  // namespace
+
+extern "C" int whf_post_carver_heights(whf_ctx *c, const whf_column *cols, uint64_t n, int vflags, int32_t *post,
+                                       int32_t *pre) {
+    if (!c || (n && (!cols || !post))) return WHF_EINVAL;
+    g_last_error[0] = 0;
+    if (n == 0) return WHF_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    TerBufs b;
+    int rc = ter_init(c, b);
+    if (rc != WHF_OK) return rc;
+    const int large = (vflags & WHF_V_LARGE) ? 1 : 0;
+    CarveRound r;
+    std::vector<int4> res;
+    std::vector<uint64_t> idx;
+    auto flush = [&]() -> int {
+        if (r.jobs.empty()) return WHF_OK;
+        const int rc2 = carved_columns(c, b, large, r.seeds, r.chunks, r.jobs, res);
+        if (rc2 != WHF_OK) return rc2;
+        for (size_t k = 0; k < r.jobs.size(); k++) {
+            post[idx[k]] = res[k].z;
+            if (pre) pre[idx[k]] = res[k].x;
+        }
+        r.clear();
+        idx.clear();
+        return WHF_OK;
+    };
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t seed = (uint64_t)cols[i].seed;
+        const int cx = cols[i].x >> 4, cz = cols[i].z >> 4;
+        int chunk = -1;
+        for (size_t k = 0; k < r.chunks.size(); k++)
+            if (r.seeds[r.chunks[k].x] == seed && r.chunks[k].y == cx && r.chunks[k].z == cz) chunk = (int)k;
+        if (chunk < 0 && (r.chunks.size() == kHutBatch || r.jobs.size() == kTerBatch)) {
+            if ((rc = flush()) != WHF_OK) return rc;
+        } else if (r.jobs.size() == kTerBatch) {
+            if ((rc = flush()) != WHF_OK) return rc;
+            chunk = -1;
+        }
+        const uint32_t slot = slot_for(seed, r.seeds, r.seen);
+        if (chunk < 0) {
+            chunk = (int)r.chunks.size();
+            r.chunks.push_back(make_int4((int)slot, cx, cz, 0));
+        }
+        r.jobs.push_back(ColJob{slot, cols[i].x, cols[i].z, chunk});
+        idx.push_back(i);
+    }
+    return flush();
+}
+
+extern "C" int whf_hut_eval(whf_ctx *c, const whf_hut_query *huts, uint64_t n, int tree, int vflags, int max_y,
+                            whf_hut_result *out) {
+    if (!c || (n && (!huts || !out))) return WHF_EINVAL;
+    if (tree < 0 || tree >= WHF_GD_TREES) return WHF_EINVAL;
+    g_last_error[0] = 0;
+    if (n == 0) return WHF_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    // the biome check at the chunk middle's pre-carver surface (eval.c:141-143)
+    std::vector<whf_biome_query> bq(n);
+    std::vector<whf_biome_result> br(n);
+    for (uint64_t i = 0; i < n; i++) bq[i] = whf_biome_query{huts[i].seed, huts[i].cx * 16, huts[i].cz * 16, WHF_OCC_SURFACE, 0};
+    int rc = whf_biome(c, bq.data(), n, tree, vflags, br.data(), nullptr);
+    if (rc != WHF_OK) return rc;
+    TerBufs b;
+    if ((rc = ter_init(c, b)) != WHF_OK) return rc;
+    const int large = (vflags & WHF_V_LARGE) ? 1 : 0;
+    CarveRound r;
+    std::vector<int4> res;
+    for (uint64_t h0 = 0; h0 < n; h0 += kHutBatch) {
+        const uint32_t nh = (uint32_t)std::min<uint64_t>(kHutBatch, n - h0);
+        r.clear();
+        for (uint32_t k = 0; k < nh; k++) {
+            const whf_hut_query &q = huts[h0 + k];
+            whf_hut_result &o = out[h0 + k];
+            std::memset(&o, 0, sizeof o);
+            // orientation and footprint (eval.c:41-50, 129-132)
+            JRand jr;
+            jr.large_feature((uint64_t)q.seed, q.cx, q.cz);
+            o.dir = jr.next_int(4);
+            const bool axis_z = o.dir == 0 || o.dir == 2;
+            o.size_x = axis_z ? 7 : 9;
+            o.size_z = axis_z ? 9 : 7;
+            const uint32_t slot = slot_for((uint64_t)q.seed, r.seeds, r.seen);
+            r.chunks.push_back(make_int4((int)slot, q.cx, q.cz, 0));
+            for (int dx = 0; dx < o.size_x; dx++)
+                for (int dz = 0; dz < o.size_z; dz++)
+                    r.jobs.push_back(ColJob{slot, q.cx * 16 + dx, q.cz * 16 + dz, (int)k});
+        }
+        if ((rc = carved_columns(c, b, large, r.seeds, r.chunks, r.jobs, res)) != WHF_OK) return rc;
+        size_t j = 0;
+        for (uint32_t k = 0; k < nh; k++) {
+            whf_hut_result &o = out[h0 + k];
+            const whf_biome_result &bz = br[h0 + k];
+            o.biome_ok = bz.ok;
+            o.winner = bz.winner;
+            o.occ_y = bz.occ_y;
+            long long sum_pre = 0, sum_post = 0;
+            bool blocked = false;
+            for (int col = 0; col < 63; col++, j++) {
+                o.wet += res[j].w;
+                sum_pre += res[j].x;
+                sum_post += res[j].z;
+                if (res[j].x - 1 >= 64) blocked = true;
+            }
+            o.flooded = o.wet == 63;
+            // a flooded hut whose pre-carver top blocks are all at or below 63 keeps the pre-carver sum
+            // (eval.c:186-205)
+            o.carve_skipped = o.flooded && !blocked;
+            o.sum_h = o.carve_skipped ? sum_pre : sum_post;
+            o.avg_y = (int32_t)(o.sum_h / 63);
+            if (!o.biome_ok) { o.ok = 0; o.reject = WHF_HUT_REJECT_BIOME; }
+            else if (o.flooded) { o.ok = 0; o.reject = WHF_HUT_REJECT_FLOOD; }
+            else if (o.avg_y > max_y) { o.ok = 0; o.reject = WHF_HUT_REJECT_Y; }
+            else { o.ok = 1; o.reject = WHF_HUT_OK; }
+        }
+    }
+    return WHF_OK;
 }

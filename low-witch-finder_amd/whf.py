@@ -24,7 +24,7 @@ CFG_PROFILE = 1
 SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags", "whf_strerror",
            "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_room_floors",
            "whf_server_open", "whf_server_close", "whf_submit", "whf_submit_deepdive", "whf_poll",
-           "whf_biome_tree", "whf_biome", "whf_surface_heights")
+           "whf_biome_tree", "whf_biome", "whf_surface_heights", "whf_post_carver_heights", "whf_hut_eval")
 HIT_DEEPDIVE = 1
 BIOME_SWAMP, BIOME_QY0, BIOME_QY_N = 6, -17, 97
 OCC_SURFACE = -(1 << 31)
@@ -102,6 +102,16 @@ class Column(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_int64), ("x", ctypes.c_int32), ("z", ctypes.c_int32)]
 
 
+class HutQuery(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int64), ("cx", ctypes.c_int32), ("cz", ctypes.c_int32)]
+
+
+class HutResult(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("dir", "size_x", "size_z", "biome_ok", "winner", "occ_y", "wet", "flooded",
+                                               "carve_skipped", "avg_y")] + \
+               [("sum_h", ctypes.c_int64), ("ok", ctypes.c_int32), ("reject", ctypes.c_int32)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("chunk_checks", ctypes.c_uint64),
                 ("hbm_budget", ctypes.c_uint64), ("flags", ctypes.c_int), ("reserved", ctypes.c_int)]
@@ -168,6 +178,12 @@ def load():
         L.whf_surface_heights.restype = ctypes.c_int
         L.whf_surface_heights.argtypes = [ctypes.c_void_p, ctypes.POINTER(Column), ctypes.c_uint64, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+        L.whf_post_carver_heights.restype = ctypes.c_int
+        L.whf_post_carver_heights.argtypes = [ctypes.c_void_p, ctypes.POINTER(Column), ctypes.c_uint64, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+        L.whf_hut_eval.restype = ctypes.c_int
+        L.whf_hut_eval.argtypes = [ctypes.c_void_p, ctypes.POINTER(HutQuery), ctypes.c_uint64, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.POINTER(HutResult)]
         L.whf_biome_tree.restype = ctypes.c_int
         L.whf_biome_tree.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         L.whf_biome.restype = ctypes.c_int
@@ -352,6 +368,35 @@ class Context:
         if rc:
             raise WhfError(rc, "whf_surface_heights")
         return h[:n], st[:n]
+
+
+    def post_carver_heights(self, cols, vflags=0):
+        """Post- and pre-carver surface heights of (seed, x, z) columns: arrays (post, pre)."""
+        n = len(cols)
+        q = (Column * max(n, 1))()
+        for i, (seed, x, z) in enumerate(cols):
+            q[i].seed = _s64(seed)
+            q[i].x, q[i].z = x, z
+        post = np.zeros(max(n, 1), np.int32)
+        pre = np.zeros(max(n, 1), np.int32)
+        p32 = ctypes.POINTER(ctypes.c_int32)
+        rc = self.L.whf_post_carver_heights(self.h, q, n, vflags, post.ctypes.data_as(p32), pre.ctypes.data_as(p32))
+        if rc:
+            raise WhfError(rc, "whf_post_carver_heights")
+        return post[:n], pre[:n]
+
+    def hut_eval(self, huts, tree, max_y=-50, vflags=0):
+        """lysh's hut verdicts for (seed, cx, cz) huts: a list of result dicts."""
+        n = len(huts)
+        q = (HutQuery * max(n, 1))()
+        for i, (seed, cx, cz) in enumerate(huts):
+            q[i].seed = _s64(seed)
+            q[i].cx, q[i].cz = cx, cz
+        res = (HutResult * max(n, 1))()
+        rc = self.L.whf_hut_eval(self.h, q, n, tree, vflags, max_y, res)
+        if rc:
+            raise WhfError(rc, "whf_hut_eval")
+        return [{f: getattr(r, f) for f, _ in HutResult._fields_} for r in res[:n]]
 
 
 def biome_tree(version):
