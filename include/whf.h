@@ -62,17 +62,27 @@ typedef struct {
     int max_height;             /* phase-1 maxHeight (lysh_phase1_check_ex's max_height, used as given); -140..100000 */
     int vflags;                 /* WHF_V_* */
     int salt;                   /* structure salt; 0 = swamp hut 14357620 */
-    int reserved;
+    int phase2;                 /* WHF_P2_*: phase-1 survivors only, or lysh's hut verdict on them (whf_hut_eval) */
+    int phase2_max_y;           /* the floor threshold of the hut verdict (lysh --phase2-max-y) */
+    int biome_tree;             /* biome tree of the hut verdict: 1 + a tree index (whf_biome_tree); 0 = the rule
+                                   flags' default, as lysh picks it (eval.c:104-106) */
 } whf_request;
 
-/* One phase-1 survivor: a hut position that passed every phase-1 gate. */
+/* Phase 2 of a request: what whf_scan returns. */
+#define WHF_P2_NONE     0   /* the phase-1 survivors (the reference GPU engine's candidates) */
+#define WHF_P2_OR_LOWER 1   /* the huts lysh accepts with avg_y <= phase2_max_y (lysh scan --list), floor_y = avg_y */
+#define WHF_P2_EXACTLY  2   /* of those, the huts with avg_y == phase2_max_y (hunt.py's "exactly", hunt.py:1342-1349) */
+
+/* One phase-1 survivor: a hut position that passed every phase-1 gate; with phase 2, a hut lysh accepts. */
 #define WHF_HIT_DEEPDIVE 1
+#define WHF_HIT_PHASE2   2   /* floor_y is the hut's lysh floor (avg_y) */
 typedef struct {
     int64_t seed;
     int32_t rx, rz;             /* region */
     int32_t cx, cz;             /* hut chunk; the hut's block is (16*cx, 16*cz) */
-    int32_t floor_y;            /* Seed Check: not computed by phase 1, INT32_MIN; Deep Dive: the room floor */
-    int32_t flags;              /* 0 for a Seed Check hit, WHF_HIT_DEEPDIVE for a Deep Dive find */
+    int32_t floor_y;            /* Seed Check: the hut's floor with phase 2, else INT32_MIN; Deep Dive: the room floor */
+    int32_t flags;              /* WHF_HIT_PHASE2 for a Seed Check hit with phase 2, WHF_HIT_DEEPDIVE for a Deep Dive
+                                   find, else 0 */
 } whf_hit;
 
 typedef struct {
@@ -97,6 +107,9 @@ typedef struct {
     double kernel_ms[8];        /* with whf_config.flags & 1: device time per kernel family (seeds, erosion build,
                                    erosion gate, compact, climate build, gate 2, ladder build, gate 3) */
     uint64_t fp64_rechecks;     /* WHF_PREC_FP32_GATE: items the FP32 gate left to the FP64 re-evaluation */
+    uint64_t phase2[5];         /* with phase 2 (lysh's p2grade, main.c:333-345): evaluated, accepted, rejected for
+                                   biome, for the floor (avg_y, or != threshold with WHF_P2_EXACTLY), for flood */
+    double phase2_ms;           /* wall time of the hut evaluation */
 } whf_stats;
 
 /* Opens a context on a device and allocates its working buffers (replaces phase1_scan's allocateBuffers,

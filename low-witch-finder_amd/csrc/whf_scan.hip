@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1062,6 +1063,8 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
     if (req->seed_mode == WHF_SEEDS_LIST && req->n_seeds && !req->seeds) return WHF_EINVAL;
     if (req->precision != WHF_PREC_FP64 && req->precision != WHF_PREC_FP32_GATE) return WHF_EINVAL;
     if (req->vflags < 0 || req->vflags >= kLayouts) return WHF_EINVAL;
+    if (req->phase2 < WHF_P2_NONE || req->phase2 > WHF_P2_EXACTLY) return WHF_EINVAL;
+    if (req->biome_tree < 0 || req->biome_tree > 8) return WHF_EINVAL;
     // gate 3 keeps one bit per check: the floodedness heights max_height..60 must fit beside the ladders
     if (req->max_height < -140 || req->max_height > 100000) return WHF_EINVAL;
     // chunk coordinates travel biased in 23 bits through the erosion kernel
@@ -1118,11 +1121,48 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
         r64.precision = WHF_PREC_FP64;
         return whf_scan(c, &r64, sink, stats);
     }
-    const uint64_t hits_total = ctl[CTL_HITS];
+    uint64_t hits_total = ctl[CTL_HITS];
     std::vector<DevHit> host_hits(std::min<uint64_t>(hits_total, kHitCap));
     if (!host_hits.empty())
         HIP_TRY(hipMemcpy(host_hits.data(), c->d_hits, host_hits.size() * sizeof(DevHit), hipMemcpyDeviceToHost));
     int rc = WHF_OK;
+    uint64_t p2[5] = {0, 0, 0, 0, 0};
+    double p2_ms = 0.0;
+    if (req->phase2 != WHF_P2_NONE) {
+        // lysh's phase 2 on the survivors (lysh_grade_scan, eval.c:381-477): the hut verdict, keep the accepted huts
+        if (hits_total > host_hits.size()) return WHF_EOVERFLOW;
+        const auto t0 = std::chrono::steady_clock::now();
+        int tree = req->biome_tree - 1;
+        if (req->biome_tree == 0) {
+            const char *dflt = (req->vflags & WHF_V_118) ? "1.18.2" : ((req->vflags & WHF_V_262) ? "26.2" : "1.21.5");
+            if (whf_biome_tree(dflt, &tree) != WHF_OK) return WHF_EINVAL;
+        }
+        std::vector<whf_hut_query> hq(host_hits.size());
+        std::vector<whf_hut_result> hr(host_hits.size());
+        for (size_t i = 0; i < host_hits.size(); i++) hq[i] = whf_hut_query{host_hits[i].seed, host_hits[i].cx, host_hits[i].cz};
+        const int rc2 = whf_hut_eval(c, hq.data(), hq.size(), tree, req->vflags & WHF_V_LARGE, req->phase2_max_y, hr.data());
+        if (rc2 != WHF_OK) return rc2;
+        size_t kept = 0;
+        for (size_t i = 0; i < host_hits.size(); i++) {
+            const whf_hut_result &h = hr[i];
+            p2[0]++;
+            bool ok = h.ok != 0;
+            int reject = h.reject;
+            if (ok && req->phase2 == WHF_P2_EXACTLY && h.avg_y != req->phase2_max_y) { ok = false; reject = WHF_HUT_REJECT_Y; }
+            if (ok) {
+                p2[1]++;
+                DevHit d = host_hits[i];
+                d.floor_y = h.avg_y;
+                d.flags |= WHF_HIT_PHASE2;
+                host_hits[kept++] = d;
+            } else {
+                p2[reject == WHF_HUT_REJECT_BIOME ? 2 : (reject == WHF_HUT_REJECT_Y ? 3 : 4)]++;
+            }
+        }
+        host_hits.resize(kept);
+        hits_total = kept;
+        p2_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
     if (sink) {
         sink->count = hits_total;
         const uint64_t m = std::min<uint64_t>(sink->capacity, host_hits.size());
@@ -1141,6 +1181,8 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
         stats->reached_ladder = cnt[CNT_LADDER];
         for (int s = 0; s < SET_COUNT; s++) stats->samplers_built[s] = cnt[CNT_BUILT + s];
         stats->fp64_rechecks = cnt[CNT_QX];
+        for (int k = 0; k < 5; k++) stats->phase2[k] = p2[k];
+        stats->phase2_ms = p2_ms;
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
         stats->device_ms = ms;

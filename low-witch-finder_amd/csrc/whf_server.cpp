@@ -97,6 +97,8 @@ void add_stats(whf_stats &a, const whf_stats &b) {
     a.device_ms += b.device_ms;
     for (int k = 0; k < 8; k++) a.kernel_ms[k] += b.kernel_ms[k];
     a.fp64_rechecks += b.fp64_rechecks;
+    for (int k = 0; k < 5; k++) a.phase2[k] += b.phase2[k];
+    a.phase2_ms += b.phase2_ms;
 }
 
 bool hit_less(const whf_hit &x, const whf_hit &y) {

@@ -38,7 +38,9 @@ struct Params {
     int maxHeight = -40;
     int vflags = 0;
     bool have_version = false, v262 = false, stats = false, fp32 = false;
-    std::string out;
+    std::string out, version;
+    bool have_p2 = false, exactly = false;
+    int p2_max_y = 0;
 };
 
 bool read_seed_list(const char *path, std::vector<int64_t> &out) {
@@ -77,6 +79,9 @@ void usage() {
         "  --min-region-z <n>    default -128\n"
         "  --max-region-z <n>    default  128   (exclusive)\n"
         "  --max-height <n>      phase-1 hut height, default -40 (below -50 counts as -50)\n"
+        "  --phase2-max-y <y>    also run the hut evaluation (lysh hut) and keep the huts it accepts with floor <= y;\n"
+        "                        gpuEst is then the hut's floor\n"
+        "  --or-lower | --exactly  with --phase2-max-y: floor <= y (default) or floor == y\n"
         "  --version <v>         game version rules (lysh names; default 1.19.x~26.1)\n"
         "  --v26-2               26.2 ridge rule\n"
         "  --out <file>          write candidates as CSV (seed,chunkX,chunkZ,blockX,blockZ,gpuEst)\n"
@@ -124,7 +129,11 @@ bool parse(const std::vector<std::string> &t, Params &p) {
             std::string v = next(a.c_str());
             if (ok && whf_version_flags(v.c_str(), &p.vflags) != WHF_OK) { std::fprintf(stderr, "unknown --version %s\n", v.c_str()); ok = false; }
             p.have_version = true;
+            p.version = v;
         }
+        else if (a == "--phase2-max-y") { std::string v = next(a.c_str()); p.p2_max_y = std::atoi(v.c_str()); p.have_p2 = true; }
+        else if (a == "--exactly") p.exactly = true;
+        else if (a == "--or-lower") p.exactly = false;
         else if (a == "--v26-2") p.v262 = true;
         else if (a == "--out") p.out = next("--out");
         else if (a == "--stats") p.stats = true;
@@ -165,6 +174,13 @@ int run(whf_ctx *ctx, const Params &p) {
     r.rx0 = p.minRX; r.rx1 = p.maxRX; r.rz0 = p.minRZ; r.rz1 = p.maxRZ;
     r.max_height = p.maxHeight < -50 ? -50 : p.maxHeight;
     r.vflags = p.vflags;
+    if (p.have_p2) {
+        r.phase2 = p.exactly ? WHF_P2_EXACTLY : WHF_P2_OR_LOWER;
+        r.phase2_max_y = p.p2_max_y;
+        int tree = 0;
+        // a version with a tree of its own selects it, a rule-set alias takes its default (eval.c:104-106)
+        r.biome_tree = whf_biome_tree(p.version.empty() ? "1.19.x~26.1" : p.version.c_str(), &tree) == WHF_OK ? tree + 1 : 0;
+    }
     std::vector<whf_hit> hits(1 << 16);
     whf_sink sink{hits.data(), hits.size(), 0};
     whf_stats st;
@@ -194,16 +210,23 @@ int run(whf_ctx *ctx, const Params &p) {
         for (int k = 0; k < WHF_KIND_N; k++) std::printf("%s%" PRIu64, k ? ", " : "", st.funnel[k]);
         std::printf("], \"tiers\": [");
         for (int k = 0; k < 7; k++) std::printf("%s%" PRIu64, k ? ", " : "", st.tiers[k]);
-        std::printf("], \"samplers_built\": [%" PRIu64 ", %" PRIu64 ", %" PRIu64 "], \"device_ms\": %.3f}\n",
-                    st.samplers_built[0], st.samplers_built[1], st.samplers_built[2], st.device_ms);
+        std::printf("], \"samplers_built\": [%" PRIu64 ", %" PRIu64 ", %" PRIu64 "], \"device_ms\": %.3f, \"phase2\": [%" PRIu64
+                    ", %" PRIu64 ", %" PRIu64 ", %" PRIu64 ", %" PRIu64 "]}\n",
+                    st.samplers_built[0], st.samplers_built[1], st.samplers_built[2], st.device_ms, st.phase2[0],
+                    st.phase2[1], st.phase2[2], st.phase2[3], st.phase2[4]);
     }
     if (!p.out.empty()) {
         FILE *f = std::fopen(p.out.c_str(), "w");
         if (!f) { std::fprintf(stderr, "cannot write %s\n", p.out.c_str()); return 1; }
         std::fprintf(f, "seed,chunkX,chunkZ,blockX,blockZ,gpuEst\n");
-        // gpuEst: phase 1 computes no height estimate; the column stays for readers that expect six fields
-        for (const whf_hit &h : hits)
-            std::fprintf(f, "%lld,%d,%d,%d,%d,nan\n", (long long)h.seed, h.cx, h.cz, h.cx * 16, h.cz * 16);
+        // gpuEst: the hut's floor with --phase2-max-y; phase 1 alone computes no height, and the column stays for
+        // readers that expect six fields
+        for (const whf_hit &h : hits) {
+            if (h.flags & WHF_HIT_PHASE2)
+                std::fprintf(f, "%lld,%d,%d,%d,%d,%.3f\n", (long long)h.seed, h.cx, h.cz, h.cx * 16, h.cz * 16, (double)h.floor_y);
+            else
+                std::fprintf(f, "%lld,%d,%d,%d,%d,nan\n", (long long)h.seed, h.cx, h.cz, h.cx * 16, h.cz * 16);
+        }
         std::fclose(f);
         std::printf("wrote %zu candidates to %s\n", hits.size(), p.out.c_str());
     }

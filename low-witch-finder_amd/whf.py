@@ -25,7 +25,8 @@ SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags"
            "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_room_floors",
            "whf_server_open", "whf_server_close", "whf_submit", "whf_submit_deepdive", "whf_poll",
            "whf_biome_tree", "whf_biome", "whf_surface_heights", "whf_post_carver_heights", "whf_hut_eval")
-HIT_DEEPDIVE = 1
+HIT_DEEPDIVE, HIT_PHASE2 = 1, 2
+P2_NONE, P2_OR_LOWER, P2_EXACTLY = 0, 1, 2
 BIOME_SWAMP, BIOME_QY0, BIOME_QY_N = 6, -17, 97
 OCC_SURFACE = -(1 << 31)
 BLOCK_SOLID, BLOCK_AIR, BLOCK_WATER, BLOCK_LAVA = 0, 1, 2, 3
@@ -41,7 +42,7 @@ class Request(ctypes.Structure):
                 ("stream", ctypes.c_uint64), ("offset", ctypes.c_uint64),
                 ("rx0", ctypes.c_int), ("rx1", ctypes.c_int), ("rz0", ctypes.c_int), ("rz1", ctypes.c_int),
                 ("max_height", ctypes.c_int), ("vflags", ctypes.c_int), ("salt", ctypes.c_int),
-                ("reserved", ctypes.c_int)]
+                ("phase2", ctypes.c_int), ("phase2_max_y", ctypes.c_int), ("biome_tree", ctypes.c_int)]
 
 
 class Hit(ctypes.Structure):
@@ -58,7 +59,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("seeds", ctypes.c_uint64), ("checks", ctypes.c_uint64), ("accepted", ctypes.c_uint64),
                 ("reached_ladder", ctypes.c_uint64), ("funnel", ctypes.c_uint64 * KIND_N),
                 ("tiers", ctypes.c_uint64 * 8), ("samplers_built", ctypes.c_uint64 * 3), ("device_ms", ctypes.c_double),
-                ("kernel_ms", ctypes.c_double * 8), ("fp64_rechecks", ctypes.c_uint64)]
+                ("kernel_ms", ctypes.c_double * 8), ("fp64_rechecks", ctypes.c_uint64),
+                ("phase2", ctypes.c_uint64 * 5), ("phase2_ms", ctypes.c_double)]
 
 
 class DdRequest(ctypes.Structure):
@@ -234,27 +236,12 @@ class Context:
         return self.L.whf_device_name(self.h).decode()
 
     def scan(self, rect, seeds=None, splitmix=None, counter=None, max_height=-50, vflags=0, capacity=1 << 16,
-             salt=0, precision=PREC_FP64):
-        """Phase-1 Seed Check. rect = (rx0, rx1, rz0, rz1); seeds = int64 array, or splitmix/counter =
-        (stream, count[, offset]). Returns (hits as list of tuples (seed, rx, rz, cx, cz), stats dict)."""
-        req = Request()
-        req.precision = precision
-        keep = None
-        if seeds is not None:
-            keep = np.ascontiguousarray(seeds, dtype=np.int64)
-            req.seed_mode = SEEDS_LIST
-            req.seeds = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
-            req.n_seeds = len(keep)
-        else:
-            gen = splitmix if splitmix is not None else counter
-            req.seed_mode = SEEDS_SPLITMIX if splitmix is not None else SEEDS_COUNTER
-            req.stream = gen[0] & 0xFFFFFFFFFFFFFFFF
-            req.n_seeds = gen[1]
-            req.offset = gen[2] if len(gen) > 2 else 0
-        req.rx0, req.rx1, req.rz0, req.rz1 = rect
-        req.max_height = max_height
-        req.vflags = vflags
-        req.salt = salt
+             salt=0, precision=PREC_FP64, phase2=None):
+        """Seed Check. rect = (rx0, rx1, rz0, rz1); seeds = int64 array, or splitmix/counter = (stream, count[,
+        offset]); phase2 = (P2_OR_LOWER or P2_EXACTLY, max_y[, tree]) adds lysh's hut verdict. Returns (hits, stats):
+        hits are tuples (seed, rx, rz, cx, cz), with the hut's floor appended under phase 2."""
+        req, keep = make_request(rect, seeds=seeds, splitmix=splitmix, counter=counter, max_height=max_height,
+                                 vflags=vflags, salt=salt, precision=precision, phase2=phase2)
         buf = (Hit * capacity)()
         sink = Sink(buf, capacity, 0)
         st = Stats()
@@ -262,11 +249,15 @@ class Context:
         if rc not in (OK, EOVERFLOW):
             raise WhfError(rc, "whf_scan")
         n = min(sink.count, capacity)
-        hits = sorted((buf[i].seed, buf[i].rx, buf[i].rz, buf[i].cx, buf[i].cz) for i in range(n))
+        if phase2 is not None:
+            hits = sorted((buf[i].seed, buf[i].rx, buf[i].rz, buf[i].cx, buf[i].cz, buf[i].floor_y) for i in range(n))
+        else:
+            hits = sorted((buf[i].seed, buf[i].rx, buf[i].rz, buf[i].cx, buf[i].cz) for i in range(n))
         stats = dict(seeds=st.seeds, checks=st.checks, accepted=st.accepted, reached_ladder=st.reached_ladder,
                      funnel=list(st.funnel), tiers=list(st.tiers)[:7], samplers_built=list(st.samplers_built),
                      device_ms=st.device_ms, kernel_ms=dict(zip(KERNEL_FAMILIES, list(st.kernel_ms))),
-                     hit_count=sink.count, overflow=rc == EOVERFLOW, fp64_rechecks=st.fp64_rechecks)
+                     hit_count=sink.count, overflow=rc == EOVERFLOW, fp64_rechecks=st.fp64_rechecks,
+                     phase2=list(st.phase2), phase2_ms=st.phase2_ms)
         return hits, stats
 
     def probe(self, seeds, xs, zs, max_height=-50, vflags=0):
@@ -413,7 +404,7 @@ def _s64(v):
 
 
 def make_request(rect, seeds=None, splitmix=None, counter=None, max_height=-50, vflags=0, salt=0,
-                 precision=PREC_FP64):
+                 precision=PREC_FP64, phase2=None):
     """A whf_request and the array it points to (keep both alive while the request runs)."""
     req = Request()
     req.precision = precision
@@ -433,13 +424,16 @@ def make_request(rect, seeds=None, splitmix=None, counter=None, max_height=-50, 
     req.max_height = max_height
     req.vflags = vflags
     req.salt = salt
+    if phase2 is not None:   # (mode, max_y, tree or None)
+        req.phase2, req.phase2_max_y = phase2[0], phase2[1]
+        req.biome_tree = 0 if len(phase2) < 3 or phase2[2] is None else phase2[2] + 1
     return req, keep
 
 
 def stats_dict(st):
     return dict(seeds=st.seeds, checks=st.checks, accepted=st.accepted, reached_ladder=st.reached_ladder,
                 funnel=list(st.funnel), tiers=list(st.tiers)[:7], samplers_built=list(st.samplers_built),
-                device_ms=st.device_ms, fp64_rechecks=st.fp64_rechecks)
+                device_ms=st.device_ms, fp64_rechecks=st.fp64_rechecks, phase2=list(st.phase2))
 
 
 class Server:

@@ -459,5 +459,8 @@ class Terrain:
         return {n: getattr(r, n) for n, _ in HutResult._fields_}
 
 
+WO_HUT_OK, WO_HUT_REJECT_Y, WO_HUT_REJECT_FLOOD, WO_HUT_REJECT_BIOME = 0, 1, 2, 3
+
+
 def hut_orientation(seed, hut_x, hut_z):
     return lib().wo_hut_orientation(u64(seed), hut_x, hut_z)
