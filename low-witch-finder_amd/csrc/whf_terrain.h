@@ -55,6 +55,32 @@ __device__ __forceinline__ double cube(double t) {
     return q + fma(e1, t, e2);
 }
 
+// The aquifer tables of one chunk, computed once per chunk and read by all of its columns (the game memoizes the same
+// values per quart position and per aquifer grid cell). Quart table: the preliminary surface and the deep-dark offset
+// (Terr::quart_surface) at quart (4 cx - 16 + i, 4 cz - 8 + j), entry i * kAqQz + j; that covers the 13 surface
+// probes of computeFluid around every anchor and column of the chunk, and the 11 x 11 skipSamplingAboveY grid.
+// Cell table: the anchor (x, y, z) and the fluid (level * 4 + type) of grid cell (cx - 1 + i, gy, cz - 1 + k), entry
+// ((gy - kAqGy0) * 3 + k) * 3 + i, for gy in [-6, 27], the cells computeSubstance reads for y in [-54, 319].
+constexpr int kAqQx = 27, kAqQz = 19, kAqQuarts = kAqQx * kAqQz;
+constexpr int kAqGy0 = -6, kAqGyN = 34, kAqCells = 3 * 3 * kAqGyN;
+
+struct ChunkAq {
+    const int *surface;    // [kAqQuarts]
+    const double *dd;      // [kAqQuarts]
+    const int4 *cells;     // [kAqCells]
+    int cx, cz;
+
+    __device__ int quart(int x, int z) const { return ((x >> 2) - (4 * cx - 16)) * kAqQz + ((z >> 2) - (4 * cz - 8)); }
+    __device__ int prelim(int x, int z) const { return surface[quart(x, z)]; }
+    __device__ bool deep_dark(int x, int y, int z) const {
+        // aquifer.c:218-228: erosion below -0.225 (folded into the offset) and depth above 0.9
+        return y_gradient((double)y, -64.0, 320.0, 1.5, -1.5) + dd[quart(x, z)] > 0.8999999761581421;
+    }
+    __device__ int4 cell(int gx, int gy, int gz) const {
+        return cells[((gy - kAqGy0) * 3 + (gz - (cz - 1))) * 3 + (gx - (cx - 1))];
+    }
+};
+
 // One seed's view of the terrain stage's tables.
 struct Terr {
     const TerrainTables *tt;
@@ -92,10 +118,9 @@ struct Terr {
         tp[2] = spline_at<kSplineMaxDepth>(S, S->root[2], cf, ef, folded, wf);
     }
 
-    // offset and factor of the preliminary surface, in double over the float spline (column_top.c:57-77)
-    __device__ void offset_factor(int x, int z, double &offset, double &factor) const {
-        double c, w, e;
-        info(x >> 2, z >> 2, c, w, e);
+    // offset and factor of the preliminary surface, in double over the float spline (column_top.c:57-77), from the
+    // quart position's continentalness, weirdness and erosion
+    __device__ void offset_factor(double c, double w, double e, double &offset, double &factor) const {
         const float cf = (float)c, ef = (float)e, wf = (float)w, nw = normalized_weirdness(wf);
         const double ro = (double)spline_at<kSplineMaxDepth>(S, S->root[0], cf, ef, nw, wf);
         const double rf = (double)spline_at<kSplineMaxDepth>(S, S->root[1], cf, ef, nw, wf);
@@ -241,9 +266,18 @@ struct Terr {
     }
 
     // ---- preliminary surface level (column_top.c:28-122) ----
-    __device__ int prelim_surface(int x, int z) const {
-        double offset, factor;
-        offset_factor(x, z, offset, factor);
+    // It depends on the block position through its quart position only (offset and factor are sampled there), as
+    // the game's per-quart cache of it assumes; so does the deep-dark test of the aquifer. One evaluation gives
+    // both: the surface, and the deep-dark offset (the offset where erosion is below -0.225, else -infinity, so
+    // that y_gradient + it > 0.9 is the test of aquifer.c:218-228 at every y).
+    __device__ void quart_surface(int qx, int qz, int &surface, double &dd_offset) const {
+        double c, w, e, offset, factor;
+        info(qx, qz, c, w, e);
+        offset_factor(c, w, e, offset, factor);
+        surface = surface_from(offset, factor);
+        dd_offset = e < -0.22499999403953552 ? offset : -__builtin_huge_val();
+    }
+    __device__ static int surface_from(double offset, double factor) {
         const double inner = (1.0 / factor) * 0.2734375 + offset * -1.0;
         const double ub = jclamp(128.0 + -128.0 * inner, -40.0, 320.0);
         const int y0 = (int)(ub / 8.0) * 8;
@@ -268,18 +302,9 @@ struct Terr {
     }
     __device__ static int level_block(int level, int type, int y) { return y < level ? type : BS_AIR; }
 
-    __device__ bool deep_dark(int x, int y, int z) const {
-        // aquifer.c:218-228: erosion below -0.225 and depth above 0.9 (float constants widened)
-        double c, w, e;
-        info(x >> 2, z >> 2, c, w, e);
-        if (!(e < -0.22499999403953552)) return false;
-        double offset, factor;
-        offset_factor(x, z, offset, factor);
-        return y_gradient((double)y, -64.0, 320.0, 1.5, -1.5) + offset > 0.8999999761581421;
-    }
-
-    __device__ void fluid(int x, int y, int z, int &out_level, int &out_type) const {
-        // aquifer.c:286-331 computeFluid with computeSurfaceLevel (:248-273) and computeFluidType (:277-289)
+    // computeFluid (aquifer.c:286-331) with computeSurfaceLevel (:248-273) and computeFluidType (:277-289); the
+    // preliminary surfaces and the deep-dark test come from the chunk's quart table
+    __device__ void fluid(const ChunkAq &A, int x, int y, int z, int &out_level, int &out_type) const {
         const int off[13][2] = {{0, 0}, {-2, -1}, {-1, -1}, {0, -1}, {1, -1}, {-3, 0}, {-2, 0},
                                 {-1, 0}, {1, 0}, {-2, 1}, {-1, 1}, {0, 1}, {1, 1}};
         int gl, gt;
@@ -287,7 +312,7 @@ struct Terr {
         int min_top = 0x7FFFFFFF;
         bool has_above = false;
         for (int i = 0; i < 13; i++) {
-            const int top = prelim_surface(x + off[i][0] * 16, z + off[i][1] * 16);
+            const int top = A.prelim(x + off[i][0] * 16, z + off[i][1] * 16);
             const int adj = top + 8;
             const bool center = i == 0;
             if (center && y - 12 > adj) { out_level = gl; out_type = gt; return; }
@@ -303,7 +328,7 @@ struct Terr {
             if (top < min_top) min_top = top;
         }
         int level;
-        if (deep_dark(x, y, z)) {
+        if (A.deep_dark(x, y, z)) {
             level = kNoFluid;
         } else {
             const int d = (min_top + 8) - y;
@@ -374,8 +399,9 @@ struct Terr {
         return 2.0 * (inner + o);
     }
 
-    // computeSubstance's block state (aquifer.c:416-563); skip_above is the column chunk's skipSamplingAboveY
-    __device__ int substance(int x, int y, int z, double density, int skip_above) const {
+    // computeSubstance's block state (aquifer.c:416-563); skip_above is the column chunk's skipSamplingAboveY, the
+    // grid cells' anchors and fluids (computed lazily by the reference, once per cell) come from the chunk's table
+    __device__ int substance(const ChunkAq &A, int x, int y, int z, double density, int skip_above) const {
         if (density > 0.0) return BS_SOLID;
         int gl, gt;
         global_fluid(y, gl, gt);
@@ -383,13 +409,12 @@ struct Terr {
         if (level_block(gl, gt, y) == BS_LAVA) return BS_LAVA;
         const int gx = floor_div(x - 5, 16), gy = floor_div(y + 1, 12), gz = floor_div(z - 5, 16);
         int d[4] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF};
-        int an[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        int4 an[4] = {make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
         for (int i = 0; i <= 1; i++)
             for (int j = -1; j <= 1; j++)
                 for (int k = 0; k <= 1; k++) {
-                    int ax, ay, az;
-                    anchor(gx + i, gy + j, gz + k, ax, ay, az);
-                    const int ddx = ax - x, ddy = ay - y, ddz = az - z;
+                    const int4 a = A.cell(gx + i, gy + j, gz + k);
+                    const int ddx = a.x - x, ddy = a.y - y, ddz = a.z - z;
                     const int dist = ddx * ddx + ddy * ddy + ddz * ddz;
                     // the four nearest, a later cell taking a tie's place (aquifer.c:478-491)
                     int at = 4;
@@ -398,13 +423,12 @@ struct Terr {
                     if (at == 4) continue;
                     for (int s = 3; s > at; s--) {
                         d[s] = d[s - 1];
-                        an[s][0] = an[s - 1][0]; an[s][1] = an[s - 1][1]; an[s][2] = an[s - 1][2];
+                        an[s] = an[s - 1];
                     }
                     d[at] = dist;
-                    an[at][0] = ax; an[at][1] = ay; an[at][2] = az;
+                    an[at] = a;
                 }
-        int l0, t0, l1, t1, l2, t2;
-        fluid(an[0][0], an[0][1], an[0][2], l0, t0);
+        const int l0 = an[0].w >> 2, t0 = an[0].w & 3;
         const double s01 = 1.0 - (double)(d[1] - d[0]) / 25.0;
         const int result = level_block(l0, t0, y);
         if (!(s01 > 0.0)) return result;
@@ -415,9 +439,9 @@ struct Terr {
         }
         double barrier = 0.0;
         bool have_barrier = false;
-        fluid(an[1][0], an[1][1], an[1][2], l1, t1);
+        const int l1 = an[1].w >> 2, t1 = an[1].w & 3;
         if (density + s01 * pressure(x, y, z, barrier, have_barrier, l0, t0, l1, t1) > 0.0) return BS_SOLID;
-        fluid(an[2][0], an[2][1], an[2][2], l2, t2);
+        const int l2 = an[2].w >> 2, t2 = an[2].w & 3;
         const double s02 = 1.0 - (double)(d[2] - d[0]) / 25.0;
         if (s02 > 0.0 && density + s01 * s02 * pressure(x, y, z, barrier, have_barrier, l0, t0, l2, t2) > 0.0)
             return BS_SOLID;

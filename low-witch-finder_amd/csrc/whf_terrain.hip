@@ -1,23 +1,20 @@
-// whf_terrain.hip — the terrain stage on gfx950: the pre-carver surface height of a batch of (seed, column) queries
-// from the overworld's interpolated final density and aquifer.
+// whf_terrain.hip — the terrain stage on gfx950: pre- and post-carver surface heights of (seed, column) queries from
+// the overworld's interpolated final density, the aquifer and the carvers, and lysh's hut evaluation over them.
 //
+// Columns are grouped by chunk; what the columns of a chunk share is computed once per chunk:
 //   k_build          the 144 named-noise samplers of a seed (climate, density, noodle and aquifer noises)
 //   k_build_blended  one lane per seed: the blended noise's 40 samplers, built in sequence from one random as the
 //                    reference builds them, and the aquifer's positional random factory
-//   k_surface        one 256-lane block per column:
-//                      A  the 121 preliminary-surface samples of the column chunk's aquifer grid (their maximum gives
-//                         skipSamplingAboveY) and the four corner terrain points;
-//                      B  the 4 x 49 corner samples of the noise column and the 4 x 3 x 49 noodle children;
-//                      C  the final density at every y (trilinear over the corners, squeezed, min with the noodle);
-//                         the top solid block y_d is its highest positive density;
-//                      D  above y_d, the aquifer decides each block up to skipSamplingAboveY, the sea above it; the
-//                         height is one above the highest block that is not air;
-//                      E  (hut evaluation) below that surface, the cells the carvers claim turn into the carver's
-//                         air or fluid where their block allows it; the post-carver height is one above the highest
-//                         block that is then not air; and whether water stands above y = 62 in the column.
-//   k_claims         one 256-lane block per target chunk: the cells its 17 x 17 x 3 carvers claim (whf_carver.h)
-// This is work per phase-1 survivor (one in 1e5..1e7 regions), so it is laid out for clarity of the reference's
-// order of operations, not for throughput: every lane calls the same pointwise functions the reference calls.
+//   k_quarts         a lane per quart of the chunk's 27 x 19 aquifer quart table (preliminary surface, deep-dark
+//                    offset), and a lane per corner column the chunk's columns use (its terrain point)
+//   k_cells          a block per chunk: skipSamplingAboveY from the 11 x 11 surface grid, then a lane per aquifer
+//                    grid cell in reach (anchor and computeFluid)
+//   k_corners        a lane per corner sample: 49 noise-column values and 3 x 49 noodle children per corner column
+//   k_claims         a block per chunk: the cells its 17 x 17 x 3 carvers claim (whf_carver.h)
+//   k_column         a wave per column: the final density top-down in windows of 64 blocks, the top solid block,
+//                    the aquifer's blocks above it (pre-carver height), then below it the carved cells' states
+//                    (post-carver height) and the flood test's fluid at y = 62
+// Work here is per phase-1 survivor, a hut at a time: a round holds up to 512 chunks so that the device is full.
 #include <cstring>
 #include <unordered_map>
 #include <vector>
@@ -30,35 +27,78 @@ using namespace whf;
 
 namespace {
 
-constexpr uint32_t kTerBatch = 1024;   // columns per device round (distinct seeds per round at most this)
-constexpr int kTerSamplers = 144;      // named-noise samplers per seed
-constexpr int kCornerJobs = 4 * 49;    // main corner samples
-constexpr int kNoodleJobs = 4 * 3 * 49;
-constexpr uint32_t kHutBatch = 16;     // huts per device round (16 x 63 columns fit one round of columns)
-constexpr int kClaimWords = 256 * 6;   // a chunk's claimed-cell bits
+constexpr uint32_t kChunkBatch = 512;           // chunks (so also distinct seeds) per device round
+constexpr uint32_t kColBatch = kChunkBatch * 64; // columns per device round
+constexpr int kTerSamplers = 144;                // named-noise samplers per seed
+constexpr int kCornerCols = 25;                  // corner columns of a chunk's 4 x 4 cells, (qx - 4 cx) * 5 + qz - 4 cz
+constexpr int kClaimWords = 256 * 6;             // a chunk's claimed-cell bits
 
-// One column of a round: its seed slot, position, and the target chunk whose carver claims apply (-1: none).
+// One column of a round: its seed slot, position, and its chunk in the round.
 struct ColJob {
     uint32_t slot;
     int32_t x, z;
     int32_t chunk;
 };
 
+// A seed slot's sampler tables, by value into every kernel.
+struct TerArgs {
+    const TerrainTables *tt;
+    const SplineTab *spline;
+    const uint8_t *tab;
+    const SPar *par;
+    const uint8_t *ltab;
+    const SPar *lpar;
+    const ulonglong2 *aq;
+    __device__ Terr at(uint32_t slot) const {
+        const ulonglong2 f = aq[slot];
+        return Terr{tt, SetView{tab, par, kTerSamplers}, slot, ltab + (size_t)slot * kLegacyPerlins * 256,
+                    lpar + (size_t)slot * kLegacyPerlins, spline, f.x, f.y};
+    }
+};
+
+// Per-chunk tables of a round; chunk k's rows start at k times the row size.
+struct ChunkBufs {
+    const int4 *chunks;        // seed slot, chunk x, chunk z, corner-column mask
+    int *qsurf;                // [kAqQuarts]
+    double *qdd;               // [kAqQuarts]
+    int4 *cells;               // [kAqCells]
+    int *skip;                 // skipSamplingAboveY
+    float4 *tp;                // [kCornerCols] terrain points
+    double *cmain;             // [kCornerCols][49]
+    double *cnood;             // [kCornerCols][3][49]
+    unsigned long long *claims;   // [kClaimWords]
+    __device__ ChunkAq aq(uint32_t k) const {
+        const int4 ch = chunks[k];
+        return ChunkAq{qsurf + (size_t)k * kAqQuarts, qdd + (size_t)k * kAqQuarts, cells + (size_t)k * kAqCells, ch.y,
+                       ch.z};
+    }
+};
+
 struct TerBufs {
     TerrainTables *tt;        // [2]: NORMAL, LARGE_BIOMES
     BuildDesc *bd;            // [2][kTerSamplers]
     SplineTab *spline;
-    uint8_t *tab;             // [kTerBatch][kTerSamplers][256]
+    uint8_t *tab;             // [kChunkBatch][kTerSamplers][256]
     SPar *par;
-    uint8_t *ltab;            // [kTerBatch][40][256]
+    uint8_t *ltab;            // [kChunkBatch][40][256]
     SPar *lpar;
-    ulonglong2 *aq;           // [kTerBatch]
-    uint64_t *seeds;          // [kTerBatch]
-    ColJob *jobs;             // [kTerBatch]
-    int4 *out;                // [kTerBatch]: pre-carver height, its top state, post-carver height, water above 62
+    ulonglong2 *aq;           // [kChunkBatch]
+    uint64_t *seeds;          // [kChunkBatch]
     float *sin_tab;           // Mth.SIN
-    int4 *chunks;             // [kHutBatch]: seed slot, chunk x, chunk z
-    unsigned long long *claims;   // [kHutBatch][kClaimWords]
+    int4 *chunks;             // [kChunkBatch]
+    int2 *pairs;              // [kChunkBatch * kCornerCols]: (chunk, corner column) of the round's corner columns
+    int *qsurf;
+    double *qdd;
+    int4 *cells;
+    int *skip;
+    float4 *tp;
+    double *cmain;
+    double *cnood;
+    unsigned long long *claims;
+    ColJob *jobs;             // [kColBatch]
+    int4 *out;                // [kColBatch]: pre-carver height, its top state, post-carver height, water above 62
+    TerArgs args(int large) const { return TerArgs{tt + large, spline, tab, par, ltab, lpar, aq}; }
+    ChunkBufs chunk_bufs() const { return ChunkBufs{chunks, qsurf, qdd, cells, skip, tp, cmain, cnood, claims}; }
 };
 
 __global__ __launch_bounds__(64) void k_build_blended(const uint64_t *__restrict__ seeds, uint32_t ns, uint64_t t_hi,
@@ -89,147 +129,79 @@ __global__ __launch_bounds__(64) void k_build_blended(const uint64_t *__restrict
     aq[s] = f;
 }
 
-template <bool kPost>
-__global__ __launch_bounds__(256) void k_surface(const TerrainTables *__restrict__ tt, const SplineTab *__restrict__ S,
-                                                 const uint8_t *__restrict__ tab, const SPar *__restrict__ par,
-                                                 const uint8_t *__restrict__ ltab, const SPar *__restrict__ lpar,
-                                                 const ulonglong2 *__restrict__ aq, const ColJob *__restrict__ jobs,
-                                                 const unsigned long long *__restrict__ claims, int4 *__restrict__ out) {
-    __shared__ double s_main[4][49];
-    __shared__ double s_nood[4][3][49];
-    __shared__ double s_dens[384];
-    __shared__ float s_tp[4][3];
-    __shared__ int s_psl, s_top_solid, s_best, s_state, s_wet;
-    const uint32_t qi = blockIdx.x;
-    const int t = threadIdx.x;
-    const ColJob col = jobs[qi];
-    const uint32_t slot = col.slot;
-    const ulonglong2 f = aq[slot];
-    const Terr T{tt, SetView{tab, par, kTerSamplers}, slot, ltab + (size_t)slot * kLegacyPerlins * 256,
-                 lpar + (size_t)slot * kLegacyPerlins, S, f.x, f.y};
-    const int x = col.x, z = col.z;
-    if (t == 0) {
-        s_psl = -0x7FFFFFFF - 1;
-        s_top_solid = -65;
-        s_best = -0x7FFFFFFF - 1;
-        s_state = BS_AIR;
-        s_wet = 0;
+__global__ __launch_bounds__(64) void k_quarts(TerArgs ta, ChunkBufs cb, uint32_t nch, const int2 *__restrict__ pairs,
+                                               uint32_t npairs) {
+    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    if (i < nch * kAqQuarts) {
+        // the preliminary surface and deep-dark offset of a quart (column_top.c:85-122, aquifer.c:218-228)
+        const uint32_t k = i / kAqQuarts, q = i - k * kAqQuarts;
+        const int4 ch = cb.chunks[k];
+        const int qi = (int)q / kAqQz, qj = (int)q - qi * kAqQz;
+        int surface;
+        double dd;
+        ta.at((uint32_t)ch.x).quart_surface(4 * ch.y - 16 + qi, 4 * ch.z - 8 + qj, surface, dd);
+        cb.qsurf[i] = surface;
+        cb.qdd[i] = dd;
+    } else if (i < nch * kAqQuarts + npairs) {
+        // the terrain point of a corner column (phase2.c:71-83)
+        const int2 p = pairs[i - nch * kAqQuarts];
+        const int4 ch = cb.chunks[p.x];
+        float t[3];
+        ta.at((uint32_t)ch.x).point261(4 * ch.y + p.y / 5, 4 * ch.z + p.y % 5, t);
+        cb.tp[(size_t)p.x * kCornerCols + p.y] = make_float4(t[0], t[1], t[2], 0.0f);
     }
-    __syncthreads();
+}
 
-    // A: skipSamplingAboveY of the column's chunk (aquifer.c:160-196): preliminary surfaces on the 11 x 11 grid of
-    // step 4 from the chunk's aquifer grid corner; corner terrain points of the column's cell (phase2.c:71-83)
-    const int min_gx = ((x >> 4) * 16 - 5) >> 4, min_gz = ((z >> 4) * 16 - 5) >> 4;
-    if (t < 121) {
-        atomicMax(&s_psl, T.prelim_surface(min_gx * 16 + 4 * (t % 11), min_gz * 16 + 4 * (t / 11)));
-    } else if (t >= 128 && t < 132) {
-        const int c = t - 128;
-        T.point261((x >> 2) + (c & 1), (z >> 2) + (c >> 1), s_tp[c]);
-    } else if (kPost && t == 192) {
-        // the hut's flood test for this column: water above y = 62 (eval.c:151-168, computeFluid aquifer.c:286-331)
-        int lv, ty;
-        T.fluid(x, 62, z, lv, ty);
-        s_wet = (ty == BS_WATER && 62 < lv) ? 1 : 0;
-    }
+__global__ __launch_bounds__(256) void k_cells(TerArgs ta, ChunkBufs cb) {
+    __shared__ int s_psl;
+    const uint32_t k = blockIdx.x;
+    const int t = threadIdx.x;
+    const int4 ch = cb.chunks[k];
+    if (t == 0) s_psl = -0x7FFFFFFF - 1;
+    __syncthreads();
+    // skipSamplingAboveY (aquifer.c:160-196): the preliminary surfaces on the 11 x 11 grid of step 4 from the
+    // chunk's aquifer grid corner (cx - 1, cz - 1), quarts 4 cx - 4 + u, 4 cz - 4 + v of the quart table
+    const int *qs = cb.qsurf + (size_t)k * kAqQuarts;
+    if (t < 121) atomicMax(&s_psl, qs[(12 + t % 11) * kAqQz + 4 + t / 11]);
     __syncthreads();
     const int skip_above = (floor_div((s_psl + 8) + 12, 12) + 1) * 12 + 11 - 1;
-
-    // B: corner samples at y = -64 + 8 i of the corners (4 cx, 4 cz) .. (4 cx + 4, 4 cz + 4)
-    for (int j = t; j < kCornerJobs + kNoodleJobs; j += 256) {
-        if (j < kCornerJobs) {
-            const int c = j / 49, i = j - c * 49;
-            const int bx = ((x >> 2) + (c & 1)) * 4, bz = ((z >> 2) + (c >> 1)) * 4, y = -64 + 8 * i;
-            s_main[c][i] = T.column(bx, y, bz, s_tp[c], T.blended(bx, y, bz));
-        } else {
-            const int jj = j - kCornerJobs;
-            const int c = jj / 147, r = jj - c * 147, w = r / 49, i = r - w * 49;
-            const int bx = ((x >> 2) + (c & 1)) * 4, bz = ((z >> 2) + (c >> 1)) * 4;
-            s_nood[c][w][i] = T.noodle(w + 1, bx, -64 + 8 * i, bz);
-        }
+    if (t == 0) cb.skip[k] = skip_above;
+    // anchors and fluids of the grid cells computeSubstance can read: y <= min(skipSamplingAboveY, 319)
+    const int gy_hi = min(kAqGy0 + kAqGyN - 1, floor_div(min(skip_above, 319) + 1, 12) + 1);
+    const int n = (gy_hi - kAqGy0 + 1) * 9;
+    const Terr T = ta.at((uint32_t)ch.x);
+    const ChunkAq A = cb.aq(k);
+    for (int c = t; c < n; c += 256) {
+        const int gy = kAqGy0 + c / 9, r = c % 9;
+        int ax, ay, az, lv, ty;
+        T.anchor(ch.y - 1 + r % 3, gy, ch.z - 1 + r / 3, ax, ay, az);
+        T.fluid(A, ax, ay, az, lv, ty);
+        cb.cells[(size_t)k * kAqCells + c] = make_int4(ax, ay, az, lv * 4 + ty);
     }
-    __syncthreads();
+}
 
-    // C: the final density of every block of the column (phase2.c:161-194), and the highest solid one
-    const double dx = (double)(x & 3) / 4.0, dz = (double)(z & 3) / 4.0;
-    for (int yi = t; yi < 384; yi += 256) {
-        const int y = -64 + yi;
-        const int cy = yi / 8;
-        const double dy = (double)(yi - cy * 8) / 8.0;
-        const double v = cell_lerp(dx, dy, dz, s_main[0], s_main[1], s_main[2], s_main[3], cy);
-        const double a = jclamp(v * 0.64, -1.0, 1.0);
-        const double squeezed = a / 2.0 - a * a * a / 24.0;
-        double ch[3];
-        for (int w = 0; w < 3; w++) ch[w] = cell_lerp(dx, dy, dz, s_nood[0][w], s_nood[1][w], s_nood[2][w], s_nood[3][w], cy);
-        const double n = T.noodle(0, x, y, z);
-        const double noodle = (n >= -1000000.0 && n < 0.0) ? 64.0 : ch[0] + 1.5 * jmax(fabs(ch[1]), fabs(ch[2]));
-        const double d = jmin(squeezed, noodle);
-        s_dens[yi] = d;
-        if (d > 0.0) atomicMax(&s_top_solid, y);
-    }
-    __syncthreads();
-
-    // D: blocks above the top solid one are air or fluid: at or below skipSamplingAboveY the aquifer decides
-    // (a barrier can also make one solid), above it the sea (water below y = 63). Top-down rounds of 256 blocks.
-    const int y_d = s_top_solid;
-    const int y_hi = min(skip_above, 319);
-    for (int y_top = y_hi; y_top > y_d; y_top -= 256) {
-        const int y = y_top - t;
-        int st = BS_AIR;
-        if (y > y_d) {
-            st = T.substance(x, y, z, s_dens[y + 64], skip_above);
-            if (st != BS_AIR) atomicMax(&s_best, y);
-        }
-        __syncthreads();
-        const int best = s_best;
-        __syncthreads();   // every lane has read s_best before a next round could raise it
-        if (best != -0x7FFFFFFF - 1) {
-            if (y == best) s_state = st;
-            break;
-        }
-    }
-    __syncthreads();
-    int h = y_d, st = BS_SOLID;   // uniform: every lane reads the same shared values
-    if (s_best != -0x7FFFFFFF - 1 && s_best > h) { h = s_best; st = s_state; }
-    if (skip_above < 62 && 62 > h) { h = 62; st = BS_WATER; }   // the sea above the aquifer's reach
-    const int pre = h >= -64 ? h + 1 : -64;
-    int post = pre;
-
-    if constexpr (kPost) {
-        // E: post-carver surface (phase2.c:337-345, 438-456): below the pre-carver surface, a claimed cell whose
-        // block is solid or water becomes lava at y <= -56 and elsewhere the aquifer's substance at density 0,
-        // unless that is solid (carver.c:145-175); other cells keep their block state. Top-down rounds of 256.
-        const unsigned long long *cw =
-            col.chunk >= 0 ? claims + (size_t)col.chunk * kClaimWords + (size_t)(((z & 15) * 16 + (x & 15)) * 6) : nullptr;
-        __syncthreads();
-        if (t == 0) s_best = -0x7FFFFFFF - 1;
-        __syncthreads();
-        int found = -0x7FFFFFFF - 1;
-        for (int y_top = pre - 1; y_top >= -64; y_top -= 256) {
-            const int y = y_top - t;
-            if (y >= -64) {
-                const double d = s_dens[y + 64];
-                int bs = d > 0.0 ? BS_SOLID : T.substance(x, y, z, d, skip_above);
-                const int b = y + 64;
-                if (cw && ((cw[b >> 6] >> (b & 63)) & 1ull) && (bs == BS_SOLID || bs == BS_WATER)) {
-                    const int cs = y <= -56 ? BS_LAVA : T.substance(x, y, z, 0.0, skip_above);
-                    if (cs != BS_SOLID) bs = cs;
-                }
-                if (bs != BS_AIR) atomicMax(&s_best, y);
-            }
-            __syncthreads();
-            found = s_best;
-            __syncthreads();
-            if (found != -0x7FFFFFFF - 1) break;
-        }
-        post = found != -0x7FFFFFFF - 1 ? found + 1 : -64;
-    }
-    if (t == 0) {
-        int4 o;
-        o.x = pre;
-        o.y = h >= -64 ? st : BS_AIR;
-        o.z = post;
-        o.w = s_wet;
-        out[qi] = o;
+__global__ __launch_bounds__(64) void k_corners(TerArgs ta, ChunkBufs cb, const int2 *__restrict__ pairs,
+                                                uint32_t npairs) {
+    // corner samples at y = -64 + 8 i (phase2.c:108-130): the noise column over the blended noise, and the three
+    // noodle children
+    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    const uint32_t nm = npairs * 49;
+    if (i < nm) {
+        const uint32_t p = i / 49, yi = i - p * 49;
+        const int2 pr = pairs[p];
+        const int4 ch = cb.chunks[pr.x];
+        const Terr T = ta.at((uint32_t)ch.x);
+        const int bx = (4 * ch.y + pr.y / 5) * 4, bz = (4 * ch.z + pr.y % 5) * 4, y = -64 + 8 * (int)yi;
+        const float4 t4 = cb.tp[(size_t)pr.x * kCornerCols + pr.y];
+        const float tp[3] = {t4.x, t4.y, t4.z};
+        cb.cmain[((size_t)pr.x * kCornerCols + pr.y) * 49 + yi] = T.column(bx, y, bz, tp, T.blended(bx, y, bz));
+    } else if (i < nm + npairs * 147) {
+        const uint32_t j = i - nm, p = j / 147, r = j - p * 147, w = r / 49, yi = r - w * 49;
+        const int2 pr = pairs[p];
+        const int4 ch = cb.chunks[pr.x];
+        const Terr T = ta.at((uint32_t)ch.x);
+        const int bx = (4 * ch.y + pr.y / 5) * 4, bz = (4 * ch.z + pr.y % 5) * 4;
+        cb.cnood[((size_t)pr.x * kCornerCols + pr.y) * 147 + w * 49 + yi] = T.noodle((int)w + 1, bx, -64 + 8 * (int)yi, bz);
     }
 }
 
@@ -252,25 +224,145 @@ __global__ __launch_bounds__(256) void k_claims(const float *__restrict__ sin_ta
     for (int i = threadIdx.x; i < kClaimWords; i += 256) dst[i] = bits[i];
 }
 
+template <bool kPost>
+__global__ __launch_bounds__(64) void k_column(TerArgs ta, ChunkBufs cb, const ColJob *__restrict__ jobs,
+                                               int4 *__restrict__ out) {
+    __shared__ double s_dens[384];   // final density of y = -64 + i, filled top-down
+    const ColJob col = jobs[blockIdx.x];
+    const int lane = threadIdx.x;
+    const uint32_t k = (uint32_t)col.chunk;
+    const int4 ch = cb.chunks[k];
+    const Terr T = ta.at(col.slot);
+    const ChunkAq A = cb.aq(k);
+    const int skip_above = cb.skip[k];
+    const int x = col.x, z = col.z;
+    // the cell's corner columns in x-then-z order (phase2.c:132-150)
+    const int c0 = ((x >> 2) - 4 * ch.y) * 5 + ((z >> 2) - 4 * ch.z);
+    const double *m00 = cb.cmain + ((size_t)k * kCornerCols + c0) * 49;
+    const double *n00 = cb.cnood + ((size_t)k * kCornerCols + c0) * 147;
+    const double dx = (double)(x & 3) / 4.0, dz = (double)(z & 3) / 4.0;
+
+    // the final density (phase2.c:161-194) of y in [lo, 319], computed in windows of 64 from the top as needed
+    int dens_lo = 320;
+    auto fill_to = [&](int y_lo) {
+        y_lo = max(y_lo, -64);
+        while (dens_lo > y_lo) {
+            const int y = dens_lo - 1 - lane;
+            if (y >= -64) {
+                const int yi = y + 64, cy = yi / 8;
+                const double dy = (double)(yi - cy * 8) / 8.0;
+                const double v = cell_lerp(dx, dy, dz, m00, m00 + 5 * 49, m00 + 49, m00 + 6 * 49, cy);
+                const double a = jclamp(v * 0.64, -1.0, 1.0);
+                const double squeezed = a / 2.0 - a * a * a / 24.0;
+                const double n = T.noodle(0, x, y, z);
+                double noodle = 64.0;
+                if (!(n >= -1000000.0 && n < 0.0)) {
+                    double chv[3];
+                    for (int w = 0; w < 3; w++)
+                        chv[w] = cell_lerp(dx, dy, dz, n00 + w * 49, n00 + 5 * 147 + w * 49, n00 + 147 + w * 49,
+                                           n00 + 6 * 147 + w * 49, cy);
+                    noodle = chv[0] + 1.5 * jmax(fabs(chv[1]), fabs(chv[2]));
+                }
+                s_dens[yi] = jmin(squeezed, noodle);
+            }
+            dens_lo = max(dens_lo - 64, -64);
+        }
+        __syncthreads();
+    };
+
+    // the top solid block: the highest positive density
+    int y_d = -65;
+    for (int y_top = 319; y_top >= -64; y_top -= 64) {
+        fill_to(y_top - 63);
+        const int y = y_top - lane;
+        const unsigned long long m = __ballot(y >= -64 && s_dens[y + 64] > 0.0);
+        if (m) {
+            y_d = y_top - (__ffsll((long long)m) - 1);
+            break;
+        }
+    }
+
+    // above it the aquifer decides each block up to skipSamplingAboveY (a barrier can make one solid), the sea
+    // above that (water below y = 63); the height is one above the highest block that is not air
+    int h = y_d, st = BS_SOLID;
+    for (int y_top = min(skip_above, 319); y_top > y_d; y_top -= 64) {
+        const int y = y_top - lane;
+        const int s = y > y_d ? T.substance(A, x, y, z, s_dens[y + 64], skip_above) : BS_AIR;
+        const unsigned long long m = __ballot(s != BS_AIR);
+        if (m) {
+            const int l = __ffsll((long long)m) - 1;
+            h = y_top - l;
+            st = __shfl(s, l);
+            break;
+        }
+    }
+    if (skip_above < 62 && 62 > h) { h = 62; st = BS_WATER; }
+    const int pre = h >= -64 ? h + 1 : -64;
+    int post = pre, wet = 0;
+
+    if constexpr (kPost) {
+        // post-carver surface (phase2.c:337-345, 438-456): below the pre-carver surface, a claimed cell whose block
+        // is solid or water becomes lava at y <= -56 and elsewhere the aquifer's substance at density 0, unless that
+        // is solid (carver.c:145-175); other cells keep their block state
+        const unsigned long long *cw = cb.claims + (size_t)k * kClaimWords + (size_t)(((z & 15) * 16 + (x & 15)) * 6);
+        post = -64;
+        for (int y_top = pre - 1; y_top >= -64; y_top -= 64) {
+            fill_to(y_top - 63);
+            const int y = y_top - lane;
+            int bs = BS_AIR;
+            if (y >= -64) {
+                const double d = s_dens[y + 64];
+                bs = d > 0.0 ? BS_SOLID : T.substance(A, x, y, z, d, skip_above);
+                const int b = y + 64;
+                if (((cw[b >> 6] >> (b & 63)) & 1ull) && (bs == BS_SOLID || bs == BS_WATER)) {
+                    const int cs = y <= -56 ? BS_LAVA : T.substance(A, x, y, z, 0.0, skip_above);
+                    if (cs != BS_SOLID) bs = cs;
+                }
+            }
+            const unsigned long long m = __ballot(bs != BS_AIR);
+            if (m) {
+                post = y_top - (__ffsll((long long)m) - 1) + 1;
+                break;
+            }
+        }
+        // the hut's flood test for this column: water above y = 62 (eval.c:151-168, computeFluid aquifer.c:286-331)
+        if (lane == 0) {
+            int lv, ty;
+            T.fluid(A, x, 62, z, lv, ty);
+            wet = (ty == BS_WATER && 62 < lv) ? 1 : 0;
+        }
+    }
+    if (lane == 0) out[blockIdx.x] = make_int4(pre, h >= -64 ? st : BS_AIR, post, wet);
+}
+
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 size_t ter_carve(uint8_t *base, TerBufs &b) {
     size_t o = 0;
     auto take = [&](size_t bytes) { uint8_t *p = base ? base + o : nullptr; o += align256(bytes); return p; };
+    const size_t nc = kChunkBatch;
     b.tt = (TerrainTables *)take(2 * sizeof(TerrainTables));
     b.bd = (BuildDesc *)take(2 * kTerSamplers * sizeof(BuildDesc));
     b.spline = (SplineTab *)take(sizeof(SplineTab));
-    b.tab = take((size_t)kTerBatch * kTerSamplers * 256);
-    b.par = (SPar *)take((size_t)kTerBatch * kTerSamplers * sizeof(SPar));
-    b.ltab = take((size_t)kTerBatch * kLegacyPerlins * 256);
-    b.lpar = (SPar *)take((size_t)kTerBatch * kLegacyPerlins * sizeof(SPar));
-    b.aq = (ulonglong2 *)take(kTerBatch * sizeof(ulonglong2));
-    b.seeds = (uint64_t *)take(kTerBatch * sizeof(uint64_t));
-    b.jobs = (ColJob *)take(kTerBatch * sizeof(ColJob));
-    b.out = (int4 *)take(kTerBatch * sizeof(int4));
+    b.tab = take(nc * kTerSamplers * 256);
+    b.par = (SPar *)take(nc * kTerSamplers * sizeof(SPar));
+    b.ltab = take(nc * kLegacyPerlins * 256);
+    b.lpar = (SPar *)take(nc * kLegacyPerlins * sizeof(SPar));
+    b.aq = (ulonglong2 *)take(nc * sizeof(ulonglong2));
+    b.seeds = (uint64_t *)take(nc * sizeof(uint64_t));
     b.sin_tab = (float *)take(65536 * sizeof(float));
-    b.chunks = (int4 *)take(kHutBatch * sizeof(int4));
-    b.claims = (unsigned long long *)take((size_t)kHutBatch * kClaimWords * sizeof(unsigned long long));
+    b.chunks = (int4 *)take(nc * sizeof(int4));
+    b.pairs = (int2 *)take(nc * kCornerCols * sizeof(int2));
+    b.qsurf = (int *)take(nc * kAqQuarts * sizeof(int));
+    b.qdd = (double *)take(nc * kAqQuarts * sizeof(double));
+    b.cells = (int4 *)take(nc * kAqCells * sizeof(int4));
+    b.skip = (int *)take(nc * sizeof(int));
+    b.tp = (float4 *)take(nc * kCornerCols * sizeof(float4));
+    b.cmain = (double *)take(nc * kCornerCols * 49 * sizeof(double));
+    b.cnood = (double *)take(nc * kCornerCols * 147 * sizeof(double));
+    b.claims = (unsigned long long *)take(nc * kClaimWords * sizeof(unsigned long long));
+    b.jobs = (ColJob *)take((size_t)kColBatch * sizeof(ColJob));
+    b.out = (int4 *)take((size_t)kColBatch * sizeof(int4));
     return o;
 }
 
@@ -353,14 +445,115 @@ int build_round(whf_ctx *c, TerBufs &b, const std::vector<uint64_t> &seeds, int 
     return check_launch();
 }
 
-// Slot of a seed in the round, adding it when new.
-uint32_t slot_for(uint64_t seed, std::vector<uint64_t> &seeds, std::unordered_map<uint64_t, uint32_t> &seen) {
-    auto it = seen.find(seed);
-    if (it == seen.end()) {
-        it = seen.emplace(seed, (uint32_t)seeds.size()).first;
-        seeds.push_back(seed);
+// A device round: up to kChunkBatch chunks (with their seeds) and kColBatch columns, each column in one of the
+// round's chunks.
+struct Round {
+    struct Key {
+        uint64_t seed;
+        int cx, cz;
+        bool operator==(const Key &o) const { return seed == o.seed && cx == o.cx && cz == o.cz; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key &k) const {
+            return std::hash<uint64_t>()(k.seed ^ ((uint64_t)(uint32_t)k.cx * 0x9E3779B97F4A7C15ull) ^
+                                         ((uint64_t)(uint32_t)k.cz << 32));
+        }
+    };
+    std::vector<uint64_t> seeds;
+    std::unordered_map<uint64_t, uint32_t> slot_of;
+    std::vector<int4> chunks;   // slot, chunk x, chunk z, mask of the corner columns its columns use
+    std::unordered_map<Key, uint32_t, KeyHash> chunk_of;
+    std::vector<ColJob> jobs;
+    void clear() {
+        seeds.clear();
+        slot_of.clear();
+        chunks.clear();
+        chunk_of.clear();
+        jobs.clear();
     }
-    return it->second;
+    bool empty() const { return jobs.empty(); }
+    // Adds a column; false when the round is full (then nothing is added).
+    bool add(uint64_t seed, int x, int z) {
+        const Key key{seed, x >> 4, z >> 4};
+        auto it = chunk_of.find(key);
+        if (jobs.size() == kColBatch || (it == chunk_of.end() && chunks.size() == kChunkBatch)) return false;
+        if (it == chunk_of.end()) {
+            auto s = slot_of.find(seed);
+            if (s == slot_of.end()) {
+                s = slot_of.emplace(seed, (uint32_t)seeds.size()).first;
+                seeds.push_back(seed);
+            }
+            it = chunk_of.emplace(key, (uint32_t)chunks.size()).first;
+            chunks.push_back(make_int4((int)s->second, key.cx, key.cz, 0));
+        }
+        int4 &ch = chunks[it->second];
+        const int c0 = ((x >> 2) - 4 * ch.y) * 5 + ((z >> 2) - 4 * ch.z);
+        ch.w |= (1 << c0) | (1 << (c0 + 1)) | (1 << (c0 + 5)) | (1 << (c0 + 6));
+        jobs.push_back(ColJob{(uint32_t)ch.x, x, z, (int32_t)it->second});
+        return true;
+    }
+};
+
+// Runs a round: res[j] = {pre-carver height, its top state, post-carver height (kPost), water above 62 (kPost)}.
+int run_round(whf_ctx *c, TerBufs &b, int large, bool post, const Round &r, std::vector<int4> &res) {
+    const hipStream_t st = c->stream;
+    int rc = build_round(c, b, r.seeds, large);
+    if (rc != WHF_OK) return rc;
+    std::vector<int2> pairs;
+    for (size_t k = 0; k < r.chunks.size(); k++)
+        for (int q = 0; q < kCornerCols; q++)
+            if ((r.chunks[k].w >> q) & 1) pairs.push_back(make_int2((int)k, q));
+    const uint32_t nch = (uint32_t)r.chunks.size(), np = (uint32_t)pairs.size(), nj = (uint32_t)r.jobs.size();
+    HIP_TRY(hipMemcpyAsync(b.chunks, r.chunks.data(), nch * sizeof(int4), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b.pairs, pairs.data(), np * sizeof(int2), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b.jobs, r.jobs.data(), nj * sizeof(ColJob), hipMemcpyHostToDevice, st));
+    const TerArgs ta = b.args(large);
+    const ChunkBufs cb = b.chunk_bufs();
+    hipLaunchKernelGGL(k_quarts, dim3((nch * kAqQuarts + np + 63) / 64), dim3(64), 0, st, ta, cb, nch,
+                       (const int2 *)b.pairs, np);
+    hipLaunchKernelGGL(k_cells, dim3(nch), dim3(256), 0, st, ta, cb);
+    hipLaunchKernelGGL(k_corners, dim3((np * 196 + 63) / 64), dim3(64), 0, st, ta, cb, (const int2 *)b.pairs, np);
+    if (post) {
+        hipLaunchKernelGGL(k_claims, dim3(nch), dim3(256), 0, st, (const float *)b.sin_tab, (const uint64_t *)b.seeds,
+                           (const int4 *)b.chunks, b.claims);
+        hipLaunchKernelGGL(k_column<true>, dim3(nj), dim3(64), 0, st, ta, cb, (const ColJob *)b.jobs, b.out);
+    } else {
+        hipLaunchKernelGGL(k_column<false>, dim3(nj), dim3(64), 0, st, ta, cb, (const ColJob *)b.jobs, b.out);
+    }
+    if ((rc = check_launch()) != WHF_OK) return rc;
+    res.resize(nj);
+    HIP_TRY(hipMemcpyAsync(res.data(), b.out, nj * sizeof(int4), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return WHF_OK;
+}
+
+// Columns in rounds; emit(i, res) receives column i's result.
+template <class Emit>
+int columns_in_rounds(whf_ctx *c, const whf_column *cols, uint64_t n, int vflags, bool post, Emit emit) {
+    TerBufs b;
+    int rc = ter_init(c, b);
+    if (rc != WHF_OK) return rc;
+    const int large = (vflags & WHF_V_LARGE) ? 1 : 0;
+    Round r;
+    std::vector<uint64_t> idx;
+    std::vector<int4> res;
+    auto flush = [&]() -> int {
+        if (r.empty()) return WHF_OK;
+        const int rc2 = run_round(c, b, large, post, r, res);
+        if (rc2 != WHF_OK) return rc2;
+        for (size_t j = 0; j < idx.size(); j++) emit(idx[j], res[j]);
+        r.clear();
+        idx.clear();
+        return WHF_OK;
+    };
+    for (uint64_t i = 0; i < n; i++) {
+        if (!r.add((uint64_t)cols[i].seed, cols[i].x, cols[i].z)) {
+            if ((rc = flush()) != WHF_OK) return rc;
+            r.add((uint64_t)cols[i].seed, cols[i].x, cols[i].z);
+        }
+        idx.push_back(i);
+    }
+    return flush();
 }
 
 }  // namespace
@@ -369,55 +562,10 @@ namespace whf {
 
 int terrain_heights(whf_ctx *c, const whf_column *cols, uint64_t n, int vflags, int32_t *height, int32_t *state) {
     if (n == 0) return WHF_OK;
-    TerBufs b;
-    int rc = ter_init(c, b);
-    if (rc != WHF_OK) return rc;
-    const hipStream_t st = c->stream;
-    const int large = (vflags & WHF_V_LARGE) ? 1 : 0;
-    std::vector<uint64_t> seeds;
-    std::vector<ColJob> jobs;
-    std::vector<int4> res(kTerBatch);
-    std::unordered_map<uint64_t, uint32_t> seen;
-    for (uint64_t q0 = 0; q0 < n; q0 += kTerBatch) {
-        const uint32_t nq = (uint32_t)std::min<uint64_t>(kTerBatch, n - q0);
-        seeds.clear();
-        seen.clear();
-        jobs.resize(nq);
-        for (uint32_t i = 0; i < nq; i++)
-            jobs[i] = ColJob{slot_for((uint64_t)cols[q0 + i].seed, seeds, seen), cols[q0 + i].x, cols[q0 + i].z, -1};
-        if ((rc = build_round(c, b, seeds, large)) != WHF_OK) return rc;
-        HIP_TRY(hipMemcpyAsync(b.jobs, jobs.data(), nq * sizeof(ColJob), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_surface<false>, dim3(nq), dim3(256), 0, st, b.tt + large, b.spline, b.tab, b.par, b.ltab,
-                           b.lpar, b.aq, b.jobs, (const unsigned long long *)nullptr, b.out);
-        if ((rc = check_launch()) != WHF_OK) return rc;
-        HIP_TRY(hipMemcpyAsync(res.data(), b.out, nq * sizeof(int4), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (uint32_t i = 0; i < nq; i++) {
-            height[q0 + i] = res[i].x;
-            if (state) state[q0 + i] = res[i].y;
-        }
-    }
-    return WHF_OK;
-}
-
-// Pre- and post-carver heights and the water-above-62 flag of columns grouped by target chunk: chunk k of the round
-// is (chunks[k].seed, chunk x, chunk z); the claims of each chunk are gathered once.
-int carved_columns(whf_ctx *c, TerBufs &b, int large, const std::vector<uint64_t> &seeds,
-                   const std::vector<int4> &chunks, const std::vector<ColJob> &jobs, std::vector<int4> &res) {
-    const hipStream_t st = c->stream;
-    int rc = build_round(c, b, seeds, large);
-    if (rc != WHF_OK) return rc;
-    HIP_TRY(hipMemcpyAsync(b.chunks, chunks.data(), chunks.size() * sizeof(int4), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(b.jobs, jobs.data(), jobs.size() * sizeof(ColJob), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_claims, dim3((uint32_t)chunks.size()), dim3(256), 0, st, b.sin_tab, b.seeds, b.chunks,
-                       b.claims);
-    hipLaunchKernelGGL(k_surface<true>, dim3((uint32_t)jobs.size()), dim3(256), 0, st, b.tt + large, b.spline, b.tab,
-                       b.par, b.ltab, b.lpar, b.aq, b.jobs, (const unsigned long long *)b.claims, b.out);
-    if ((rc = check_launch()) != WHF_OK) return rc;
-    res.resize(jobs.size());
-    HIP_TRY(hipMemcpyAsync(res.data(), b.out, jobs.size() * sizeof(int4), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    return WHF_OK;
+    return columns_in_rounds(c, cols, n, vflags, false, [&](uint64_t i, const int4 &o) {
+        height[i] = o.x;
+        if (state) state[i] = o.y;
+    });
 }
 
 }  // namespace whf
@@ -431,65 +579,16 @@ extern "C" int whf_surface_heights(whf_ctx *c, const whf_column *cols, uint64_t 
     return terrain_heights(c, cols, n, vflags, height, state);
 }
 
-namespace {
-
-// Rounds of columns grouped by target chunk: at most kHutBatch chunks and kTerBatch columns per round.
-struct CarveRound {
-    std::vector<uint64_t> seeds;
-    std::unordered_map<uint64_t, uint32_t> seen;
-    std::vector<int4> chunks;
-    std::vector<ColJob> jobs;
-    void clear() { seeds.clear(); seen.clear(); chunks.clear(); jobs.clear(); }
-};
-
-}  // namespace
-
 extern "C" int whf_post_carver_heights(whf_ctx *c, const whf_column *cols, uint64_t n, int vflags, int32_t *post,
                                        int32_t *pre) {
     if (!c || (n && (!cols || !post))) return WHF_EINVAL;
     g_last_error[0] = 0;
     if (n == 0) return WHF_OK;
     HIP_TRY(hipSetDevice(c->device));
-    TerBufs b;
-    int rc = ter_init(c, b);
-    if (rc != WHF_OK) return rc;
-    const int large = (vflags & WHF_V_LARGE) ? 1 : 0;
-    CarveRound r;
-    std::vector<int4> res;
-    std::vector<uint64_t> idx;
-    auto flush = [&]() -> int {
-        if (r.jobs.empty()) return WHF_OK;
-        const int rc2 = carved_columns(c, b, large, r.seeds, r.chunks, r.jobs, res);
-        if (rc2 != WHF_OK) return rc2;
-        for (size_t k = 0; k < r.jobs.size(); k++) {
-            post[idx[k]] = res[k].z;
-            if (pre) pre[idx[k]] = res[k].x;
-        }
-        r.clear();
-        idx.clear();
-        return WHF_OK;
-    };
-    for (uint64_t i = 0; i < n; i++) {
-        const uint64_t seed = (uint64_t)cols[i].seed;
-        const int cx = cols[i].x >> 4, cz = cols[i].z >> 4;
-        int chunk = -1;
-        for (size_t k = 0; k < r.chunks.size(); k++)
-            if (r.seeds[r.chunks[k].x] == seed && r.chunks[k].y == cx && r.chunks[k].z == cz) chunk = (int)k;
-        if (chunk < 0 && (r.chunks.size() == kHutBatch || r.jobs.size() == kTerBatch)) {
-            if ((rc = flush()) != WHF_OK) return rc;
-        } else if (r.jobs.size() == kTerBatch) {
-            if ((rc = flush()) != WHF_OK) return rc;
-            chunk = -1;
-        }
-        const uint32_t slot = slot_for(seed, r.seeds, r.seen);
-        if (chunk < 0) {
-            chunk = (int)r.chunks.size();
-            r.chunks.push_back(make_int4((int)slot, cx, cz, 0));
-        }
-        r.jobs.push_back(ColJob{slot, cols[i].x, cols[i].z, chunk});
-        idx.push_back(i);
-    }
-    return flush();
+    return columns_in_rounds(c, cols, n, vflags, true, [&](uint64_t i, const int4 &o) {
+        post[i] = o.z;
+        if (pre) pre[i] = o.x;
+    });
 }
 
 extern "C" int whf_hut_eval(whf_ctx *c, const whf_hut_query *huts, uint64_t n, int tree, int vflags, int max_y,
@@ -508,10 +607,10 @@ extern "C" int whf_hut_eval(whf_ctx *c, const whf_hut_query *huts, uint64_t n, i
     TerBufs b;
     if ((rc = ter_init(c, b)) != WHF_OK) return rc;
     const int large = (vflags & WHF_V_LARGE) ? 1 : 0;
-    CarveRound r;
+    Round r;
     std::vector<int4> res;
-    for (uint64_t h0 = 0; h0 < n; h0 += kHutBatch) {
-        const uint32_t nh = (uint32_t)std::min<uint64_t>(kHutBatch, n - h0);
+    for (uint64_t h0 = 0; h0 < n; h0 += kChunkBatch) {
+        const uint32_t nh = (uint32_t)std::min<uint64_t>(kChunkBatch, n - h0);
         r.clear();
         for (uint32_t k = 0; k < nh; k++) {
             const whf_hut_query &q = huts[h0 + k];
@@ -524,13 +623,10 @@ extern "C" int whf_hut_eval(whf_ctx *c, const whf_hut_query *huts, uint64_t n, i
             const bool axis_z = o.dir == 0 || o.dir == 2;
             o.size_x = axis_z ? 7 : 9;
             o.size_z = axis_z ? 9 : 7;
-            const uint32_t slot = slot_for((uint64_t)q.seed, r.seeds, r.seen);
-            r.chunks.push_back(make_int4((int)slot, q.cx, q.cz, 0));
             for (int dx = 0; dx < o.size_x; dx++)
-                for (int dz = 0; dz < o.size_z; dz++)
-                    r.jobs.push_back(ColJob{slot, q.cx * 16 + dx, q.cz * 16 + dz, (int)k});
+                for (int dz = 0; dz < o.size_z; dz++) r.add((uint64_t)q.seed, q.cx * 16 + dx, q.cz * 16 + dz);
         }
-        if ((rc = carved_columns(c, b, large, r.seeds, r.chunks, r.jobs, res)) != WHF_OK) return rc;
+        if ((rc = run_round(c, b, large, true, r, res)) != WHF_OK) return rc;
         size_t j = 0;
         for (uint32_t k = 0; k < nh; k++) {
             whf_hut_result &o = out[h0 + k];
