@@ -2,9 +2,10 @@
 """Benchmark: the GPU Seed Check at scale on one MI355X, beside the reference CPU scanner on the host.
 
 Workload (one step): 2^26 world seeds from SplitMix64(seed=2), hut regions x, z in [-8, 7] (256 per seed,
-1.718e10 hut-region checks), version 1.20 rules, phase-1 maxHeight -50, through the plain FP64 kernel
-chain (bit-exact with the reference CPU scanner). The seeds are generated on the device, so the inputs
-are resident before the timed region starts.
+1.718e10 hut-region checks), version 1.20 rules, threshold -52 'or lower': phase 1 at maxHeight -50, then
+lysh's hut verdict (biome at the surface, flood test, 63-column post-carver average y <= -52) on every
+phase-1 survivor, so a step ends with the (seed, chunk, floor) finds, bit-exact with the reference CPU
+scanner. The seeds are generated on the device, so the inputs are resident before the timed region starts.
 
 Prints one JSON line: throughput (hut-region checks/s), the roofline of the dominant kernel (measured
 with HIP events on the library's stream during a profiled step after the timed steps), and the CPU
@@ -77,15 +78,20 @@ def cpu_baseline(n_seeds_max, seconds):
     threads = min(16, os.cpu_count() or 1)
     out = subprocess.run([exe, "--stream", "2", "--count", str(n_seeds_max), "--rx0", "-8", "--rx1", "8",
                           "--rz0", "-8", "--rz1", "8", "--version", "1.20", "--max-y", "-50", "--threads",
-                          str(threads), "--time-limit", str(seconds)], check=True, capture_output=True,
-                         text=True).stdout
+                          str(threads), "--time-limit", str(seconds), "--phase2-max-y", str(THRESHOLD)],
+                         check=True, capture_output=True, text=True).stdout
     r = json.loads(out)
     return dict(value=round(r["checks_per_s"], 1), unit="checks/s", cores=threads, kind="reference",
                 sample="first %d seeds of the same SplitMix64(2) stream x 256 regions (time-bounded %.0f s), "
-                       "reference lysh_scan_rect per seed, one seed per host thread" % (r["seeds"], seconds),
-                funnel=r["funnel"], tiers=r["tiers"], seeds=r["seeds"])
+                       "reference lysh_scan_rect per seed with its phase-2 hook (lysh_eval_hut, max-y %d) on the "
+                       "survivors, one seed per host thread" % (r["seeds"], seconds, THRESHOLD),
+                funnel=r["funnel"], tiers=r["tiers"], seeds=r["seeds"], phase2_evaluated=r["phase2_evaluated"],
+                phase2_accepted=r["phase2_accepted"],
+                phase2_ms_per_hut=round(1e3 * r["phase2_seconds"] / r["phase2_evaluated"], 3)
+                if r["phase2_evaluated"] else None)
 
 
+THRESHOLD = -52      # the Seed Check's floor threshold, 'or lower' (phase 1 runs at maxHeight -50)
 DD_RECT = (-2, 2, -2, 2)
 DD_FLOOR = -55
 DD_TOTAL = 1 << 32   # structure seeds of the full Deep Dive workload
@@ -166,8 +172,8 @@ def deep_dive(whf, sweep_n, expand_n, cpu_seconds):
 def serving_stream(n, seed, versions=("1.20",)):
     """The serving workload: n requests, sizes round(2^U(0,20)) seeds from SplitMix64(request index), radius
     uniform over {512, 1024, 2048, 4096} blocks -> regions +-(r // 512 + 1), threshold uniform over -50..-54 with
-    'or lower' / 'exactly' at 50/50 (-54 is always 'or lower'), version 1.20. Phase 1 runs at maxHeight -50 (the
-    threshold is the finisher's)."""
+    'or lower' / 'exactly' at 50/50 (-54 is always 'or lower'), version 1.20. Phase 1 runs at maxHeight -50, and
+    lysh's hut verdict applies the request's threshold to the survivors."""
     import numpy as np
     rng = np.random.default_rng(seed)
     sizes = np.rint(2.0 ** rng.uniform(0, 20, n)).astype(np.int64)
@@ -211,7 +217,9 @@ def serve(whf, reqs, rate, slots, chunk_checks, background=None):
                     time.sleep(d)
                 with lock:
                     ids.append(srv.submit(q["rect"], capacity=1024, splitmix=q["splitmix"], max_height=-50,
-                                          vflags=whf.version_flags(q["version"])))
+                                          vflags=whf.version_flags(q["version"]),
+                                          phase2=(whf.P2_EXACTLY if q["exactly"] else whf.P2_OR_LOWER,
+                                                  q["threshold"], whf.biome_tree(q["version"]))))
 
         th = threading.Thread(target=feeder)
         t0 = time.perf_counter()
@@ -296,7 +304,8 @@ def main():
     whf = __graft_entry__.load_whf()
     rect = (-8, 8, -8, 8)
     vflags = whf.version_flags("1.20")
-    kw = dict(splitmix=(2, args.seeds), max_height=-50, vflags=vflags, capacity=1 << 20)
+    kw = dict(splitmix=(2, args.seeds), max_height=-50, vflags=vflags, capacity=1 << 20,
+              phase2=(whf.P2_OR_LOWER, THRESHOLD, whf.biome_tree("1.20")))
     timing = {}
     results = {}
     with whf.Context(0) as ctx:
@@ -313,7 +322,7 @@ def main():
         err = ctx.noise_error([int(x) for x in range(1, 257)], 4096)
     h64, s64 = results[whf.PREC_FP64]
     h32, last = results[whf.PREC_FP32_GATE]
-    identical = h64 == h32 and all(s64[k] == last[k] for k in ("funnel", "tiers", "accepted", "checks"))
+    identical = h64 == h32 and all(s64[k] == last[k] for k in ("funnel", "tiers", "accepted", "checks", "phase2"))
     # the headline is the faster of the two bit-identical paths; both are reported
     best = min(timing, key=timing.get)
     secs = timing[best]
@@ -356,10 +365,17 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(secs * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if best == whf.PREC_FP64 else "f32-gate+f64", "data": "synthetic",
         "config": {"workload": "Seed Check at scale: %d SplitMix64(2) seeds x regions [-8,7]^2 (256 per seed), "
-                               "version 1.20, phase-1 maxHeight -50; value = the faster of the plain FP64 chain and "
-                               "the FP32-gated erosion stage (identical finds)" % args.seeds,
-                   "seeds": args.seeds, "regions_per_seed": 256, "version": "1.20",
-                   "precision": "fp64" if best == whf.PREC_FP64 else "fp32-gate"},
+                               "version 1.20, threshold %d or-lower (phase-1 maxHeight -50, then lysh's hut verdict "
+                               "on the survivors); value = the faster of the plain FP64 chain and the FP32-gated "
+                               "erosion stage (identical finds)" % (args.seeds, THRESHOLD),
+                   "seeds": args.seeds, "regions_per_seed": 256, "version": "1.20", "threshold": THRESHOLD,
+                   "mode": "or-lower", "precision": "fp64" if best == whf.PREC_FP64 else "fp32-gate"},
+        "finds": len(h64), "phase1_survivors": last["accepted"],
+        "phase2": {"evaluated": last["phase2"][0], "accepted": last["phase2"][1],
+                   "rejected_biome": last["phase2"][2], "rejected_y": last["phase2"][3],
+                   "rejected_flood": last["phase2"][4], "ms_per_step": round(last["phase2_ms"], 3),
+                   "ms_per_hut": round(last["phase2_ms"] / max(1, last["phase2"][0]), 4),
+                   "cpu_reference_ms_per_hut": cpu["phase2_ms_per_hut"] if cpu else None},
         "fp64_value": round(checks / timing[whf.PREC_FP64], 1), "fp64_ms_per_step": round(timing[whf.PREC_FP64] * 1e3, 3),
         "fp32_gate_value": round(checks / timing[whf.PREC_FP32_GATE], 1),
         "fp32_gate_ms_per_step": round(timing[whf.PREC_FP32_GATE] * 1e3, 3),

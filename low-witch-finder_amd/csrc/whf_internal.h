@@ -157,6 +157,10 @@ void launch_build(whf_ctx *c, const uint64_t *seeds, const BuildDesc *bd, int se
                   uint8_t *tab, SPar *par, unsigned long long *built);
 // Zero the batch's queue counts and the gate flags of its first ns seeds.
 void launch_reset(whf_ctx *c, uint32_t ns);
+// lysh's phase 2 on phase-1 survivors of a request (whf_scan.hip): keeps the accepted huts in place with their floor;
+// adds to p2 (evaluated, accepted, rejected biome, height, flood) and to *ms.
+int phase2_filter(whf_ctx *c, const whf_request *req, whf_hit *hits, uint64_t n, uint64_t *kept, uint64_t p2[5],
+                  double *ms);
 // Pre-carver surface heights of n columns (seed, x, z) and the state of each column's top block (0 solid,
 // 2 water, 3 lava; 1 for a column of air); whf_terrain.hip.
 int terrain_heights(whf_ctx *c, const whf_column *cols, uint64_t n, int vflags, int32_t *height, int32_t *state);

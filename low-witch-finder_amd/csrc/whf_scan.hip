@@ -971,6 +971,49 @@ int run_batch(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint6
 
 }  // namespace
 
+namespace whf {
+
+// lysh's phase 2 on phase-1 survivors (lysh_grade_scan, eval.c:381-477): the hut verdict of each, keeping the accepted
+// huts (in place, in order) with their floor; counters evaluated, accepted, rejected for the biome, the height
+// (avg_y above the threshold, or not equal to it with 'exactly') and the flood.
+int phase2_filter(whf_ctx *c, const whf_request *req, whf_hit *hits, uint64_t n, uint64_t *kept, uint64_t p2[5],
+                  double *ms) {
+    const auto t0 = std::chrono::steady_clock::now();
+    *kept = 0;
+    int tree = req->biome_tree - 1;
+    if (req->biome_tree == 0) {
+        const char *dflt = (req->vflags & WHF_V_118) ? "1.18.2" : ((req->vflags & WHF_V_262) ? "26.2" : "1.21.5");
+        if (whf_biome_tree(dflt, &tree) != WHF_OK) return WHF_EINVAL;
+    }
+    std::vector<whf_hut_query> hq(n);
+    std::vector<whf_hut_result> hr(n);
+    for (uint64_t i = 0; i < n; i++) hq[i] = whf_hut_query{hits[i].seed, hits[i].cx, hits[i].cz};
+    const int rc = whf_hut_eval(c, hq.data(), n, tree, req->vflags & WHF_V_LARGE, req->phase2_max_y, hr.data());
+    if (rc != WHF_OK) return rc;
+    uint64_t k = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        const whf_hut_result &h = hr[i];
+        p2[0]++;
+        bool ok = h.ok != 0;
+        int reject = h.reject;
+        if (ok && req->phase2 == WHF_P2_EXACTLY && h.avg_y != req->phase2_max_y) { ok = false; reject = WHF_HUT_REJECT_Y; }
+        if (ok) {
+            p2[1]++;
+            whf_hit d = hits[i];
+            d.floor_y = h.avg_y;
+            d.flags |= WHF_HIT_PHASE2;
+            hits[k++] = d;
+        } else {
+            p2[reject == WHF_HUT_REJECT_BIOME ? 2 : (reject == WHF_HUT_REJECT_Y ? 3 : 4)]++;
+        }
+    }
+    *kept = k;
+    *ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return WHF_OK;
+}
+
+}  // namespace whf
+
 // ================================================================================================ C ABI
 
 extern "C" {
@@ -1129,39 +1172,13 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
     uint64_t p2[5] = {0, 0, 0, 0, 0};
     double p2_ms = 0.0;
     if (req->phase2 != WHF_P2_NONE) {
-        // lysh's phase 2 on the survivors (lysh_grade_scan, eval.c:381-477): the hut verdict, keep the accepted huts
         if (hits_total > host_hits.size()) return WHF_EOVERFLOW;
-        const auto t0 = std::chrono::steady_clock::now();
-        int tree = req->biome_tree - 1;
-        if (req->biome_tree == 0) {
-            const char *dflt = (req->vflags & WHF_V_118) ? "1.18.2" : ((req->vflags & WHF_V_262) ? "26.2" : "1.21.5");
-            if (whf_biome_tree(dflt, &tree) != WHF_OK) return WHF_EINVAL;
-        }
-        std::vector<whf_hut_query> hq(host_hits.size());
-        std::vector<whf_hut_result> hr(host_hits.size());
-        for (size_t i = 0; i < host_hits.size(); i++) hq[i] = whf_hut_query{host_hits[i].seed, host_hits[i].cx, host_hits[i].cz};
-        const int rc2 = whf_hut_eval(c, hq.data(), hq.size(), tree, req->vflags & WHF_V_LARGE, req->phase2_max_y, hr.data());
+        uint64_t kept = 0;
+        const int rc2 = phase2_filter(c, req, reinterpret_cast<whf_hit *>(host_hits.data()), host_hits.size(), &kept,
+                                      p2, &p2_ms);
         if (rc2 != WHF_OK) return rc2;
-        size_t kept = 0;
-        for (size_t i = 0; i < host_hits.size(); i++) {
-            const whf_hut_result &h = hr[i];
-            p2[0]++;
-            bool ok = h.ok != 0;
-            int reject = h.reject;
-            if (ok && req->phase2 == WHF_P2_EXACTLY && h.avg_y != req->phase2_max_y) { ok = false; reject = WHF_HUT_REJECT_Y; }
-            if (ok) {
-                p2[1]++;
-                DevHit d = host_hits[i];
-                d.floor_y = h.avg_y;
-                d.flags |= WHF_HIT_PHASE2;
-                host_hits[kept++] = d;
-            } else {
-                p2[reject == WHF_HUT_REJECT_BIOME ? 2 : (reject == WHF_HUT_REJECT_Y ? 3 : 4)]++;
-            }
-        }
         host_hits.resize(kept);
         hits_total = kept;
-        p2_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     if (sink) {
         sink->count = hits_total;

@@ -9,7 +9,9 @@
 // request with the least work left (the oldest on ties), runs it with whf_scan and folds the chunk's hits and
 // counters into the request. A request admitted while others run therefore starts when the first slot frees up, and the chunks of
 // several requests run at once on the device. Each chunk is a plain whf_scan over a sub-range of the request's
-// seeds, and counters add over chunks, so a request's hits and counters are those of whf_scan on it alone.
+// seeds, and counters add over chunks, so a request's hits and counters are those of whf_scan on it alone. A request
+// with lysh's phase 2 runs its chunks as phase 1 only; the slot that completes the last chunk runs the hut verdicts
+// of all the request's survivors in one batch, as whf_scan does at the end of a scan.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -25,6 +27,11 @@
 
 // Device bytes one context holds (whf_scan.hip).
 __attribute__((visibility("hidden"))) uint64_t whf_ctx_device_bytes(const whf_ctx *c);
+namespace whf {
+// lysh's phase 2 on a request's phase-1 survivors, in place (whf_scan.hip).
+__attribute__((visibility("hidden"))) int phase2_filter(whf_ctx *c, const whf_request *req, whf_hit *hits, uint64_t n,
+                                                        uint64_t *kept, uint64_t p2[5], double *ms);
+}
 
 namespace {
 
@@ -254,24 +261,47 @@ struct whf_server {
             }
             whf_request sub = j->req;
             sub.n_seeds = n;
+            sub.phase2 = WHF_P2_NONE;
             if (sub.seed_mode == WHF_SEEDS_LIST) sub.seeds = j->req.seeds + s0;
             else sub.offset = j->req.offset + s0;
             whf_sink sink{buf.data(), kChunkHits, 0};
             whf_stats st{};
             const int rc = n ? whf_scan(c, &sub, &sink, &st) : WHF_OK;
-            std::lock_guard<std::mutex> lk(mu);
-            if (rc == WHF_OK || rc == WHF_EOVERFLOW) {
-                add_stats(j->stats, st);
-                const uint64_t m = std::min<uint64_t>(sink.count, kChunkHits);
-                j->hits.insert(j->hits.end(), buf.begin(), buf.begin() + m);
-                if (rc == WHF_EOVERFLOW && j->status == WHF_OK) j->status = WHF_EOVERFLOW;
-            } else if (j->status == WHF_OK || j->status == WHF_EOVERFLOW) {
-                j->status = rc;
+            bool verdicts = false;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (rc == WHF_OK || rc == WHF_EOVERFLOW) {
+                    add_stats(j->stats, st);
+                    const uint64_t m = std::min<uint64_t>(sink.count, kChunkHits);
+                    j->hits.insert(j->hits.end(), buf.begin(), buf.begin() + m);
+                    if (rc == WHF_EOVERFLOW && j->status == WHF_OK) j->status = WHF_EOVERFLOW;
+                } else if (j->status == WHF_OK || j->status == WHF_EOVERFLOW) {
+                    j->status = rc;
+                }
+                j->done_seeds += n;
+                j->in_flight--;
+                if (j->done_seeds >= j->req.n_seeds && j->in_flight == 0) {
+                    if (j->req.phase2 != WHF_P2_NONE && j->status == WHF_OK) verdicts = true;
+                    else finish(j);
+                }
+                work_cv.notify_all();
             }
-            j->done_seeds += n;
-            j->in_flight--;
-            if (j->done_seeds >= j->req.n_seeds && j->in_flight == 0) finish(j);
-            work_cv.notify_all();
+            if (verdicts) {
+                // no chunk of j is left or in flight, so this thread alone touches its hits now
+                uint64_t kept = 0, p2[5] = {0, 0, 0, 0, 0};
+                double ms = 0.0;
+                const int rc2 = whf::phase2_filter(c, &j->req, j->hits.data(), j->hits.size(), &kept, p2, &ms);
+                std::lock_guard<std::mutex> lk(mu);
+                if (rc2 == WHF_OK) {
+                    j->hits.resize(kept);
+                    for (int k = 0; k < 5; k++) j->stats.phase2[k] += p2[k];
+                    j->stats.phase2_ms += ms;
+                } else {
+                    j->status = rc2;
+                }
+                finish(j);
+                work_cv.notify_all();
+            }
         }
     }
 };

@@ -32,6 +32,9 @@ constexpr uint32_t kColBatch = kChunkBatch * 64; // columns per device round
 constexpr int kTerSamplers = 144;                // named-noise samplers per seed
 constexpr int kCornerCols = 25;                  // corner columns of a chunk's 4 x 4 cells, (qx - 4 cx) * 5 + qz - 4 cz
 constexpr int kClaimWords = 256 * 6;             // a chunk's claimed-cell bits
+constexpr int kTunnelQ = 256;                    // cave tunnels queued per chunk (64 bytes each, in LDS)
+constexpr int kEllQ = 640;                       // ellipsoids queued per pass (48 bytes each)
+constexpr int kCanyonSlots = 4;                  // canyon width tables per chunk in LDS
 
 // One column of a round: its seed slot, position, and its chunk in the round.
 struct ColJob {
@@ -208,20 +211,88 @@ __global__ __launch_bounds__(64) void k_corners(TerArgs ta, ChunkBufs cb, const 
 __global__ __launch_bounds__(256) void k_claims(const float *__restrict__ sin_tab, const uint64_t *__restrict__ seeds,
                                                 const int4 *__restrict__ chunks,
                                                 unsigned long long *__restrict__ claims) {
-    // the cells of one target chunk that its 17 x 17 origin chunks' three carvers claim (carver.c:498-523)
+    // the cells of one target chunk that its 17 x 17 origin chunks' three carvers claim (carver.c:498-523). No lane
+    // walks a whole cave system or claims a whole ellipsoid: (1) the carvers' start rolls, cave rooms and canyon walks,
+    // queueing each cave tunnel; (2) the queued tunnels, queueing their branch tunnels; (3) the branch tunnels. Every
+    // pass queues the ellipsoids within reach, and then each wave claims the cells of every fourth queued ellipsoid,
+    // a column per lane. The claimed set is a union, so the order changes nothing; a piece that finds its queue full
+    // runs in the lane that made it.
     __shared__ unsigned long long bits[kClaimWords];
+    __shared__ TunnelJob tq[kTunnelQ];
+    __shared__ Ell eq[kEllQ];
+    __shared__ float cw[kCanyonSlots][384];
+    __shared__ int s_top, s_all, s_ell, s_can;
+    const int t = threadIdx.x;
     const int4 ch = chunks[blockIdx.x];
-    for (int i = threadIdx.x; i < kClaimWords; i += 256) bits[i] = 0ull;
+    for (int i = t; i < kClaimWords; i += 256) bits[i] = 0ull;
+    if (t == 0) {
+        s_top = 0;
+        s_ell = 0;
+        s_can = 0;
+    }
     __syncthreads();
     const CarveClaims cc{sin_tab, bits, ch.y * 16, ch.z * 16, (double)(ch.y * 16 + 8), (double)(ch.z * 16 + 8)};
     const uint64_t world = seeds[ch.x];
-    for (int job = threadIdx.x; job < 17 * 17 * 3; job += 256) {
+    auto push_ell = [&](const Ell &e, const float *width) {
+        if (e.canyon != kOwnWidths) {
+            const int at = atomicAdd(&s_ell, 1);
+            if (at < kEllQ) {
+                eq[at] = e;
+                return;
+            }
+        }
+        cc.claim(e, width, 0, 1);
+    };
+    auto widths = [&](int &id) -> float * {
+        const int at = atomicAdd(&s_can, 1);
+        if (at >= kCanyonSlots) return nullptr;
+        id = at;
+        return cw[at];
+    };
+    auto flush = [&]() {
+        __syncthreads();
+        const int n = min(s_ell, kEllQ);
+        for (int e = t >> 6; e < n; e += 4) {
+            const Ell el = eq[e];
+            cc.claim(el, el.canyon >= 0 ? cw[el.canyon] : nullptr, t & 63, 64);
+        }
+        __syncthreads();
+        if (t == 0) s_ell = 0;
+        __syncthreads();
+    };
+    auto no_branch = [](const TunnelJob &) {};
+    auto run_branch = [&](const TunnelJob &b) { cc.tunnel<false>(b, no_branch, push_ell); };
+    auto push_top = [&](const TunnelJob &j) {
+        const int at = atomicAdd(&s_top, 1);
+        if (at < kTunnelQ) tq[at] = j;
+        else cc.tunnel<true>(j, run_branch, push_ell);
+    };
+    for (int job = t; job < 17 * 17 * 3; job += 256) {
         const int o = job / 3, ci = job - o * 3;
-        cc.carver(world, ch.y + o / 17 - 8, ch.z + o % 17 - 8, ci);
+        cc.carver(world, ch.y + o / 17 - 8, ch.z + o % 17 - 8, ci, push_top, push_ell, widths);
     }
+    flush();
+    const int n_top = min(s_top, kTunnelQ);
+    if (t == 0) s_all = n_top;
     __syncthreads();
+    auto push_branch = [&](const TunnelJob &b) {
+        const int at = atomicAdd(&s_all, 1);
+        if (at < kTunnelQ) tq[at] = b;
+        else cc.tunnel<false>(b, no_branch, push_ell);
+    };
+    for (int i = t; i < n_top; i += 256) {
+        const TunnelJob j = tq[i];
+        cc.tunnel<true>(j, push_branch, push_ell);
+    }
+    flush();
+    const int n_all = min(s_all, kTunnelQ);
+    for (int i = n_top + t; i < n_all; i += 256) {
+        const TunnelJob j = tq[i];
+        cc.tunnel<false>(j, no_branch, push_ell);
+    }
+    flush();
     unsigned long long *dst = claims + (size_t)blockIdx.x * kClaimWords;
-    for (int i = threadIdx.x; i < kClaimWords; i += 256) dst[i] = bits[i];
+    for (int i = t; i < kClaimWords; i += 256) dst[i] = bits[i];
 }
 
 template <bool kPost>
@@ -598,15 +669,13 @@ extern "C" int whf_hut_eval(whf_ctx *c, const whf_hut_query *huts, uint64_t n, i
     g_last_error[0] = 0;
     if (n == 0) return WHF_OK;
     HIP_TRY(hipSetDevice(c->device));
-    // the biome check at the chunk middle's pre-carver surface (eval.c:141-143)
-    std::vector<whf_biome_query> bq(n);
-    std::vector<whf_biome_result> br(n);
-    for (uint64_t i = 0; i < n; i++) bq[i] = whf_biome_query{huts[i].seed, huts[i].cx * 16, huts[i].cz * 16, WHF_OCC_SURFACE, 0};
-    int rc = whf_biome(c, bq.data(), n, tree, vflags, br.data(), nullptr);
-    if (rc != WHF_OK) return rc;
     TerBufs b;
-    if ((rc = ter_init(c, b)) != WHF_OK) return rc;
+    int rc = ter_init(c, b);
+    if (rc != WHF_OK) return rc;
     const int large = (vflags & WHF_V_LARGE) ? 1 : 0;
+    // per round of huts: the 63 footprint columns of each, and its chunk middle, whose pre-carver surface is the
+    // biome check's height (eval.c:141-143); all in the hut's chunk
+    std::vector<whf_biome_query> bq(n);
     Round r;
     std::vector<int4> res;
     for (uint64_t h0 = 0; h0 < n; h0 += kChunkBatch) {
@@ -625,15 +694,13 @@ extern "C" int whf_hut_eval(whf_ctx *c, const whf_hut_query *huts, uint64_t n, i
             o.size_z = axis_z ? 9 : 7;
             for (int dx = 0; dx < o.size_x; dx++)
                 for (int dz = 0; dz < o.size_z; dz++) r.add((uint64_t)q.seed, q.cx * 16 + dx, q.cz * 16 + dz);
+            r.add((uint64_t)q.seed, q.cx * 16 + 8, q.cz * 16 + 8);
         }
         if ((rc = run_round(c, b, large, true, r, res)) != WHF_OK) return rc;
         size_t j = 0;
         for (uint32_t k = 0; k < nh; k++) {
+            const whf_hut_query &q = huts[h0 + k];
             whf_hut_result &o = out[h0 + k];
-            const whf_biome_result &bz = br[h0 + k];
-            o.biome_ok = bz.ok;
-            o.winner = bz.winner;
-            o.occ_y = bz.occ_y;
             long long sum_pre = 0, sum_post = 0;
             bool blocked = false;
             for (int col = 0; col < 63; col++, j++) {
@@ -642,17 +709,28 @@ extern "C" int whf_hut_eval(whf_ctx *c, const whf_hut_query *huts, uint64_t n, i
                 sum_post += res[j].z;
                 if (res[j].x - 1 >= 64) blocked = true;
             }
+            bq[h0 + k] = whf_biome_query{q.seed, q.cx * 16, q.cz * 16, res[j].x - 1, 0};
+            j++;
             o.flooded = o.wet == 63;
             // a flooded hut whose pre-carver top blocks are all at or below 63 keeps the pre-carver sum
             // (eval.c:186-205)
             o.carve_skipped = o.flooded && !blocked;
             o.sum_h = o.carve_skipped ? sum_pre : sum_post;
             o.avg_y = (int32_t)(o.sum_h / 63);
-            if (!o.biome_ok) { o.ok = 0; o.reject = WHF_HUT_REJECT_BIOME; }
-            else if (o.flooded) { o.ok = 0; o.reject = WHF_HUT_REJECT_FLOOD; }
-            else if (o.avg_y > max_y) { o.ok = 0; o.reject = WHF_HUT_REJECT_Y; }
-            else { o.ok = 1; o.reject = WHF_HUT_OK; }
         }
+    }
+    std::vector<whf_biome_result> br(n);
+    if ((rc = whf_biome(c, bq.data(), n, tree, vflags, br.data(), nullptr)) != WHF_OK) return rc;
+    for (uint64_t i = 0; i < n; i++) {
+        whf_hut_result &o = out[i];
+        o.biome_ok = br[i].ok;
+        o.winner = br[i].winner;
+        o.occ_y = br[i].occ_y;
+        // lysh's reject order: biome, flood, height (eval.c:221-224)
+        if (!o.biome_ok) { o.ok = 0; o.reject = WHF_HUT_REJECT_BIOME; }
+        else if (o.flooded) { o.ok = 0; o.reject = WHF_HUT_REJECT_FLOOD; }
+        else if (o.avg_y > max_y) { o.ok = 0; o.reject = WHF_HUT_REJECT_Y; }
+        else { o.ok = 1; o.reject = WHF_HUT_OK; }
     }
     return WHF_OK;
 }

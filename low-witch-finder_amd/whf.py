@@ -457,7 +457,7 @@ class Server:
         rc = self.L.whf_submit(self.h, ctypes.byref(req), ctypes.byref(sink), ctypes.byref(rid))
         if rc:
             raise WhfError(rc, "whf_submit")
-        self.pending[rid.value] = (req, keep, buf, sink)
+        self.pending[rid.value] = (req, keep, buf, sink, kw.get("phase2") is not None)
         return rid.value
 
     def submit_deepdive(self, s_first, count, rect, floor_max=-55, expand=True, max_height=-50, vflags=0, salt=0,
@@ -499,9 +499,13 @@ class Server:
                                               funnel=list(st.funnel), tiers=list(st.tiers)[:7]),
                                 submit_s=e.submit_s, start_s=e.start_s, done_s=e.done_s))
                 continue
-            req, keep, buf, sink = p
+            req, keep, buf, sink, p2 = p
             m = min(sink.count, sink.capacity)
-            hits = sorted((buf[i].seed, buf[i].rx, buf[i].rz, buf[i].cx, buf[i].cz) for i in range(m))
+            if p2:
+                hits = sorted((buf[i].seed, buf[i].rx, buf[i].rz, buf[i].cx, buf[i].cz, buf[i].floor_y)
+                              for i in range(m))
+            else:
+                hits = sorted((buf[i].seed, buf[i].rx, buf[i].rz, buf[i].cx, buf[i].cz) for i in range(m))
             out.append(dict(id=e.id, status=e.status, hits=hits, stats=stats_dict(e.stats), submit_s=e.submit_s,
                             start_s=e.start_s, done_s=e.done_s))
         return out

@@ -1,16 +1,19 @@
 /* lysh_cpu_bench — CPU baseline harness around the reference scanner (test infrastructure).
  *
  * Links the reference's own lysh objects (built by oracle/Makefile from engine/tools/lysh/src) and runs
- * its phase-1 Seed Check exactly as `lysh scan` does for one seed: lysh_scan_rect (search.c:198) with
- * threads = 1 and no phase-2 hook, which is the first pass of cmd_scan (main.c:139-216). Seeds come from
- * the sequential SplitMix64 stream (Vigna) and are handed out one at a time to T worker threads, so a
- * T-core host runs T seeds at once (lysh caps threads per seed at the number of rz rows, search.c:205-207).
+ * its Seed Check as `lysh scan` does for one seed: lysh_scan_rect (search.c:198) with threads = 1, which is
+ * the first pass of cmd_scan (main.c:139-216); with --phase2-max-y, every phase-1 survivor also gets lysh's hut
+ * verdict through the scanner's own phase-2 hook (search.c:167-192 calling lysh_eval_hut, eval.c:121-226), the
+ * work of lysh's grading pass (eval.c:381-477). Seeds come from the sequential SplitMix64 stream (Vigna) and are
+ * handed out one at a time to T worker threads, so a T-core host runs T seeds at once (lysh caps threads per seed
+ * at the number of rz rows, search.c:205-207).
  *
  *   lysh_cpu_bench --stream S --count N [--offset O] --rx0 A --rx1 B --rz0 C --rz1 D
- *                  [--max-y M] [--version V] [--threads T] [--time-limit SEC]
+ *                  [--max-y M] [--version V] [--threads T] [--time-limit SEC] [--phase2-max-y Y [--exactly 1]]
  *
  * Prints one JSON line: seeds and region checks done, seconds, checks/s, and the summed funnel
- * (reject kinds, main.c:96-100 order) and erosion tier histogram, which must equal the GPU's counters.
+ * (reject kinds, main.c:96-100 order) and erosion tier histogram, which must equal the GPU's counters; with
+ * phase 2, the huts evaluated and accepted (avg_y <= Y, or == Y with --exactly 1) and the phase-2 seconds.
  */
 #include <inttypes.h>
 #include <pthread.h>
@@ -26,6 +29,7 @@
 static uint64_t g_stream, g_offset;
 static long long g_count;
 static int g_rx0, g_rx1, g_rz0, g_rz1, g_maxy, g_threads = 1;
+static int g_p2 = 0, g_p2_maxy = 0, g_exactly = 0;
 static double g_limit = 0.0;
 static lysh_phase1_opts g_opts;
 static long long g_next;
@@ -50,7 +54,24 @@ typedef struct {
     long long seeds, scanned, accepted, reached;
     long long reject[LYSH_P1_REJECT_KIND_N];
     long long tier[LYSH_TIER_MAX];
+    long long p2_eval, p2_ok;
+    double p2_seconds;
 } acc_t;
+
+/* phase-2 hook: the worker's accumulator rides on the scanner's per-thread context (search.h:36-45) */
+static void p2_ctx_init(void *user, lysh_search_ctx *ctx, int thread_index) {
+    (void)thread_index;
+    ctx->p2_slot = user;
+}
+static int p2_hook(void *user, lysh_search_ctx *ctx, int rx, int rz, int hut_x, int hut_z) {
+    (void)user; (void)rx; (void)rz;
+    acc_t *a = (acc_t *)ctx->p2_slot;
+    lysh_hut_result r = lysh_eval_hut(ctx, hut_x, hut_z, g_p2_maxy);
+    const int ok = r.ok && (!g_exactly || r.avg_y == g_p2_maxy);
+    a->p2_eval++;
+    a->p2_ok += ok;
+    return ok;
+}
 
 static void *worker(void *arg) {
     acc_t *a = (acc_t *)arg;
@@ -70,6 +91,11 @@ static void *worker(void *arg) {
         o.max_height = g_maxy;
         o.salt = LYSH_SWAMP_HUT_SALT;
         o.threads = 1;
+        if (g_p2) {
+            o.phase2_hook = p2_hook;
+            o.phase2_user = a;
+            o.phase2_ctx_init = p2_ctx_init;
+        }
         lysh_scan_result r;
         if (lysh_scan_rect(&o, g_rx0, g_rx1, g_rz0, g_rz1, &r)) continue;
         a->seeds++;
@@ -78,6 +104,7 @@ static void *worker(void *arg) {
         a->reached += r.reached_ladder;
         for (int k = 0; k < LYSH_P1_REJECT_KIND_N; k++) a->reject[k] += r.reject_hist[k];
         for (int k = 0; k < LYSH_TIER_MAX; k++) a->tier[k] += r.tier_hist[k];
+        a->p2_seconds += r.p2_seconds;
     }
     return NULL;
 }
@@ -113,6 +140,8 @@ int main(int argc, char **argv) {
         else if (!strcmp(a, "--max-y")) g_maxy = atoi(v);
         else if (!strcmp(a, "--threads")) g_threads = atoi(v);
         else if (!strcmp(a, "--time-limit")) g_limit = atof(v);
+        else if (!strcmp(a, "--phase2-max-y")) { g_p2 = 1; g_p2_maxy = atoi(v); }
+        else if (!strcmp(a, "--exactly")) g_exactly = atoi(v);
         else if (!strcmp(a, "--version")) { if (set_version(v)) { fprintf(stderr, "bad --version %s\n", v); return 2; } }
         else { fprintf(stderr, "unknown option %s\n", a); return 2; }
     }
@@ -130,6 +159,7 @@ int main(int argc, char **argv) {
         s.seeds += acc[t].seeds; s.scanned += acc[t].scanned; s.accepted += acc[t].accepted; s.reached += acc[t].reached;
         for (int k = 0; k < LYSH_P1_REJECT_KIND_N; k++) s.reject[k] += acc[t].reject[k];
         for (int k = 0; k < LYSH_TIER_MAX; k++) s.tier[k] += acc[t].tier[k];
+        s.p2_eval += acc[t].p2_eval; s.p2_ok += acc[t].p2_ok; s.p2_seconds += acc[t].p2_seconds;
     }
     double secs = now_s() - g_t0;
     printf("{\"seeds\": %lld, \"checks\": %lld, \"seconds\": %.6f, \"checks_per_s\": %.1f, \"threads\": %d, "
@@ -138,7 +168,8 @@ int main(int argc, char **argv) {
     for (int k = 0; k < LYSH_P1_REJECT_KIND_N; k++) printf("%s%lld", k ? ", " : "", s.reject[k]);
     printf("], \"tiers\": [");
     for (int k = 0; k < 7; k++) printf("%s%lld", k ? ", " : "", s.tier[k]);
-    printf("]}\n");
+    printf("], \"phase2\": %d, \"phase2_evaluated\": %lld, \"phase2_accepted\": %lld, \"phase2_seconds\": %.6f}\n",
+           g_p2, s.p2_eval, s.p2_ok, s.p2_seconds);
     free(acc);
     free(th);
     return 0;

@@ -71,6 +71,34 @@ def test_mix_equals_alone_all_at_once():
 
 
 @pytest.mark.timeout(600)
+def test_mix_with_phase2_equals_alone():
+    """Requests with lysh's hut verdict (or-lower and exactly, several thresholds and versions) at a phase-1 height
+    that leaves survivors: finds with their floors and the phase-2 counters equal whf_scan alone."""
+    rng = np.random.default_rng(9)
+    versions = ["1.18", "1.20", "1.21", "26.2"]
+    reqs = []
+    for i in range(16):
+        v = versions[i % 4]
+        mode = whf.P2_EXACTLY if i % 2 else whf.P2_OR_LOWER
+        thr = int(rng.choice([60, 64, 70, 75]))
+        reqs.append(((-8, 8, -8, 8), dict(splitmix=(4242 + i, int(round(2 ** rng.uniform(9, 13)))), max_height=60,
+                                          vflags=whf.version_flags(v), phase2=(mode, thr, whf.biome_tree(v)))))
+    print("scanning %d phase-2 requests alone" % len(reqs), flush=True)
+    ref = alone(reqs)
+    assert sum(st["phase2"][0] for _, st in ref) > 20, "too few survivors to exercise phase 2"
+    print("serving them together", flush=True)
+    with whf.Server(0, slots=4, chunk_checks=1 << 18) as srv:
+        ids = [srv.submit(rect, capacity=1 << 16, **kw) for rect, kw in reqs]
+        got = {}
+        while len(got) < len(ids):
+            for e in srv.poll(timeout_ms=60000):
+                got[e["id"]] = e
+    for rid, r in zip(ids, ref):
+        check_same(got[rid], r)
+        assert got[rid]["stats"]["phase2"] == r[1]["phase2"]
+
+
+@pytest.mark.timeout(600)
 def test_mix_equals_alone_staggered_arrivals():
     """Requests arrive while others run (one big request first, then a stream of small ones)."""
     reqs = [((-9, 9, -9, 9), dict(splitmix=(31337, 1 << 20), max_height=-50))] + request_mix(24, 2, 10)
