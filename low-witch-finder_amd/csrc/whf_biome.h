@@ -11,7 +11,7 @@
 namespace whf {
 
 constexpr int kSplineMaxDepth = 6;     // nesting the device spline evaluator unrolls
-constexpr int kTreeMaxDepth = 12;      // frames of the device tree walk
+constexpr int kTreeMaxDepth = 8;       // levels of the device tree walk
 
 // Device copy of the overworld terrain spline.
 struct SplineTab {
@@ -77,68 +77,87 @@ __device__ __noinline__ float spline_at(const SplineTab *__restrict__ S, int nod
     }
 }
 
-__device__ __forceinline__ int64_t node_dist(const whf_gd_node &n, const int64_t t[6]) {
+// An R-tree node as the device walks it: the game data's node with its children made contiguous (the children of
+// an internal node are nodes first .. first + kids - 1, in the game's order), so that a node's children load
+// together instead of one after another through the pre-order end links. The host builds it from whf_gd_node.
+struct TreeNode {
+    int16_t range[12];
+    uint32_t first;
+    uint8_t kids;
+    uint8_t biome;
+    uint16_t pad;
+};
+static_assert(sizeof(TreeNode) == 32, "tree node layout");
+
+// The six quantized climate targets, by value.
+struct Target {
+    int64_t t[6];
+};
+
+__device__ __forceinline__ int64_t node_dist(const TreeNode &n, const Target &q) {
     // biome.c:23-31: squared distance from the target to the node's box, summed over the six parameters
     int64_t s = 0;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-        const int64_t above = t[k] - (int64_t)n.range[2 * k + 1];
-        const int64_t below = (int64_t)n.range[2 * k] - t[k];
+        const int64_t above = q.t[k] - (int64_t)n.range[2 * k + 1];
+        const int64_t below = (int64_t)n.range[2 * k] - q.t[k];
         const int64_t d = above > 0 ? above : (below > 0 ? below : 0);
         s += d * d;
     }
     return s;
 }
 
-// Climate.RTree.SubTree.search as lysh walks it (biome.c:34-47), without recursion: a stack of frames, one per
-// internal node on the path. A frame scans its children in order; a child whose box is nearer than the best leaf
-// so far is entered (a leaf child is its own answer), and its answer replaces the best only when strictly nearer.
-// A frame's best distance is always the distance of its best leaf (or "none"), so a child frame starts from it.
-__device__ int tree_leaf(const whf_gd_node *__restrict__ g, const int64_t t[6]) {
-    if (g[0].kids == 0) return 0;
-    int fnode[kTreeMaxDepth], fc[kTreeMaxDepth], fk[kTreeMaxDepth], fres[kTreeMaxDepth];
-    int64_t fbest[kTreeMaxDepth], fd[kTreeMaxDepth];
-    int sp = 0;
-    fnode[0] = 0; fc[0] = 1; fk[0] = 0; fres[0] = -1; fbest[0] = INT64_MAX; fd[0] = 0;
-    int ret = -1;
-    bool back = false;
-    for (;;) {
-        if (back) {
-            // the child at fc[sp] (distance fd[sp]) returned leaf `ret`
-            const int64_t dg = ret == fc[sp] ? fd[sp] : node_dist(g[ret], t);
-            if (fbest[sp] > dg) { fbest[sp] = dg; fres[sp] = ret; }
-            fc[sp] = g[fc[sp]].end;
-            fk[sp]++;
-            back = false;
+// Climate.RTree.SubTree.search as lysh walks it (biome.c:34-47), for an internal node: its children in order; a
+// child whose box is nearer than the best leaf so far is entered (a leaf child is its own answer), and its answer
+// replaces the best only when strictly nearer. `best` is always the distance of `res` (or the maximum while there
+// is none), which is what the reference's callee recomputes from the leaf it is handed. Children are read 8 at a
+// time; D bounds the depth (one function per level).
+constexpr int kTreeGroup = 8;
+template <int D>
+__device__ __noinline__ void tree_walk(const TreeNode *__restrict__ g, int node, Target q, int64_t &best, int &res) {
+    const TreeNode nd = g[node];
+    const int first = (int)nd.first, kids = nd.kids;
+    for (int k0 = 0; k0 < kids; k0 += kTreeGroup) {
+        int64_t d[kTreeGroup];
+        int ck[kTreeGroup];
+#pragma unroll
+        for (int k = 0; k < kTreeGroup; k++) {
+            d[k] = 0;
+            ck[k] = 0;
+            if (k0 + k < kids) {
+                const TreeNode c = g[first + k0 + k];
+                d[k] = node_dist(c, q);
+                ck[k] = c.kids;
+            }
         }
-        const int kids = g[fnode[sp]].kids;
-        bool down = false;
-        while (fk[sp] < kids) {
-            const int c = fc[sp];
-            const whf_gd_node nc = g[c];
-            const int64_t d = node_dist(nc, t);
-            if (fbest[sp] > d) {
-                if (nc.kids == 0) {
-                    fbest[sp] = d;
-                    fres[sp] = c;
-                } else {
-                    fd[sp] = d;
-                    const int s1 = sp + 1;
-                    fnode[s1] = c; fc[s1] = c + 1; fk[s1] = 0; fres[s1] = fres[sp]; fbest[s1] = fbest[sp];
-                    sp = s1;
-                    down = true;
-                    break;
+#pragma unroll
+        for (int k = 0; k < kTreeGroup; k++) {
+            if (k0 + k < kids && best > d[k]) {
+                const int c = first + k0 + k;
+                if (ck[k] == 0) {
+                    best = d[k];
+                    res = c;
+                } else if constexpr (D > 1) {
+                    int64_t b2 = best;
+                    int r2 = res;
+                    tree_walk<D - 1>(g, c, q, b2, r2);
+                    if (best > b2) {
+                        best = b2;
+                        res = r2;
+                    }
                 }
             }
-            fc[sp] = nc.end;
-            fk[sp]++;
         }
-        if (down) continue;
-        ret = fres[sp];
-        if (sp == 0) return ret;
-        sp--;
-        back = true;
     }
+}
+
+// The leaf the reference's search returns for the target (its node index in the device layout).
+__device__ __forceinline__ int tree_leaf(const TreeNode *__restrict__ g, const Target &q) {
+    if (g[0].kids == 0) return 0;
+    int64_t best = INT64_MAX;
+    int res = -1;
+    tree_walk<kTreeMaxDepth>(g, 0, q, best, res);
+    return res;
 }
 
 }  // namespace whf

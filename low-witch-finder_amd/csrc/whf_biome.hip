@@ -41,7 +41,7 @@ struct BioPoint {
 
 // Device buffers of the stage, carved from one allocation.
 struct BioBufs {
-    whf_gd_node *nodes;       // all trees
+    TreeNode *nodes;          // all trees, children contiguous
     SplineTab *spline;
     BioTables *tabs;          // [2]: NORMAL, LARGE_BIOMES
     BuildDesc *bd;            // [2][kBioSamplers]
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(64) void k_bio_point(const BioTables *__restrict__ 
     out[i] = p;
 }
 
-__global__ __launch_bounds__(128) void k_bio_tree(const whf_gd_node *__restrict__ g, const BioPoint *__restrict__ pts,
+__global__ __launch_bounds__(128) void k_bio_tree(const TreeNode *__restrict__ g, const BioPoint *__restrict__ pts,
                                                   const whf_biome_query *__restrict__ q,
                                                   whf_biome_result *__restrict__ res, uint8_t *__restrict__ win,
                                                   unsigned long long *__restrict__ checks) {
@@ -104,11 +104,11 @@ __global__ __launch_bounds__(128) void k_bio_tree(const whf_gd_node *__restrict_
     const int occ_y = q[qi].occ_y;
     const bool active = lane <= kQyLanes;   // lanes 0..96: quart heights; lane 97: the query's own height
     int winner = -1;
-    int64_t t[6] = {p.t[0], p.t[1], p.t[2], p.t[3], 0, p.t[5]};
+    Target t{{p.t[0], p.t[1], p.t[2], p.t[3], 0, p.t[5]}};
     const int qy = lane < kQyLanes ? WHF_BIOME_QY0 + lane : (occ_y >> 2);
     if (active) {
         // depth = gradient at block y 4 qy + overworld offset, in double, quantized from float (biome.c:131-141)
-        t[4] = quantize((float)(depth_gradient((double)(qy * 4)) + p.offset));
+        t.t[4] = quantize((float)(depth_gradient((double)(qy * 4)) + p.offset));
         winner = g[tree_leaf(g, t)].biome;
         if (lane < kQyLanes) win[(size_t)qi * kQyLanes + lane] = (uint8_t)winner;
     }
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(128) void k_bio_tree(const whf_gd_node *__restrict_
     if ((lane & 63) == 0) res[qi].swamp_qy[lane >> 6] = m;
     if (lane == kQyLanes) {
         whf_biome_result &r = res[qi];
-        for (int k = 0; k < 6; k++) r.target[k] = t[k];
+        for (int k = 0; k < 6; k++) r.target[k] = t.t[k];
         r.qx = p.qx;
         r.qy = qy;
         r.qz = p.qz;
@@ -134,7 +134,7 @@ size_t bio_carve(uint8_t *base, BioBufs &b) {
     size_t o = 0;
     auto take = [&](size_t bytes) { uint8_t *p = base ? base + o : nullptr; o += align256(bytes); return p; };
     const uint32_t nn = whf_gd_tree_base[WHF_GD_TREES];
-    b.nodes = (whf_gd_node *)take(nn * sizeof(whf_gd_node));
+    b.nodes = (TreeNode *)take(nn * sizeof(TreeNode));
     b.spline = (SplineTab *)take(sizeof(SplineTab));
     b.tabs = (BioTables *)take(2 * sizeof(BioTables));
     b.bd = (BuildDesc *)take(2 * kBioSamplers * sizeof(BuildDesc));
@@ -168,6 +168,31 @@ void bio_layout(int large, BioTables &bt, std::vector<BuildDesc> &bd) {
     for (const SamplerRef &r : set) bd.push_back(make_build_desc(r));
 }
 
+// The trees in the device layout: each tree keeps its node range, with the nodes renumbered breadth first so that
+// the children of a node are consecutive, in the game's order (the pre-order data chains them through end links).
+std::vector<TreeNode> relay_trees() {
+    const uint32_t nn = whf_gd_tree_base[WHF_GD_TREES];
+    std::vector<TreeNode> out(nn);
+    std::vector<uint32_t> order;
+    for (int t = 0; t < WHF_GD_TREES; t++) {
+        const uint32_t base = whf_gd_tree_base[t];
+        const whf_gd_node *g = whf_gd_nodes + base;
+        order.assign(1, 0);   // tree-local pre-order indices in the new order
+        for (size_t i = 0; i < order.size(); i++) {
+            const whf_gd_node &src = g[order[i]];
+            TreeNode &dst = out[base + i];
+            std::memcpy(dst.range, src.range, sizeof dst.range);
+            dst.kids = src.kids;
+            dst.biome = src.biome;
+            dst.pad = 0;
+            dst.first = (uint32_t)order.size();
+            uint32_t c = order[i] + 1;
+            for (int k = 0; k < src.kids; k++, c = g[c].end) order.push_back(c);
+        }
+    }
+    return out;
+}
+
 int bio_init(whf_ctx *c, BioBufs &b) {
     if (c->d_bio) {
         bio_carve((uint8_t *)c->d_bio, b);
@@ -188,8 +213,9 @@ int bio_init(whf_ctx *c, BioBufs &b) {
     c->bio_bytes = bytes;
     bio_carve((uint8_t *)base, b);
     const hipStream_t st = c->stream;
-    HIP_TRY(hipMemcpyAsync(b.nodes, whf_gd_nodes, whf_gd_tree_base[WHF_GD_TREES] * sizeof(whf_gd_node),
-                           hipMemcpyHostToDevice, st));
+    const std::vector<TreeNode> dev_nodes = relay_trees();
+    HIP_TRY(hipMemcpyAsync(b.nodes, dev_nodes.data(), dev_nodes.size() * sizeof(TreeNode), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
     SplineTab sp;
     std::memset(&sp, 0, sizeof sp);
     std::memcpy(sp.node, whf_gd_spline_nodes, whf_gd_n_spline_nodes * sizeof(whf_gd_spline_node));
@@ -241,7 +267,7 @@ int whf_biome(whf_ctx *c, const whf_biome_query *queries, uint64_t n, int tree, 
     if (rc != WHF_OK) return rc;
     const hipStream_t st = c->stream;
     const int large = (vflags & WHF_V_LARGE) ? 1 : 0;
-    const whf_gd_node *g = b.nodes + whf_gd_tree_base[tree];
+    const TreeNode *g = b.nodes + whf_gd_tree_base[tree];
     std::vector<uint64_t> seeds;
     std::vector<uint32_t> slot;
     std::unordered_map<uint64_t, uint32_t> seen;
