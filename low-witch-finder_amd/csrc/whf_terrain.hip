@@ -3,8 +3,9 @@
 //
 // Columns are grouped by chunk; what the columns of a chunk share is computed once per chunk:
 //   k_build          the 144 named-noise samplers of a seed (climate, density, noodle and aquifer noises)
-//   k_build_blended  one lane per seed: the blended noise's 40 samplers, built in sequence from one random as the
-//                    reference builds them, and the aquifer's positional random factory
+//   k_build_blended  a wave per seed: the blended noise's 40 samplers, which the reference draws one after another
+//                    from one random, a lane each after a jump ahead in that random; and the aquifer's positional
+//                    random factory
 //   k_quarts         a lane per quart of the chunk's 27 x 19 aquifer quart table (preliminary surface, deep-dark
 //                    offset), and a lane per corner column the chunk's columns use (its terrain point)
 //   k_cells          a block per chunk: skipSamplingAboveY from the 11 x 11 surface grid, then a lane per aquifer
@@ -88,6 +89,7 @@ struct TerBufs {
     ulonglong2 *aq;           // [kChunkBatch]
     uint64_t *seeds;          // [kChunkBatch]
     float *sin_tab;           // Mth.SIN
+    ulonglong2 *jumps;        // [kLegacyPerlins][128]: the engine's 259 k-step jump matrices
     int4 *chunks;             // [kChunkBatch]
     int2 *pairs;              // [kChunkBatch * kCornerCols]: (chunk, corner column) of the round's corner columns
     int *qsurf;
@@ -104,32 +106,64 @@ struct TerBufs {
     ChunkBufs chunk_bufs() const { return ChunkBufs{chunks, qsurf, qdd, cells, skip, tp, cmain, cnood, claims}; }
 };
 
-__global__ __launch_bounds__(64) void k_build_blended(const uint64_t *__restrict__ seeds, uint32_t ns, uint64_t t_hi,
-                                                      uint64_t t_lo, uint64_t a_hi, uint64_t a_lo,
+// The state of an Xoroshiro128 engine after n steps, n fixed: a 128 x 128 matrix over GF(2) (the engine is linear),
+// stored as its 128 columns; column j is the image of state bit j (bits 0..63 of lo, then of hi).
+__device__ __forceinline__ Xr xr_jump(const ulonglong2 *__restrict__ cols, Xr s) {
+    uint64_t lo = 0, hi = 0;
+#pragma unroll 4
+    for (int j = 0; j < 128; j++) {
+        const uint64_t bit = j < 64 ? (s.lo >> j) & 1ull : (s.hi >> (j - 64)) & 1ull;
+        const ulonglong2 c = cols[j];
+        lo ^= c.x & (0ull - bit);
+        hi ^= c.y & (0ull - bit);
+    }
+    return Xr{lo, hi};
+}
+
+__global__ __launch_bounds__(64) void k_build_blended(const uint64_t *__restrict__ seeds, uint64_t t_hi, uint64_t t_lo,
+                                                      uint64_t a_hi, uint64_t a_lo, const ulonglong2 *__restrict__ jumps,
                                                       uint8_t *__restrict__ ltab, SPar *__restrict__ lpar,
                                                       ulonglong2 *__restrict__ aq) {
+    // a wave per seed. The game builds the blended noise's 40 samplers (lower, upper and main stacks, each from its
+    // coarsest octave down) one after another from one random, createRandom("minecraft:terrain") of the world's
+    // deriver (interp_noise.c:19-48, noise.c:167-209). A sampler takes 3 + 256 draws of one engine step each unless
+    // a bounded draw rejects (odds below 2^-24 per draw), so lane k jumps to step 259 k and builds sampler k; a lane
+    // whose engine did not end where the next lane's began saw a rejection, and then lane 0 builds them all again
+    // in sequence.
     __shared__ uint32_t words[64 * 64];
     const uint32_t lane = threadIdx.x;
-    const uint32_t s = blockIdx.x * 64 + lane;
-    if (s >= ns) return;   // each lane shuffles in its own LDS column only
+    const uint32_t s = blockIdx.x;
     const uint64_t seed = seeds[s];
-    // deriver of the world random, createRandom("minecraft:terrain"); then lower, upper and main stacks, each
-    // from its coarsest octave down, all from this one random (interp_noise.c:19-48, noise.c:167-209)
     Xr r = xr_seed(seed);
     Xr der = xr_fork(r);
-    Xr pr = xr_make(t_hi ^ der.lo, t_lo ^ der.hi);
-    for (int k = 0; k < kLegacyPerlins; k++)
-        build_sampler(pr, words, lane, ltab + ((size_t)s * kLegacyPerlins + k) * 256, lpar[(size_t)s * kLegacyPerlins + k]);
-    // the aquifer's random: forkPositional of the world random, fromHashOf("minecraft:aquifer"), forkPositional
-    // (aquifer.c:123-135)
-    Xr w = xr_seed(seed);
-    const uint64_t b_lo = xr_next(w);
-    const uint64_t b_hi = xr_next(w);
-    Xr a = xr_make(a_hi ^ b_lo, a_lo ^ b_hi);
-    ulonglong2 f;
-    f.x = xr_next(a);
-    f.y = xr_next(a);
-    aq[s] = f;
+    const Xr p0 = xr_make(t_hi ^ der.lo, t_lo ^ der.hi);
+    uint8_t *tab = ltab + (size_t)s * kLegacyPerlins * 256;
+    SPar *par = lpar + (size_t)s * kLegacyPerlins;
+    Xr start = p0, pr = p0;
+    if (lane < (uint32_t)kLegacyPerlins) {
+        start = xr_jump(jumps + (size_t)lane * 128, p0);
+        pr = start;
+        build_sampler(pr, words, lane, tab + (size_t)lane * 256, par[lane]);
+    }
+    const uint64_t next_lo = __shfl_down(start.lo, 1), next_hi = __shfl_down(start.hi, 1);
+    const bool off = lane + 1 < (uint32_t)kLegacyPerlins && (pr.lo != next_lo || pr.hi != next_hi);
+    if (__ballot(off) && lane == 0) {
+        __threadfence_block();
+        Xr q = p0;
+        for (int k = 0; k < kLegacyPerlins; k++) build_sampler(q, words, 0, tab + (size_t)k * 256, par[k]);
+    }
+    if (lane == 0) {
+        // the aquifer's random: forkPositional of the world random, fromHashOf("minecraft:aquifer"), forkPositional
+        // (aquifer.c:123-135)
+        Xr w = xr_seed(seed);
+        const uint64_t b_lo = xr_next(w);
+        const uint64_t b_hi = xr_next(w);
+        Xr a = xr_make(a_hi ^ b_lo, a_lo ^ b_hi);
+        ulonglong2 f;
+        f.x = xr_next(a);
+        f.y = xr_next(a);
+        aq[s] = f;
+    }
 }
 
 __global__ __launch_bounds__(64) void k_quarts(TerArgs ta, ChunkBufs cb, uint32_t nch, const int2 *__restrict__ pairs,
@@ -422,6 +456,7 @@ size_t ter_carve(uint8_t *base, TerBufs &b) {
     b.aq = (ulonglong2 *)take(nc * sizeof(ulonglong2));
     b.seeds = (uint64_t *)take(nc * sizeof(uint64_t));
     b.sin_tab = (float *)take(65536 * sizeof(float));
+    b.jumps = (ulonglong2 *)take((size_t)kLegacyPerlins * 128 * sizeof(ulonglong2));
     b.chunks = (int4 *)take(nc * sizeof(int4));
     b.pairs = (int2 *)take(nc * kCornerCols * sizeof(int2));
     b.qsurf = (int *)take(nc * kAqQuarts * sizeof(int));
@@ -456,6 +491,58 @@ void ter_layout(int large, TerrainTables &tt, std::vector<BuildDesc> &bd) {
     for (int n = 0; n < T_COUNT; n++) fill_noise_desc(tt.noise[n], pid[n], set, SET_COUNT);
     bd.clear();
     for (const SamplerRef &r : set) bd.push_back(make_build_desc(r));
+}
+
+// Host side of the jump matrices: the Xoroshiro128 engine step (whf_device.h xr_next) as a GF(2) matrix, its
+// 259th power P, and P^k for k < kLegacyPerlins, each checked against stepping the engine.
+struct Gf2Mat {
+    uint64_t lo[128], hi[128];   // column j
+};
+void gf2_apply(const Gf2Mat &m, uint64_t lo, uint64_t hi, uint64_t &olo, uint64_t &ohi) {
+    olo = ohi = 0;
+    for (int j = 0; j < 128; j++)
+        if (j < 64 ? (lo >> j) & 1 : (hi >> (j - 64)) & 1) {
+            olo ^= m.lo[j];
+            ohi ^= m.hi[j];
+        }
+}
+Gf2Mat gf2_mul(const Gf2Mat &a, const Gf2Mat &b) {   // a after b
+    Gf2Mat c;
+    for (int j = 0; j < 128; j++) gf2_apply(a, b.lo[j], b.hi[j], c.lo[j], c.hi[j]);
+    return c;
+}
+void engine_step(uint64_t &lo, uint64_t &hi) {
+    const uint64_t h = hi ^ lo;
+    lo = ((lo << 49) | (lo >> 15)) ^ h ^ (h << 21);
+    hi = (h << 28) | (h >> 36);
+}
+int jump_matrices(std::vector<ulonglong2> &out) {
+    Gf2Mat step, id;
+    for (int j = 0; j < 128; j++) {
+        uint64_t lo = j < 64 ? 1ull << j : 0, hi = j < 64 ? 0 : 1ull << (j - 64);
+        id.lo[j] = lo;
+        id.hi[j] = hi;
+        engine_step(lo, hi);
+        step.lo[j] = lo;
+        step.hi[j] = hi;
+    }
+    Gf2Mat p = id, sq = step;
+    for (unsigned e = 259; e; e >>= 1) {
+        if (e & 1) p = gf2_mul(sq, p);
+        sq = gf2_mul(sq, sq);
+    }
+    out.assign((size_t)kLegacyPerlins * 128, ulonglong2{});
+    Gf2Mat m = id;
+    uint64_t tlo = 0x0123456789ABCDEFull, thi = 0xFEDCBA9876543210ull;   // a test state stepped alongside
+    for (int k = 0; k < kLegacyPerlins; k++) {
+        uint64_t jlo, jhi;
+        gf2_apply(m, 0x0123456789ABCDEFull, 0xFEDCBA9876543210ull, jlo, jhi);
+        if (jlo != tlo || jhi != thi) return WHF_EDEVICE;
+        for (int j = 0; j < 128; j++) out[(size_t)k * 128 + j] = ulonglong2{m.lo[j], m.hi[j]};
+        m = gf2_mul(p, m);
+        for (int i = 0; i < 259; i++) engine_step(tlo, thi);
+    }
+    return WHF_OK;
 }
 
 int ter_init(whf_ctx *c, TerBufs &b) {
@@ -498,6 +585,12 @@ int ter_init(whf_ctx *c, TerBufs &b) {
     for (int r = 0; r < 3; r++) sp.root[r] = whf_gd_spline_root[r];
     HIP_TRY(hipMemcpyAsync(b.spline, &sp, sizeof sp, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b.sin_tab, whf_gd_mc_sin, 65536 * sizeof(float), hipMemcpyHostToDevice, st));
+    std::vector<ulonglong2> jm;
+    if (jump_matrices(jm) != WHF_OK) {
+        std::snprintf(g_last_error, sizeof g_last_error, "engine jump matrices do not match stepping");
+        return WHF_EDEVICE;
+    }
+    HIP_TRY(hipMemcpyAsync(b.jumps, jm.data(), jm.size() * sizeof(ulonglong2), hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
     return WHF_OK;
 }
@@ -511,8 +604,8 @@ int build_round(whf_ctx *c, TerBufs &b, const std::vector<uint64_t> &seeds, int 
     md5_longs("minecraft:aquifer", a_hi, a_lo);
     HIP_TRY(hipMemcpyAsync(b.seeds, seeds.data(), ns * sizeof(uint64_t), hipMemcpyHostToDevice, st));
     launch_build(c, b.seeds, b.bd + large * kTerSamplers, kTerSamplers, ns, b.tab, b.par, c->d_cnt + CNT_BIO_BUILT);
-    hipLaunchKernelGGL(k_build_blended, dim3((ns + 63) / 64), dim3(64), 0, st, b.seeds, ns, t_hi, t_lo, a_hi, a_lo,
-                       b.ltab, b.lpar, b.aq);
+    hipLaunchKernelGGL(k_build_blended, dim3(ns), dim3(64), 0, st, (const uint64_t *)b.seeds, t_hi, t_lo, a_hi, a_lo,
+                       (const ulonglong2 *)b.jumps, b.ltab, b.lpar, b.aq);
     return check_launch();
 }
 

@@ -104,3 +104,17 @@ def test_many_columns_few_seeds(ctx):
 def test_invalid_and_empty(ctx):
     h, st = ctx.surface_heights([])
     assert len(h) == 0 and len(st) == 0
+
+
+def test_blended_rejection_seeds(ctx):
+    """Seeds whose blended-noise draws reject a bounded draw before the last sampler (the device's per-sampler jump
+    ahead is then off, and it builds the samplers again in sequence): heights equal the oracle's."""
+    from test_oracle_p2 import BLENDED_REJECTION_SEEDS
+    rng = random.Random(23)
+    rows = [(seed, rng.randint(-3000, 3000), rng.randint(-3000, 3000)) for seed, _ in BLENDED_REJECTION_SEEDS
+            for _ in range(6)]
+    # mixed with seeds that take the parallel build, in one round
+    rows += [(rng.getrandbits(64), rng.randint(-3000, 3000), rng.randint(-3000, 3000)) for _ in range(10)]
+    h, st = ctx.surface_heights(rows)
+    for i, r in enumerate(rows):
+        assert (int(h[i]), int(st[i])) == O.Terrain(r[0]).height(r[1], r[2]), r

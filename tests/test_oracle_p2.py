@@ -134,3 +134,57 @@ def test_surface_is_on_a_block():
         assert t.substance(x, h - 1, z, t.density(x, h - 1, z)) == st
         if h < 320:
             assert t.substance(x, h, z, t.density(x, h, z)) == O.AIR
+
+
+# Seeds whose blended-noise random (createRandom("minecraft:terrain") of the world's deriver, interp_noise.c:19-48)
+# rejects a bounded draw of Xoroshiro128++'s nextInt while building sampler k < 39: the samplers after it start one
+# or more engine steps later than 259 k. Used by the GPU parity tests of the terrain stage.
+BLENDED_REJECTION_SEEDS = [(5219354412081042641, 2), (347959725458519193, 13), (5518476781235771321, 18),
+                           (3913600556879318232, 36), (7453945117803196337, 38)]
+
+
+def first_rejecting_sampler(seed):
+    """The first of the 40 blended-noise samplers whose permutation draws reject, or None: the game's Xoroshiro128++
+    (seed upgrade, deriver, MD5 of the name) and Java 17's bounded nextInt, restated from their published
+    definitions."""
+    import hashlib
+    m64 = (1 << 64) - 1
+
+    def mix64(z):
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m64
+        return z ^ (z >> 31)
+
+    def nxt(s):
+        lo, hi = s
+        out = ((((lo + hi) & m64) << 17 | ((lo + hi) & m64) >> 47) + lo) & m64
+        h = hi ^ lo
+        return out, ((((lo << 49) | (lo >> 15)) & m64) ^ h ^ ((h << 21) & m64), ((h << 28) | (h >> 36)) & m64)
+
+    d = hashlib.md5(b"minecraft:terrain").digest()
+    t_hi, t_lo = int.from_bytes(d[:8], "big"), int.from_bytes(d[8:], "big")
+    l = (seed & m64) ^ 0x6A09E667F3BCC909
+    s = (mix64(l), mix64((l - 0x61C8864680B583EB) & m64))
+    a, s = nxt(s)
+    b, s = nxt(s)
+    s = (t_hi ^ a, t_lo ^ b)
+    for k in range(40):
+        for _ in range(3):
+            _, s = nxt(s)
+        for i in range(256):
+            bound = 256 - i
+            lim = (1 << 32) % bound
+            out, s = nxt(s)
+            rejected = False
+            while ((out & 0xFFFFFFFF) * bound) & 0xFFFFFFFF < lim:
+                rejected = True
+                out, s = nxt(s)
+            if rejected:
+                return k
+    return None
+
+
+def test_blended_rejection_seeds():
+    for seed, k in BLENDED_REJECTION_SEEDS:
+        assert first_rejecting_sampler(seed) == k
+    assert first_rejecting_sampler(1) is None
