@@ -49,6 +49,14 @@ WORK_UNIT = {"build_erosion": "Perlin sampler built = 271 Xoroshiro128++ steps x
                              % OPS_PER_Y0_EVAL}
 
 
+def note(msg):
+    """A progress line on stderr after each finished step or stage (the JSON result goes to stdout)."""
+    print("[bench %.1fs] %s" % (time.perf_counter() - T0, msg), file=sys.stderr, flush=True)
+
+
+T0 = time.perf_counter()
+
+
 def algorithmic_ops(st):
     """Algorithmic lane-operations per kernel family of one scan, from the scan's own counters."""
     built = st["samplers_built"]
@@ -132,10 +140,12 @@ def deep_dive(whf, sweep_n, expand_n, cpu_seconds):
         t0 = time.perf_counter()
         spots, _, st = ctx.deepdive(s0, sweep_n, DD_RECT, DD_FLOOR, spot_capacity=1 << 21)
         t_sweep = time.perf_counter() - t0
+        note("deep dive sweep of %d structure seeds: %.3f s" % (sweep_n, t_sweep))
         t0 = time.perf_counter()
         xspots, finds, xs = ctx.deepdive(s0, expand_n, DD_RECT, DD_FLOOR, expand=True, max_height=-50,
                                          spot_capacity=1 << 21)
         t_exp_total = time.perf_counter() - t0
+        note("deep dive expansion of %d structure seeds: %.3f s" % (expand_n, t_exp_total))
     with whf.Context(0, profile=True) as ctx:
         _, _, ps = ctx.deepdive(s0, min(sweep_n, 1 << 26), DD_RECT, DD_FLOOR, spot_capacity=1 << 21)
         _, _, px = ctx.deepdive(s0, expand_n, DD_RECT, DD_FLOOR, expand=True, max_height=-50, spot_capacity=1 << 21)
@@ -249,6 +259,7 @@ def serving(whf, n, load, slots, chunk_checks, versions=("1.20",), background=No
     ev, wall, _ = serve(whf, sat, None, slots, chunk_checks)
     capacity = sum(e["stats"]["checks"] for e in ev) / wall
     rate = load * capacity / mean_checks
+    note("serving capacity %.3g checks/s from %d requests at once" % (capacity, len(sat)))
     ev, wall, dd = serve(whf, reqs, rate, slots, chunk_checks, background)
     lat = np.array([e["done_s"] - e["submit_s"] for e in ev]) * 1e3
     big = np.array([q["splitmix"][1] >= 1 << 19 for q in reqs])
@@ -312,14 +323,32 @@ def main():
         for prec in (whf.PREC_FP64, whf.PREC_FP32_GATE):
             for _ in range(args.warmup):
                 ctx.scan(rect, precision=prec, **kw)
+            note("warmup done (precision %d)" % prec)
             # whf_scan returns after synchronizing the library's stream, so the wall clock between calls brackets
             # exactly the device work of the K steps
             t0 = time.perf_counter()
             for _ in range(args.steps):
                 hits, st = ctx.scan(rect, precision=prec, **kw)
             timing[prec] = (time.perf_counter() - t0) / args.steps
+            note("%d timed steps (precision %d): %.3f s per step, %d finds" % (args.steps, prec, timing[prec], len(hits)))
             results[prec] = (hits, st)
         err = ctx.noise_error([int(x) for x in range(1, 257)], 4096)
+        # phase 2 on its own: the step's phase-1 survivors evaluated in one call (throughput), and one hut per call
+        # (the latency a small request sees)
+        kw1 = {k: v for k, v in kw.items() if k != "phase2"}
+        surv, _ = ctx.scan(rect, precision=whf.PREC_FP64, **kw1)
+        huts = [(h[0], h[3], h[4]) for h in surv]
+        tree = whf.biome_tree("1.20")
+        ctx.hut_eval(huts[:8], tree, THRESHOLD, vflags)
+        t0 = time.perf_counter()
+        ctx.hut_eval(huts, tree, THRESHOLD, vflags)
+        p2_batch_s = time.perf_counter() - t0
+        singles = huts[:20]
+        t0 = time.perf_counter()
+        for h in singles:
+            ctx.hut_eval([h], tree, THRESHOLD, vflags)
+        p2_single_s = (time.perf_counter() - t0) / max(1, len(singles))
+        note("phase 2 alone: %d huts in %.1f ms, %.2f ms for one" % (len(huts), p2_batch_s * 1e3, p2_single_s * 1e3))
     h64, s64 = results[whf.PREC_FP64]
     h32, last = results[whf.PREC_FP32_GATE]
     identical = h64 == h32 and all(s64[k] == last[k] for k in ("funnel", "tiers", "accepted", "checks", "phase2"))
@@ -350,14 +379,18 @@ def main():
                     launch_ms=round(fam[dominant] / n_batches, 4), ops_per_launch=r.get("ops_per_launch"),
                     work_unit=WORK_UNIT.get(dominant))
 
+    note("profiled step done")
     cpu = None if args.no_cpu else cpu_baseline(min(args.seeds, 1 << 20), args.cpu_seconds)
+    note("cpu baseline done")
     srv = None if args.serve_requests <= 0 else serving(whf, args.serve_requests, args.serve_load, args.serve_slots,
                                                         args.serve_chunk)
+    note("serving done")
     mixed = None
     if args.serve_requests > 0 and args.mixed_dd_seeds > 0:
         mixed = serving(whf, args.serve_requests, args.serve_load, args.serve_slots, args.serve_chunk,
                         versions=("1.18", "1.20", "1.21"),
                         background=(splitmix_first(3) & ((1 << 48) - 1), args.mixed_dd_seeds))
+        note("mixed serving done")
     dd = None if args.no_deep_dive else deep_dive(whf, args.dd_seeds, args.dd_expand,
                                                   0 if args.no_cpu else min(10.0, args.cpu_seconds))
     line = {
@@ -375,6 +408,9 @@ def main():
                    "rejected_biome": last["phase2"][2], "rejected_y": last["phase2"][3],
                    "rejected_flood": last["phase2"][4], "ms_per_step": round(last["phase2_ms"], 3),
                    "ms_per_hut": round(last["phase2_ms"] / max(1, last["phase2"][0]), 4),
+                   "batch": {"huts": len(huts), "ms": round(p2_batch_s * 1e3, 3),
+                             "ms_per_hut": round(p2_batch_s * 1e3 / max(1, len(huts)), 4)},
+                   "single_hut_ms": round(p2_single_s * 1e3, 3),
                    "cpu_reference_ms_per_hut": cpu["phase2_ms_per_hut"] if cpu else None},
         "fp64_value": round(checks / timing[whf.PREC_FP64], 1), "fp64_ms_per_step": round(timing[whf.PREC_FP64] * 1e3, 3),
         "fp32_gate_value": round(checks / timing[whf.PREC_FP32_GATE], 1),

@@ -76,6 +76,22 @@ def test_hut_eval_random_against_oracle(ctx):
     assert not bad, bad[:3]
 
 
+def test_hut_eval_two_rounds(ctx):
+    """More huts than one device round holds (512), with repeated seeds and repeated huts: a sample, including the
+    rounds' first and last huts, equals the oracle."""
+    rng = random.Random(31)
+    seeds = [rng.getrandbits(64) for _ in range(40)]
+    huts = [(seeds[k % 40], rng.randint(-20000, 20000), rng.randint(-20000, 20000)) for k in range(600)]
+    huts[600 - 1] = huts[3]
+    print("whf_hut_eval: %d huts" % len(huts), flush=True)
+    got = ctx.hut_eval(huts, 3)
+    assert got[599] == got[3]
+    for i in [0, 1, 510, 511, 512, 513, 598] + rng.sample(range(600), 30):
+        s, cx, cz = huts[i]
+        want = O.Terrain(s).hut_eval(3, cx * 16, cz * 16, -50)
+        assert all(got[i][k] == want[k] for k in FIELDS), (i, huts[i])
+
+
 def test_post_heights_random_against_oracle(ctx):
     rng = random.Random(29)
     cols = []
