@@ -127,30 +127,34 @@ __global__ __launch_bounds__(64) void k_build_blended(const uint64_t *__restrict
     // a wave per seed. The game builds the blended noise's 40 samplers (lower, upper and main stacks, each from its
     // coarsest octave down) one after another from one random, createRandom("minecraft:terrain") of the world's
     // deriver (interp_noise.c:19-48, noise.c:167-209). A sampler takes 3 + 256 draws of one engine step each unless
-    // a bounded draw rejects (odds below 2^-24 per draw), so lane k jumps to step 259 k and builds sampler k; a lane
-    // whose engine did not end where the next lane's began saw a rejection, and then lane 0 builds them all again
-    // in sequence.
+    // a bounded draw rejects (odds below 2^-24 per draw), so lane k jumps to step 259 k and builds sampler k. A lane
+    // whose engine did not end where the next lane's began saw a rejection (exactly: the engine's period is
+    // 2^128 - 1); its end is where the next sampler starts, so the lanes after it jump again from there.
     __shared__ uint32_t words[64 * 64];
     const uint32_t lane = threadIdx.x;
     const uint32_t s = blockIdx.x;
     const uint64_t seed = seeds[s];
     Xr r = xr_seed(seed);
     Xr der = xr_fork(r);
-    const Xr p0 = xr_make(t_hi ^ der.lo, t_lo ^ der.hi);
+    Xr base = xr_make(t_hi ^ der.lo, t_lo ^ der.hi);   // start of sampler `first`
     uint8_t *tab = ltab + (size_t)s * kLegacyPerlins * 256;
     SPar *par = lpar + (size_t)s * kLegacyPerlins;
-    Xr start = p0, pr = p0;
-    if (lane < (uint32_t)kLegacyPerlins) {
-        start = xr_jump(jumps + (size_t)lane * 128, p0);
-        pr = start;
-        build_sampler(pr, words, lane, tab + (size_t)lane * 256, par[lane]);
-    }
-    const uint64_t next_lo = __shfl_down(start.lo, 1), next_hi = __shfl_down(start.hi, 1);
-    const bool off = lane + 1 < (uint32_t)kLegacyPerlins && (pr.lo != next_lo || pr.hi != next_hi);
-    if (__ballot(off) && lane == 0) {
-        __threadfence_block();
-        Xr q = p0;
-        for (int k = 0; k < kLegacyPerlins; k++) build_sampler(q, words, 0, tab + (size_t)k * 256, par[k]);
+    for (uint32_t first = 0;;) {
+        Xr start = base, pr = base;
+        const bool mine = lane >= first && lane < (uint32_t)kLegacyPerlins;
+        if (mine) {
+            start = xr_jump(jumps + (size_t)(lane - first) * 128, base);
+            pr = start;
+            build_sampler(pr, words, lane, tab + (size_t)lane * 256, par[lane]);
+        }
+        const uint64_t next_lo = __shfl_down(start.lo, 1), next_hi = __shfl_down(start.hi, 1);
+        const bool off = mine && lane + 1 < (uint32_t)kLegacyPerlins && (pr.lo != next_lo || pr.hi != next_hi);
+        const unsigned long long m = __ballot(off);
+        if (!m) break;
+        const int k = __ffsll((long long)m) - 1;   // the first lane that ended off: its samplers are right
+        base.lo = __shfl(pr.lo, k);
+        base.hi = __shfl(pr.hi, k);
+        first = (uint32_t)k + 1;
     }
     if (lane == 0) {
         // the aquifer's random: forkPositional of the world random, fromHashOf("minecraft:aquifer"), forkPositional
