@@ -109,6 +109,16 @@ struct whf_ctx {
     void *d_ter = nullptr;                // terrain stage buffers and tables (whf_terrain.hip), allocated on first use
     size_t ter_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Seed Check pipelining: a batch's gates 2 and 3 run on tail_stream beside the next batch's sampler builds and
+    // erosion gate, so the buffers they read exist twice (set 0 above, set 1 here) and batches alternate between
+    // them; ev_front[p] marks a front's end, ev_tail[p] the end of the tail that read set p
+    hipStream_t tail_stream = nullptr;
+    uint64_t *d_seeds1 = nullptr;         // [kMaxSeeds]
+    uint8_t *d_tabC1 = nullptr;
+    whf::SPar *d_parC1 = nullptr;
+    uint4 *d_q1_1 = nullptr, *d_q2_1 = nullptr;
+    uint32_t *d_slotC1 = nullptr, *d_seedC1 = nullptr, *d_needL1 = nullptr, *d_ctl1 = nullptr;
+    hipEvent_t ev_front[2] = {nullptr, nullptr}, ev_tail[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> prof_ev;  // pairs per kernel launch when profiling
     std::vector<int> prof_fam;
     int n_cu = 256;
@@ -116,15 +126,17 @@ struct whf_ctx {
 
 namespace whf {
 
-// Brackets the launches of one kernel family with HIP events on the library's stream when profiling.
+// Brackets the launches of one kernel family with HIP events on the stream they go to (the library's stream
+// unless given) when profiling.
 struct Prof {
     whf_ctx *c;
     int fam;
-    Prof(whf_ctx *c_, int f) : c(c_), fam(f) {
+    hipStream_t st;
+    Prof(whf_ctx *c_, int f, hipStream_t s = nullptr) : c(c_), fam(f), st(s ? s : c_->stream) {
         if (c->profile) {
             hipEvent_t e;
             (void)hipEventCreate(&e);
-            (void)hipEventRecord(e, c->stream);
+            (void)hipEventRecord(e, st);
             c->prof_ev.push_back(e);
             c->prof_fam.push_back(fam);
         }
@@ -133,7 +145,7 @@ struct Prof {
         if (c->profile) {
             hipEvent_t e;
             (void)hipEventCreate(&e);
-            (void)hipEventRecord(e, c->stream);
+            (void)hipEventRecord(e, st);
             c->prof_ev.push_back(e);
         }
     }
