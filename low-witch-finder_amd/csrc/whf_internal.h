@@ -87,6 +87,18 @@ enum Fam {
 
 }  // namespace whf
 
+namespace whf {
+// The buffers of one batch that its later stages read: its seeds, erosion survivors (Q1) and their seeds' flags,
+// set-C slots and tables, gate-2 survivors (Q2) and their seeds' flags, and its control words.
+struct BatchSet {
+    uint64_t *seeds;
+    uint8_t *tabC;
+    SPar *parC;
+    uint4 *q1, *q2;
+    uint32_t *needC, *slotC, *seedC, *needL, *ctl;
+};
+}  // namespace whf
+
 struct whf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -109,16 +121,14 @@ struct whf_ctx {
     void *d_ter = nullptr;                // terrain stage buffers and tables (whf_terrain.hip), allocated on first use
     size_t ter_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // Seed Check pipelining: a batch's gates 2 and 3 run on tail_stream beside the next batch's sampler builds and
-    // erosion gate, so the buffers they read exist twice (set 0 above, set 1 here) and batches alternate between
-    // them; ev_front[p] marks a front's end, ev_tail[p] the end of the tail that read set p
-    hipStream_t tail_stream = nullptr;
-    uint64_t *d_seeds1 = nullptr;         // [kMaxSeeds]
-    uint8_t *d_tabC1 = nullptr;
-    whf::SPar *d_parC1 = nullptr;
-    uint4 *d_q1_1 = nullptr, *d_q2_1 = nullptr;
-    uint32_t *d_slotC1 = nullptr, *d_seedC1 = nullptr, *d_needL1 = nullptr, *d_ctl1 = nullptr;
-    hipEvent_t ev_front[2] = {nullptr, nullptr}, ev_tail[2] = {nullptr, nullptr};
+    // Seed Check pipelining: a batch runs in three stages on three streams (the library's stream: seeds, set E and
+    // the erosion gate; stage_stream[0]: set C and gate 2; stage_stream[1]: set L and gate 3), so three batches are
+    // on the device at once. What a batch's later stages read exists three times (batch_set[0] aliases the buffers
+    // above, 1 and 2 are their own) and batch k uses set k % 3; ev_stage[s][set] marks the end of stage s of the
+    // batch on that set
+    hipStream_t stage_stream[2] = {nullptr, nullptr};
+    whf::BatchSet batch_set[3] = {};
+    hipEvent_t ev_stage[3][3] = {};
     std::vector<hipEvent_t> prof_ev;  // pairs per kernel launch when profiling
     std::vector<int> prof_fam;
     int n_cu = 256;

@@ -814,19 +814,23 @@ int ctx_alloc(whf_ctx *c) {
     HIP_TRY(hipMalloc(&c->d_seedL, S * 4));
     HIP_TRY(hipMalloc(&c->d_ctl, 64));
     static_assert(CTL_N * sizeof(uint32_t) <= 64, "control words");
-    HIP_TRY(hipMalloc(&c->d_seeds1, S * sizeof(uint64_t)));
-    HIP_TRY(hipMalloc(&c->d_tabC1, S * kSetSize[SET_C] * 256));
-    HIP_TRY(hipMalloc(&c->d_parC1, S * kSetSize[SET_C] * sizeof(SPar)));
-    HIP_TRY(hipMalloc(&c->d_q1_1, (size_t)kMaxItems * sizeof(uint4)));
-    HIP_TRY(hipMalloc(&c->d_q2_1, (size_t)kMaxItems * sizeof(uint4)));
-    HIP_TRY(hipMalloc(&c->d_slotC1, S * 4));
-    HIP_TRY(hipMalloc(&c->d_seedC1, S * 4));
-    HIP_TRY(hipMalloc(&c->d_needL1, S * 4));
-    HIP_TRY(hipMalloc(&c->d_ctl1, 64));
-    for (int p = 0; p < 2; p++) {
-        HIP_TRY(hipEventCreateWithFlags(&c->ev_front[p], hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&c->ev_tail[p], hipEventDisableTiming));
+    c->batch_set[0] = BatchSet{c->d_seeds, c->d_tab[SET_C], c->d_par[SET_C], c->d_q1, c->d_q2, c->d_needC,
+                               c->d_slotC, c->d_seedC, c->d_needL, c->d_ctl};
+    for (int k = 1; k < 3; k++) {
+        BatchSet &b = c->batch_set[k];
+        HIP_TRY(hipMalloc(&b.seeds, S * sizeof(uint64_t)));
+        HIP_TRY(hipMalloc(&b.tabC, S * kSetSize[SET_C] * 256));
+        HIP_TRY(hipMalloc(&b.parC, S * kSetSize[SET_C] * sizeof(SPar)));
+        HIP_TRY(hipMalloc(&b.q1, (size_t)kMaxItems * sizeof(uint4)));
+        HIP_TRY(hipMalloc(&b.q2, (size_t)kMaxItems * sizeof(uint4)));
+        HIP_TRY(hipMalloc(&b.needC, S * 4));
+        HIP_TRY(hipMalloc(&b.slotC, S * 4));
+        HIP_TRY(hipMalloc(&b.seedC, S * 4));
+        HIP_TRY(hipMalloc(&b.needL, S * 4));
+        HIP_TRY(hipMalloc(&b.ctl, 64));
     }
+    for (int s = 0; s < 3; s++)
+        for (int k = 0; k < 3; k++) HIP_TRY(hipEventCreateWithFlags(&c->ev_stage[s][k], hipEventDisableTiming));
     HIP_TRY(hipMalloc(&c->d_cnt, CNT_N * sizeof(unsigned long long)));
     HIP_TRY(hipMalloc(&c->d_hits, (size_t)kHitCap * sizeof(DevHit)));
     HIP_TRY(hipMalloc(&c->d_gt, kLayouts * sizeof(GateTables)));
@@ -862,8 +866,8 @@ __attribute__((visibility("hidden"))) uint64_t whf_ctx_device_bytes(const whf_ct
     uint64_t b = B * 8 + 4 * B * 4 + 2 * S * 4 + 2ull * kMaxItems * sizeof(uint4) + (uint64_t)kQxCap * sizeof(uint4) +
                  (uint64_t)kHitCap * sizeof(DevHit) + kLayouts * (sizeof(GateTables) + kBdPerLayout * sizeof(BuildDesc));
     for (int s = 0; s < SET_COUNT; s++) b += S * kSetSize[s] * (256 + sizeof(SPar));
-    // the second buffer set of the pipelined Seed Check
-    b += S * 8 + S * kSetSize[SET_C] * (256 + sizeof(SPar)) + 2ull * kMaxItems * sizeof(uint4) + 3 * S * 4 + 64;
+    // the second and third buffer sets of the pipelined Seed Check
+    b += 2 * (S * 8 + S * kSetSize[SET_C] * (256 + sizeof(SPar)) + 2ull * kMaxItems * sizeof(uint4) + 4 * S * 4 + 64);
     return b + (c ? c->dd_bytes + c->bio_bytes + c->ter_bytes : 0);
 }
 
@@ -897,30 +901,14 @@ void launch_reset(whf_ctx *c, uint32_t ns) {
                        c->d_needL, ns);
 }
 
-// The buffers of one batch that its gates 2 and 3 read: set 0 (also the Deep Dive's) or set 1.
-struct BatchBufs {
-    uint64_t *seeds;
-    uint8_t *tabC;
-    SPar *parC;
-    uint4 *q1, *q2;
-    uint32_t *slotC, *seedC, *needL, *ctl;
-};
-
-BatchBufs batch_bufs(whf_ctx *c, int set) {
-    if (set == 0) return BatchBufs{c->d_seeds, c->d_tab[SET_C], c->d_par[SET_C], c->d_q1, c->d_q2, c->d_slotC,
-                                   c->d_seedC, c->d_needL, c->d_ctl};
-    return BatchBufs{c->d_seeds1, c->d_tabC1, c->d_parC1, c->d_q1_1, c->d_q2_1, c->d_slotC1, c->d_seedC1, c->d_needL1,
-                     c->d_ctl1};
-}
-
-// Set C for the seeds with erosion survivors: slots (compact) and tables (build).
-void front_c(whf_ctx *c, const BatchBufs &b, int layout, uint32_t ns, hipStream_t st) {
+// Set C for the seeds with erosion survivors (slots by compaction, then tables) and gate 2.
+void stage_c(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags, hipStream_t st) {
     const uint32_t blocks_n = std::max(1u, (ns + 255) / 256);
     const uint32_t persist = (uint32_t)c->n_cu * 8;
     const uint32_t nb = std::min(ns, kMaxSeeds);   // at most kMaxSeeds seeds get tables; gates count the rest
     {
         Prof p(c, F_COMPACT, st);
-        hipLaunchKernelGGL(k_compact, dim3(blocks_n), dim3(256), 0, st, c->d_needC, ns, b.slotC, b.seedC,
+        hipLaunchKernelGGL(k_compact, dim3(blocks_n), dim3(256), 0, st, b.needC, ns, b.slotC, b.seedC,
                            b.ctl + CTL_NC, kMaxSeeds);
     }
     {
@@ -929,18 +917,18 @@ void front_c(whf_ctx *c, const BatchBufs &b, int layout, uint32_t ns, hipStream_
                            b.seeds, bd_of(c, layout, SET_C), 28, b.seedC, b.ctl + CTL_NC, kMaxSeeds, b.tabC, b.parC,
                            c->d_cnt + CNT_BUILT + SET_C);
     }
-}
-
-// Gate 2, set L for its survivors' seeds, gate 3 -> hits.
-void tail(whf_ctx *c, const BatchBufs &b, int layout, uint32_t ns, int vflags, int max_height, hipStream_t st) {
-    const uint32_t blocks_n = std::max(1u, (ns + 255) / 256);
-    const uint32_t persist = (uint32_t)c->n_cu * 8;
-    const uint32_t nb = std::min(ns, kMaxSeeds);
     {
         Prof p(c, F_GATE2, st);
         hipLaunchKernelGGL(k_gate2, dim3(persist), dim3(kGateWG), 0, st, b.seeds, c->d_gt + layout, b.tabC, b.parC,
                            b.slotC, b.q1, b.q2, b.ctl, b.needL, vflags, c->d_cnt);
     }
+}
+
+// Set L for the seeds with gate-2 survivors, and gate 3 -> hits.
+void stage_l(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags, int max_height, hipStream_t st) {
+    const uint32_t blocks_n = std::max(1u, (ns + 255) / 256);
+    const uint32_t persist = (uint32_t)c->n_cu * 8;
+    const uint32_t nb = std::min(ns, kMaxSeeds);
     {
         Prof p(c, F_COMPACT, st);
         hipLaunchKernelGGL(k_compact, dim3(blocks_n), dim3(256), 0, st, b.needL, ns, c->d_slotL, c->d_seedL,
@@ -961,9 +949,9 @@ void tail(whf_ctx *c, const BatchBufs &b, int layout, uint32_t ns, int vflags, i
 }
 
 int run_tail(whf_ctx *c, int layout, uint32_t ns, int vflags, int max_height) {
-    const BatchBufs b = batch_bufs(c, 0);
-    front_c(c, b, layout, ns, c->stream);
-    tail(c, b, layout, ns, vflags, max_height, c->stream);
+    const BatchSet &b = c->batch_set[0];
+    stage_c(c, b, layout, ns, vflags, c->stream);
+    stage_l(c, b, layout, ns, vflags, max_height, c->stream);
     return check_launch();
 }
 
@@ -971,14 +959,14 @@ int run_tail(whf_ctx *c, int layout, uint32_t ns, int vflags, int max_height) {
 
 namespace {
 
-// The front of one batch (seeds already in b.seeds): seeds [s0, s0 + ns) of the request, flat region range
-// [f0, f0 + fn): reset, set E, the erosion gate, set C. Its tail (gates 2 and 3) goes to the tail stream.
-int run_front(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_t f0, uint32_t fn, uint32_t W,
-              const BatchBufs &b) {
+// The first stage of one batch (seeds already in b.seeds): seeds [s0, s0 + ns) of the request, flat region range
+// [f0, f0 + fn): reset, set E, the erosion gate.
+int stage_e(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_t f0, uint32_t fn, uint32_t W,
+            const BatchSet &b) {
     const hipStream_t st = c->stream;
     {
         Prof p(c, F_SEEDS);
-        hipLaunchKernelGGL(k_reset, dim3(std::max(1u, (ns + 255) / 256)), dim3(256), 0, st, b.ctl, c->d_needC,
+        hipLaunchKernelGGL(k_reset, dim3(std::max(1u, (ns + 255) / 256)), dim3(256), 0, st, b.ctl, b.needC,
                            b.needL, ns);
     }
     {
@@ -1004,7 +992,7 @@ int run_front(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint6
         const FastDiv Wd = make_fastdiv(W), Rd = make_fastdiv(chunk);
 #define WHF_EROSION_ARGS                                                                                          \
     grid, dim3(kEroWG), 0, st, b.seeds, ns, c->d_tab[SET_E], c->d_par[SET_E], gt, req->rx0, req->rz0, Wd, Rd, f0, fn,  \
-        chunk, n_chunks, salt, single, b.q1, b.ctl, c->d_needC, c->d_qx, c->d_cnt
+        chunk, n_chunks, salt, single, b.q1, b.ctl, b.needC, c->d_qx, c->d_cnt
         if (wide && f32) hipLaunchKernelGGL((k_erosion<4, true>), WHF_EROSION_ARGS);
         else if (wide) hipLaunchKernelGGL((k_erosion<4, false>), WHF_EROSION_ARGS);
         else if (f32) hipLaunchKernelGGL((k_erosion<8, true>), WHF_EROSION_ARGS);
@@ -1012,9 +1000,8 @@ int run_front(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint6
 #undef WHF_EROSION_ARGS
         if (f32 && !single)
             hipLaunchKernelGGL(k_erosion_exact, dim3((uint32_t)c->n_cu * 2), dim3(256), 0, st, c->d_tab[SET_E],
-                               c->d_par[SET_E], gt, c->d_qx, b.q1, b.ctl, c->d_needC, c->d_cnt);
+                               c->d_par[SET_E], gt, c->d_qx, b.q1, b.ctl, b.needC, c->d_cnt);
     }
-    front_c(c, b, layout, ns, st);
     return check_launch();
 }
 
@@ -1120,11 +1107,11 @@ int whf_open(const whf_config *cfg, whf_ctx **out) {
         delete c;
         return WHF_EDEVICE;
     }
-    if (hipStreamCreateWithFlags(&c->tail_stream, hipStreamNonBlocking) != hipSuccess) {   // default priority too
-        (void)hipStreamDestroy(c->stream);
-        delete c;
-        return WHF_EDEVICE;
-    }
+    for (hipStream_t &s : c->stage_stream)
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {   // default priority too
+            whf_close(c);
+            return WHF_EDEVICE;
+        }
     int rc = ctx_alloc(c);
     if (rc != WHF_OK) {
         whf_close(c);
@@ -1140,20 +1127,25 @@ void whf_close(whf_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_seeds, c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_par[0], c->d_par[1], c->d_par[2],
                     c->d_q1, c->d_q2, c->d_qx, c->d_needC, c->d_needL, c->d_slotC, c->d_slotL, c->d_seedC, c->d_seedL,
-                    c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd, c->d_dd, c->d_bio, c->d_ter,
-                    c->d_seeds1, c->d_tabC1, c->d_parC1, c->d_q1_1, c->d_q2_1, c->d_slotC1, c->d_seedC1, c->d_needL1,
-                    c->d_ctl1};
-    if (c->tail_stream) (void)hipStreamSynchronize(c->tail_stream);
+                    c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd, c->d_dd, c->d_bio, c->d_ter};
+    for (hipStream_t s : c->stage_stream)
+        if (s) (void)hipStreamSynchronize(s);
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    for (int k = 1; k < 3; k++) {
+        const BatchSet &b = c->batch_set[k];
+        void *own[] = {b.seeds, b.tabC, b.parC, b.q1, b.q2, b.needC, b.slotC, b.seedC, b.needL, b.ctl};
+        for (void *p : own)
+            if (p) (void)hipFree(p);
+    }
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    for (int p = 0; p < 2; p++) {
-        if (c->ev_front[p]) (void)hipEventDestroy(c->ev_front[p]);
-        if (c->ev_tail[p]) (void)hipEventDestroy(c->ev_tail[p]);
-    }
-    if (c->tail_stream) (void)hipStreamDestroy(c->tail_stream);
+    for (int s = 0; s < 3; s++)
+        for (int k = 0; k < 3; k++)
+            if (c->ev_stage[s][k]) (void)hipEventDestroy(c->ev_stage[s][k]);
+    for (hipStream_t s : c->stage_stream)
+        if (s) (void)hipStreamDestroy(s);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1197,20 +1189,20 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
         regions_per = kMaxItems;
         seeds_per = 1;
     }
-    // batch k uses buffer set k % 2: its front (seeds, set E, erosion, set C) runs on the library's stream and its
-    // tail (gates 2 and 3) on the tail stream, beside the front of batch k + 1; the front of batch k + 2 waits
-    // for the tail of batch k, the last reader of that set
-    const hipStream_t ts = c->tail_stream;
-    HIP_TRY(hipEventRecord(c->ev_front[1], st));   // the tail stream starts after the counters' reset
-    HIP_TRY(hipStreamWaitEvent(ts, c->ev_front[1], 0));
-    bool tail_of[2] = {false, false};
+    // batch k uses buffer set k % 3 and runs in three stages, each on its own stream: seeds, set E and the erosion
+    // gate; set C and gate 2; set L and gate 3. A stage waits for the batch's previous stage, and the first stage
+    // of batch k + 3 for the last stage of batch k, the last reader of that set; stages of one kind run in order
+    const hipStream_t ss[3] = {st, c->stage_stream[0], c->stage_stream[1]};
+    HIP_TRY(hipEventRecord(c->ev_stage[0][0], st));   // the other streams start after the counters' reset
+    for (int s = 1; s < 3; s++) HIP_TRY(hipStreamWaitEvent(ss[s], c->ev_stage[0][0], 0));
+    bool used[3] = {false, false, false};
     int set = 0;
     for (uint64_t s0 = 0; s0 < req->n_seeds; s0 += seeds_per) {
         const uint32_t ns = (uint32_t)std::min<uint64_t>(seeds_per, req->n_seeds - s0);
         for (uint64_t f0 = 0; f0 < R; f0 += regions_per) {
             const uint32_t fn = (uint32_t)std::min<uint64_t>(regions_per, R - f0);
-            const BatchBufs b = batch_bufs(c, set);
-            if (tail_of[set]) HIP_TRY(hipStreamWaitEvent(st, c->ev_tail[set], 0));
+            const BatchSet &b = c->batch_set[set];
+            if (used[set]) HIP_TRY(hipStreamWaitEvent(st, c->ev_stage[2][set], 0));
             if (req->seed_mode == WHF_SEEDS_LIST) {
                 HIP_TRY(hipMemcpyAsync(b.seeds, req->seeds + s0, ns * sizeof(uint64_t), hipMemcpyHostToDevice, st));
             } else {
@@ -1218,19 +1210,24 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
                 hipLaunchKernelGGL(k_seeds, dim3((ns + 255) / 256), dim3(256), 0, st, req->seed_mode, req->stream,
                                    req->offset + s0, ns, b.seeds);
             }
-            int rc = run_front(c, req, layout, ns, f0, fn, W, b);
+            int rc = stage_e(c, req, layout, ns, f0, fn, W, b);
             if (rc) return rc;
-            HIP_TRY(hipEventRecord(c->ev_front[set], st));
-            HIP_TRY(hipStreamWaitEvent(ts, c->ev_front[set], 0));
-            tail(c, b, layout, ns, req->vflags, req->max_height, ts);
+            HIP_TRY(hipEventRecord(c->ev_stage[0][set], st));
+            HIP_TRY(hipStreamWaitEvent(ss[1], c->ev_stage[0][set], 0));
+            stage_c(c, b, layout, ns, req->vflags, ss[1]);
+            HIP_TRY(hipEventRecord(c->ev_stage[1][set], ss[1]));
+            HIP_TRY(hipStreamWaitEvent(ss[2], c->ev_stage[1][set], 0));
+            stage_l(c, b, layout, ns, req->vflags, req->max_height, ss[2]);
             if ((rc = check_launch()) != WHF_OK) return rc;
-            HIP_TRY(hipEventRecord(c->ev_tail[set], ts));
-            tail_of[set] = true;
-            set ^= 1;
+            HIP_TRY(hipEventRecord(c->ev_stage[2][set], ss[2]));
+            used[set] = true;
+            set = (set + 1) % 3;
         }
     }
-    HIP_TRY(hipEventRecord(c->ev_tail[set], ts));   // everything on the tail stream so far
-    HIP_TRY(hipStreamWaitEvent(st, c->ev_tail[set], 0));
+    for (int s = 1; s < 3; s++) {   // everything on the other streams so far
+        HIP_TRY(hipEventRecord(c->ev_stage[s][0], ss[s]));
+        HIP_TRY(hipStreamWaitEvent(st, c->ev_stage[s][0], 0));
+    }
     HIP_TRY(hipEventRecord(c->ev1, st));
     unsigned long long cnt[CNT_N];
     uint32_t ctl[CTL_N];
