@@ -67,16 +67,16 @@ def algorithmic_ops(st):
             "build_ladder": built[2] * OPS_PER_SAMPLER, "erosion_gate": evals * OPS_PER_Y0_EVAL}
 
 
-def traffic_from_profiles(family):
-    """HBM bytes per launch of a kernel family from the committed rocprofv3 PMC summary (FETCH_SIZE and
-    WRITE_SIZE scaled by the copy-kernel calibration), or None when it was not measured."""
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE
+    scaled by the copy-kernel calibration), or None when it was not measured."""
     path = os.path.join(REPO, "profiles", "traffic_c2.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    return t.get(KERNEL_OF[family])
+    return t.get(kernel)
 
 
 def cpu_baseline(n_seeds_max, seconds):
@@ -374,10 +374,15 @@ def main():
                                launch_ms=round(ms / n_batches, 4), ops_per_launch=int(ops // n_batches))
     dominant = max(fam, key=fam.get)
     r = rooflines.get(dominant, {})
+    kernel = KERNEL_OF[dominant]
+    if dominant == "erosion_gate" and best == whf.PREC_FP32_GATE:
+        kernel = "k_erosion<4, true>"   # the FP32-gated instance: FP32 corner blend, FP64 coordinates and sums
     roofline = dict(bound="valu", achieved=r.get("achieved"), peak=round(PEAK_VALU_OPS / 1e12, 2), unit="Tops/s",
-                    frac=r.get("frac"), traffic=traffic_from_profiles(dominant), kernel=KERNEL_OF[dominant],
+                    frac=r.get("frac"), traffic=traffic_from_profiles(kernel), kernel=kernel,
                     launch_ms=round(fam[dominant] / n_batches, 4), ops_per_launch=r.get("ops_per_launch"),
-                    work_unit=WORK_UNIT.get(dominant))
+                    work_unit=WORK_UNIT.get(dominant) if kernel != "k_erosion<4, true>" else
+                    "y=0 Perlin evaluation = %d ops (FP32 corner blend, FP64 coordinates and sums); evaluations per "
+                    "region from the tier histogram" % OPS_PER_Y0_EVAL)
     # the whole step: the algorithmic lane-ops of every counted family over the step's wall time. A batch's gates
     # 2 and 3 run beside the next batch's builds and erosion gate, so a kernel's launch time includes the device
     # it shares; this is the figure the overlap improves.

@@ -122,8 +122,8 @@ struct whf_ctx {
     size_t ter_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Seed Check pipelining: a batch runs in three stages on three streams (the library's stream: seeds, set E and
-    // the erosion gate; stage_stream[0]: set C and gate 2; stage_stream[1]: set L and gate 3), so three batches are
-    // on the device at once. What a batch's later stages read exists three times (batch_set[0] aliases the buffers
+    // the erosion gate; stage_stream[0]: set C and gate 2; stage_stream[1]: set L and gate 3, both created by the
+    // first scan of more than one batch), so three batches are on the device at once. What a batch's later stages read exists three times (batch_set[0] aliases the buffers
     // above, 1 and 2 are their own) and batch k uses set k % 3; ev_stage[s][set] marks the end of stage s of the
     // batch on that set
     hipStream_t stage_stream[2] = {nullptr, nullptr};

@@ -1107,11 +1107,6 @@ int whf_open(const whf_config *cfg, whf_ctx **out) {
         delete c;
         return WHF_EDEVICE;
     }
-    for (hipStream_t &s : c->stage_stream)
-        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {   // default priority too
-            whf_close(c);
-            return WHF_EDEVICE;
-        }
     int rc = ctx_alloc(c);
     if (rc != WHF_OK) {
         whf_close(c);
@@ -1192,9 +1187,19 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
     // batch k uses buffer set k % 3 and runs in three stages, each on its own stream: seeds, set E and the erosion
     // gate; set C and gate 2; set L and gate 3. A stage waits for the batch's previous stage, and the first stage
     // of batch k + 3 for the last stage of batch k, the last reader of that set; stages of one kind run in order
-    const hipStream_t ss[3] = {st, c->stage_stream[0], c->stage_stream[1]};
-    HIP_TRY(hipEventRecord(c->ev_stage[0][0], st));   // the other streams start after the counters' reset
-    for (int s = 1; s < 3; s++) HIP_TRY(hipStreamWaitEvent(ss[s], c->ev_stage[0][0], 0));
+    // A scan of one batch (a server's chunk, a small request) has nothing to overlap: all its stages go to the
+    // library's stream, and the stage streams, which share the device's few hardware queues with the other
+    // contexts' streams, stay idle.
+    const uint64_t n_batches = ((req->n_seeds + seeds_per - 1) / seeds_per) * ((R + regions_per - 1) / regions_per);
+    const bool piped = n_batches > 1;
+    if (piped)   // created on first use, of default priority like the library's stream
+        for (hipStream_t &s : c->stage_stream)
+            if (!s) HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const hipStream_t ss[3] = {st, piped ? c->stage_stream[0] : st, piped ? c->stage_stream[1] : st};
+    if (piped) {
+        HIP_TRY(hipEventRecord(c->ev_stage[0][0], st));   // the other streams start after the counters' reset
+        for (int s = 1; s < 3; s++) HIP_TRY(hipStreamWaitEvent(ss[s], c->ev_stage[0][0], 0));
+    }
     bool used[3] = {false, false, false};
     int set = 0;
     for (uint64_t s0 = 0; s0 < req->n_seeds; s0 += seeds_per) {
@@ -1224,7 +1229,7 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
             set = (set + 1) % 3;
         }
     }
-    for (int s = 1; s < 3; s++) {   // everything on the other streams so far
+    for (int s = 1; s < 3 && piped; s++) {   // everything on the other streams so far
         HIP_TRY(hipEventRecord(c->ev_stage[s][0], ss[s]));
         HIP_TRY(hipStreamWaitEvent(st, c->ev_stage[s][0], 0));
     }
