@@ -1207,7 +1207,7 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
         for (uint64_t f0 = 0; f0 < R; f0 += regions_per) {
             const uint32_t fn = (uint32_t)std::min<uint64_t>(regions_per, R - f0);
             const BatchSet &b = c->batch_set[set];
-            if (used[set]) HIP_TRY(hipStreamWaitEvent(st, c->ev_stage[2][set], 0));
+            if (piped && used[set]) HIP_TRY(hipStreamWaitEvent(st, c->ev_stage[2][set], 0));
             if (req->seed_mode == WHF_SEEDS_LIST) {
                 HIP_TRY(hipMemcpyAsync(b.seeds, req->seeds + s0, ns * sizeof(uint64_t), hipMemcpyHostToDevice, st));
             } else {
@@ -1217,14 +1217,18 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
             }
             int rc = stage_e(c, req, layout, ns, f0, fn, W, b);
             if (rc) return rc;
-            HIP_TRY(hipEventRecord(c->ev_stage[0][set], st));
-            HIP_TRY(hipStreamWaitEvent(ss[1], c->ev_stage[0][set], 0));
+            if (piped) {
+                HIP_TRY(hipEventRecord(c->ev_stage[0][set], st));
+                HIP_TRY(hipStreamWaitEvent(ss[1], c->ev_stage[0][set], 0));
+            }
             stage_c(c, b, layout, ns, req->vflags, ss[1]);
-            HIP_TRY(hipEventRecord(c->ev_stage[1][set], ss[1]));
-            HIP_TRY(hipStreamWaitEvent(ss[2], c->ev_stage[1][set], 0));
+            if (piped) {
+                HIP_TRY(hipEventRecord(c->ev_stage[1][set], ss[1]));
+                HIP_TRY(hipStreamWaitEvent(ss[2], c->ev_stage[1][set], 0));
+            }
             stage_l(c, b, layout, ns, req->vflags, req->max_height, ss[2]);
             if ((rc = check_launch()) != WHF_OK) return rc;
-            HIP_TRY(hipEventRecord(c->ev_stage[2][set], ss[2]));
+            if (piped) HIP_TRY(hipEventRecord(c->ev_stage[2][set], ss[2]));
             used[set] = true;
             set = (set + 1) % 3;
         }
