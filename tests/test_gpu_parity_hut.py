@@ -76,6 +76,22 @@ def test_hut_eval_random_against_oracle(ctx):
     assert not bad, bad[:3]
 
 
+@pytest.mark.parametrize("version", ["1.18.2", "26.2"])
+def test_hut_eval_rule_sets_against_oracle(ctx, version):
+    """Huts under the 1.18 and 26.2 rule sets and with LARGE_BIOMES (the terrain and climate noises of the large
+    preset), against the oracle."""
+    rng = random.Random(37 if version == "26.2" else 41)
+    tree = whf.biome_tree(version)
+    for large in (0, O.V_LARGE):
+        vflags = whf.version_flags(version) | large
+        huts = [(rng.getrandbits(64), rng.randint(-20000, 20000), rng.randint(-20000, 20000)) for _ in range(12)]
+        print("whf_hut_eval: %s, vflags %d" % (version, vflags), flush=True)
+        got = ctx.hut_eval(huts, tree, -50, vflags)
+        for (s, cx, cz), g in zip(huts, got):
+            want = O.Terrain(s, vflags).hut_eval(tree, cx * 16, cz * 16, -50)
+            assert all(g[k] == want[k] for k in FIELDS), ((s, cx, cz), vflags, g, want)
+
+
 def test_hut_eval_two_rounds(ctx):
     """More huts than one device round holds (512), with repeated seeds and repeated huts: a sample, including the
     rounds' first and last huts, equals the oracle."""
