@@ -150,8 +150,9 @@ def deep_dive(whf, sweep_n, expand_n, cpu_seconds):
         _, _, ps = ctx.deepdive(s0, min(sweep_n, 1 << 26), DD_RECT, DD_FLOOR, spot_capacity=1 << 21)
         _, _, px = ctx.deepdive(s0, expand_n, DD_RECT, DD_FLOOR, expand=True, max_height=-50, spot_capacity=1 << 21)
     sweep_rate = sweep_n / t_sweep
-    # expansion rate from the profiled stage time (the sweep part of that call is excluded)
-    x_sec = px["stage_ms"][2] / 1e3
+    # expansion time: the expanding call's wall time less its own sweep of expand_n structure seeds at the measured
+    # sweep rate (the expansion's kernels run on three streams, so summing their event times would overcount)
+    x_sec = t_exp_total - expand_n / sweep_rate
     x_rate = xs["expanded_seeds"] / x_sec if x_sec > 0 else None
     spots_per_seed = st["spots"] / sweep_n
     total_s = DD_TOTAL / sweep_rate + (DD_TOTAL * spots_per_seed * 65536 / x_rate if x_rate else 0)

@@ -46,10 +46,14 @@ extern "C" {
 
 typedef struct whf_ctx whf_ctx;
 
+#define WHF_CFG_PROFILE    1   /* whf_config.flags: time kernel families with HIP events (whf_stats.kernel_ms) */
+#define WHF_CFG_ONE_STREAM 2   /* whf_config.flags: run every stage of a scan on one stream (no batch pipeline;
+                                  for contexts that share the device's hardware queues with many others) */
+
 typedef struct {
     int device;                 /* HIP device ordinal */
-    int flags;                  /* reserved, 0 */
-    uint64_t hbm_budget;        /* bytes of device memory the context may hold; 0 = 8 GiB */
+    int flags;                  /* WHF_CFG_* */
+    uint64_t hbm_budget;        /* unused by whf_open (a context holds about 9 GB); whf_server_config caps slots */
 } whf_config;
 
 typedef struct {

@@ -331,24 +331,30 @@ int dd_sweep(whf_ctx *c, const whf_dd_request *req, const DdBufs &b, uint64_t s_
 int dd_expand(whf_ctx *c, const whf_dd_request *req, const whf_dd_spot *d_spots, uint32_t n_spots) {
     const hipStream_t st = c->stream;
     const int layout = req->vflags;
+    // each batch of 16 spots: its world seeds and the lazy erosion gate on the library's stream, then the Seed
+    // Check's gates 2 and 3 on the pipeline's stage streams beside the next batch
+    Pipeline pl(c);
+    if (int rc = pl.begin((n_spots + kDdSpotsPerExpand - 1) / kDdSpotsPerExpand)) return rc;
     for (uint32_t s0 = 0; s0 < n_spots; s0 += kDdSpotsPerExpand) {
         const uint32_t k = std::min(kDdSpotsPerExpand, n_spots - s0);
         const uint32_t ns = k << 16;
-        launch_reset(c, ns);
+        const BatchSet *bp = nullptr;
+        if (int rc = pl.acquire(bp)) return rc;
+        const BatchSet &b = *bp;
+        reset_set(c, b, ns);
         {
             Prof p(c, F_DD_EXPAND);
-            hipLaunchKernelGGL(k_dd_xseeds, dim3((ns + 255) / 256), dim3(256), 0, st, d_spots + s0, ns, c->d_seeds);
-            HIP_TRY(hipMemsetAsync(c->d_ctl + CTL_DD_WORK, 0, sizeof(uint32_t), st));
+            hipLaunchKernelGGL(k_dd_xseeds, dim3((ns + 255) / 256), dim3(256), 0, st, d_spots + s0, ns, b.seeds);
+            HIP_TRY(hipMemsetAsync(b.ctl + CTL_DD_WORK, 0, sizeof(uint32_t), st));
             // persistent lanes: about one workgroup per LDS slot of the device (16 KB each)
             const uint32_t grid = std::min((ns + 63) / 64, (uint32_t)c->n_cu * 10);
-            hipLaunchKernelGGL(k_dd_erosion, dim3(grid), dim3(64), 0, st, c->d_seeds, ns, d_spots + s0,
+            hipLaunchKernelGGL(k_dd_erosion, dim3(grid), dim3(64), 0, st, b.seeds, ns, d_spots + s0,
                                bd_of(c, layout, SET_E), c->d_gt + layout, (req->vflags & WHF_V_SINGLE) ? 1 : 0,
-                               c->d_q1, c->d_ctl, c->d_needC, c->d_cnt);
+                               b.q1, b.ctl, b.needC, c->d_cnt);
         }
-        int rc = run_tail(c, layout, ns, req->vflags, req->max_height);
-        if (rc) return rc;
+        if (int rc = pl.tail(b, layout, ns, req->vflags, req->max_height)) return rc;
     }
-    return WHF_OK;
+    return pl.end();
 }
 
 }  // namespace

@@ -127,6 +127,7 @@ struct whf_ctx {
     // above, 1 and 2 are their own) and batch k uses set k % 3; ev_stage[s][set] marks the end of stage s of the
     // batch on that set
     hipStream_t stage_stream[2] = {nullptr, nullptr};
+    bool one_stream = false;              // WHF_CFG_ONE_STREAM: stages stay on the library's stream
     whf::BatchSet batch_set[3] = {};
     hipEvent_t ev_stage[3][3] = {};
     std::vector<hipEvent_t> prof_ev;  // pairs per kernel launch when profiling
@@ -189,5 +190,24 @@ int terrain_heights(whf_ctx *c, const whf_column *cols, uint64_t n, int vflags, 
 // Gates 2 and 3 of a batch whose erosion survivors are already in Q1 (with needC set): compact, build set C,
 // gate 2, compact, build set L, gate 3 -> hits.
 int run_tail(whf_ctx *c, int layout, uint32_t ns, int vflags, int max_height);
+// Resets a batch set's control words and the seed flags of its first ns seeds, on the library's stream.
+void reset_set(whf_ctx *c, const BatchSet &b, uint32_t ns);
+
+// The three-stage batch pipeline (whf_scan.hip). Batch k uses buffer set k % 3: the caller takes the set with
+// acquire(), issues the batch's first stage into it on the library's stream, and hands it to tail(), which issues
+// set C + gate 2 and set L + gate 3 on the two stage streams. end() makes the library's stream wait for all of it.
+// With one batch, or a context opened with WHF_CFG_ONE_STREAM, every stage goes to the library's stream.
+struct Pipeline {
+    whf_ctx *c;
+    bool piped = false;
+    hipStream_t ss[3] = {nullptr, nullptr, nullptr};
+    bool used[3] = {false, false, false};
+    int set = 0;
+    explicit Pipeline(whf_ctx *c_) : c(c_) {}
+    int begin(uint64_t n_batches);
+    int acquire(const BatchSet *&b);
+    int tail(const BatchSet &b, int layout, uint32_t ns, int vflags, int max_height);
+    int end();
+};
 
 }  // namespace whf

@@ -816,17 +816,17 @@ int ctx_alloc(whf_ctx *c) {
     static_assert(CTL_N * sizeof(uint32_t) <= 64, "control words");
     c->batch_set[0] = BatchSet{c->d_seeds, c->d_tab[SET_C], c->d_par[SET_C], c->d_q1, c->d_q2, c->d_needC,
                                c->d_slotC, c->d_seedC, c->d_needL, c->d_ctl};
-    for (int k = 1; k < 3; k++) {
+    for (int k = 1; k < 3; k++) {   // per-seed arrays for the largest batch (a Deep Dive expansion's 2^20 seeds)
         BatchSet &b = c->batch_set[k];
-        HIP_TRY(hipMalloc(&b.seeds, S * sizeof(uint64_t)));
+        HIP_TRY(hipMalloc(&b.seeds, B * sizeof(uint64_t)));
         HIP_TRY(hipMalloc(&b.tabC, S * kSetSize[SET_C] * 256));
         HIP_TRY(hipMalloc(&b.parC, S * kSetSize[SET_C] * sizeof(SPar)));
         HIP_TRY(hipMalloc(&b.q1, (size_t)kMaxItems * sizeof(uint4)));
         HIP_TRY(hipMalloc(&b.q2, (size_t)kMaxItems * sizeof(uint4)));
-        HIP_TRY(hipMalloc(&b.needC, S * 4));
-        HIP_TRY(hipMalloc(&b.slotC, S * 4));
+        HIP_TRY(hipMalloc(&b.needC, B * 4));
+        HIP_TRY(hipMalloc(&b.slotC, B * 4));
         HIP_TRY(hipMalloc(&b.seedC, S * 4));
-        HIP_TRY(hipMalloc(&b.needL, S * 4));
+        HIP_TRY(hipMalloc(&b.needL, B * 4));
         HIP_TRY(hipMalloc(&b.ctl, 64));
     }
     for (int s = 0; s < 3; s++)
@@ -867,7 +867,8 @@ __attribute__((visibility("hidden"))) uint64_t whf_ctx_device_bytes(const whf_ct
                  (uint64_t)kHitCap * sizeof(DevHit) + kLayouts * (sizeof(GateTables) + kBdPerLayout * sizeof(BuildDesc));
     for (int s = 0; s < SET_COUNT; s++) b += S * kSetSize[s] * (256 + sizeof(SPar));
     // the second and third buffer sets of the pipelined Seed Check
-    b += 2 * (S * 8 + S * kSetSize[SET_C] * (256 + sizeof(SPar)) + 2ull * kMaxItems * sizeof(uint4) + 4 * S * 4 + 64);
+    b += 2 * (B * 8 + S * kSetSize[SET_C] * (256 + sizeof(SPar)) + 2ull * kMaxItems * sizeof(uint4) + 3 * B * 4 +
+              S * 4 + 64);
     return b + (c ? c->dd_bytes + c->bio_bytes + c->ter_bytes : 0);
 }
 
@@ -955,6 +956,64 @@ int run_tail(whf_ctx *c, int layout, uint32_t ns, int vflags, int max_height) {
     return check_launch();
 }
 
+void reset_set(whf_ctx *c, const BatchSet &b, uint32_t ns) {
+    Prof p(c, F_SEEDS);
+    hipLaunchKernelGGL(k_reset, dim3(std::max(1u, (ns + 255) / 256)), dim3(256), 0, c->stream, b.ctl, b.needC, b.needL,
+                       ns);
+}
+
+int Pipeline::begin(uint64_t n_batches) {
+    const hipStream_t st = c->stream;
+    // one batch has nothing to overlap; then the stage streams, which share the device's few hardware queues with
+    // other contexts' streams, stay idle
+    piped = n_batches > 1 && !c->one_stream;
+    if (piped)   // created on first use, of default priority like the library's stream
+        for (hipStream_t &s : c->stage_stream)
+            if (!s) HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    ss[0] = st;
+    ss[1] = piped ? c->stage_stream[0] : st;
+    ss[2] = piped ? c->stage_stream[1] : st;
+    if (piped) {
+        HIP_TRY(hipEventRecord(c->ev_stage[0][0], st));   // the stage streams start after what st has so far
+        for (int s = 1; s < 3; s++) HIP_TRY(hipStreamWaitEvent(ss[s], c->ev_stage[0][0], 0));
+    }
+    return WHF_OK;
+}
+
+int Pipeline::acquire(const BatchSet *&b) {
+    b = &c->batch_set[set];
+    // stage L of the batch three back is the last reader of this set
+    if (piped && used[set]) HIP_TRY(hipStreamWaitEvent(ss[0], c->ev_stage[2][set], 0));
+    return WHF_OK;
+}
+
+int Pipeline::tail(const BatchSet &b, int layout, uint32_t ns, int vflags, int max_height) {
+    if (piped) {
+        HIP_TRY(hipEventRecord(c->ev_stage[0][set], ss[0]));
+        HIP_TRY(hipStreamWaitEvent(ss[1], c->ev_stage[0][set], 0));
+    }
+    stage_c(c, b, layout, ns, vflags, ss[1]);
+    if (piped) {
+        HIP_TRY(hipEventRecord(c->ev_stage[1][set], ss[1]));
+        HIP_TRY(hipStreamWaitEvent(ss[2], c->ev_stage[1][set], 0));
+    }
+    stage_l(c, b, layout, ns, vflags, max_height, ss[2]);
+    const int rc = check_launch();
+    if (rc != WHF_OK) return rc;
+    if (piped) HIP_TRY(hipEventRecord(c->ev_stage[2][set], ss[2]));
+    used[set] = true;
+    set = (set + 1) % 3;
+    return WHF_OK;
+}
+
+int Pipeline::end() {
+    for (int s = 1; s < 3 && piped; s++) {   // everything on the stage streams so far
+        HIP_TRY(hipEventRecord(c->ev_stage[s][0], ss[s]));
+        HIP_TRY(hipStreamWaitEvent(ss[0], c->ev_stage[s][0], 0));
+    }
+    return WHF_OK;
+}
+
 }  // namespace whf
 
 namespace {
@@ -964,11 +1023,7 @@ namespace {
 int stage_e(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_t f0, uint32_t fn, uint32_t W,
             const BatchSet &b) {
     const hipStream_t st = c->stream;
-    {
-        Prof p(c, F_SEEDS);
-        hipLaunchKernelGGL(k_reset, dim3(std::max(1u, (ns + 255) / 256)), dim3(256), 0, st, b.ctl, b.needC,
-                           b.needL, ns);
-    }
+    reset_set(c, b, ns);
     {
         Prof p(c, F_BUILD_E);
         const uint32_t jobs = ns * 8;
@@ -1097,7 +1152,8 @@ int whf_open(const whf_config *cfg, whf_ctx **out) {
     whf_ctx *c = new (std::nothrow) whf_ctx();
     if (!c) return WHF_ENOMEM;
     c->device = dev;
-    c->profile = cfg && (cfg->flags & 1);
+    c->profile = cfg && (cfg->flags & WHF_CFG_PROFILE);
+    c->one_stream = cfg && (cfg->flags & WHF_CFG_ONE_STREAM);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
         std::snprintf(c->name, sizeof c->name, "%s (%s)", prop.name, prop.gcnArchName);
@@ -1184,30 +1240,17 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
         regions_per = kMaxItems;
         seeds_per = 1;
     }
-    // batch k uses buffer set k % 3 and runs in three stages, each on its own stream: seeds, set E and the erosion
-    // gate; set C and gate 2; set L and gate 3. A stage waits for the batch's previous stage, and the first stage
-    // of batch k + 3 for the last stage of batch k, the last reader of that set; stages of one kind run in order
-    // A scan of one batch (a server's chunk, a small request) has nothing to overlap: all its stages go to the
-    // library's stream, and the stage streams, which share the device's few hardware queues with the other
-    // contexts' streams, stay idle.
+    // batch k runs in three stages (Pipeline): seeds, set E and the erosion gate; set C and gate 2; set L and gate 3
     const uint64_t n_batches = ((req->n_seeds + seeds_per - 1) / seeds_per) * ((R + regions_per - 1) / regions_per);
-    const bool piped = n_batches > 1;
-    if (piped)   // created on first use, of default priority like the library's stream
-        for (hipStream_t &s : c->stage_stream)
-            if (!s) HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    const hipStream_t ss[3] = {st, piped ? c->stage_stream[0] : st, piped ? c->stage_stream[1] : st};
-    if (piped) {
-        HIP_TRY(hipEventRecord(c->ev_stage[0][0], st));   // the other streams start after the counters' reset
-        for (int s = 1; s < 3; s++) HIP_TRY(hipStreamWaitEvent(ss[s], c->ev_stage[0][0], 0));
-    }
-    bool used[3] = {false, false, false};
-    int set = 0;
+    Pipeline pl(c);
+    if (int rc = pl.begin(n_batches)) return rc;
     for (uint64_t s0 = 0; s0 < req->n_seeds; s0 += seeds_per) {
         const uint32_t ns = (uint32_t)std::min<uint64_t>(seeds_per, req->n_seeds - s0);
         for (uint64_t f0 = 0; f0 < R; f0 += regions_per) {
             const uint32_t fn = (uint32_t)std::min<uint64_t>(regions_per, R - f0);
-            const BatchSet &b = c->batch_set[set];
-            if (piped && used[set]) HIP_TRY(hipStreamWaitEvent(st, c->ev_stage[2][set], 0));
+            const BatchSet *bp = nullptr;
+            if (int rc = pl.acquire(bp)) return rc;
+            const BatchSet &b = *bp;
             if (req->seed_mode == WHF_SEEDS_LIST) {
                 HIP_TRY(hipMemcpyAsync(b.seeds, req->seeds + s0, ns * sizeof(uint64_t), hipMemcpyHostToDevice, st));
             } else {
@@ -1217,26 +1260,10 @@ int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stat
             }
             int rc = stage_e(c, req, layout, ns, f0, fn, W, b);
             if (rc) return rc;
-            if (piped) {
-                HIP_TRY(hipEventRecord(c->ev_stage[0][set], st));
-                HIP_TRY(hipStreamWaitEvent(ss[1], c->ev_stage[0][set], 0));
-            }
-            stage_c(c, b, layout, ns, req->vflags, ss[1]);
-            if (piped) {
-                HIP_TRY(hipEventRecord(c->ev_stage[1][set], ss[1]));
-                HIP_TRY(hipStreamWaitEvent(ss[2], c->ev_stage[1][set], 0));
-            }
-            stage_l(c, b, layout, ns, req->vflags, req->max_height, ss[2]);
-            if ((rc = check_launch()) != WHF_OK) return rc;
-            if (piped) HIP_TRY(hipEventRecord(c->ev_stage[2][set], ss[2]));
-            used[set] = true;
-            set = (set + 1) % 3;
+            if ((rc = pl.tail(b, layout, ns, req->vflags, req->max_height)) != WHF_OK) return rc;
         }
     }
-    for (int s = 1; s < 3 && piped; s++) {   // everything on the other streams so far
-        HIP_TRY(hipEventRecord(c->ev_stage[s][0], ss[s]));
-        HIP_TRY(hipStreamWaitEvent(st, c->ev_stage[s][0], 0));
-    }
+    if (int rc = pl.end()) return rc;
     HIP_TRY(hipEventRecord(c->ev1, st));
     unsigned long long cnt[CNT_N];
     uint32_t ctl[CTL_N];

@@ -317,7 +317,8 @@ int whf_server_open(const whf_server_config *cfg, whf_server **out) {
     if (!s) return WHF_ENOMEM;
     if (cfg && cfg->chunk_checks) s->chunk_checks = cfg->chunk_checks;
     const uint64_t budget = cfg ? cfg->hbm_budget : 0;
-    whf_config cc{cfg ? cfg->device : 0, 0, 0};
+    // one stream per slot: the slots' contexts already overlap on the device, which has few hardware queues to share
+    whf_config cc{cfg ? cfg->device : 0, WHF_CFG_ONE_STREAM, 0};
     for (int i = 0; i < slots; i++) {
         whf_ctx *c = nullptr;
         int rc = whf_open(&cc, &c);

@@ -19,6 +19,7 @@ V_118, V_262, V_LARGE, V_SINGLE = 1, 2, 4, 8
 PREC_FP64, PREC_FP32_GATE = 0, 1
 KIND_N = 11
 CFG_PROFILE = 1
+CFG_ONE_STREAM = 2   # every stage of a scan on one stream (no batch pipeline)
 
 # Exported symbols of include/whf.h, checked by the tests.
 SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags", "whf_strerror",
@@ -212,10 +213,10 @@ def version_flags(name):
 class Context:
     """One device context (whf_open / whf_close)."""
 
-    def __init__(self, device=0, profile=False):
+    def __init__(self, device=0, profile=False, one_stream=False):
         self.L = load()
         self.h = ctypes.c_void_p()
-        cfg = Config(device, CFG_PROFILE if profile else 0, 0)
+        cfg = Config(device, (CFG_PROFILE if profile else 0) | (CFG_ONE_STREAM if one_stream else 0), 0)
         rc = self.L.whf_open(ctypes.byref(cfg), ctypes.byref(self.h))
         if rc:
             raise WhfError(rc, "whf_open")
