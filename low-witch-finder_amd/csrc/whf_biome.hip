@@ -10,8 +10,9 @@
 //                lane 97 for the query's own height; the swamp verdicts are gathered by wave ballot
 //
 // The trees (2.25 MB for all eight versions, about 290 KB each) do not fit the 160 KB LDS; a walk touches a few
-// hundred 32-byte nodes near the root, which stay in L2 across the batch. All of this is rare work next to the
-// phase-1 gates: it runs for survivors, not for every region.
+// hundred 32-byte nodes near the root, which stay in L2 across the batch. The host renumbers each tree breadth
+// first at init so that a node's children are consecutive and load together (relay_trees). All of this is rare
+// work next to the phase-1 gates: it runs for survivors, not for every region.
 #include <cstring>
 #include <unordered_map>
 #include <vector>
