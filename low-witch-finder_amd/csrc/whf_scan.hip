@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
                                                     uint4 *__restrict__ q1, uint32_t *__restrict__ ctl,
                                                     uint32_t *__restrict__ needC, uint4 *__restrict__ qx,
                                                     unsigned long long *__restrict__ cnt) {
-    __shared__ uint16_t s_tab[ST * 8 * 256];     // byte-pair tables: P(i) | P(i + 1) << 8
+    __shared__ __align__(16) uint16_t s_tab[ST * 8 * 256];   // byte-pair tables: P(i) | P(i + 1) << 8
     __shared__ SPar s_par[ST * 8];
     __shared__ int2 s_item[kEroPass];
     __shared__ uint16_t s_li[2][kEroPass];       // alive lists: item index and running sum
@@ -165,11 +165,24 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
     // regions per seed in this workgroup: the full chunk (Rd) unless this is the last, partial chunk
     const FastDiv Rn = rn == Rd.d ? Rd : FastDiv{rn, 0u, 0u};
 
-    {   // stage the tile's erosion tables as byte pairs, and the origins
+    {   // stage the tile's erosion tables as byte pairs, and the origins: a lane takes 16 bytes of a table with one
+        // 16-byte load, and the byte after them (the next 16 bytes' first, or the table's first after its last), and
+        // writes their 16 pairs
         const uint8_t *src = tabE + (size_t)seed0 * 8 * 256;
-        for (uint32_t w = tid; w < ns * 8 * 256; w += kEroWG) {
-            const uint32_t t = w >> 8, i = w & 255;
-            s_tab[w] = (uint16_t)(src[w] | (uint32_t)src[(t << 8) | ((i + 1) & 255)] << 8);
+        for (uint32_t q = tid; q < ns * 8 * 16; q += kEroWG) {
+            const uint32_t t = q >> 4, c = q & 15;
+            const uint4 v = reinterpret_cast<const uint4 *>(src)[q];
+            const uint32_t w[5] = {v.x, v.y, v.z, v.w, (uint32_t)src[(t << 8) | (((c + 1) & 15) << 4)]};
+            uint32_t o[8];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {   // bytes 4k .. 4k + 3 and the next one
+                const uint32_t cur = w[k], nxt = w[k + 1] & 255u;
+                o[2 * k] = (cur & 0xFFFFu) | (((cur >> 8) & 0xFFFFu) << 16);
+                o[2 * k + 1] = (cur >> 16) | (((cur >> 24) | (nxt << 8)) << 16);
+            }
+            uint4 *dst = reinterpret_cast<uint4 *>(s_tab + (size_t)q * 16);
+            dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+            dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
         }
         for (uint32_t w = tid; w < ns * 8; w += kEroWG) s_par[w] = parE[(size_t)seed0 * 8 + w];
         if (tid < 8) s_cnt[tid] = 0;
