@@ -199,6 +199,18 @@ typedef struct {
  * SolveRun (hunt.py:947-957). Not re-entrant per context. */
 int whf_deepdive(whf_ctx *ctx, const whf_dd_request *req, whf_dd_sink *spots, whf_sink *finds, whf_dd_stats *stats);
 
+/* The terrain stage of the reference's --spots mode (hut_lookup_v4_source.cu:1262-1292, the run hunt.py starts
+ * per chunk of solver hits, hunt.py:954-957): the caller fills s48, cx and cz of n <= 2^20 spots (the "S cx cz"
+ * lines of its spots file); the library fills in place rx = cx >> 5, rz = cz >> 5, the chunk's carver seed X and,
+ * where X is a mineshaft start, the room floor (INT32_MIN where the walk does not complete, and for a spot that is
+ * not a start, which the reference reports as "not a mineshaft start" and skips) and piece count (0 exactly for a
+ * spot that is not a start). Every start,
+ * whatever its room floor, is expanded to its 2^16 world seeds and checked by the phase-1 chain at its chunk
+ * (req->max_height, req->vflags; the other request fields are ignored); finds in `finds` as in whf_deepdive.
+ * stats: structure_seeds = tests = n, starts, walks, spots = starts expanded, and the expansion's counters. */
+int whf_dd_terrain(whf_ctx *ctx, const whf_dd_request *req, whf_dd_spot *spots, uint64_t n, whf_sink *finds,
+                   whf_dd_stats *stats);
+
 /* Layout walks of given carver seeds, each as if it were a start: status (0 complete, 2 capped), room floor
  * and piece count per seed (mstree's emit, mstree.cpp:17-34, without the piece list). For parity checks. */
 int whf_room_floors(whf_ctx *ctx, const uint64_t *carvers, uint64_t n, int32_t *status, int32_t *room_floor,

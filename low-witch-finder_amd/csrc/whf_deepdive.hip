@@ -123,6 +123,54 @@ __global__ __launch_bounds__(64) void k_dd_walks(const uint64_t *__restrict__ ca
     }
 }
 
+// Spots named by the caller (structure seed and chunk, the reference's --spots lines): per spot, a group of G
+// lanes computes the chunk's carver seed and the start test and, for a start, walks the whole layout; fills the
+// spot's region, carver seed, room floor (INT32_MIN unless a start whose walk completed) and piece count in place.
+// start[q] = 1 for a start.
+template <int G>
+__global__ __launch_bounds__(64) void k_dd_spot_walks(whf_dd_spot *__restrict__ spots, uint32_t n,
+                                                       uint32_t *__restrict__ start,
+                                                       unsigned long long *__restrict__ cnt) {
+    constexpr int NG = 64 / G;
+    __shared__ uint2 s_box[NG][kMsMaxPieces];
+    __shared__ uint32_t s_frm[NG][kMsMaxFrames];
+    const int lane = (int)threadIdx.x, grp = lane / G, lig = lane % G;
+    unsigned long long c[5] = {0, 0, 0, 0, 0};   // done, cut, capped, draws, pieces
+    for (uint32_t q = blockIdx.x * NG + (uint32_t)grp; q < n; q += gridDim.x * NG) {
+        whf_dd_spot sp = spots[q];
+        const uint64_t s48 = sp.s48 & kLcgM48;
+        uint64_t st = (s48 ^ kLcgMul) & kLcgM48;
+        const uint64_t a = lcg_next_long(st);
+        const uint64_t b = lcg_next_long(st);
+        const uint64_t X = carver_seed(s48, a, b, sp.cx, sp.cz);
+        const bool is_start = ms_is_start(X);   // the same for the whole group
+        int floor = INT_MIN, pieces = 0;
+        if (is_start) {
+            const MsOut o = ms_room_floor<G>(X, INT_MIN, s_box[grp], s_frm[grp], lig, grp * G);
+            if (o.status == MS_DONE) floor = o.floor;
+            pieces = o.pieces;
+            c[o.status]++;
+            c[3] += o.draws;
+            c[4] += (unsigned long long)o.pieces;
+        }
+        if (lig == 0) {
+            sp.s48 = s48;
+            sp.rx = sp.cx >> 5;
+            sp.rz = sp.cz >> 5;
+            sp.carver = X;
+            sp.room_floor = floor;
+            sp.pieces = pieces;
+            spots[q] = sp;
+            start[q] = is_start ? 1u : 0u;
+        }
+    }
+    for (int k = 0; k < 5; k++) {
+        unsigned long long v = lig == 0 ? c[k] : 0ull;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+        if (lane == 0 && v) atomicAdd(&cnt[CNT_DD_DONE + k], v);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------- expansion
 
 __global__ void k_dd_xseeds(const whf_dd_spot *__restrict__ spots, uint32_t ns, uint64_t *__restrict__ seeds) {
@@ -357,6 +405,74 @@ int dd_expand(whf_ctx *c, const whf_dd_request *req, const whf_dd_spot *d_spots,
     return pl.end();
 }
 
+// After the last batch of a Deep Dive call: the finds (floor_y = the room floor of their spot) and the counters
+// that the sweep and the spots entry share.
+int dd_finish(whf_ctx *c, bool expand, uint64_t spots_total,
+              const std::map<std::tuple<uint64_t, int, int>, int> &floor_of, whf_sink *finds, whf_dd_stats *stats) {
+    const hipStream_t st = c->stream;
+    HIP_TRY(hipEventRecord(c->ev1, st));
+    unsigned long long cnt[CNT_N];
+    uint32_t ctl[CTL_N];
+    HIP_TRY(hipMemcpyAsync(cnt, c->d_cnt, sizeof cnt, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ctl, c->d_ctl, sizeof ctl, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (cnt[CNT_SLOT_OVER]) {
+        std::snprintf(g_last_error, sizeof g_last_error, "%llu gate survivors beyond the sampler-table slots",
+                      (unsigned long long)cnt[CNT_SLOT_OVER]);
+        return WHF_EOVERFLOW;
+    }
+    int rc = WHF_OK;
+    const uint64_t hits_total = expand ? ctl[CTL_HITS] : 0;
+    if (finds) {
+        std::vector<DevHit> hh(std::min<uint64_t>(hits_total, kHitCap));
+        if (!hh.empty()) HIP_TRY(hipMemcpy(hh.data(), c->d_hits, hh.size() * sizeof(DevHit), hipMemcpyDeviceToHost));
+        finds->count = hits_total;
+        const uint64_t m = std::min<uint64_t>(finds->capacity, hh.size());
+        for (uint64_t i = 0; i < m; i++) {
+            whf_hit h;
+            std::memcpy(&h, &hh[i], sizeof h);
+            auto it = floor_of.find(std::make_tuple((uint64_t)h.seed & kLcgM48, h.cx, h.cz));
+            h.floor_y = it == floor_of.end() ? INT32_MIN : it->second;
+            h.flags = WHF_HIT_DEEPDIVE;
+            finds->hits[i] = h;
+        }
+        if (hits_total > finds->capacity || hh.size() < hits_total) rc = WHF_EOVERFLOW;
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->walks[0] = cnt[CNT_DD_DONE];
+        stats->walks[1] = cnt[CNT_DD_CUT];
+        stats->walks[2] = cnt[CNT_DD_CAPPED];
+        stats->starts = stats->walks[0] + stats->walks[1] + stats->walks[2];
+        stats->spots = spots_total;
+        stats->draws = cnt[CNT_DD_DRAWS];
+        stats->pieces = cnt[CNT_DD_PIECES];
+        if (expand) {
+            stats->expanded_seeds = spots_total << 16;
+            uint64_t ero = 0;
+            for (int t = 0; t < 7; t++) { stats->tiers[t] = cnt[CNT_TIER0 + t]; ero += cnt[CNT_TIER0 + t]; }
+            stats->funnel[0] = ero;
+            for (int k = 1; k < WHF_KIND_N; k++) stats->funnel[k] = cnt[CNT_KIND0 + k];
+            stats->accepted = cnt[CNT_KIND0 + 10];
+            for (int s = 0; s < SET_COUNT; s++) stats->samplers_built[s] = cnt[CNT_BUILT + s];
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        stats->device_ms = ms;
+        if (c->profile) {
+            for (size_t i = 0; i + 1 < c->prof_ev.size(); i += 2) {
+                float m2 = 0.f;
+                (void)hipEventElapsedTime(&m2, c->prof_ev[i], c->prof_ev[i + 1]);
+                const int f = c->prof_fam[i / 2];
+                if (f == F_DD_SWEEP) stats->stage_ms[0] += m2;
+                else if (f == F_DD_FLOOR) stats->stage_ms[1] += m2;
+                else stats->stage_ms[2] += m2;   // the expansion and the Seed Check gates it runs
+            }
+        }
+    }
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -417,71 +533,73 @@ int whf_deepdive(whf_ctx *c, const whf_dd_request *req, whf_dd_sink *spots, whf_
             }
         }
     }
-    HIP_TRY(hipEventRecord(c->ev1, st));
-    unsigned long long cnt[CNT_N];
-    uint32_t ctl[CTL_N];
-    HIP_TRY(hipMemcpyAsync(cnt, c->d_cnt, sizeof cnt, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(ctl, c->d_ctl, sizeof ctl, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (cnt[CNT_SLOT_OVER]) {
-        std::snprintf(g_last_error, sizeof g_last_error, "%llu gate survivors beyond the sampler-table slots",
-                      (unsigned long long)cnt[CNT_SLOT_OVER]);
-        return WHF_EOVERFLOW;
-    }
-    rc = WHF_OK;
+    rc = dd_finish(c, req->expand != 0, spots_total, floor_of, finds, stats);
+    if (rc != WHF_OK && rc != WHF_EOVERFLOW) return rc;
     if (spots) {
         spots->count = spots_total;
         if (spots_total > spots->capacity) rc = WHF_EOVERFLOW;
     }
-    const uint64_t hits_total = req->expand ? ctl[CTL_HITS] : 0;
-    if (finds) {
-        std::vector<DevHit> hh(std::min<uint64_t>(hits_total, kHitCap));
-        if (!hh.empty()) HIP_TRY(hipMemcpy(hh.data(), c->d_hits, hh.size() * sizeof(DevHit), hipMemcpyDeviceToHost));
-        finds->count = hits_total;
-        const uint64_t m = std::min<uint64_t>(finds->capacity, hh.size());
-        for (uint64_t i = 0; i < m; i++) {
-            whf_hit h;
-            std::memcpy(&h, &hh[i], sizeof h);
-            auto it = floor_of.find(std::make_tuple((uint64_t)h.seed & kLcgM48, h.cx, h.cz));
-            h.floor_y = it == floor_of.end() ? INT32_MIN : it->second;
-            h.flags = WHF_HIT_DEEPDIVE;
-            finds->hits[i] = h;
-        }
-        if (hits_total > finds->capacity || hh.size() < hits_total) rc = WHF_EOVERFLOW;
-    }
     if (stats) {
-        std::memset(stats, 0, sizeof *stats);
         stats->structure_seeds = req->count;
         stats->tests = req->count * regions;
-        stats->walks[0] = cnt[CNT_DD_DONE];
-        stats->walks[1] = cnt[CNT_DD_CUT];
-        stats->walks[2] = cnt[CNT_DD_CAPPED];
-        stats->starts = stats->walks[0] + stats->walks[1] + stats->walks[2];
-        stats->spots = spots_total;
-        stats->draws = cnt[CNT_DD_DRAWS];
-        stats->pieces = cnt[CNT_DD_PIECES];
-        if (req->expand) {
-            stats->expanded_seeds = spots_total << 16;
-            uint64_t ero = 0;
-            for (int t = 0; t < 7; t++) { stats->tiers[t] = cnt[CNT_TIER0 + t]; ero += cnt[CNT_TIER0 + t]; }
-            stats->funnel[0] = ero;
-            for (int k = 1; k < WHF_KIND_N; k++) stats->funnel[k] = cnt[CNT_KIND0 + k];
-            stats->accepted = cnt[CNT_KIND0 + 10];
-            for (int s = 0; s < SET_COUNT; s++) stats->samplers_built[s] = cnt[CNT_BUILT + s];
+    }
+    return rc;
+}
+
+int whf_dd_terrain(whf_ctx *c, const whf_dd_request *req, whf_dd_spot *spots, uint64_t n, whf_sink *finds,
+                   whf_dd_stats *stats) {
+    if (!c || !req || (n && !spots)) return WHF_EINVAL;
+    g_last_error[0] = 0;
+    if (n > kDdSpotCap) return WHF_EINVAL;
+    if (req->vflags < 0 || req->vflags >= kLayouts) return WHF_EINVAL;
+    if (req->max_height < -140 || req->max_height > 100000) return WHF_EINVAL;
+    if (finds && finds->capacity && !finds->hits) return WHF_EINVAL;
+    HIP_TRY(hipSetDevice(c->device));
+    DdBufs b;
+    int rc = dd_alloc(c, b);
+    if (rc) return rc;
+    const hipStream_t st = c->stream;
+    HIP_TRY(hipMemsetAsync(c->d_cnt, 0, CNT_N * sizeof(unsigned long long), st));
+    HIP_TRY(hipMemsetAsync(c->d_ctl, 0, CTL_N * sizeof(uint32_t), st));
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    c->prof_ev.clear();
+    c->prof_fam.clear();
+    HIP_TRY(hipEventRecord(c->ev0, st));
+    std::map<std::tuple<uint64_t, int, int>, int> floor_of;
+    uint64_t n_starts = 0;
+    if (n) {
+        // the spots in b.spots and their start flags in the start queue's memory (kDdStartCap * 24 bytes >= n * 4)
+        uint32_t *d_start = reinterpret_cast<uint32_t *>(b.starts);
+        std::vector<uint32_t> is_start(n);
+        HIP_TRY(hipMemcpyAsync(b.spots, spots, n * sizeof(whf_dd_spot), hipMemcpyHostToDevice, st));
+        {
+            Prof p(c, F_DD_FLOOR);
+            const uint32_t groups = 64 / kDdGroup;
+            const uint32_t grid = std::max(1u, std::min(((uint32_t)n + groups - 1) / groups, (uint32_t)c->n_cu * 16));
+            hipLaunchKernelGGL(k_dd_spot_walks<kDdGroup>, dim3(grid), dim3(64), 0, st, b.spots, (uint32_t)n, d_start,
+                               c->d_cnt);
         }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-        stats->device_ms = ms;
-        if (c->profile) {
-            for (size_t i = 0; i + 1 < c->prof_ev.size(); i += 2) {
-                float m2 = 0.f;
-                (void)hipEventElapsedTime(&m2, c->prof_ev[i], c->prof_ev[i + 1]);
-                const int f = c->prof_fam[i / 2];
-                if (f == F_DD_SWEEP) stats->stage_ms[0] += m2;
-                else if (f == F_DD_FLOOR) stats->stage_ms[1] += m2;
-                else stats->stage_ms[2] += m2;   // the expansion and the Seed Check gates it runs
-            }
+        HIP_TRY(hipMemcpyAsync(spots, b.spots, n * sizeof(whf_dd_spot), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(is_start.data(), d_start, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if ((rc = check_launch())) return rc;
+        // every start is expanded, whatever its room floor (the reference's --spots mode tests the start only)
+        std::vector<whf_dd_spot> starts;
+        for (uint64_t i = 0; i < n; i++) {
+            if (!is_start[i]) continue;
+            starts.push_back(spots[i]);
+            floor_of[std::make_tuple(spots[i].s48, spots[i].cx, spots[i].cz)] = spots[i].room_floor;
         }
+        n_starts = starts.size();
+        if (n_starts) {
+            HIP_TRY(hipMemcpyAsync(b.spots, starts.data(), n_starts * sizeof(whf_dd_spot), hipMemcpyHostToDevice, st));
+            if ((rc = dd_expand(c, req, b.spots, (uint32_t)n_starts))) return rc;
+        }
+    }
+    rc = dd_finish(c, true, n_starts, floor_of, finds, stats);
+    if (stats) {
+        stats->structure_seeds = n;
+        stats->tests = n;
     }
     return rc;
 }

@@ -8,8 +8,20 @@
 //                     file starts "seed,chunkX,chunkZ,layoutX,...", hut_lookup_v4_source.cu:897; its airHi/airLo
 //                     columns come from the reference's layout catalogue, which this engine does not use, so the
 //                     room floor stands in their place).
+//
+// --spots FILE runs the terrain stage of the reference's --spots mode instead of the sweep (hut_lookup_v4_source.cu:
+// 1262-1292, started by hunt.py per chunk of solver hits, hunt.py:954-957): "S cx cz" lines (others skipped), each
+// spot's start test, and every start expanded to its 2^16 world seeds through the phase-1 chain (--max-height);
+// --out gets the reference's survivor file, "seed,chunkX,chunkZ,layoutX,airHi,airLo" with air columns 0 as the
+// reference writes them in this mode. The reference's other options on that command line (--catalog, --batch,
+// --ero-split, --ero-t1/-c0/-t3/-c1, --quiet) are accepted and have no effect here. --phase2-max-y T keeps only the
+// survivors lysh's hut verdict accepts at avg_y <= T (hunt.py's finisher asks lysh for that first, hunt.py:281-287).
+// A spot that is not a mineshaft start is reported and skipped and makes the exit status 1, as in the reference.
 // Exit status 0 on success, 2 for a bad command line, 1 for a device or output error.
+#include <algorithm>
+#include <chrono>
 #include <cinttypes>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,7 +39,9 @@ void usage() {
         "                     [--stats]\n"
         "  structure seeds S + i (mod 2^48), i < N; hut regions [a,b) x [c,d); keep mineshaft rooms with floor <= F\n"
         "  (default -55); --expand checks each kept room's 2^16 world seeds with the phase-1 chain (maxHeight H,\n"
-        "  default -50; version V, default 1.19.x~26.1)\n");
+        "  default -50; version V, default 1.19.x~26.1)\n"
+        "       deepdive_scan --spots FILE [--max-height H] [--version V] [--phase2-max-y T] [--out FILE] [--stats]\n"
+        "  the terrain stage of the spots in FILE (\"S cx cz\" lines); --out gets seed,chunkX,chunkZ,layoutX,airHi,airLo\n");
 }
 
 bool num(const char *s, long long &v) {
@@ -36,13 +50,115 @@ bool num(const char *s, long long &v) {
     return e && *e == 0 && e != s;
 }
 
+
+// The spots mode: reads FILE, runs whf_dd_terrain on up to 2^20 spots at a time, writes the survivors.
+int run_spots(const std::string &path, const whf_dd_request &req, const std::string &version, bool have_p2,
+              long long p2_max_y, const std::string &out_path, bool stats) {
+    FILE *sf = std::fopen(path.c_str(), "r");
+    if (!sf) { std::fprintf(stderr, "cannot read %s\n", path.c_str()); return 1; }
+    std::vector<whf_dd_spot> spots;
+    char ln[512];
+    while (std::fgets(ln, sizeof ln, sf)) {
+        long long S;
+        int cx, cz;
+        if (ln[0] == '#' || std::sscanf(ln, "%lld %d %d", &S, &cx, &cz) != 3) continue;
+        whf_dd_spot sp;
+        std::memset(&sp, 0, sizeof sp);
+        sp.s48 = (uint64_t)S & 0xFFFFFFFFFFFFull;
+        sp.cx = cx;
+        sp.cz = cz;
+        spots.push_back(sp);
+    }
+    std::fclose(sf);
+    int tree = 0;
+    if (have_p2 && whf_biome_tree(version.c_str(), &tree) != WHF_OK) {
+        std::fprintf(stderr, "unknown version %s\n", version.c_str());
+        return 2;
+    }
+    FILE *out = nullptr;
+    if (!out_path.empty()) {
+        out = std::fopen(out_path.c_str(), "w");
+        if (!out) { std::fprintf(stderr, "cannot write %s\n", out_path.c_str()); return 1; }
+        std::fprintf(out, "seed,chunkX,chunkZ,layoutX,airHi,airLo\n");
+    }
+    whf_ctx *ctx = nullptr;
+    whf_config cfg;
+    std::memset(&cfg, 0, sizeof cfg);
+    int rc = whf_open(&cfg, &ctx);
+    if (rc != WHF_OK) {
+        std::fprintf(stderr, "whf_open: %s\n", whf_strerror(rc));
+        if (out) std::fclose(out);
+        return 1;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t kChunk = 1u << 20;
+    std::vector<whf_hit> finds(1 << 20);
+    uint64_t n_bad = 0, n_surv = 0, n_terrain = 0, starts = 0, expanded = 0;
+    double device_ms = 0.0;
+    for (size_t s0 = 0; s0 < spots.size() && rc == WHF_OK; s0 += kChunk) {
+        const size_t n = std::min(kChunk, spots.size() - s0);
+        whf_dd_spot *sp = spots.data() + s0;
+        whf_sink fs{finds.data(), finds.size(), 0};
+        whf_dd_stats st;
+        rc = whf_dd_terrain(ctx, &req, sp, n, &fs, &st);
+        if (rc != WHF_OK) { std::fprintf(stderr, "whf_dd_terrain: %s\n", whf_strerror(rc)); break; }
+        for (size_t i = 0; i < n; i++)
+            if (sp[i].pieces == 0) {
+                n_bad++;
+                std::fprintf(stderr, "*** not a mineshaft start: S=%" PRIu64 " chunk (%d,%d)\n", sp[i].s48, sp[i].cx,
+                             sp[i].cz);
+            }
+        starts += st.starts;
+        expanded += st.expanded_seeds;
+        device_ms += st.device_ms;
+        n_terrain += fs.count;
+        std::vector<whf_hit> keep(finds.begin(), finds.begin() + (std::ptrdiff_t)fs.count);
+        if (have_p2 && !keep.empty()) {
+            std::vector<whf_hut_query> hq(keep.size());
+            std::vector<whf_hut_result> hr(keep.size());
+            for (size_t i = 0; i < keep.size(); i++) hq[i] = whf_hut_query{keep[i].seed, keep[i].cx, keep[i].cz};
+            rc = whf_hut_eval(ctx, hq.data(), hq.size(), tree, req.vflags, (int)p2_max_y, hr.data());
+            if (rc != WHF_OK) { std::fprintf(stderr, "whf_hut_eval: %s\n", whf_strerror(rc)); break; }
+            size_t k = 0;
+            for (size_t i = 0; i < keep.size(); i++)
+                if (hr[i].ok) keep[k++] = keep[i];
+            keep.resize(k);
+        }
+        n_surv += keep.size();
+        if (out)
+            for (const whf_hit &h : keep) {
+                uint64_t X = 0;   // the spot's carver seed
+                for (size_t i = 0; i < n; i++)
+                    if (sp[i].s48 == ((uint64_t)h.seed & 0xFFFFFFFFFFFFull) && sp[i].cx == h.cx && sp[i].cz == h.cz) {
+                        X = sp[i].carver;
+                        break;
+                    }
+                std::fprintf(out, "%" PRId64 ",%d,%d,%" PRIu64 ",0,0\n", h.seed, h.cx, h.cz, X);
+            }
+    }
+    whf_close(ctx);
+    if (out) std::fclose(out);
+    if (rc != WHF_OK) return 1;
+    const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("spots: %zu spots (%" PRIu64 " not a start), %" PRIu64 " terrain survivors -> %s; terrain %.3f s "
+                "(%.1f ms per spot)\n", spots.size() - (size_t)n_bad, n_bad, n_surv,
+                out_path.empty() ? "-" : out_path.c_str(), t, spots.empty() ? 0.0 : 1e3 * t / (double)spots.size());
+    if (stats)
+        std::printf("{\"spots\": %zu, \"not_start\": %" PRIu64 ", \"starts\": %" PRIu64 ", \"expanded_seeds\": %" PRIu64
+                    ", \"phase1\": %" PRIu64 ", \"survivors\": %" PRIu64 ", \"device_ms\": %.3f, \"wall_s\": %.3f}\n",
+                    spots.size(), n_bad, starts, expanded, n_terrain, n_surv, device_ms, t);
+    return n_bad ? 1 : 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
     unsigned long long s_first = 0, count = 0;
     long long rx0 = 0, rx1 = 0, rz0 = 0, rz1 = 0, floor_max = -55, max_h = -50;
     bool expand = false, stats = false, have_s = false, have_n = false, have_rect[4] = {false, false, false, false};
-    std::string version = "1.19.x~26.1", hits_path, out_path;
+    long long p2_max_y = 0;
+    bool have_p2 = false;
+    std::string version = "1.19.x~26.1", hits_path, out_path, spots_path;
     for (int i = 1; i < argc; i++) {
         const std::string a = argv[i];
         auto next = [&](const char *&v) -> bool {
@@ -66,7 +182,24 @@ int main(int argc, char **argv) {
         else if (a == "--version") { if (!next(v)) { usage(); return 2; } version = v; }
         else if (a == "--dump-hits") { if (!next(v)) { usage(); return 2; } hits_path = v; }
         else if (a == "--out") { if (!next(v)) { usage(); return 2; } out_path = v; }
+        else if (a == "--spots") { if (!next(v)) { usage(); return 2; } spots_path = v; }
+        else if (a == "--phase2-max-y") { if (!next(v) || !num(v, p2_max_y)) { usage(); return 2; } have_p2 = true; }
+        else if (a == "--quiet" || a == "--ero-split") {}
+        else if (a == "--catalog" || a == "--batch" || a == "--ero-t1" || a == "--ero-c0" || a == "--ero-t3" ||
+                 a == "--ero-c1") { if (!next(v)) { usage(); return 2; } }
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return 2; }
+    }
+    if (!spots_path.empty()) {
+        whf_dd_request req;
+        std::memset(&req, 0, sizeof req);
+        req.expand = 1;
+        req.max_height = (int)max_h;
+        if (whf_version_flags(version.c_str(), &req.vflags) != WHF_OK) {
+            std::fprintf(stderr, "unknown version %s\n", version.c_str());
+            usage();
+            return 2;
+        }
+        return run_spots(spots_path, req, version, have_p2, p2_max_y, out_path, stats);
     }
     if (!have_s || !have_n || !(have_rect[0] && have_rect[1] && have_rect[2] && have_rect[3]) || rx1 <= rx0 ||
         rz1 <= rz0) {

@@ -23,7 +23,7 @@ CFG_ONE_STREAM = 2   # every stage of a scan on one stream (no batch pipeline)
 
 # Exported symbols of include/whf.h, checked by the tests.
 SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags", "whf_strerror",
-           "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_room_floors",
+           "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_dd_terrain", "whf_room_floors",
            "whf_server_open", "whf_server_close", "whf_submit", "whf_submit_deepdive", "whf_poll",
            "whf_biome_tree", "whf_biome", "whf_surface_heights", "whf_post_carver_heights", "whf_hut_eval")
 HIT_DEEPDIVE, HIT_PHASE2 = 1, 2
@@ -174,6 +174,9 @@ def load():
         L.whf_deepdive.restype = ctypes.c_int
         L.whf_deepdive.argtypes = [ctypes.c_void_p, ctypes.POINTER(DdRequest), ctypes.POINTER(DdSink),
                                    ctypes.POINTER(Sink), ctypes.POINTER(DdStats)]
+        L.whf_dd_terrain.restype = ctypes.c_int
+        L.whf_dd_terrain.argtypes = [ctypes.c_void_p, ctypes.POINTER(DdRequest), ctypes.POINTER(DdSpot), ctypes.c_uint64,
+                                     ctypes.POINTER(Sink), ctypes.POINTER(DdStats)]
         L.whf_room_floors.restype = ctypes.c_int
         L.whf_room_floors.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
@@ -208,6 +211,14 @@ def version_flags(name):
     if rc:
         raise WhfError(rc, "version %r" % name)
     return v.value
+
+
+def _dd_stats(st, rc, fsink):
+    return dict(structure_seeds=st.structure_seeds, tests=st.tests, starts=st.starts, walks=list(st.walks),
+                spots=st.spots, draws=st.draws, pieces=st.pieces, expanded_seeds=st.expanded_seeds,
+                accepted=st.accepted, funnel=list(st.funnel), tiers=list(st.tiers)[:7],
+                samplers_built=list(st.samplers_built), device_ms=st.device_ms, stage_ms=list(st.stage_ms),
+                overflow=rc == EOVERFLOW, find_count=fsink.count)
 
 
 class Context:
@@ -304,12 +315,28 @@ class Context:
         spots = sorted((p.s48, p.rx, p.rz, p.cx, p.cz, p.carver, p.room_floor, p.pieces)
                        for p in sbuf[:min(ssink.count, spot_capacity)])
         finds = sorted((h.seed, h.rx, h.rz, h.cx, h.cz, h.floor_y) for h in fbuf[:min(fsink.count, find_capacity)])
-        stats = dict(structure_seeds=st.structure_seeds, tests=st.tests, starts=st.starts, walks=list(st.walks),
-                     spots=st.spots, draws=st.draws, pieces=st.pieces, expanded_seeds=st.expanded_seeds,
-                     accepted=st.accepted, funnel=list(st.funnel), tiers=list(st.tiers)[:7],
-                     samplers_built=list(st.samplers_built), device_ms=st.device_ms, stage_ms=list(st.stage_ms),
-                     overflow=rc == EOVERFLOW, find_count=fsink.count)
-        return spots, finds, stats
+        return spots, finds, _dd_stats(st, rc, fsink)
+
+    def dd_terrain(self, spots, max_height=-50, vflags=0, find_capacity=1 << 16):
+        """The --spots terrain stage: spots are (s48, cx, cz). Returns (the spots filled in, as tuples
+        (s48, rx, rz, cx, cz, carver, room_floor, pieces), in input order, pieces = 0 where the spot is not a
+        mineshaft start; finds as tuples (seed, rx, rz, cx, cz, room_floor), sorted; stats dict)."""
+        req = DdRequest(0, 0, 0, 1, 0, 1, 0, 1, max_height, vflags, 0, 0)
+        n = len(spots)
+        sbuf = (DdSpot * max(1, n))()
+        for i, (s48, cx, cz) in enumerate(spots):
+            sbuf[i].s48 = s48 & 0xFFFFFFFFFFFFFFFF
+            sbuf[i].cx = cx
+            sbuf[i].cz = cz
+        fbuf = (Hit * find_capacity)()
+        fsink = Sink(fbuf, find_capacity, 0)
+        st = DdStats()
+        rc = self.L.whf_dd_terrain(self.h, ctypes.byref(req), sbuf, n, ctypes.byref(fsink), ctypes.byref(st))
+        if rc not in (OK, EOVERFLOW):
+            raise WhfError(rc, "whf_dd_terrain")
+        out = [(p.s48, p.rx, p.rz, p.cx, p.cz, p.carver, p.room_floor, p.pieces) for p in sbuf[:n]]
+        finds = sorted((h.seed, h.rx, h.rz, h.cx, h.cz, h.floor_y) for h in fbuf[:min(fsink.count, find_capacity)])
+        return out, finds, _dd_stats(st, rc, fsink)
 
     def room_floors(self, carvers):
         """Full layout walks of the given carver seeds: arrays (status, room_floor, pieces)."""
