@@ -97,7 +97,8 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
 // Tiered erosion (noise.c:432-459) over (seed, region) items, level-synchronously: level L evaluates octave slot
 // L of both sub-samplers (evaluations 2L and 2L + 1) for every item still alive, as one interleaved pair, then
 // applies the tier checks that fall after those two evaluations in the reference's order. Items that survive are
-// compacted into the next level's list in LDS. The second evaluation of a pair is computed even when the check
+// compacted in place into one LDS list (each level reads its entries into registers before the first append). The
+// second evaluation of a pair is computed even when the check
 // after the first one rejects; it is then discarded, so the decisions and the tier histogram are the reference's.
 
 struct LPair {
@@ -106,7 +107,9 @@ struct LPair {
 };
 
 constexpr int kEroWG = 256;
-constexpr int kEroPass = 512;          // items per pass (2 per lane)
+// Items per pass (4 per lane): a whole tile of 4 seeds x 256 regions, so that levels 2 and 3 still fill whole waves
+// (measured against 512, 768, 1536 and 2048: 1024 is fastest; 768 splits such a tile into a full pass and a thin one)
+constexpr int kEroPass = 1024;
 constexpr int kCzBias = 1 << 22;       // chunk z is stored biased so that 4 tile bits fit above it
 
 __device__ __forceinline__ void unpack_item(int2 e, int &cx, int &cz, int &sl) {
@@ -149,8 +152,8 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
     __shared__ __align__(16) uint16_t s_tab[ST * 8 * 256];   // byte-pair tables: P(i) | P(i + 1) << 8
     __shared__ SPar s_par[ST * 8];
     __shared__ int2 s_item[kEroPass];
-    __shared__ uint16_t s_li[2][kEroPass];       // alive lists: item index and running sum
-    __shared__ double s_ls[2][kEroPass];
+    __shared__ uint16_t s_li[kEroPass];          // alive list, compacted in place: item index and running sum
+    __shared__ double s_ls[kEroPass];
     __shared__ uint32_t s_n[4];                  // list sizes after levels 0, 1, 2
     __shared__ uint32_t s_cnt[8];
 
@@ -249,28 +252,38 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
             } else {
                 const uint32_t pos = wave_append(alive, &s_n[0]);
                 if (alive) {
-                    s_li[0][pos] = (uint16_t)i;
-                    s_ls[0][pos] = sum;
+                    s_li[pos] = (uint16_t)i;
+                    s_ls[pos] = sum;
                 }
             }
         }
         __syncthreads();
         if (!single) {
             // ---- levels 1..3: evaluations 2L, 2L + 1 and tiers 2L - 1, 2L ----
+            // a lane holds list entries tid + k * kEroWG; all are read before the barrier, so the survivors can be
+            // appended to the same list
+            constexpr int kPer = kEroPass / kEroWG;
             for (int L = 1; L <= 3; L++) {
-                const int cur = (L - 1) & 1, nxt = L & 1;
                 const uint32_t n = s_n[L - 1];
                 const int k0 = 2 * L, k1 = 2 * L + 1;
-                for (uint32_t i0 = tid & ~63u; i0 < n; i0 += kEroWG) {
-                    const uint32_t q = i0 + lane_id();
+                uint32_t idx[kPer];
+                double run[kPer];
+#pragma unroll
+                for (int k = 0; k < kPer; k++) {
+                    const uint32_t q = tid + (uint32_t)k * kEroWG;
+                    idx[k] = q < n ? s_li[q] : 0u;
+                    run[k] = q < n ? s_ls[q] : 0.0;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < kPer; k++) {
+                    if ((tid & ~63u) + (uint32_t)k * kEroWG >= n) break;   // wave-uniform
+                    const uint32_t q = tid + (uint32_t)k * kEroWG;
                     bool alive = false, undecided = false;
-                    uint32_t idx = 0;
-                    double sum = 0.0;
+                    double sum = run[k];
                     int cx = 0, cz = 0, sl = 0;
                     if (q < n) {
-                        idx = s_li[cur][q];
-                        sum = s_ls[cur][q];
-                        unpack_item(s_item[idx], cx, cz, sl);
+                        unpack_item(s_item[idx[k]], cx, cz, sl);
                         const double x = (double)(cx * 16 + 8) / 4.0, z = (double)(cz * 16 + 8) / 4.0;
                         const LPair Q[2] = {LPair{s_tab + (sl * 8 + k0) * 256}, LPair{s_tab + (sl * 8 + k1) * 256}};
                         const SPar *P[2] = {&s_par[sl * 8 + k0], &s_par[sl * 8 + k1]};
@@ -301,8 +314,8 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
                     if (L < 3) {
                         const uint32_t pos = wave_append(alive, &s_n[L]);
                         if (alive) {
-                            s_li[nxt][pos] = (uint16_t)idx;
-                            s_ls[nxt][pos] = sum;
+                            s_li[pos] = (uint16_t)idx[k];
+                            s_ls[pos] = sum;
                         }
                     } else {
                         // survivors of the erosion gate -> Q1
