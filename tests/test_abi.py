@@ -88,3 +88,20 @@ def test_deepdive_cli_argument_errors_exit_2_before_touching_the_gpu():
         assert r.returncode == 2, (args, r.returncode, r.stderr)
         assert "usage: deepdive_scan" in r.stdout
     assert subprocess.run([whf.DEEPDIVE_SCAN, "--help"], capture_output=True, timeout=60).returncode == 0
+
+
+def test_deepdive_spots_cli_errors_before_touching_the_gpu(tmp_path):
+    """--spots mode: bad options exit 2 and a missing spots file exits 1, all before the device is opened; the
+    options of hut_lookup_v4's terrain command line that have no effect here are accepted."""
+    sp = tmp_path / "spots.txt"
+    sp.write_text("# no spots\n")
+    for args in (["--spots"], ["--spots", str(sp), "--version", "1.17"], ["--spots", str(sp), "--max-height", "x"],
+                 ["--spots", str(sp), "--phase2-max-y"], ["--spots", str(sp), "--catalog"],
+                 ["--spots", str(sp), "--batch"]):
+        r = subprocess.run([whf.DEEPDIVE_SCAN, *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2, (args, r.returncode, r.stderr)
+    r = subprocess.run([whf.DEEPDIVE_SCAN, "--catalog", "cat.txt", "--spots", str(tmp_path / "missing.txt"),
+                        "--batch", "4096", "--ero-split", "--ero-t1", "0.5", "--ero-c0", "2", "--ero-t3", "0.6",
+                        "--ero-c1", "3", "--max-height", "-54", "--quiet", "--out", str(tmp_path / "o.csv")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "cannot read" in r.stderr
