@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwhf_hip.so")
 PHASE1_SCAN = os.path.join(HERE, "bin", "phase1_scan")
 DEEPDIVE_SCAN = os.path.join(HERE, "bin", "deepdive_scan")
+BIOME_MAP = os.path.join(HERE, "bin", "biome_map")
 
 OK, EINVAL, ENOMEM, EDEVICE, EOVERFLOW, ENOTSUP = 0, -1, -2, -3, -4, -5
 SEEDS_LIST, SEEDS_SPLITMIX, SEEDS_COUNTER = 0, 1, 2

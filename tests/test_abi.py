@@ -105,3 +105,13 @@ def test_deepdive_spots_cli_errors_before_touching_the_gpu(tmp_path):
                         "--ero-c1", "3", "--max-height", "-54", "--quiet", "--out", str(tmp_path / "o.csv")],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "cannot read" in r.stderr
+
+
+def test_biome_map_cli_errors_exit_2_before_touching_the_gpu():
+    """bin/biome_map takes the map tool's command line (biome_map.c:62-84): the same usage errors exit 2."""
+    base = ["--seed", "5", "--x0", "0", "--z0", "0", "--w", "4", "--h", "4"]
+    for args in (base[:-2], base + ["--ss", "5"], base + ["--step", "0"], ["--seed", "5", "--x0", "0", "--z0", "0",
+                 "--w", "5000", "--h", "4"], base + ["--ver", "1.17"], base + ["--bogus", "1"], ["--stdin"],
+                 base + ["--seed", "x"]):
+        r = subprocess.run([whf.BIOME_MAP, *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2, (args, r.returncode, r.stderr)
