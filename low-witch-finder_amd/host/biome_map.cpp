@@ -12,7 +12,8 @@
 //     does), otherwise the answer at the chunk middle's pre-carver surface.
 //
 // V is a game version name (default 26.3): its biome tree and, for the surface, its terrain rules. All biome checks
-// of a band of rows go to the device in one whf_biome call; each distinct chunk is asked once.
+// of a band of rows go to the device in one whf_biome call; each distinct chunk is asked once. Unlike the reference,
+// coordinates beyond +-2^30 blocks (chunks beyond +-2^26) are refused as a usage error: the world spans +-3e7.
 // Exit status 0 on success, 2 for a bad command line, 1 for a device or output error.
 #include <cstdint>
 #include <cstdio>
@@ -126,7 +127,13 @@ int run_map(whf_ctx *ctx, int64_t seed, long long x0, long long z0, int w, int h
 int run_stdin(whf_ctx *ctx, int64_t seed, int tree, int vflags) {
     std::vector<uint64_t> chunks;
     long long cx, cz;
-    while (std::scanf("%lld %lld", &cx, &cz) == 2) chunks.push_back(chunk_key(cx, cz));
+    while (std::scanf("%lld %lld", &cx, &cz) == 2) {
+        if (std::llabs(cx) > (1ll << 26) || std::llabs(cz) > (1ll << 26)) {
+            std::fprintf(stderr, "biome_map: chunk %lld %lld is outside the world\n", cx, cz);
+            return 2;
+        }
+        chunks.push_back(chunk_key(cx, cz));
+    }
     if (chunks.empty()) return 0;
     static const int ys[6] = {40, 48, 56, 64, 72, 80};
     std::vector<int> win;
@@ -180,8 +187,12 @@ int main(int argc, char **argv) {
         else { std::fprintf(stderr, "biome_map: unknown arg %s\n", a); return 2; }
     }
     if (from_stdin) w = h = 1;
+    // block coordinates stay within +-2^30 (the world is +-3e7), so 16 * chunk fits the queries' 32-bit fields
+    const long long lim = 1ll << 30;
+    const bool in_world = std::llabs(x0) <= lim && std::llabs(z0) <= lim && std::llabs(x0 + w * step) <= lim &&
+                          std::llabs(z0 + h * step) <= lim;
     if (have != 31 || w <= 0 || h <= 0 || w > 4096 || h > 4096 || step < 1 || step > (1 << 24) || ss < 1 || ss > 4 ||
-        y < -64 || y > 319) {
+        y < -64 || y > 319 || !in_world) {
         usage();
         return 2;
     }

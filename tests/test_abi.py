@@ -112,6 +112,6 @@ def test_biome_map_cli_errors_exit_2_before_touching_the_gpu():
     base = ["--seed", "5", "--x0", "0", "--z0", "0", "--w", "4", "--h", "4"]
     for args in (base[:-2], base + ["--ss", "5"], base + ["--step", "0"], ["--seed", "5", "--x0", "0", "--z0", "0",
                  "--w", "5000", "--h", "4"], base + ["--ver", "1.17"], base + ["--bogus", "1"], ["--stdin"],
-                 base + ["--seed", "x"]):
+                 base + ["--seed", "x"], ["--seed", "5", "--x0", str(1 << 31), "--z0", "0", "--w", "4", "--h", "4"]):
         r = subprocess.run([whf.BIOME_MAP, *args], capture_output=True, text=True, timeout=60)
         assert r.returncode == 2, (args, r.returncode, r.stderr)
