@@ -17,7 +17,8 @@ namespace whf {
 constexpr int kWave = 64;
 
 // Device control words of a batch.
-enum Ctl { CTL_Q1 = 0, CTL_Q2, CTL_NC, CTL_NL, CTL_QX, CTL_HITS, CTL_DD_STARTS, CTL_DD_SPOTS, CTL_DD_WORK, CTL_N };
+// CTL_QM: gate-2 survivors of temperature and ridge, before the cave ladder. Words below CTL_HITS are per batch.
+enum Ctl { CTL_Q1 = 0, CTL_Q2, CTL_NC, CTL_NL, CTL_QX, CTL_QM, CTL_HITS, CTL_DD_STARTS, CTL_DD_SPOTS, CTL_DD_WORK, CTL_N };
 
 // Device counters of a scan (u64).
 enum Cnt {

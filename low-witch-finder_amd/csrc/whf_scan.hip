@@ -8,8 +8,9 @@
 //   k_erosion    placement + tiered erosion per (seed, region) with the seed tile's set-E tables in LDS; a
 //                per-lane work queue keeps lanes busy through the tier early exits; survivors -> queue Q1.
 //   k_compact    seeds flagged by a gate -> dense slot list for the next build.
-//   k_gate2      temperature, ridge, Entrance@50, Entrance@60 per Q1 entry; survivors -> Q2.
-//   k_gate3      cave@maxHeight, both ladders, continentalness, floodedness per Q2 entry; survivors -> hits.
+//   k_gate2a     temperature and ridge per Q1 entry; survivors -> Q2.
+//   k_gate2b     Entrance@50, Entrance@60 per Q2 entry; survivors -> Q1 (free again by then).
+//   k_gate3      cave@maxHeight, both ladders, continentalness, floodedness per gate-2 survivor; survivors -> hits.
 // Every decision follows lysh_phase1_check_ex (engine/tools/lysh/src/phase1.c:130-200) bit for bit, so the
 // survivors and the funnel counters equal the reference CPU scanner's.
 #include <hip/hip_runtime.h>
@@ -430,14 +431,16 @@ __device__ __forceinline__ void block_count(uint32_t *s_cnt, int kind) { atomicA
 
 constexpr int kGateWG = 256;
 
-__global__ __launch_bounds__(kGateWG) void k_gate2(const uint64_t *__restrict__ seeds, const GateTables *__restrict__ gt,
-                                                    const uint8_t *__restrict__ tabC, const SPar *__restrict__ parC,
-                                                    const uint32_t *__restrict__ slotC, const uint4 *__restrict__ q1,
-                                                    uint4 *__restrict__ q2, uint32_t *__restrict__ ctl,
-                                                    uint32_t *__restrict__ needL, int vflags,
-                                                    unsigned long long *__restrict__ cnt) {
-    __shared__ uint32_t s_cnt[16];
-    if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
+// Gate 2 in two kernels. k_gate2a: temperature and ridge with the version rules on Q1 -> Q2 (count CTL_QM); it
+// needs few registers, so many waves hide its gathers. k_gate2b: the cave ladder's Entrance@50 and @60 on those
+// survivors -> Q1, which gate 3 then reads (count CTL_Q2).
+__global__ __launch_bounds__(kGateWG) void k_gate2a(const GateTables *__restrict__ gt, const uint8_t *__restrict__ tabC,
+                                                     const SPar *__restrict__ parC, const uint32_t *__restrict__ slotC,
+                                                     const uint4 *__restrict__ q1, uint4 *__restrict__ qm,
+                                                     uint32_t *__restrict__ ctl, int vflags,
+                                                     unsigned long long *__restrict__ cnt) {
+    __shared__ uint32_t s_cnt[4];
+    if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t n = ctl[CTL_Q1];
     const SetView C{tabC, parC, kSetSize[SET_C]};
@@ -449,13 +452,12 @@ __global__ __launch_bounds__(kGateWG) void k_gate2(const uint64_t *__restrict__ 
         if (q < n) {
             e = q1[q];
             const uint32_t slot = slotC[e.x];
-            const int hut_x = (int)e.y * 16, hut_z = (int)e.z * 16;
             int kind = -1;
             if (slot >= kMaxSeeds) {
                 atomicAdd(&cnt[CNT_SLOT_OVER], 1ull);
                 kind = 0;   // no tables: neither counted nor kept
             } else if (!single) {
-                const double qx = (hut_x + 8) / 4.0, qz = (hut_z + 8) / 4.0;
+                const double qx = ((int)e.y * 16 + 8) / 4.0, qz = ((int)e.z * 16 + 8) / 4.0;
                 const double t = noise_y0(gt->noise[G_TEMPERATURE], C, slot, qx, qz);
                 const bool t_bad = (vflags & WHF_V_118) ? (t < -0.45) : (t > 0.2 || t < -0.45);
                 if (t_bad) kind = 1;
@@ -465,30 +467,55 @@ __global__ __launch_bounds__(kGateWG) void k_gate2(const uint64_t *__restrict__ 
                     else if ((vflags & WHF_V_262) && r <= -0.91) kind = 2;
                 }
             }
-            if (kind < 0) {
-                atomicAdd(&s_cnt[15], 1u);   // reached the cave ladder
-                CaveCtx cv{gt, C, SetView{nullptr, nullptr, 0}, slot, 0};
-                const int x = hut_x + 3, z = hut_z + 3;
-                if (cv.entrance_nonneg(x, 50, z)) kind = 3;
-                else if (cv.entrance_nonneg(x, 60, z)) kind = 4;
-            }
             if (kind > 0) block_count(s_cnt, kind);
             else if (kind < 0) survive = true;
         }
+        const uint32_t pos = wave_append(survive, &ctl[CTL_QM]);
+        if (survive) qm[pos] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x >= 1 && threadIdx.x < 3 && s_cnt[threadIdx.x])
+        atomicAdd(&cnt[CNT_KIND0 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(kGateWG) void k_gate2b(const GateTables *__restrict__ gt, const uint8_t *__restrict__ tabC,
+                                                     const SPar *__restrict__ parC, const uint32_t *__restrict__ slotC,
+                                                     const uint4 *__restrict__ qm, uint4 *__restrict__ qout,
+                                                     uint32_t *__restrict__ ctl, uint32_t *__restrict__ needL,
+                                                     unsigned long long *__restrict__ cnt) {
+    __shared__ uint32_t s_cnt[8];
+    if (threadIdx.x < 8) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t n = ctl[CTL_QM];
+    const SetView C{tabC, parC, kSetSize[SET_C]};
+    for (uint32_t q0 = blockIdx.x * kGateWG + (threadIdx.x & ~63u); q0 < n; q0 += gridDim.x * kGateWG) {
+        const uint32_t q = q0 + lane_id();
+        bool survive = false;
+        uint4 e = make_uint4(0, 0, 0, 0);
+        if (q < n) {
+            e = qm[q];
+            atomicAdd(&s_cnt[0], 1u);   // reached the cave ladder
+            CaveCtx cv{gt, C, SetView{nullptr, nullptr, 0}, slotC[e.x], 0};
+            const int x = (int)e.y * 16 + 3, z = (int)e.z * 16 + 3;
+            int kind = -1;
+            if (cv.entrance_nonneg(x, 50, z)) kind = 3;
+            else if (cv.entrance_nonneg(x, 60, z)) kind = 4;
+            if (kind > 0) block_count(s_cnt, kind);
+            else survive = true;
+        }
         const uint32_t pos = wave_append(survive, &ctl[CTL_Q2]);
         if (survive) {
-            q2[pos] = e;
+            qout[pos] = e;
             needL[e.x] = 1u;
         }
     }
     __syncthreads();
-    if (threadIdx.x < 16 && s_cnt[threadIdx.x]) {
-        const int i = threadIdx.x;
-        atomicAdd(&cnt[i == 15 ? CNT_LADDER : CNT_KIND0 + i], (unsigned long long)s_cnt[i]);
-    }
+    if (threadIdx.x == 0 && s_cnt[0]) atomicAdd(&cnt[CNT_LADDER], (unsigned long long)s_cnt[0]);
+    if ((threadIdx.x == 3 || threadIdx.x == 4) && s_cnt[threadIdx.x])
+        atomicAdd(&cnt[CNT_KIND0 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
 }
 
-// Gate 3 per Q2 entry, in two stages per chunk of 256 entries. Stage A: cave@maxHeight, one lane per entry
+// Gate 3 per gate-2 survivor, in two stages per chunk of 256 entries. Stage A: cave@maxHeight, one lane per entry
 // (it decides 3 in 4 entries). Stage B: every remaining check of the survivors at once, one lane per (entry,
 // check): the negative ladder, the positive ladder, continentalness and the floodedness heights, in lysh's order
 // (phase1.c:151-196). Each check is a pure function of the entry, so the entry's reject kind is its first failing
@@ -509,7 +536,7 @@ __global__ __launch_bounds__(kGateWG) void k_gate3(const uint64_t *__restrict__ 
                                                     const uint8_t *__restrict__ tabC, const SPar *__restrict__ parC,
                                                     const uint8_t *__restrict__ tabL, const SPar *__restrict__ parL,
                                                     const uint32_t *__restrict__ slotC,
-                                                    const uint32_t *__restrict__ slotL, const uint4 *__restrict__ q2,
+                                                    const uint32_t *__restrict__ slotL, const uint4 *__restrict__ surv2,
                                                     const uint32_t *__restrict__ ctl, int vflags, int max_h,
                                                     DevHit *__restrict__ hits, uint32_t *__restrict__ hit_ctr,
                                                     uint32_t hit_cap, unsigned long long *__restrict__ cnt) {
@@ -532,7 +559,7 @@ __global__ __launch_bounds__(kGateWG) void k_gate3(const uint64_t *__restrict__ 
             const uint32_t q = base + threadIdx.x;
             bool keep = false;
             if (q < n) {
-                const uint4 e = q2[q];
+                const uint4 e = surv2[q];
                 const CaveCtx cv{gt, C, L, slotC[e.x], slotL[e.x]};
                 const int x = (int)e.y * 16 + 3, z = (int)e.z * 16 + 3;
                 if (cv.slotC >= kMaxSeeds || cv.slotL >= kMaxSeeds) atomicAdd(&cnt[CNT_SLOT_OVER], 1ull);
@@ -548,7 +575,7 @@ __global__ __launch_bounds__(kGateWG) void k_gate3(const uint64_t *__restrict__ 
         for (uint32_t w = threadIdx.x; w < ns * (uint32_t)n_checks; w += kGateWG) {
             const uint32_t li = w / (uint32_t)n_checks;
             const int j = (int)(w - li * (uint32_t)n_checks);
-            const uint4 e = q2[s_list[li]];
+            const uint4 e = surv2[s_list[li]];
             const CaveCtx cv{gt, C, L, slotC[e.x], slotL[e.x]};
             const int hut_x = (int)e.y * 16, hut_z = (int)e.z * 16, x = hut_x + 3, z = hut_z + 3;
             bool fail;
@@ -583,7 +610,7 @@ __global__ __launch_bounds__(kGateWG) void k_gate3(const uint64_t *__restrict__ 
                     const int kind = j < n_neg ? 6 : (j < n_neg + 4 ? 7 : ((!single && j == n_neg + 4) ? 8 : 9));
                     atomicAdd(&s_cnt[kind], 1u);
                 }
-                e = q2[s_list[li]];
+                e = surv2[s_list[li]];
             }
             const uint32_t pos = wave_append(accept, hit_ctr);
             if (accept && pos < hit_cap) {
@@ -946,8 +973,10 @@ void stage_c(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags,
     }
     {
         Prof p(c, F_GATE2, st);
-        hipLaunchKernelGGL(k_gate2, dim3(persist), dim3(kGateWG), 0, st, b.seeds, c->d_gt + layout, b.tabC, b.parC,
-                           b.slotC, b.q1, b.q2, b.ctl, b.needL, vflags, c->d_cnt);
+        hipLaunchKernelGGL(k_gate2a, dim3(persist), dim3(kGateWG), 0, st, c->d_gt + layout, b.tabC, b.parC, b.slotC,
+                           b.q1, b.q2, b.ctl, vflags, c->d_cnt);
+        hipLaunchKernelGGL(k_gate2b, dim3(persist), dim3(kGateWG), 0, st, c->d_gt + layout, b.tabC, b.parC, b.slotC,
+                           b.q2, b.q1, b.ctl, b.needL, c->d_cnt);
     }
 }
 
@@ -970,7 +999,7 @@ void stage_l(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags,
     {
         Prof p(c, F_GATE3, st);
         hipLaunchKernelGGL(k_gate3, dim3(persist), dim3(kGateWG), 0, st, b.seeds, c->d_gt + layout, b.tabC, b.parC,
-                           c->d_tab[SET_L], c->d_par[SET_L], b.slotC, c->d_slotL, b.q2, b.ctl, vflags, max_height,
+                           c->d_tab[SET_L], c->d_par[SET_L], b.slotC, c->d_slotL, b.q1, b.ctl, vflags, max_height,
                            c->d_hits, c->d_ctl + CTL_HITS, kHitCap, c->d_cnt);
     }
 }
