@@ -250,8 +250,10 @@ int whf_biome_tree(const char *version, int *tree);
  * coordinates, column_top.c:124-130 overworld offset for depth), quantized, and the nearest leaf of biome tree
  * `tree` (0..7, Climate.RTree.SubTree.search, biome.c:34-47). Besides the answer at occ_y, each result carries the
  * swamp verdict at every quart height the surface can have; `winners` (may be NULL) receives the winner at each
- * of them, WHF_BIOME_QY_N bytes per query. vflags: WHF_V_LARGE selects the LARGE_BIOMES climate noises; other
- * bits are ignored. Not re-entrant per context. */
+ * of them, WHF_BIOME_QY_N bytes per query. vflags: WHF_V_LARGE selects the LARGE_BIOMES climate noises;
+ * WHF_BIO_OCC_ONLY computes only the answer at occ_y (one tree walk per query instead of 98, for maps; swamp_qy is
+ * then 0 and `winners` must be NULL); other bits are ignored. Not re-entrant per context. */
+#define WHF_BIO_OCC_ONLY 0x100
 int whf_biome(whf_ctx *ctx, const whf_biome_query *queries, uint64_t n, int tree, int vflags, whf_biome_result *out,
               uint8_t *winners);
 

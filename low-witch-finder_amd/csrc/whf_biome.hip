@@ -8,6 +8,7 @@
 //                height-independent quantized parameters
 //   k_bio_tree   one 128-lane block per query: lane i < 97 walks the version's biome tree for quart height -17 + i,
 //                lane 97 for the query's own height; the swamp verdicts are gathered by wave ballot
+//   k_bio_tree1  with WHF_BIO_OCC_ONLY: one lane per query, its own height only
 //
 // The trees (2.25 MB for all eight versions, about 290 KB each) do not fit the 160 KB LDS; a walk touches a few
 // hundred 32-byte nodes near the root, which stay in L2 across the batch. The host renumbers each tree breadth
@@ -126,6 +127,35 @@ __global__ __launch_bounds__(128) void k_bio_tree(const TreeNode *__restrict__ g
         r.ok = winner == WHF_BIOME_SWAMP;
         atomicAdd(checks, (unsigned long long)(kQyLanes + 1));
     }
+}
+
+// WHF_BIO_OCC_ONLY: only the answer at each query's own height, one lane per query (a map needs no per-height
+// verdicts, which take the other 97 lanes of k_bio_tree).
+__global__ __launch_bounds__(64) void k_bio_tree1(const TreeNode *__restrict__ g, const BioPoint *__restrict__ pts,
+                                                  const whf_biome_query *__restrict__ q,
+                                                  whf_biome_result *__restrict__ res, uint32_t n,
+                                                  unsigned long long *__restrict__ checks) {
+    const uint32_t qi = blockIdx.x * 64 + threadIdx.x;
+    const bool live = qi < n;
+    if (live) {
+        const BioPoint p = pts[qi];
+        const int occ_y = q[qi].occ_y;
+        const int qy = occ_y >> 2;
+        Target t{{p.t[0], p.t[1], p.t[2], p.t[3], 0, p.t[5]}};
+        t.t[4] = quantize((float)(depth_gradient((double)(qy * 4)) + p.offset));
+        const int winner = g[tree_leaf(g, t)].biome;
+        whf_biome_result &r = res[qi];
+        for (int k = 0; k < 6; k++) r.target[k] = t.t[k];
+        r.qx = p.qx;
+        r.qy = qy;
+        r.qz = p.qz;
+        r.occ_y = occ_y;
+        r.winner = winner;
+        r.ok = winner == WHF_BIOME_SWAMP;
+        r.swamp_qy[0] = r.swamp_qy[1] = 0;
+    }
+    const unsigned long long m = __ballot(live);
+    if (threadIdx.x == 0 && m) atomicAdd(checks, (unsigned long long)__popcll(m));
 }
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -260,6 +290,9 @@ int whf_biome(whf_ctx *c, const whf_biome_query *queries, uint64_t n, int tree, 
               uint8_t *winners) {
     if (!c || (n && (!queries || !out))) return WHF_EINVAL;
     if (tree < 0 || tree >= WHF_GD_TREES) return WHF_EINVAL;
+    const bool occ_only = (vflags & WHF_BIO_OCC_ONLY) != 0;
+    if (occ_only && winners) return WHF_EINVAL;
+    vflags &= ~WHF_BIO_OCC_ONLY;
     g_last_error[0] = 0;
     if (n == 0) return WHF_OK;
     HIP_TRY(hipSetDevice(c->device));
@@ -313,8 +346,12 @@ int whf_biome(whf_ctx *c, const whf_biome_query *queries, uint64_t n, int tree, 
         launch_build(c, b.seeds, b.bd + large * kBioSamplers, kBioSamplers, ns, b.tab, b.par, c->d_cnt + CNT_BIO_BUILT);
         hipLaunchKernelGGL(k_bio_point, dim3((nq + 63) / 64), dim3(64), 0, st, b.tabs + large, b.spline, b.tab, b.par,
                            b.q, b.slot, nq, b.pt);
-        hipLaunchKernelGGL(k_bio_tree, dim3(nq), dim3(128), 0, st, g, b.pt, b.q, b.res, b.win,
-                           c->d_cnt + CNT_BIO_CHECKS);
+        if (occ_only)
+            hipLaunchKernelGGL(k_bio_tree1, dim3((nq + 63) / 64), dim3(64), 0, st, g, b.pt, b.q, b.res, nq,
+                               c->d_cnt + CNT_BIO_CHECKS);
+        else
+            hipLaunchKernelGGL(k_bio_tree, dim3(nq), dim3(128), 0, st, g, b.pt, b.q, b.res, b.win,
+                               c->d_cnt + CNT_BIO_CHECKS);
         rc = check_launch();
         if (rc != WHF_OK) return rc;
         HIP_TRY(hipMemcpyAsync(out + q0, b.res, nq * sizeof(whf_biome_result), hipMemcpyDeviceToHost, st));

@@ -101,7 +101,7 @@ int run_map(whf_ctx *ctx, int64_t seed, long long x0, long long z0, int w, int h
                     }
                 }
         std::vector<int> win;
-        if (biome_of(ctx, seed, chunks, y, tree, vflags, win, nullptr)) return 1;
+        if (biome_of(ctx, seed, chunks, y, tree, vflags | WHF_BIO_OCC_ONLY, win, nullptr)) return 1;
         out.resize((size_t)(j1 - j0) * w);
         size_t s = 0;
         for (int j = j0; j < j1; j++)
@@ -154,7 +154,7 @@ int run_stdin(whf_ctx *ctx, int64_t seed, int tree, int vflags) {
     }
     if (!edge.empty()) {
         std::vector<int> ew;
-        if (biome_of(ctx, seed, edge, WHF_OCC_SURFACE, tree, vflags, ew, nullptr)) return 1;
+        if (biome_of(ctx, seed, edge, WHF_OCC_SURFACE, tree, vflags | WHF_BIO_OCC_ONLY, ew, nullptr)) return 1;
         for (size_t k = 0; k < edge.size(); k++) out[edge_at[k]] = (uint8_t)ew[k];
     }
     return std::fwrite(out.data(), 1, out.size(), stdout) == out.size() ? 0 : 1;

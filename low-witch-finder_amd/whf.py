@@ -30,6 +30,7 @@ SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags"
 HIT_DEEPDIVE, HIT_PHASE2 = 1, 2
 P2_NONE, P2_OR_LOWER, P2_EXACTLY = 0, 1, 2
 BIOME_SWAMP, BIOME_QY0, BIOME_QY_N = 6, -17, 97
+BIO_OCC_ONLY = 0x100
 OCC_SURFACE = -(1 << 31)
 BLOCK_SOLID, BLOCK_AIR, BLOCK_WATER, BLOCK_LAVA = 0, 1, 2, 3
 

@@ -148,3 +148,20 @@ def test_invalid_arguments(ctx):
     with pytest.raises(whf.WhfError):
         ctx.biome([(1, 0, 0, 64)], -1)
     assert ctx.biome([], 3) == []
+
+
+@pytest.mark.parametrize("tree,vflags", [(3, 0), (7, whf.V_LARGE)])
+def test_occ_only_mode_equals_the_full_answer(ctx, tree, vflags):
+    """WHF_BIO_OCC_ONLY (one tree walk per query, for maps) gives every field of the answer at occ_y that the full
+    mode gives (swamp_qy is then 0), over 3,000 random queries at given heights and at the pre-carver surface."""
+    rng = random.Random(11 + tree)
+    qs = [(rng.getrandbits(64) - (1 << 63), rng.randint(-20000, 20000) * 16, rng.randint(-20000, 20000) * 16,
+           rng.choice([rng.randint(-64, 300), whf.OCC_SURFACE]) if i % 7 else whf.OCC_SURFACE) for i in range(3000)]
+    full = ctx.biome(qs, tree, vflags)
+    one = ctx.biome(qs, tree, vflags | whf.BIO_OCC_ONLY)
+    for f, o in zip(full, one):
+        assert o["swamp_qy"] == 0
+        assert {k: v for k, v in o.items() if k != "swamp_qy"} == {k: v for k, v in f.items() if k != "swamp_qy"}
+    assert sum(o["ok"] for o in one) >= 1
+    with pytest.raises(whf.WhfError):
+        ctx.biome(qs[:4], tree, whf.BIO_OCC_ONLY, winners=True)
