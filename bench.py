@@ -40,7 +40,7 @@ OPS_PER_SAMPLER = 271 * 21
 # subtractions, 8 gradients x (3 multiplies + 2 adds), 2 fades x 7, 7 lerps x 3, and weight, sum, amplitude.
 OPS_PER_Y0_EVAL = 4 + 2 + 2 + 4 + 8 * 5 + 2 * 7 + 7 * 3 + 3
 KERNEL_OF = {"seeds": "k_seeds", "build_erosion": "k_build (set E)", "erosion_gate": "k_erosion<4>",
-             "compact": "k_compact", "build_climate": "k_build (set C)", "gate2": "k_gate2",
+             "compact": "k_compact", "build_climate": "k_build (set C)", "gate2": "k_gate2a + k_gate2b",
              "build_ladder": "k_build (set L)", "gate3": "k_gate3"}
 WORK_UNIT = {"build_erosion": "Perlin sampler built = 271 Xoroshiro128++ steps x 21 int32 lane-ops",
              "build_climate": "Perlin sampler built = 271 Xoroshiro128++ steps x 21 int32 lane-ops",
@@ -68,15 +68,17 @@ def algorithmic_ops(st):
 
 
 def traffic_from_profiles(kernel):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE
-    scaled by the copy-kernel calibration), or None when it was not measured."""
+    """HBM bytes per launch of a kernel (or of "a + b", kernels launched together as one family) from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration), or None when it was not
+    measured."""
     path = os.path.join(REPO, "profiles", "traffic_c2.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    return t.get(kernel)
+    parts = [t.get(k) for k in kernel.split(" + ")]
+    return None if None in parts else sum(parts)
 
 
 def cpu_baseline(n_seeds_max, seconds):
