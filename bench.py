@@ -27,26 +27,28 @@ sys.path.insert(0, REPO)
 import __graft_entry__  # noqa: E402
 
 METRIC = "hut-region checks/sec, FP32-gate vs FP64, bit-exact finds; p50/p99 latency"
-# MI355X peak VALU lane-operations per second: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz (32-bit integer or FP64
-# add/mul issue; the same figure as the 78.6 TFLOP/s FP64 vector peak counted without the FMA factor 2).
+# MI355X published peaks (dense vector): FP32 157.3 TFLOP/s, FP64 78.6 TFLOP/s. The 32-bit VALU lane-operation
+# issue rate, 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T/s, bounds the integer work of the sampler builds (the
+# published FP64 figure counts an FMA as 2 FLOP at this same issue rate).
+PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
 PEAK_VALU_OPS = 256 * 4 * 16 * 2.4e9
-# Algorithmic cost of one Perlin sampler build: 259 Perlin draws + 12 derivation draws of Xoroshiro128++,
-# each 21 32-bit lane operations (18 for the 64-bit state update and output, 3 for the bounded draw).
-OPS_PER_SAMPLER = 271 * 21
-
-
-# Algorithmic FP64 operations of one y = 0 Perlin evaluation as the reference writes it (noise.c:292-318 with
-# its octave accumulation noise.c:441-449): 4 coordinate multiplies, 2 origin adds, 2 floors, 4 fraction
-# subtractions, 8 gradients x (3 multiplies + 2 adds), 2 fades x 7, 7 lerps x 3, and weight, sum, amplitude.
-OPS_PER_Y0_EVAL = 4 + 2 + 2 + 4 + 8 * 5 + 2 * 7 + 7 * 3 + 3
-KERNEL_OF = {"seeds": "k_seeds", "build_erosion": "k_build (set E)", "erosion_gate": "k_erosion<4>",
+# Algorithmic work of one y = 0 Perlin evaluation (noise.c:292-318 with the octave accumulation noise.c:441-449):
+# the corner blend is 51 FLOP (8 gradients x 2, 2 fades x 7 with the y fade precomputed, 7 lerps x 3) and the
+# coordinates and accumulation 8 FLOP (2 origin adds, 2 fraction subtractions, weight multiply and add, amplitude
+# scale, lacunarity scale). The FP32-gated erosion kernel does the blend in FP32 and the rest in FP64; the plain
+# kernel does all 59 in FP64.
+FLOP_BLEND_PER_EVAL = 51
+FLOP_COORD_PER_EVAL = 8
+# Algorithmic integer work of one Perlin sampler build: 259 Xoroshiro128++ draws of the permutation and origin
+# (noise.c:61-78), each about 25 32-bit lane operations (64-bit state update and output on 32-bit halves, the
+# bounded draw's multiply-high), plus the 256 byte swaps.
+OPS_PER_SAMPLER = 259 * 25
+# The same figure as round numbers per Xoroshiro step, kept for the secondary lane-op fields.
+OPS_PER_Y0_EVAL_LANE = 4 + 2 + 2 + 4 + 8 * 5 + 2 * 7 + 7 * 3 + 3
+KERNEL_OF = {"seeds": "k_seeds", "build_erosion": "k_build (set E)", "erosion_gate": "k_erosion<4, false>",
              "compact": "k_compact", "build_climate": "k_build (set C)", "gate2": "k_gate2a + k_gate2b",
              "build_ladder": "k_build (set L)", "gate3": "k_gate3"}
-WORK_UNIT = {"build_erosion": "Perlin sampler built = 271 Xoroshiro128++ steps x 21 int32 lane-ops",
-             "build_climate": "Perlin sampler built = 271 Xoroshiro128++ steps x 21 int32 lane-ops",
-             "build_ladder": "Perlin sampler built = 271 Xoroshiro128++ steps x 21 int32 lane-ops",
-             "erosion_gate": "y=0 Perlin evaluation = %d FP64 ops; evaluations per region from the tier histogram"
-                             % OPS_PER_Y0_EVAL}
 
 
 def note(msg):
@@ -57,14 +59,31 @@ def note(msg):
 T0 = time.perf_counter()
 
 
-def algorithmic_ops(st):
-    """Algorithmic lane-operations per kernel family of one scan, from the scan's own counters."""
+def erosion_evals(st):
+    """y = 0 Perlin evaluations of the tiered erosion gate in one scan, from the scan's own counters: tier t fires
+    after t + 2 evaluations, and a region that passes the gate takes all 8."""
+    return sum((t + 2) * n for t, n in enumerate(st["tiers"])) + 8 * (st["checks"] - st["funnel"][0])
+
+
+def family_work(st, fp32_gate):
+    """Algorithmic work per kernel family of one scan: (fp32 FLOP, fp64 FLOP, int32 lane-ops)."""
     built = st["samplers_built"]
-    tiers, funnel = st["tiers"], st["funnel"]
-    # tier t fires after evaluation t + 1, i.e. after t + 2 evaluations; survivors take all 8
-    evals = sum((t + 2) * n for t, n in enumerate(tiers)) + 8 * (st["checks"] - funnel[0])
-    return {"build_erosion": built[0] * OPS_PER_SAMPLER, "build_climate": built[1] * OPS_PER_SAMPLER,
-            "build_ladder": built[2] * OPS_PER_SAMPLER, "erosion_gate": evals * OPS_PER_Y0_EVAL}
+    ev = erosion_evals(st)
+    ero = (ev * FLOP_BLEND_PER_EVAL, ev * FLOP_COORD_PER_EVAL, 0) if fp32_gate else \
+        (0, ev * (FLOP_BLEND_PER_EVAL + FLOP_COORD_PER_EVAL), 0)
+    return {"build_erosion": (0, 0, built[0] * OPS_PER_SAMPLER), "build_climate": (0, 0, built[1] * OPS_PER_SAMPLER),
+            "build_ladder": (0, 0, built[2] * OPS_PER_SAMPLER), "erosion_gate": ero}
+
+
+def roofline_of(work, seconds):
+    """Fractions of the published peaks: FP32 and FP64 FLOP/s against 157.3 and 78.6 TFLOP/s, integer lane-ops
+    against the 39.3 T/s issue rate. The kernel's fraction of the vector unit is their sum (the three kinds of work
+    share one issue port)."""
+    f32, f64, i32 = (w / seconds for w in work)
+    fr = dict(fp32=f32 / PEAK_FP32, fp64=f64 / PEAK_FP64, int32=i32 / PEAK_VALU_OPS)
+    return dict(fp32_tflops=round(f32 / 1e12, 3), fp64_tflops=round(f64 / 1e12, 3), int32_tops=round(i32 / 1e12, 3),
+                frac_fp32=round(fr["fp32"], 4), frac_fp64=round(fr["fp64"], 4), frac_int32=round(fr["int32"], 4),
+                frac=round(sum(fr.values()), 4))
 
 
 def traffic_from_profiles(kernel):
@@ -81,24 +100,108 @@ def traffic_from_profiles(kernel):
     return None if None in parts else sum(parts)
 
 
-def cpu_baseline(n_seeds_max, seconds):
+def host_cpus():
+    """The host's CPUs as this process sees them: nproc (affinity), os.cpu_count(), the cgroup CPU quota, the CPU
+    model, and the thread count the CPU baseline uses (the smallest of affinity and quota: every core this process
+    may run on)."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = affinity if quota is None else max(1, min(affinity, int(quota)))
+    return dict(nproc=affinity, cpu_count=os.cpu_count(), cgroup_quota_cpus=quota, model=model, threads=threads)
+
+
+def cpu_baseline(n_seeds_max, seconds, host):
     exe = os.path.join(REPO, "oracle", "_ref", "lysh_cpu_bench")
     if not os.path.exists(exe):
         return None
-    threads = min(16, os.cpu_count() or 1)
+    threads = host["threads"]
     out = subprocess.run([exe, "--stream", "2", "--count", str(n_seeds_max), "--rx0", "-8", "--rx1", "8",
                           "--rz0", "-8", "--rz1", "8", "--version", "1.20", "--max-y", "-50", "--threads",
-                          str(threads), "--time-limit", str(seconds), "--phase2-max-y", str(THRESHOLD)],
+                          str(threads), "--time-limit", str(seconds), "--phase2-max-y", str(THRESHOLD), "--list", "1"],
                          check=True, capture_output=True, text=True).stdout
     r = json.loads(out)
     return dict(value=round(r["checks_per_s"], 1), unit="checks/s", cores=threads, kind="reference",
                 sample="first %d seeds of the same SplitMix64(2) stream x 256 regions (time-bounded %.0f s), "
                        "reference lysh_scan_rect per seed with its phase-2 hook (lysh_eval_hut, max-y %d) on the "
                        "survivors, one seed per host thread" % (r["seeds"], seconds, THRESHOLD),
-                funnel=r["funnel"], tiers=r["tiers"], seeds=r["seeds"], phase2_evaluated=r["phase2_evaluated"],
-                phase2_accepted=r["phase2_accepted"],
+                host=dict(host, cflags=r["cflags"], compiler="gcc " + r["compiler"]),
+                funnel=r["funnel"], tiers=r["tiers"], seeds=r["seeds"], checks=r["checks"], accepted=r["accepted"],
+                reached_ladder=r["reached_ladder"], phase2_evaluated=r["phase2_evaluated"],
+                phase2_accepted=r["phase2_accepted"], phase2_rejected=r["phase2_rejected"],
+                finds=[tuple(f) for f in r["finds"]],
                 phase2_ms_per_hut=round(1e3 * r["phase2_seconds"] / r["phase2_evaluated"], 3)
                 if r["phase2_evaluated"] else None)
+
+
+def cpu_parity(whf, cpu, precisions):
+    """The GPU scan of exactly the seeds the CPU baseline scanned (its stream prefix), with the same threshold and
+    version, compared with the reference scanner's own counters and finds: checks, the reject funnel, the erosion tier
+    histogram, phase-1 survivors and cells reaching the cave ladder, the phase-2 verdict counters and the list of finds
+    (seed, chunk x, chunk z, floor). One comparison per GPU precision."""
+    rect = (-8, 8, -8, 8)
+    out = dict(seeds=cpu["seeds"], compared=["checks", "funnel", "tiers", "accepted", "reached_ladder",
+                                             "phase2 counters", "finds (seed, cx, cz, floor)"])
+    want_finds = sorted(cpu["finds"])
+    want_p2 = [cpu["phase2_evaluated"], cpu["phase2_accepted"]] + list(cpu["phase2_rejected"])
+    equal = True
+    with whf.Context(0) as ctx:
+        for prec in precisions:
+            hits, st = ctx.scan(rect, splitmix=(2, cpu["seeds"]), max_height=-50, vflags=whf.version_flags("1.20"),
+                                capacity=1 << 20, precision=prec,
+                                phase2=(whf.P2_OR_LOWER, THRESHOLD, whf.biome_tree("1.20")))
+            got_finds = sorted((h[0], h[3], h[4], h[5]) for h in hits)
+            diff = [k for k, a, b in (("checks", st["checks"], cpu["checks"]), ("funnel", st["funnel"], cpu["funnel"]),
+                                      ("tiers", st["tiers"], cpu["tiers"]), ("accepted", st["accepted"], cpu["accepted"]),
+                                      ("reached_ladder", st["reached_ladder"], cpu["reached_ladder"]),
+                                      ("phase2", st["phase2"], want_p2), ("finds", got_finds, want_finds)) if a != b]
+            name = "fp64" if prec == whf.PREC_FP64 else "fp32_gate"
+            out[name] = dict(equal=not diff, differs=diff, finds=len(got_finds), phase2=st["phase2"])
+            equal = equal and not diff
+    out["equal"] = equal
+    out["finds"] = [list(f) for f in want_finds]
+    return out
+
+
+def cpu_product_mode(n_seeds, threads):
+    """hunt.py's CPU mode: one `lysh scan --list --no-stats` process per seed (hunt.py:1390-1392), `threads` processes
+    at a time, over the first n_seeds seeds of the SplitMix64(2) stream x 256 regions, threshold -52."""
+    from concurrent.futures import ThreadPoolExecutor
+    exe = os.path.join(REPO, "oracle", "_ref", "lysh")
+    if not os.path.exists(exe):
+        return None
+    seeds = [splitmix_at(2, i) for i in range(n_seeds)]
+
+    def one(seed):
+        return subprocess.run([exe, "scan", "--seed", str(seed), "--rx0", "-8", "--rx1", "8", "--rz0", "-8", "--rz1",
+                               "8", "--threads", "1", "--max-y", "-50", "--phase2-max-y", str(THRESHOLD), "--version",
+                               "1.20", "--list", "--no-stats"], check=True, capture_output=True, text=True).stdout
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        outs = list(ex.map(one, seeds))
+    secs = time.perf_counter() - t0
+    hits = sum(o.count("\nhit ") + o.startswith("hit ") for o in outs)
+    return dict(value=round(n_seeds * 256 / secs, 1), unit="checks/s", cores=threads, kind="reference",
+                sample="one `lysh scan --list --no-stats --threads 1` process per seed, %d processes at a time, over "
+                       "the first %d seeds of the same stream x 256 regions, phase-2 max-y %d" % (threads, n_seeds,
+                                                                                            THRESHOLD),
+                seconds=round(secs, 3), ms_per_seed_per_process=round(1e3 * secs * threads / n_seeds, 3), hits=hits)
 
 
 THRESHOLD = -52      # the Seed Check's floor threshold, 'or lower' (phase 1 runs at maxHeight -50)
@@ -107,20 +210,27 @@ DD_FLOOR = -55
 DD_TOTAL = 1 << 32   # structure seeds of the full Deep Dive workload
 
 
-def splitmix_first(stream):
-    """First output of sequential SplitMix64 seeded with `stream` (Vigna's splitmix64.c)."""
+def splitmix_at(stream, i):
+    """Output i (from 0) of sequential SplitMix64 seeded with `stream` (Vigna's splitmix64.c), as a signed 64-bit
+    world seed."""
     m = (1 << 64) - 1
-    z = (stream + 0x9E3779B97F4A7C15) & m
+    z = (stream + (i + 1) * 0x9E3779B97F4A7C15) & m
     z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
     z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
-    return z ^ (z >> 31)
+    z ^= z >> 31
+    return z - (1 << 64) if z >> 63 else z
+
+
+def splitmix_first(stream):
+    """First output of sequential SplitMix64 seeded with `stream`, unsigned."""
+    return splitmix_at(stream, 0) & ((1 << 64) - 1)
 
 
 def deep_dive_cpu(s0, seconds):
     exe = os.path.join(REPO, "oracle", "_ref", "mstree_cpu_bench")
     if not os.path.exists(exe):
         return None
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cpus()["threads"]
     out = subprocess.run([exe, "--s-first", str(s0), "--count", str(1 << 26), "--rx0", str(DD_RECT[0]), "--rx1",
                           str(DD_RECT[1]), "--rz0", str(DD_RECT[2]), "--rz1", str(DD_RECT[3]), "--floor", str(DD_FLOOR),
                           "--threads", str(threads), "--time-limit", str(seconds)], check=True, capture_output=True,
@@ -300,6 +410,8 @@ def main():
     ap.add_argument("--seeds", type=int, default=1 << 26)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--product-seeds", type=int, default=1024,
+                    help="seeds of the product-mode CPU timing (one lysh process per seed)")
     ap.add_argument("--dd-seeds", type=int, default=1 << 28, help="structure seeds of the Deep Dive sweep")
     ap.add_argument("--dd-expand", type=int, default=1 << 22, help="structure seeds whose spots are expanded")
     ap.add_argument("--no-deep-dive", action="store_true")
@@ -360,57 +472,80 @@ def main():
     secs = timing[best]
     checks = last["checks"]
 
-    # profiled step: per-kernel-family device time from HIP events recorded on the library's stream
+    # profiled step: per-kernel-family device time from HIP events recorded on the stream each kernel runs on
+    fp32 = best == whf.PREC_FP32_GATE
     with whf.Context(0, profile=True) as ctx:
         _, prof = ctx.scan(rect, precision=best, **kw)
     fam = prof["kernel_ms"]
     n_batches = -(-args.seeds // (1 << 17))
     built = prof["samplers_built"]
-    work = algorithmic_ops(prof)
+    work = family_work(prof, fp32)
     rooflines = {}
     for name, ms in fam.items():
-        ops = work.get(name)
-        if ops is None or ms <= 0:
+        if name not in work or ms <= 0:
             continue
-        achieved = ops / (ms / 1e3)
-        rooflines[name] = dict(achieved=round(achieved / 1e12, 3), frac=round(achieved / PEAK_VALU_OPS, 4),
-                               launch_ms=round(ms / n_batches, 4), ops_per_launch=int(ops // n_batches))
+        rooflines[name] = dict(roofline_of(work[name], ms / 1e3), launch_ms=round(ms / n_batches, 4))
     # the same step with every stage on one stream (no overlap): each kernel's own duration, the figure a
     # kernel-level optimisation moves; the pipelined figures above include the device the launch shares
     with whf.Context(0, profile=True, one_stream=True) as ctx:
         _, prof1 = ctx.scan(rect, precision=best, **kw)
     rooflines_alone = {}
     for name, ms in prof1["kernel_ms"].items():
-        ops = work.get(name)
-        if ops is None or ms <= 0:
+        if name not in work or ms <= 0:
             continue
-        achieved = ops / (ms / 1e3)
-        rooflines_alone[name] = dict(achieved=round(achieved / 1e12, 3), frac=round(achieved / PEAK_VALU_OPS, 4),
-                                     launch_ms=round(ms / n_batches, 4))
-    dominant = max(fam, key=fam.get)
-    r = rooflines.get(dominant, {})
-    kernel = KERNEL_OF[dominant]
-    if dominant == "erosion_gate" and best == whf.PREC_FP32_GATE:
-        kernel = "k_erosion<4, true>"   # the FP32-gated instance: FP32 corner blend, FP64 coordinates and sums
-    roofline = dict(bound="valu", achieved=r.get("achieved"), peak=round(PEAK_VALU_OPS / 1e12, 2), unit="Tops/s",
-                    frac=r.get("frac"), traffic=traffic_from_profiles(kernel), kernel=kernel,
-                    launch_ms=round(fam[dominant] / n_batches, 4), ops_per_launch=r.get("ops_per_launch"),
-                    launch_ms_alone=rooflines_alone.get(dominant, {}).get("launch_ms"),
-                    frac_alone=rooflines_alone.get(dominant, {}).get("frac"),
-                    work_unit=WORK_UNIT.get(dominant) if kernel != "k_erosion<4, true>" else
-                    "y=0 Perlin evaluation = %d ops (FP32 corner blend, FP64 coordinates and sums); evaluations per "
-                    "region from the tier histogram" % OPS_PER_Y0_EVAL)
-    # the whole step: the algorithmic lane-ops of every counted family over the step's wall time. A batch's gates
-    # 2 and 3 run beside the next batch's builds and erosion gate, so a kernel's launch time includes the device
-    # it shares; this is the figure the overlap improves.
-    step_ops = sum(work.values())
-    step_roofline = dict(bound="valu", unit="Tops/s", peak=round(PEAK_VALU_OPS / 1e12, 2),
-                         achieved=round(step_ops / secs / 1e12, 3), frac=round(step_ops / secs / PEAK_VALU_OPS, 4),
-                         ops_per_step=int(step_ops), families=sorted(work))
+        rooflines_alone[name] = dict(roofline_of(work[name], ms / 1e3), launch_ms=round(ms / n_batches, 4))
+    # the noise kernel: the erosion gate of the faster precision
+    kernel = "k_erosion<4, true>" if fp32 else "k_erosion<4, false>"
+    evals = erosion_evals(prof)
+    r = rooflines["erosion_gate"]
+    ra = rooflines_alone["erosion_gate"]
+    # achieved in TFLOP/s of the peak named: for the FP32-gated kernel an FP64 FLOP counts as 2 FP32 FLOP (the ratio
+    # of the two published peaks), so achieved / peak = frac
+    peak = PEAK_FP32 if fp32 else PEAK_FP64
+    roofline = dict(bound="valu", achieved=round(r["frac"] * peak / 1e12, 3), peak=peak / 1e12,
+                    unit="TFLOP/s (FP32-equivalent)" if fp32 else "TFLOP/s", frac=r["frac"],
+                    traffic=traffic_from_profiles(kernel), kernel=kernel,
+                    launch_ms=r["launch_ms"], evals_per_launch=evals // n_batches,
+                    flop_per_eval={"fp32": FLOP_BLEND_PER_EVAL if fp32 else 0,
+                                   "fp64": FLOP_COORD_PER_EVAL + (0 if fp32 else FLOP_BLEND_PER_EVAL)},
+                    fp32_tflops=r["fp32_tflops"], fp64_tflops=r["fp64_tflops"], frac_fp32=r["frac_fp32"],
+                    frac_fp64=r["frac_fp64"], launch_ms_alone=ra["launch_ms"], frac_alone=ra["frac"],
+                    frac_fp32_alone=ra["frac_fp32"], frac_fp64_alone=ra["frac_fp64"],
+                    lane_ops_per_eval=OPS_PER_Y0_EVAL_LANE,
+                    lane_op_frac=round(evals * OPS_PER_Y0_EVAL_LANE / (fam["erosion_gate"] / 1e3) / PEAK_VALU_OPS, 4),
+                    work_unit="y=0 Perlin evaluation = %d FLOP corner blend (%s) + %d FLOP coordinates and "
+                              "accumulation (FP64); evaluations per launch from the tier histogram; frac = FP32 "
+                              "FLOP/s / 157.3 T + FP64 FLOP/s / 78.6 T" % (FLOP_BLEND_PER_EVAL,
+                                                                           "FP32" if fp32 else "FP64",
+                                                                           FLOP_COORD_PER_EVAL))
+    # the largest kernel by device time, the sampler build (three sets), on its own line
+    bms = sum(fam[k] for k in ("build_erosion", "build_climate", "build_ladder"))
+    bms1 = sum(prof1["kernel_ms"][k] for k in ("build_erosion", "build_climate", "build_ladder"))
+    bwork = (0, 0, sum(built) * OPS_PER_SAMPLER)
+    rb = roofline_of(bwork, bms / 1e3)
+    roofline_build = dict(bound="valu", kernel="k_build (sets E, C, L)", unit="Tops/s",
+                          peak=round(PEAK_VALU_OPS / 1e12, 2), achieved=rb["int32_tops"], frac=rb["frac"],
+                          frac_alone=roofline_of(bwork, bms1 / 1e3)["frac"], samplers=built,
+                          launch_ms=round(bms / n_batches / 3, 4), ms_per_step=round(bms, 3),
+                          ms_per_step_alone=round(bms1, 3),
+                          traffic={k: traffic_from_profiles("k_build (set %s)" % k) for k in "ECL"},
+                          work_unit="Perlin sampler built = 259 Xoroshiro128++ draws x 25 int32 lane-ops")
+    # the whole step: all counted work over the step's wall time. A batch's gates 2 and 3 run beside the next
+    # batch's builds and erosion gate, so a kernel's launch time includes the device it shares; this is the figure
+    # the overlap improves.
+    step_work = tuple(sum(w[i] for w in work.values()) for i in range(3))
+    step_roofline = dict(roofline_of(step_work, secs), families=sorted(work), bound="valu")
 
     note("profiled step done")
-    cpu = None if args.no_cpu else cpu_baseline(min(args.seeds, 1 << 20), args.cpu_seconds)
+    host = host_cpus()
+    cpu = None if args.no_cpu else cpu_baseline(min(args.seeds, 1 << 20), args.cpu_seconds, host)
     note("cpu baseline done")
+    parity = None
+    if cpu:
+        parity = cpu_parity(whf, cpu, (whf.PREC_FP64, whf.PREC_FP32_GATE))
+        note("cpu parity: %s over %d seeds" % (parity["equal"], parity["seeds"]))
+    product = None if args.no_cpu else cpu_product_mode(args.product_seeds, host["threads"])
+    note("cpu product mode done")
     srv = None if args.serve_requests <= 0 else serving(whf, args.serve_requests, args.serve_load, args.serve_slots,
                                                         args.serve_chunk)
     note("serving done")
@@ -449,7 +584,9 @@ def main():
         "fp64_rechecks": last["fp64_rechecks"],
         "fp32_error": {"stage": "erosion (8 octave samplers, y=0)", "max_abs_per_octave": err,
                        "bound_used": whf.fp32_bound()},
-        "roofline": roofline, "step_roofline": step_roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "roofline_build": roofline_build, "step_roofline": step_roofline,
+        "cpu_baseline": cpu and {k: v for k, v in cpu.items() if k != "finds"},
+        "cpu_parity": parity, "cpu_product_mode": product,
         "funnel": last["funnel"], "tiers": last["tiers"], "accepted": last["accepted"],
         "kernel_ms": {k: round(v, 3) for k, v in fam.items()}, "rooflines": rooflines,
         "rooflines_alone": rooflines_alone, "samplers_built": built,

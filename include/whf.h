@@ -351,7 +351,9 @@ int whf_server_open(const whf_server_config *cfg, whf_server **out);
 void whf_server_close(whf_server *srv);
 
 /* Queues a request (copied; for WHF_SEEDS_LIST the seeds array must stay valid until its event). The sink is
- * filled when the request completes, its hits sorted by (seed, rx, rz). Thread-safe. */
+ * filled when the request completes, its hits sorted by (seed, rx, rz). With phase 2, a request whose phase-1
+ * survivors overflow the library's survivor buffer completes with WHF_EOVERFLOW and an empty sink (count 0), as
+ * whf_scan returns WHF_EOVERFLOW without filling the sink. Thread-safe. */
 int whf_submit(whf_server *srv, const whf_request *req, whf_sink *sink, uint64_t *id);
 
 /* Queues a background Deep Dive (request copied; see whf_deepdive). Slots run its chunks of 2^18 structure seeds

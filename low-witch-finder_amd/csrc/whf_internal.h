@@ -28,7 +28,9 @@ enum Cnt {
     CNT_BUILT = 21,                      // 21 + set
     CNT_QX = 24,                         // FP32 gate: items re-evaluated in FP64
     CNT_QX_OVER = 25,                    // FP32 gate: re-evaluation queue overflow (the scan is redone in FP64)
-    CNT_SLOT_OVER = 26,                  // more gate survivors than sampler-table slots (the batch is redone smaller)
+    CNT_SLOT_OVER = 26,                  // a gate found a seed without a sampler-table slot: cannot happen in
+                                         // whf_scan (a batch holds at most kMaxSeeds seeds); the Deep Dive
+                                         // expansion, whose batches hold more, returns WHF_EOVERFLOW
     CNT_DD_DONE = 27,                    // Deep Dive layout walks: complete, cut below the line, capped
     CNT_DD_CUT = 28,
     CNT_DD_CAPPED = 29,

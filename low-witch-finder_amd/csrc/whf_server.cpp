@@ -223,8 +223,12 @@ struct whf_server {
             uint64_t s0 = 0, n = 0;
             {
                 std::unique_lock<std::mutex> lk(mu);
+                // wake for interactive work, for a Deep Dive chunk this slot may take, or to exit once stopped with
+                // nothing queued; while Deep Dive chunks are queued but the other slots hold every Deep Dive place,
+                // a stopping slot sleeps until one of them finishes its chunk (run_dd_chunk notifies)
                 work_cv.wait(lk, [&] {
-                    return stop || !jobs.empty() || (!dd_jobs.empty() && dd_running < dd_limit);
+                    return !jobs.empty() || (!dd_jobs.empty() && dd_running < dd_limit) ||
+                           (stop && dd_jobs.empty());
                 });
                 if (jobs.empty()) {
                     if (!dd_jobs.empty() && dd_running < dd_limit) {
@@ -244,8 +248,7 @@ struct whf_server {
                         run_dd_chunk(c, d, d0, dn, dd_spots, dd_finds);
                         continue;
                     }
-                    if (stop && dd_jobs.empty()) return;   // stop with nothing queued
-                    continue;
+                    return;   // stopped with nothing queued (the wait's only other way out)
                 }
                 // shortest remaining work first (ties: the older request), so small requests overtake big ones
                 auto best = jobs.begin();
@@ -281,8 +284,14 @@ struct whf_server {
                 j->done_seeds += n;
                 j->in_flight--;
                 if (j->done_seeds >= j->req.n_seeds && j->in_flight == 0) {
-                    if (j->req.phase2 != WHF_P2_NONE && j->status == WHF_OK) verdicts = true;
-                    else finish(j);
+                    if (j->req.phase2 != WHF_P2_NONE && j->status == WHF_OK) {
+                        verdicts = true;
+                    } else {
+                        // a phase-1 overflow leaves no complete survivor set to judge: as whf_scan, report
+                        // WHF_EOVERFLOW with an empty sink rather than unjudged survivors
+                        if (j->req.phase2 != WHF_P2_NONE) j->hits.clear();
+                        finish(j);
+                    }
                 }
                 work_cv.notify_all();
             }

@@ -1,22 +1,26 @@
-"""Writes csrc/gen/whf_game_data.c: the biome search trees, the terrain spline and the game's sine table, read from
-the reference's generated tables (engine/tools/lysh/src/biome_trees.h, terrain_table.h, sin_table.h) in the layout
-csrc/whf_game_data.h declares.
+"""Writes csrc/gen/whf_game_data.c, the biome search trees, the terrain spline and the game's sine table in the layout
+csrc/whf_game_data.h declares, from the committed data file data/whf_game_data.json.xz.
 
 The data are the game's (dumped from each Minecraft version by the reference's tools/accuracy/dump_trees.py, its
-TerrainTable export and its JVM dump of Mth.SIN); they have to be identical entry for entry, so they are converted,
-not retyped. The
-script checks the structure it reads (subtree ends, child counts, spline pool bounds) and fails loudly when
-the input does not parse.
+TerrainTable export and its JVM dump of Mth.SIN, shipped in the reference as engine/tools/lysh/src/biome_trees.h,
+terrain_table.h and sin_table.h under lysh's MIT licence; data/NOTICE.md gives the provenance and the licence). They
+have to be identical entry for entry, so they are converted, not retyped: with --ref the script parses the
+reference's headers, checks the structure it reads (subtree ends, child counts, spline pool bounds, sine special
+values), and writes the data file (--pack) or fails when the committed data file differs from them.
 
-    python gen_game_data.py [--ref /root/reference] [--out csrc/gen/whf_game_data.c]
+    python gen_game_data.py [--out csrc/gen/whf_game_data.c]          from the committed data file
+    python gen_game_data.py --ref /root/reference [--pack]              check (or rewrite) the data file first
 """
 import argparse
+import json
+import lzma
 import os
 import re
 import struct
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+DATA = os.path.join(HERE, "data", "whf_game_data.json.xz")
 
 
 def _ints(text):
@@ -156,7 +160,8 @@ def c_float(v):
 
 def emit(trees, versions, spline, sin_table, out):
     floats, child, nodes, roots, depth = spline
-    lines = ["/* Generated by gen_game_data.py from the reference's biome_trees.h and terrain_table.h; do not edit. */",
+    lines = ["/* Generated by gen_game_data.py from data/whf_game_data.json.xz (the game's tables; data/NOTICE.md); "
+             "do not edit. */",
              '#include "../whf_game_data.h"', ""]
     base = [0]
     for ranges, kids, biome, end, _ in trees:
@@ -197,18 +202,82 @@ def emit(trees, versions, spline, sin_table, out):
     os.replace(tmp, out)
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--ref", default=os.environ.get("WHF_REFERENCE", "/root/reference"))
-    ap.add_argument("--out", default=os.path.join(HERE, "csrc", "gen", "whf_game_data.c"))
-    a = ap.parse_args(argv)
-    src_dir = os.path.join(a.ref, "engine", "tools", "lysh", "src")
+def _bits(v):
+    return struct.unpack("<I", struct.pack("<f", v))[0]
+
+
+def _float(b):
+    return struct.unpack("<f", struct.pack("<I", b))[0]
+
+
+def to_data(trees, versions, spline, sin_table):
+    """The parsed tables as a JSON-able dict; floats as their IEEE-754 single bit patterns, so nothing rounds."""
+    floats, child, nodes, roots, depth = spline
+    return {
+        "format": 1,
+        "trees": [{"ranges": [v for r in ranges for v in r], "kids": kids, "biome": biome, "end": end, "depth": d}
+                  for ranges, kids, biome, end, d in trees],
+        "versions": [list(v) for v in versions],
+        "spline": {"floats": [_bits(v) for v in floats], "child": child,
+                   "nodes": [list(n[:6]) + [_bits(n[6])] for n in nodes], "roots": roots, "depth": depth},
+        "sin": [_bits(v) for v in sin_table],
+    }
+
+
+def from_data(d):
+    if d.get("format") != 1:
+        raise SystemExit("unknown game data format")
+    trees = []
+    for t in d["trees"]:
+        r = t["ranges"]
+        ranges = [r[12 * i:12 * i + 12] for i in range(len(t["kids"]))]
+        trees.append((ranges, t["kids"], t["biome"], t["end"], t["depth"]))
+    versions = [tuple(v) for v in d["versions"]]
+    sp = d["spline"]
+    nodes = [tuple(n[:6]) + (_float(n[6]),) for n in sp["nodes"]]
+    spline = ([_float(b) for b in sp["floats"]], sp["child"], nodes, sp["roots"], sp["depth"])
+    return trees, versions, spline, [_float(b) for b in d["sin"]]
+
+
+def read_reference(ref):
+    src_dir = os.path.join(ref, "engine", "tools", "lysh", "src")
     with open(os.path.join(src_dir, "biome_trees.h")) as f:
         trees, versions = parse_trees(f.read())
     with open(os.path.join(src_dir, "terrain_table.h")) as f:
         spline = parse_spline(f.read())
     with open(os.path.join(src_dir, "sin_table.h")) as f:
         sin_table = parse_sin(f.read())
+    return to_data(trees, versions, spline, sin_table)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--ref", help="the reference tree: check the data file against its tables")
+    ap.add_argument("--pack", action="store_true", help="with --ref: (re)write the data file from its tables")
+    ap.add_argument("--data", default=DATA)
+    ap.add_argument("--out", default=os.path.join(HERE, "csrc", "gen", "whf_game_data.c"))
+    a = ap.parse_args(argv)
+    if a.ref:
+        want = read_reference(a.ref)
+        if a.pack:
+            os.makedirs(os.path.dirname(a.data), exist_ok=True)
+            with lzma.open(a.data + ".tmp", "wt", preset=9 | lzma.PRESET_EXTREME) as f:
+                json.dump(want, f, separators=(",", ":"))
+            os.replace(a.data + ".tmp", a.data)
+            print("wrote %s" % a.data)
+        else:
+            with lzma.open(a.data, "rt") as f:
+                if json.load(f) != want:
+                    raise SystemExit("%s differs from the reference's tables (run with --pack)" % a.data)
+            print("%s equals the reference's tables" % a.data)
+    with lzma.open(a.data, "rt") as f:
+        trees, versions, spline, sin_table = from_data(json.load(f))
+    # the same structural checks as on the reference's headers
+    for t, (ranges, kids, biome, end, depth) in enumerate(trees):
+        if check_tree(t, ranges, kids, end) != depth:
+            raise SystemExit("tree %d: depth mismatch" % t)
+    if sin_table[0] != 0.0 or sin_table[16384] != 1.0 or len(sin_table) != 65536:
+        raise SystemExit("sin table: unexpected values")
     emit(trees, versions, spline, sin_table, a.out)
     print("game data: %d trees (%d nodes), %d versions, %d spline nodes (depth %d) -> %s"
           % (len(trees), sum(len(t[1]) for t in trees), len(versions), len(spline[2]), spline[4], a.out))

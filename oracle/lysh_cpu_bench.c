@@ -10,10 +10,16 @@
  *
  *   lysh_cpu_bench --stream S --count N [--offset O] --rx0 A --rx1 B --rz0 C --rz1 D
  *                  [--max-y M] [--version V] [--threads T] [--time-limit SEC] [--phase2-max-y Y [--exactly 1]]
+ *                  [--list 1]
+ *
+ * Seeds are handed out in stream order and a handed-out seed is always finished, so the seeds scanned are exactly
+ * the first `seeds` of the stream even when the time limit stops the run.
  *
  * Prints one JSON line: seeds and region checks done, seconds, checks/s, and the summed funnel
  * (reject kinds, main.c:96-100 order) and erosion tier histogram, which must equal the GPU's counters; with
- * phase 2, the huts evaluated and accepted (avg_y <= Y, or == Y with --exactly 1) and the phase-2 seconds.
+ * phase 2, the huts evaluated and accepted (avg_y <= Y, or == Y with --exactly 1), the rejections by reason
+ * (biome, floor, flood: the order of lysh's grading counters, main.c:333-345) and the phase-2 seconds; with
+ * --list 1 also every accepted hut as [seed, chunk x, chunk z, avg_y]. `cflags` and `compiler` name the build.
  */
 #include <inttypes.h>
 #include <pthread.h>
@@ -35,6 +41,11 @@ static lysh_phase1_opts g_opts;
 static long long g_next;
 static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;
 static double g_t0;
+static int g_list = 0;
+
+#ifndef LYSH_BENCH_CFLAGS
+#define LYSH_BENCH_CFLAGS "unknown"
+#endif
 
 static double now_s(void) {
     struct timespec ts;
@@ -55,7 +66,10 @@ typedef struct {
     long long reject[LYSH_P1_REJECT_KIND_N];
     long long tier[LYSH_TIER_MAX];
     long long p2_eval, p2_ok;
+    long long p2_rej[3];          /* biome, floor, flood */
     double p2_seconds;
+    long long (*finds)[4];        /* --list: seed, chunk x, chunk z, avg_y */
+    long long n_finds, cap_finds;
 } acc_t;
 
 /* phase-2 hook: the worker's accumulator rides on the scanner's per-thread context (search.h:36-45) */
@@ -70,6 +84,23 @@ static int p2_hook(void *user, lysh_search_ctx *ctx, int rx, int rz, int hut_x, 
     const int ok = r.ok && (!g_exactly || r.avg_y == g_p2_maxy);
     a->p2_eval++;
     a->p2_ok += ok;
+    if (!ok) {
+        /* the reject order of lysh's verdict (eval.c:221-224); 'exactly' misses count as floor rejections */
+        if (r.reject == LYSH_HUT_REJECT_BIOME) a->p2_rej[0]++;
+        else if (r.reject == LYSH_HUT_REJECT_FLOOD) a->p2_rej[2]++;
+        else a->p2_rej[1]++;
+    } else if (g_list) {
+        if (a->n_finds == a->cap_finds) {
+            a->cap_finds = a->cap_finds ? 2 * a->cap_finds : 64;
+            a->finds = realloc(a->finds, (size_t)a->cap_finds * sizeof *a->finds);
+            if (!a->finds) { fprintf(stderr, "out of memory\n"); exit(1); }
+        }
+        long long *f = a->finds[a->n_finds++];
+        f[0] = (long long)ctx->seed;
+        f[1] = hut_x >> 4;
+        f[2] = hut_z >> 4;
+        f[3] = r.avg_y;
+    }
     return ok;
 }
 
@@ -142,6 +173,7 @@ int main(int argc, char **argv) {
         else if (!strcmp(a, "--time-limit")) g_limit = atof(v);
         else if (!strcmp(a, "--phase2-max-y")) { g_p2 = 1; g_p2_maxy = atoi(v); }
         else if (!strcmp(a, "--exactly")) g_exactly = atoi(v);
+        else if (!strcmp(a, "--list")) g_list = atoi(v);
         else if (!strcmp(a, "--version")) { if (set_version(v)) { fprintf(stderr, "bad --version %s\n", v); return 2; } }
         else { fprintf(stderr, "unknown option %s\n", a); return 2; }
     }
@@ -160,6 +192,7 @@ int main(int argc, char **argv) {
         for (int k = 0; k < LYSH_P1_REJECT_KIND_N; k++) s.reject[k] += acc[t].reject[k];
         for (int k = 0; k < LYSH_TIER_MAX; k++) s.tier[k] += acc[t].tier[k];
         s.p2_eval += acc[t].p2_eval; s.p2_ok += acc[t].p2_ok; s.p2_seconds += acc[t].p2_seconds;
+        for (int k = 0; k < 3; k++) s.p2_rej[k] += acc[t].p2_rej[k];
     }
     double secs = now_s() - g_t0;
     printf("{\"seeds\": %lld, \"checks\": %lld, \"seconds\": %.6f, \"checks_per_s\": %.1f, \"threads\": %d, "
@@ -168,8 +201,25 @@ int main(int argc, char **argv) {
     for (int k = 0; k < LYSH_P1_REJECT_KIND_N; k++) printf("%s%lld", k ? ", " : "", s.reject[k]);
     printf("], \"tiers\": [");
     for (int k = 0; k < 7; k++) printf("%s%lld", k ? ", " : "", s.tier[k]);
-    printf("], \"phase2\": %d, \"phase2_evaluated\": %lld, \"phase2_accepted\": %lld, \"phase2_seconds\": %.6f}\n",
-           g_p2, s.p2_eval, s.p2_ok, s.p2_seconds);
+    printf("], \"phase2\": %d, \"phase2_evaluated\": %lld, \"phase2_accepted\": %lld, "
+           "\"phase2_rejected\": [%lld, %lld, %lld], \"phase2_seconds\": %.6f, \"cflags\": \"%s\", "
+           "\"compiler\": \"%s\"",
+           g_p2, s.p2_eval, s.p2_ok, s.p2_rej[0], s.p2_rej[1], s.p2_rej[2], s.p2_seconds, LYSH_BENCH_CFLAGS,
+           __VERSION__);
+    if (g_list) {
+        printf(", \"finds\": [");
+        int first = 1;
+        for (int t = 0; t < g_threads; t++) {
+            for (long long i = 0; i < acc[t].n_finds; i++) {
+                const long long *f = acc[t].finds[i];
+                printf("%s[%lld, %lld, %lld, %lld]", first ? "" : ", ", f[0], f[1], f[2], f[3]);
+                first = 0;
+            }
+            free(acc[t].finds);
+        }
+        printf("]");
+    }
+    printf("}\n");
     free(acc);
     free(th);
     return 0;

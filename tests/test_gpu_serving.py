@@ -168,3 +168,21 @@ def test_background_deepdive_and_requests_equal_alone():
         assert d["dd_stats"][k] == dd_st[k], k
     # the interactive requests were not held behind the whole Deep Dive
     assert min(got[i]["done_s"] for i in ids) < d["done_s"]
+
+
+@pytest.mark.timeout(300)
+def test_close_with_queued_deepdive_returns_without_spinning():
+    """Closing a server while a background Deep Dive still has chunks queued and every Deep Dive place is taken
+    (3 slots, at most 1 Deep Dive chunk at a time): close finishes the Deep Dive and returns, and the two idle slots
+    sleep meanwhile instead of spinning on the scheduler's lock (the process's CPU time over the close stays below
+    twice its wall time; a thread waiting on the device may spin, two more spinning threads would pass it)."""
+    s0 = O.splitmix(3, 0) & ((1 << 48) - 1)
+    srv = whf.Server(0, slots=3)
+    srv.submit_deepdive(s0, 16 << 18, (-2, 2, -2, 2), floor_max=-55, expand=True, max_height=-50)
+    time.sleep(0.05)
+    t0, c0 = time.perf_counter(), time.process_time()
+    srv.close()
+    wall, cpu = time.perf_counter() - t0, time.process_time() - c0
+    print("close: %.3f s wall, %.3f s cpu" % (wall, cpu), flush=True)
+    assert wall > 0.05, "the Deep Dive finished before close; the case was not exercised"
+    assert cpu < 2.0 * wall + 0.05
