@@ -199,6 +199,17 @@ typedef struct {
  * SolveRun (hunt.py:947-957). Not re-entrant per context. */
 int whf_deepdive(whf_ctx *ctx, const whf_dd_request *req, whf_dd_sink *spots, whf_sink *finds, whf_dd_stats *stats);
 
+/* The sweep of whf_deepdive (req->expand must be 0) without allocating: no hipMalloc, hipFree or other allocating
+ * call, host or device, inside the call. It has no counterpart in the reference; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid allocation), which open the
+ * context beforehand. Its start queue, spots and counters live in a static device array of about 8.4 MB per device,
+ * set up by whf_open; a sweep batch holds at most 2^17 mineshaft starts (batches are sized to expect half of that,
+ * more returns WHF_EOVERFLOW with a message) and at most 4096 hut regions are accepted. Spots are copied straight into
+ * `spots` (count as in whf_deepdive, WHF_EOVERFLOW when more than its capacity); stats as whf_deepdive's with expand =
+ * 0, stage times zero. Every call on a device shares that device's array: calls that overlap on one device, from any
+ * context or thread, race and must not be made. */
+int whf_deepdive_noalloc(whf_ctx *ctx, const whf_dd_request *req, whf_dd_sink *spots, whf_dd_stats *stats);
+
 /* The terrain stage of the reference's --spots mode (hut_lookup_v4_source.cu:1262-1292, the run hunt.py starts
  * per chunk of solver hits, hunt.py:954-957): the caller fills s48, cx and cz of n <= 2^20 spots (the "S cx cz"
  * lines of its spots file); the library fills in place rx = cx >> 5, rz = cz >> 5, the chunk's carver seed X and,

@@ -18,6 +18,7 @@
 #include <climits>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 
 #include "whf_internal.h"
@@ -323,6 +324,31 @@ __global__ __launch_bounds__(64) void k_dd_erosion(const uint64_t *__restrict__ 
     if (lane == 0 && built) atomicAdd(&cnt[CNT_BUILT + SET_E], built);
 }
 
+// ---------------------------------------------------------------------------------------------- no-alloc sweep
+
+// whf_deepdive_noalloc's working memory: the control words, the counters, one sweep batch's start queue and its
+// spots. A file-scope device array, so the entry point allocates nothing; every device the library runs on holds its
+// own copy (about 8.4 MB) until the process ends, and every call on a device shares it.
+constexpr uint32_t kNaStartCap = 1u << 17;      // starts per sweep batch; batches expect half of it
+constexpr uint32_t kNaSpotCap = kNaStartCap;    // a spot is a start, so the spot queue cannot overflow
+constexpr size_t kNaCtlOff = 0;                 // uint32_t[CTL_N]
+constexpr size_t kNaCntOff = 256;               // unsigned long long[CNT_N]
+constexpr size_t kNaStartsOff = 1024;           // DdStart[kNaStartCap]
+constexpr size_t kNaSpotsOff = kNaStartsOff + (((size_t)kNaStartCap * sizeof(DdStart) + 255) & ~(size_t)255);
+constexpr size_t kNaScratchBytes = kNaSpotsOff + (size_t)kNaSpotCap * sizeof(whf_dd_spot);
+static_assert(CTL_N * sizeof(uint32_t) <= kNaCntOff - kNaCtlOff, "control words fit");
+static_assert(CNT_N * sizeof(unsigned long long) <= kNaStartsOff - kNaCntOff, "counters fit");
+
+static __device__ __align__(256) uint8_t scratch[kNaScratchBytes];
+
+// Zeroes the control words and, with all != 0, the counters (a kernel, not hipMemsetAsync: callers may capture the
+// stream into a graph).
+__global__ void k_na_clear(uint32_t *__restrict__ ctl, unsigned long long *__restrict__ cnt, int all) {
+    const int t = (int)threadIdx.x;
+    if (t < CTL_N) ctl[t] = 0u;
+    if (all && t < CNT_N) cnt[t] = 0ull;
+}
+
 // ---------------------------------------------------------------------------------------------- host
 
 struct DdBufs {
@@ -475,6 +501,29 @@ int dd_finish(whf_ctx *c, bool expand, uint64_t spots_total,
 
 }  // namespace
 
+namespace whf {
+
+// The device address of `scratch` on the context's device, looked up once per device (with that device current) and
+// kept for every later context on it: in host code the array's name is only a host-side stand-in.
+int dd_scratch_init(whf_ctx *c) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void *addr[kMaxDevices] = {};
+    if (c->device < 0 || c->device >= kMaxDevices) return WHF_EDEVICE;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[c->device]) {
+        HIP_TRY(hipSetDevice(c->device));
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return WHF_EDEVICE;
+        addr[c->device] = p;
+    }
+    c->dd_scratch = addr[c->device];
+    return WHF_OK;
+}
+
+}  // namespace whf
+
 extern "C" {
 
 int whf_deepdive(whf_ctx *c, const whf_dd_request *req, whf_dd_sink *spots, whf_sink *finds, whf_dd_stats *stats) {
@@ -542,6 +591,100 @@ int whf_deepdive(whf_ctx *c, const whf_dd_request *req, whf_dd_sink *spots, whf_
     if (stats) {
         stats->structure_seeds = req->count;
         stats->tests = req->count * regions;
+    }
+    return rc;
+}
+
+int whf_deepdive_noalloc(whf_ctx *c, const whf_dd_request *req, whf_dd_sink *spots, whf_dd_stats *stats) {
+    if (!c || !req) return WHF_EINVAL;
+    g_last_error[0] = 0;
+    if (req->expand != 0) {
+        std::snprintf(g_last_error, sizeof g_last_error, "whf_deepdive_noalloc sweeps only (expand must be 0)");
+        return WHF_EINVAL;
+    }
+    if (req->rx1 <= req->rx0 || req->rz1 <= req->rz0) return WHF_EINVAL;
+    const int64_t lim = 65536;
+    if (req->rx0 < -lim || req->rx1 > lim || req->rz0 < -lim || req->rz1 > lim) return WHF_EINVAL;
+    const uint64_t regions = (uint64_t)(req->rx1 - req->rx0) * (uint64_t)(req->rz1 - req->rz0);
+    if (regions > 4096) {
+        std::snprintf(g_last_error, sizeof g_last_error, "%llu hut regions: at most 4096",
+                      (unsigned long long)regions);
+        return WHF_EINVAL;
+    }
+    if (req->count > (1ull << 48)) return WHF_EINVAL;
+    if (req->floor_max < -1000 || req->floor_max > 1000) return WHF_EINVAL;
+    if (spots && spots->capacity && !spots->spots) return WHF_EINVAL;
+    if (!c->dd_scratch) return WHF_EDEVICE;
+    HIP_TRY(hipSetDevice(c->device));
+    const hipStream_t st = c->stream;
+    uint8_t *base = static_cast<uint8_t *>(c->dd_scratch);
+    uint32_t *ctl = reinterpret_cast<uint32_t *>(base + kNaCtlOff);
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(base + kNaCntOff);
+    DdStart *starts = reinterpret_cast<DdStart *>(base + kNaStartsOff);
+    whf_dd_spot *d_spots = reinterpret_cast<whf_dd_spot *>(base + kNaSpotsOff);
+    HIP_TRY(hipEventRecord(c->ev0, st));
+    hipLaunchKernelGGL(k_na_clear, dim3(1), dim3(64), 0, st, ctl, cnt, 1);
+
+    // structure seeds per batch: about kNaStartCap / 2 expected starts (start probability 0.004 = 1/250)
+    const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(kDdSweepMax, (uint64_t)(kNaStartCap / 2) * 250 / regions));
+    const int nrx = req->rx1 - req->rx0, nrz = req->rz1 - req->rz0;
+    const int salt = req->salt ? req->salt : 14357620;
+    const int cut_rel = -62 - req->floor_max;
+    uint64_t spots_total = 0;
+    for (uint64_t done = 0; done < req->count; done += per) {
+        const uint32_t n = (uint32_t)std::min<uint64_t>(per, req->count - done);
+        if (done) hipLaunchKernelGGL(k_na_clear, dim3(1), dim3(64), 0, st, ctl, cnt, 0);
+        hipLaunchKernelGGL(k_dd_sweep, dim3((n + 255) / 256), dim3(256), 0, st, (req->s_first + done) & kLcgM48, n,
+                           req->rx0, nrx, req->rz0, nrz, salt, starts, kNaStartCap, ctl);
+        uint32_t n_starts = 0;
+        HIP_TRY(hipMemcpyAsync(&n_starts, ctl + CTL_DD_STARTS, sizeof n_starts, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (int rc = check_launch()) return rc;
+        if (n_starts > kNaStartCap) {
+            std::snprintf(g_last_error, sizeof g_last_error, "start queue overflow (%u starts in one batch, at most %u)",
+                          n_starts, kNaStartCap);
+            return WHF_EOVERFLOW;
+        }
+        const uint32_t groups = 64 / kDdGroup;
+        const uint32_t grid = std::max(1u, std::min((n_starts + groups - 1) / groups, (uint32_t)c->n_cu * 16));
+        hipLaunchKernelGGL(k_dd_floor<kDdGroup>, dim3(grid), dim3(64), 0, st, starts, ctl, kNaStartCap, cut_rel,
+                           req->floor_max, d_spots, kNaSpotCap, ctl, cnt);
+        uint32_t ns = 0;
+        HIP_TRY(hipMemcpyAsync(&ns, ctl + CTL_DD_SPOTS, sizeof ns, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (int rc = check_launch()) return rc;
+        // straight into the caller's sink, as far as it holds them
+        if (spots && spots_total < spots->capacity && ns) {
+            const uint64_t m = std::min<uint64_t>(ns, spots->capacity - spots_total);
+            HIP_TRY(hipMemcpyAsync(spots->spots + spots_total, d_spots, m * sizeof(whf_dd_spot), hipMemcpyDeviceToHost,
+                                   st));
+            HIP_TRY(hipStreamSynchronize(st));
+        }
+        spots_total += ns;
+    }
+    HIP_TRY(hipEventRecord(c->ev1, st));
+    unsigned long long h[CNT_N];
+    HIP_TRY(hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    int rc = WHF_OK;
+    if (spots) {
+        spots->count = spots_total;
+        if (spots_total > spots->capacity) rc = WHF_EOVERFLOW;
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->structure_seeds = req->count;
+        stats->tests = req->count * regions;
+        stats->walks[0] = h[CNT_DD_DONE];
+        stats->walks[1] = h[CNT_DD_CUT];
+        stats->walks[2] = h[CNT_DD_CAPPED];
+        stats->starts = stats->walks[0] + stats->walks[1] + stats->walks[2];
+        stats->spots = spots_total;
+        stats->draws = h[CNT_DD_DRAWS];
+        stats->pieces = h[CNT_DD_PIECES];
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        stats->device_ms = ms;
     }
     return rc;
 }

@@ -118,6 +118,7 @@ struct whf_ctx {
     whf::GateTables *d_gt = nullptr;      // [kLayouts]
     whf::BuildDesc *d_bd = nullptr;       // [kLayouts][8 + 28 + 36]
     void *d_dd = nullptr;                 // Deep Dive buffers (whf_deepdive.hip), allocated on first use
+    void *dd_scratch = nullptr;           // device address of whf_deepdive_noalloc's static scratch on this device
     size_t dd_bytes = 0;
     void *d_bio = nullptr;                // biome stage buffers and tables (whf_biome.hip), allocated on first use
     size_t bio_bytes = 0;
@@ -193,6 +194,9 @@ int terrain_heights(whf_ctx *c, const whf_column *cols, uint64_t n, int vflags, 
 // Gates 2 and 3 of a batch whose erosion survivors are already in Q1 (with needC set): compact, build set C,
 // gate 2, compact, build set L, gate 3 -> hits.
 int run_tail(whf_ctx *c, int layout, uint32_t ns, int vflags, int max_height);
+// Looks up (once per device) the device address of whf_deepdive_noalloc's static scratch array and keeps it in the
+// context (whf_deepdive.hip).
+int dd_scratch_init(whf_ctx *c);
 // Resets a batch set's control words and the seed flags of its first ns seeds, on the library's stream.
 void reset_set(whf_ctx *c, const BatchSet &b, uint32_t ns);
 

@@ -1223,6 +1223,10 @@ int whf_open(const whf_config *cfg, whf_ctx **out) {
         whf_close(c);
         return rc == WHF_EDEVICE ? WHF_ENOMEM : rc;
     }
+    if ((rc = dd_scratch_init(c)) != WHF_OK) {
+        whf_close(c);
+        return rc;
+    }
     *out = c;
     return WHF_OK;
 }

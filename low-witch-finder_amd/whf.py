@@ -24,7 +24,8 @@ CFG_ONE_STREAM = 2   # every stage of a scan on one stream (no batch pipeline)
 
 # Exported symbols of include/whf.h, checked by the tests.
 SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags", "whf_strerror",
-           "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_dd_terrain", "whf_room_floors",
+           "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_deepdive_noalloc", "whf_dd_terrain",
+           "whf_room_floors",
            "whf_server_open", "whf_server_close", "whf_submit", "whf_submit_deepdive", "whf_poll",
            "whf_biome_tree", "whf_biome", "whf_surface_heights", "whf_post_carver_heights", "whf_hut_eval")
 HIT_DEEPDIVE, HIT_PHASE2 = 1, 2
@@ -176,6 +177,9 @@ def load():
         L.whf_deepdive.restype = ctypes.c_int
         L.whf_deepdive.argtypes = [ctypes.c_void_p, ctypes.POINTER(DdRequest), ctypes.POINTER(DdSink),
                                    ctypes.POINTER(Sink), ctypes.POINTER(DdStats)]
+        L.whf_deepdive_noalloc.restype = ctypes.c_int
+        L.whf_deepdive_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(DdRequest), ctypes.POINTER(DdSink),
+                                           ctypes.POINTER(DdStats)]
         L.whf_dd_terrain.restype = ctypes.c_int
         L.whf_dd_terrain.argtypes = [ctypes.c_void_p, ctypes.POINTER(DdRequest), ctypes.POINTER(DdSpot), ctypes.c_uint64,
                                      ctypes.POINTER(Sink), ctypes.POINTER(DdStats)]
@@ -318,6 +322,21 @@ class Context:
                        for p in sbuf[:min(ssink.count, spot_capacity)])
         finds = sorted((h.seed, h.rx, h.rz, h.cx, h.cz, h.floor_y) for h in fbuf[:min(fsink.count, find_capacity)])
         return spots, finds, _dd_stats(st, rc, fsink)
+
+    def deepdive_noalloc(self, s_first, count, rect, floor_max=-55, salt=0, spot_capacity=1 << 16):
+        """The Deep Dive sweep through whf_deepdive_noalloc (no allocation in the call). Returns (spots as in
+        deepdive, stats dict)."""
+        req = DdRequest(s_first & 0xFFFFFFFFFFFFFFFF, count, rect[0], rect[1], rect[2], rect[3], floor_max, 0, -50, 0,
+                        salt, 0)
+        sbuf = (DdSpot * max(1, spot_capacity))()
+        ssink = DdSink(sbuf, spot_capacity, 0)
+        st = DdStats()
+        rc = self.L.whf_deepdive_noalloc(self.h, ctypes.byref(req), ctypes.byref(ssink), ctypes.byref(st))
+        if rc not in (OK, EOVERFLOW):
+            raise WhfError(rc, "whf_deepdive_noalloc")
+        spots = sorted((p.s48, p.rx, p.rz, p.cx, p.cz, p.carver, p.room_floor, p.pieces)
+                       for p in sbuf[:min(ssink.count, spot_capacity)])
+        return spots, _dd_stats(st, rc, Sink(None, 0, 0))
 
     def dd_terrain(self, spots, max_height=-50, vflags=0, find_capacity=1 << 16):
         """The --spots terrain stage: spots are (s48, cx, cz). Returns (the spots filled in, as tuples
