@@ -379,11 +379,41 @@ int whf_submit_deepdive(whf_server *srv, const whf_dd_request *req, whf_dd_sink 
  * or a negative error code. Thread-safe. */
 int whf_poll(whf_server *srv, whf_event *events, int max_events, int timeout_ms);
 
+/* ---- Tuning: launch shapes chosen per device ---- */
+
+/* The launch shapes the library leaves open. whf_open starts from the built-in defaults, then applies the entry for
+ * its device (whf_device_name without the arch suffix, or "default") from the JSON file named by the WHF_TUNED
+ * environment variable, or else from tuned.json beside libwhf_hip.so, if one exists; low-witch-finder_amd/tune.py
+ * measures the candidates on the device and writes that file. No counterpart in the reference's engine, whose launch
+ * shapes are constants; hunt.py keeps a tuned configuration per GPU the same way (hunt.py:927-945, 959-1021). Every
+ * choice gives the same answers; only the speed changes. A field of 0 means the default. */
+typedef struct {
+    int ero_wg;                 /* erosion gate workgroup: 256 or 512 lanes (default 256) */
+    int ero_ilp;                /* FP32-gated erosion gate: items evaluated at once per lane, 1 or 2 (default 2;
+                                   512-lane workgroups take 1) */
+    int ero_wg_per_cu;          /* erosion gate: persistent workgroups per compute unit, 1..64, or -1 for one
+                                   workgroup per tile (default -1) */
+    int gate_wg_per_cu;         /* workgroups per compute unit of the persistent gate-2 and gate-3 kernels, 1..32
+                                   (default 8) */
+    uint64_t chunk_checks;      /* serving: (seed, region) checks per scheduled chunk when whf_server_config leaves
+                                   it 0 (default 2^24) */
+    int reserved[8];
+} whf_tuning;
+
+/* The context's launch shapes. */
+int whf_get_tuning(whf_ctx *ctx, whf_tuning *out);
+/* Sets the context's launch shapes (0 fields take the defaults); WHF_EINVAL for a value out of range. */
+int whf_set_tuning(whf_ctx *ctx, const whf_tuning *t);
+
 /* Version name -> WHF_V_* flags, accepting lysh's version names (main.c:104-125, biome_trees.h:70499-70526). */
 int whf_version_flags(const char *name, int *vflags);
 
 /* Text of a return code. */
 const char *whf_strerror(int code);
+
+/* Identifier of this library build (a digest of its sources): tuned.json entries record the build they were
+ * measured with, and whf_open applies only an entry measured with its own build. */
+const char *whf_build_id(void);
 
 /* Name of the device the context runs on (for logs), or "" when unknown. */
 const char *whf_device_name(whf_ctx *ctx);

@@ -328,79 +328,127 @@ __device__ __forceinline__ void perlin_y0_x2(const Pairs *Q, const SPar *const *
 }
 
 // ---------------------------------------------------------------------------------------------- FP32 gate core
-// The same two evaluations with the lattice part (origin add, floor, fractions) in FP64 and the corner blend in
-// FP32: the fractions are rounded to float once, then gradient dots, fades and lerps run in single precision.
-// Worst-case error against the FP64 value, term by term (absolute):
-//   float inputs (fractions, hy, hy - 1, fade(hy)) are within 2^-25 of their doubles;
-//   a gradient dot (|.| <= 2): 2 x 2^-25 propagated + 2^-24 rounding                     <= 1.2e-7;
-//   fade(t) on [0, 1]: |f'| <= 1.875 on the input, six roundings at magnitudes <= 15    <= 1.6e-6;
-//   first lerp level: fade error x |b - a| <= 4, plus the dots' errors and roundings    <= 7.4e-6;
-//   second level (y fade, exact to 2^-25):                                              <= 8.0e-6;
-//   third level: 8.0e-6 + fade error x 4 + roundings                                     <= 1.5e-5.
+// The same two evaluations for the FP32 gate. The lattice part (origin add, floor, fraction) stays FP64, so the cell
+// is the reference's. The fractional coordinates are then truncated to 23-bit fixed point, X = floor(x * 2^23), and
+// each corner's gradient dot +-X +-Y (or Z) is an exact integer: the gradient's three components come from 2-bit
+// fields of three 32-bit constants, indexed by the hash, and meet the coordinates in 24-bit multiply-adds. A dot is
+// at most 2^24 in magnitude, so it converts to float exactly. Fades and lerps run in FP32 (with FMA) on the dots
+// scaled by 2^23, which scales the result by 2^23 exactly; the caller's weights carry the 2^-23.
+// Worst-case error against the FP64 value (absolute, unscaled):
+//   fixed-point coordinates: each within 2^-23 below its double, so a dot is within 2^-22 = 2.4e-7;
+//   fade(t) from the float of t (within 2^-25): |f'| <= 1.875 on the input, five roundings    <= 1.7e-6;
+//   first lerp level: dots' error + rounding of b - a and of the FMA + fade error x 4           <= 7.3e-6;
+//   second level (y fade, the float of the FP64 fade, within 2^-25 x 4):                        <= 7.6e-6;
+//   third level: 7.6e-6 + roundings + fade error x 4                                            <= 1.5e-5.
 // The gate uses four times that, kFp32PerlinBound = 2^-14, as the margin around every threshold; the measured
 // maximum (whf_noise_error) is reported beside it.
 constexpr double kFp32PerlinBound = 6.103515625e-05;   // 2^-14
+constexpr int kFixBits = 23;
+constexpr double kFixOne = 8388608.0;                   // 2^23
+constexpr double kFixInv = 1.0 / 8388608.0;             // 2^-23, exact
 
-__device__ __forceinline__ float grad_sel_f(uint32_t h, float x, float y, float z) {
-    h &= 15u;
-    float u = h < 8u ? x : y;
-    float v = h < 4u ? y : ((h == 12u || h == 14u) ? x : z);
-    if (h & 1u) u = -u;
-    if (h & 2u) v = -v;
-    return u + v;
+// What the FP32 gate needs of a y = 0 sampler: origin x and z, the lattice y, the fixed-point y offsets of the two
+// corner rows (hy and hy - 1) and the y fade as a float.
+struct EPar {
+    double ox, oz;
+    int jy, hq, hmq;
+    float fy;
+};
+static_assert(sizeof(EPar) == 32, "EPar layout");
+
+__device__ __forceinline__ EPar make_epar(const SPar &s) {
+    EPar e;
+    e.ox = s.ox;
+    e.oz = s.oz;
+    e.jy = s.jy;
+    e.hq = (int)(s.hy * kFixOne);   // hy in [0, 1): truncation = floor
+    e.hmq = e.hq - (1 << kFixBits);
+    e.fy = (float)s.fy;
+    return e;
 }
-__device__ __forceinline__ float fade_f(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
-__device__ __forceinline__ float lerp_f(float t, float a, float b) { return a + t * (b - a); }
 
-template <typename Pairs>
-__device__ __forceinline__ void perlin_y0_x2_f32(const Pairs *Q, const SPar *const *s, const double *x,
-                                                 const double *z, double *out) {
-    float g[2], l[2], gm[2], lm[2];
-    int i[2], k[2];
-    uint32_t ab[2], cd[2], ef[2];
+// The gradient table of kGrad as 2-bit two's-complement fields (bits 2h, 2h + 1 of c[axis]).
+struct GradLut {
+    uint32_t c[3];
+};
+constexpr GradLut make_grad_lut() {
+    GradLut l{{0, 0, 0}};
+    for (int h = 0; h < 16; h++)
+        for (int a = 0; a < 3; a++) {
+            const bool neg = (kGrad.neg[a] >> h) & 1u, zero = (kGrad.zero[a] >> h) & 1u;
+            const uint32_t f = zero ? 0u : (neg ? 3u : 1u);
+            l.c[a] |= f << (2 * h);
+        }
+    return l;
+}
+constexpr GradLut kGradLut = make_grad_lut();
+
+// Gradient dot of hash h (off = 2 * (h & 15) in its low 5 bits) with fixed-point (X, Y, Z): exact.
+__device__ __forceinline__ int grad_q(uint32_t off, int X, int Y, int Z) {
+    const int gx = __builtin_amdgcn_sbfe((int)kGradLut.c[0], off, 2u);
+    const int gy = __builtin_amdgcn_sbfe((int)kGradLut.c[1], off, 2u);
+    const int gz = __builtin_amdgcn_sbfe((int)kGradLut.c[2], off, 2u);
+    return __mul24(gx, X) + __mul24(gy, Y) + __mul24(gz, Z);
+}
+__device__ __forceinline__ float fade_q(float t) {
+    const float p = __builtin_fmaf(t, __builtin_fmaf(t, 6.0f, -15.0f), 10.0f);
+    return t * t * t * p;
+}
+__device__ __forceinline__ float lerp_q(float t, float a, float b) { return __builtin_fmaf(t, b - a, a); }
+
+// N independent y = 0 evaluations (pair tables Q[t], samplers s[t], points (x[t], z[t])) scaled by 2^23, written as
+// lock-step lanes of N-arrays so the compiler interleaves their dependency chains.
+template <int N, typename Pairs>
+__device__ __forceinline__ void perlin_y0_xn_q(const Pairs *Q, const EPar *const *s, const double *x, const double *z,
+                                               double *out) {
+    int X[N], Z[N], i[N], k[N];
+    float gf[N], lf[N];
+    uint32_t ab[N], cd[N], ef[N];
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
+    for (int t = 0; t < N; t++) {
         const double d = x[t] + s[t]->ox, f = z[t] + s[t]->oz;
         const double fi = floor(d), fk = floor(f);
         i[t] = (int)fi;
         k[t] = (int)fk;
         const double gd = d - fi, ld = f - fk;
-        g[t] = (float)gd;
-        l[t] = (float)ld;
-        gm[t] = (float)(gd - 1);
-        lm[t] = (float)(ld - 1);
+        X[t] = (int)(gd * kFixOne);
+        Z[t] = (int)(ld * kFixOne);
+        gf[t] = (float)gd;
+        lf[t] = (float)ld;
     }
 #pragma unroll
-    for (int t = 0; t < 2; t++) ab[t] = Q[t](i[t]);
+    for (int t = 0; t < N; t++) ab[t] = Q[t](i[t]);
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
+    for (int t = 0; t < N; t++) {
         const int j = s[t]->jy;
         cd[t] = Q[t]((int)(ab[t] & 255u) + j);
         ef[t] = Q[t]((int)(ab[t] >> 8) + j);
     }
-    uint32_t h0[2], h1[2], h2[2], h3[2];
+    uint32_t h0[N], h1[N], h2[N], h3[N];
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-        h0[t] = Q[t]((int)(cd[t] & 255u) + k[t]);
-        h1[t] = Q[t]((int)(ef[t] & 255u) + k[t]);
-        h2[t] = Q[t]((int)(cd[t] >> 8) + k[t]);
-        h3[t] = Q[t]((int)(ef[t] >> 8) + k[t]);
+    for (int t = 0; t < N; t++) {
+        h0[t] = Q[t]((int)(cd[t] & 255u) + k[t]);   // P(c+k),  P(c+k+1)
+        h1[t] = Q[t]((int)(ef[t] & 255u) + k[t]);   // P(e+k),  P(e+k+1)
+        h2[t] = Q[t]((int)(cd[t] >> 8) + k[t]);     // P(d+k),  P(d+k+1)
+        h3[t] = Q[t]((int)(ef[t] >> 8) + k[t]);     // P(f+k),  P(f+k+1)
     }
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-        const float h = (float)s[t]->hy, hm = (float)s[t]->hy1, fy = (float)s[t]->fy;
-        const float c0 = grad_sel_f(h0[t], g[t], h, l[t]);
-        const float c1 = grad_sel_f(h1[t], gm[t], h, l[t]);
-        const float c2 = grad_sel_f(h2[t], g[t], hm, l[t]);
-        const float c3 = grad_sel_f(h3[t], gm[t], hm, l[t]);
-        const float c4 = grad_sel_f(h0[t] >> 8, g[t], h, lm[t]);
-        const float c5 = grad_sel_f(h1[t] >> 8, gm[t], h, lm[t]);
-        const float c6 = grad_sel_f(h2[t] >> 8, g[t], hm, lm[t]);
-        const float c7 = grad_sel_f(h3[t] >> 8, gm[t], hm, lm[t]);
-        const float fx = fade_f(g[t]), fz = fade_f(l[t]);
-        const float lo = lerp_f(fy, lerp_f(fx, c0, c1), lerp_f(fx, c2, c3));
-        const float hi = lerp_f(fy, lerp_f(fx, c4, c5), lerp_f(fx, c6, c7));
-        out[t] = (double)lerp_f(fz, lo, hi);
+    for (int t = 0; t < N; t++) {
+        const int Y = s[t]->hq, Ym = s[t]->hmq;
+        const int Xm = X[t] - (1 << kFixBits), Zm = Z[t] - (1 << kFixBits);
+        // low byte of a pair: hash bits at 0..3, field offset 2h; high byte: bits 8..11
+        const float c0 = (float)grad_q(h0[t] << 1, X[t], Y, Z[t]);
+        const float c1 = (float)grad_q(h1[t] << 1, Xm, Y, Z[t]);
+        const float c2 = (float)grad_q(h2[t] << 1, X[t], Ym, Z[t]);
+        const float c3 = (float)grad_q(h3[t] << 1, Xm, Ym, Z[t]);
+        const float c4 = (float)grad_q((h0[t] >> 7) & 30u, X[t], Y, Zm);
+        const float c5 = (float)grad_q((h1[t] >> 7) & 30u, Xm, Y, Zm);
+        const float c6 = (float)grad_q((h2[t] >> 7) & 30u, X[t], Ym, Zm);
+        const float c7 = (float)grad_q((h3[t] >> 7) & 30u, Xm, Ym, Zm);
+        const float fx = fade_q(gf[t]), fz = fade_q(lf[t]), fy = s[t]->fy;
+        const float lo = lerp_q(fy, lerp_q(fx, c0, c1), lerp_q(fx, c2, c3));
+        const float hi = lerp_q(fy, lerp_q(fx, c4, c5), lerp_q(fx, c6, c7));
+        out[t] = (double)lerp_q(fz, lo, hi);
     }
 }
 

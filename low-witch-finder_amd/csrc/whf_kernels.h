@@ -29,6 +29,7 @@ struct NoiseDesc {
 // (= amp * persistence of its slot); thr[t] is checked after evaluation t + 1 (noise.c:359-459).
 struct EroDesc {
     double sc[8], lac[8], w[8];
+    double w23[8];      // w * 2^-23: the weights of the FP32 gate, whose values come scaled by 2^23 (exact)
     double amplitude;
     double thr[7];
     double margin[8];   // FP32 gate: bound on |estimate - FP64 value| of sum * amplitude after evaluation k

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <dlfcn.h>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -25,6 +27,8 @@
 #include <new>
 #include <string>
 #include <vector>
+
+#include <type_traits>
 
 #include "whf_internal.h"
 
@@ -107,9 +111,9 @@ struct LPair {
     __device__ __forceinline__ uint32_t operator()(int i) const { return p[i & 255]; }
 };
 
-constexpr int kEroWG = 256;
-// Items per pass (4 per lane): a whole tile of 4 seeds x 256 regions, so that levels 2 and 3 still fill whole waves
-// (measured against 512, 768, 1536 and 2048: 1024 is fastest; 768 splits such a tile into a full pass and a thin one)
+// Items per pass: a whole tile of 4 seeds x 256 regions, so that levels 2 and 3 still fill whole waves (measured
+// against 512, 768, 1536 and 2048: 1024 is fastest; 768 splits such a tile into a full pass and a thin one). The
+// workgroup size (256 or 512 lanes, 4 or 2 items per lane) is a tuning choice (whf_tuning.ero_wg).
 constexpr int kEroPass = 1024;
 constexpr int kCzBias = 1 << 22;       // chunk z is stored biased so that 4 tile bits fit above it
 
@@ -139,10 +143,39 @@ __device__ __forceinline__ void push_recheck(bool pred, uint32_t slot, int cx, i
     }
 }
 
+// Evaluations k0 and k0 + 1 of IL items (tile seed sl[u], point (x[u], z[u])): v[2u] and v[2u + 1]. The 2 * IL
+// evaluations are independent, so their lookup chains interleave.
+template <bool F32, int IL, typename Par>
+__device__ __forceinline__ void ero_evals(const uint16_t *s_tab, const Par *s_par, const EroDesc &ed, int k0,
+                                          const uint32_t *sl, const double *x, const double *z, double *v) {
+    const int k1 = k0 + 1;
+    LPair Q[2 * IL];
+    const Par *P[2 * IL];
+    double xs[2 * IL], zs[2 * IL];
+#pragma unroll
+    for (int u = 0; u < IL; u++) {
+        Q[2 * u] = LPair{s_tab + (sl[u] * 8 + k0) * 256};
+        Q[2 * u + 1] = LPair{s_tab + (sl[u] * 8 + k1) * 256};
+        P[2 * u] = &s_par[sl[u] * 8 + k0];
+        P[2 * u + 1] = &s_par[sl[u] * 8 + k1];
+        xs[2 * u] = x[u] * ed.sc[k0] * ed.lac[k0];
+        xs[2 * u + 1] = x[u] * ed.sc[k1] * ed.lac[k1];
+        zs[2 * u] = z[u] * ed.sc[k0] * ed.lac[k0];
+        zs[2 * u + 1] = z[u] * ed.sc[k1] * ed.lac[k1];
+        if (!(fabs(x[u]) < ed.coord_lim && fabs(z[u]) < ed.coord_lim)) {
+            for (int t = 2 * u; t < 2 * u + 2; t++) { xs[t] = keep_precision(xs[t]); zs[t] = keep_precision(zs[t]); }
+        }
+    }
+    if constexpr (F32) perlin_y0_xn_q<2 * IL>(Q, P, xs, zs, v);
+    else perlin_y0_x2(Q, P, xs, zs, v);   // the plain kernel takes one item per lane
+}
+
 // ST seeds per workgroup. Items of a pass are (seed-in-tile, region) pairs laid out seed-major. F32 selects the
-// FP32-gated corner blend (perlin_y0_x2_f32) with FP64 re-evaluation of undecided items.
-template <int ST, bool F32>
-__global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__ seeds, uint32_t n_seeds,
+// FP32-gated fixed-point corner blend (perlin_y0_xn_q, values scaled by 2^23, weights w23) with FP64 re-evaluation of
+// undecided items. IL items per lane are evaluated at once (the FP32 gate's evaluations are short, and without a
+// second item their LDS lookup chains leave the device waiting; IL = 1 for the plain kernel).
+template <int ST, bool F32, int WG, int IL>
+__global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ seeds, uint32_t n_seeds,
                                                     const uint8_t *__restrict__ tabE, const SPar *__restrict__ parE,
                                                     const GateTables *__restrict__ gt, int rx0, int rz0, FastDiv Wd,
                                                     FastDiv Rd, uint64_t flat0, uint32_t flat_n, uint32_t chunk,
@@ -150,17 +183,29 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
                                                     uint4 *__restrict__ q1, uint32_t *__restrict__ ctl,
                                                     uint32_t *__restrict__ needC, uint4 *__restrict__ qx,
                                                     unsigned long long *__restrict__ cnt) {
+    static_assert(F32 || IL == 1, "the plain kernel evaluates one item per lane");
     __shared__ __align__(16) uint16_t s_tab[ST * 8 * 256];   // byte-pair tables: P(i) | P(i + 1) << 8
-    __shared__ SPar s_par[ST * 8];
+    using Par = typename std::conditional<F32, EPar, SPar>::type;
+    __shared__ Par s_par[ST * 8];
+    __shared__ uint64_t s_seed[ST];
     __shared__ int2 s_item[kEroPass];
-    __shared__ uint16_t s_li[kEroPass];          // alive list, compacted in place: item index and running sum
-    __shared__ double s_ls[kEroPass];
+    // alive list, compacted in place: item index and running sum (the FP32 gate keeps its sum as a float between
+    // levels, which its margins allow for, and so fits five workgroups per CU)
+    using Sum = typename std::conditional<F32, float, double>::type;
+    __shared__ uint16_t s_li[kEroPass];
+    __shared__ Sum s_ls[kEroPass];
     __shared__ uint32_t s_n[4];                  // list sizes after levels 0, 1, 2
     __shared__ uint32_t s_cnt[8];
 
     const uint32_t tid = threadIdx.x;
-    const uint32_t tile = blockIdx.x / n_chunks;
-    const uint32_t ck = blockIdx.x - tile * n_chunks;
+    const uint32_t lane = lane_id();
+    if (tid < 8) s_cnt[tid] = 0;
+    // persistent workgroups: each takes blocks (tile, region chunk) blockIdx.x, + gridDim.x, ... in turn; the barrier
+    // at the end of a block's last pass keeps its tables until every wave is done with them
+    const uint32_t n_blocks = ((n_seeds + ST - 1) / ST) * n_chunks;
+    for (uint32_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    const uint32_t tile = blk / n_chunks;
+    const uint32_t ck = blk - tile * n_chunks;
     const uint32_t seed0 = tile * ST;
     const uint32_t ns = min((uint32_t)ST, n_seeds - seed0);
     const uint32_t rbase = ck * chunk;
@@ -173,7 +218,7 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
         // 16-byte load, and the byte after them (the next 16 bytes' first, or the table's first after its last), and
         // writes their 16 pairs
         const uint8_t *src = tabE + (size_t)seed0 * 8 * 256;
-        for (uint32_t q = tid; q < ns * 8 * 16; q += kEroWG) {
+        for (uint32_t q = tid; q < ns * 8 * 16; q += WG) {
             const uint32_t t = q >> 4, c = q & 15;
             const uint4 v = reinterpret_cast<const uint4 *>(src)[q];
             const uint32_t w[5] = {v.x, v.y, v.z, v.w, (uint32_t)src[(t << 8) | (((c + 1) & 15) << 4)]};
@@ -188,12 +233,16 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
             dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
             dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
         }
-        for (uint32_t w = tid; w < ns * 8; w += kEroWG) s_par[w] = parE[(size_t)seed0 * 8 + w];
-        if (tid < 8) s_cnt[tid] = 0;
+        for (uint32_t w = tid; w < ns * 8; w += WG) {
+            if constexpr (F32) s_par[w] = make_epar(parE[(size_t)seed0 * 8 + w]);
+            else s_par[w] = parE[(size_t)seed0 * 8 + w];
+        }
+        if (tid < ns) s_seed[tid] = seeds[seed0 + tid];
         if (tid < 4) s_n[tid] = 0;
     }
     const EroDesc &ed = gt->ero;
     const double amplitude = ed.amplitude;
+    const double *w = F32 ? ed.w23 : ed.w;   // the FP32 gate's values are scaled by 2^23
     // first region of this chunk as (row, column); a region offset ro then needs one 32-bit division
     const uint64_t fbase = flat0 + rbase;
     const uint32_t row0 = (uint32_t)(fbase / W);
@@ -203,67 +252,73 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
     const uint64_t total = (uint64_t)ns * rn;
     for (uint64_t pass0 = 0; pass0 < total; pass0 += kEroPass) {
         const uint32_t npass = (uint32_t)min((uint64_t)kEroPass, total - pass0);
-        // ---- level 0: placement, evaluations 0 and 1, tier 0 ----
-        for (uint32_t i0 = tid & ~63u; i0 < npass; i0 += kEroWG) {
-            const uint32_t i = i0 + lane_id();
-            bool alive = false, undecided = false;
-            double sum = 0.0;
-            int cx = 0, cz = 0;
-            uint32_t sl = 0;
-            if (i < npass) {
-                const uint32_t it = (uint32_t)(pass0 + i);
-                sl = Rn.m ? Rn.div(it) : it / rn;
-                const uint32_t c = col0 + (it - sl * rn);
-                const uint32_t drow = Wd.div(c);
-                hut_chunk(seeds[seed0 + sl], rx0 + (int)(c - drow * W), rz0 + (int)(row0 + drow), salt, cx, cz);
-                s_item[i] = make_int2(cx, (cz + kCzBias) | (int)(sl << 23));
-                if (single) {
-                    alive = true;
-                } else {
-                    const double x = (double)(cx * 16 + 8) / 4.0, z = (double)(cz * 16 + 8) / 4.0;
-                    const LPair Q[2] = {LPair{s_tab + (sl * 8 + 0) * 256}, LPair{s_tab + (sl * 8 + 1) * 256}};
-                    const SPar *P[2] = {&s_par[sl * 8 + 0], &s_par[sl * 8 + 1]};
-                    const bool raw = fabs(x) < ed.coord_lim && fabs(z) < ed.coord_lim;
-                    double xs[2] = {x * ed.sc[0] * ed.lac[0], x * ed.sc[1] * ed.lac[1]};
-                    double zs[2] = {z * ed.sc[0] * ed.lac[0], z * ed.sc[1] * ed.lac[1]};
-                    if (!raw) {
-                        for (int t = 0; t < 2; t++) { xs[t] = keep_precision(xs[t]); zs[t] = keep_precision(zs[t]); }
-                    }
-                    double v[2];
-                    if (F32) perlin_y0_x2_f32(Q, P, xs, zs, v);
-                    else perlin_y0_x2(Q, P, xs, zs, v);
-                    sum += ed.w[0] * v[0];
-                    sum += ed.w[1] * v[1];
-                    const int dcs = tier_decide<F32>(sum * amplitude, ed.thr[0], ed.margin[1]);
+        // ---- level 0: placement, evaluations 0 and 1, tier 0; IL items per lane ----
+        for (uint32_t i0 = tid & ~63u; i0 < npass; i0 += WG * IL) {
+            uint32_t i[IL], sl[IL];
+            int cx[IL], cz[IL];
+            bool in[IL], alive[IL], undecided[IL];
+            double sum[IL], x[IL], z[IL];
+#pragma unroll
+            for (int u = 0; u < IL; u++) {
+                i[u] = i0 + (uint32_t)u * WG + lane;
+                in[u] = i[u] < npass;
+                alive[u] = undecided[u] = false;
+                sum[u] = 0.0;
+                cx[u] = cz[u] = 0;
+                sl[u] = 0;
+                if (in[u]) {
+                    const uint32_t it = (uint32_t)(pass0 + i[u]);
+                    sl[u] = Rn.m ? Rn.div(it) : it / rn;
+                    const uint32_t c = col0 + (it - sl[u] * rn);
+                    const uint32_t drow = Wd.div(c);
+                    hut_chunk(s_seed[sl[u]], rx0 + (int)(c - drow * W), rz0 + (int)(row0 + drow), salt, cx[u], cz[u]);
+                    s_item[i[u]] = make_int2(cx[u], (cz[u] + kCzBias) | (int)(sl[u] << 23));
+                }
+                x[u] = (double)(cx[u] * 16 + 8) / 4.0;
+                z[u] = (double)(cz[u] * 16 + 8) / 4.0;
+            }
+            if (single) {
+#pragma unroll
+                for (int u = 0; u < IL; u++) alive[u] = in[u];
+            } else {
+                double v[2 * IL];
+                ero_evals<F32, IL>(s_tab, s_par, ed, 0, sl, x, z, v);
+#pragma unroll
+                for (int u = 0; u < IL; u++) {
+                    if (!in[u]) continue;
+                    sum[u] += w[0] * v[2 * u];
+                    sum[u] += w[1] * v[2 * u + 1];
+                    const int dcs = tier_decide<F32>(sum[u] * amplitude, ed.thr[0], ed.margin[1]);
                     if (dcs == 0) atomicAdd(&s_cnt[0], 1u);
-                    else if (dcs == 1) alive = true;
-                    else undecided = true;
+                    else if (dcs == 1) alive[u] = true;
+                    else undecided[u] = true;
                 }
             }
-            if constexpr (F32) push_recheck(undecided, seed0 + sl, cx, cz, qx, ctl, cnt);
-            if (single) {
-                const uint32_t pos = wave_append(alive, &ctl[CTL_Q1]);
-                if (alive) {
-                    const int2 e = s_item[i];
-                    int cx, cz, sl;
-                    unpack_item(e, cx, cz, sl);
-                    q1[pos] = make_uint4(seed0 + (uint32_t)sl, (uint32_t)cx, (uint32_t)cz, 0u);
-                    needC[seed0 + sl] = 1u;
-                }
-            } else {
-                const uint32_t pos = wave_append(alive, &s_n[0]);
-                if (alive) {
-                    s_li[pos] = (uint16_t)i;
-                    s_ls[pos] = sum;
+#pragma unroll
+            for (int u = 0; u < IL; u++) {
+                if constexpr (F32) push_recheck(undecided[u], seed0 + sl[u], cx[u], cz[u], qx, ctl, cnt);
+                if (single) {
+                    const uint32_t pos = wave_append(alive[u], &ctl[CTL_Q1]);
+                    if (alive[u]) {
+                        q1[pos] = make_uint4(seed0 + sl[u], (uint32_t)cx[u], (uint32_t)cz[u], 0u);
+                        needC[seed0 + sl[u]] = 1u;
+                    }
+                } else {
+                    const uint32_t pos = wave_append(alive[u], &s_n[0]);
+                    if (alive[u]) {
+                        s_li[pos] = (uint16_t)i[u];
+                        s_ls[pos] = (Sum)sum[u];
+                    }
                 }
             }
         }
         __syncthreads();
         if (!single) {
             // ---- levels 1..3: evaluations 2L, 2L + 1 and tiers 2L - 1, 2L ----
-            // a lane holds list entries tid + k * kEroWG; all are read before the barrier, so the survivors can be
+            // a lane holds list entries tid + k * WG; all are read before the barrier, so the survivors can be
             // appended to the same list
-            constexpr int kPer = kEroPass / kEroWG;
+            constexpr int kPer = kEroPass / WG;
+            static_assert(kPer % IL == 0, "list entries per lane come in groups of IL");
             for (int L = 1; L <= 3; L++) {
                 const uint32_t n = s_n[L - 1];
                 const int k0 = 2 * L, k1 = 2 * L + 1;
@@ -271,59 +326,65 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
                 double run[kPer];
 #pragma unroll
                 for (int k = 0; k < kPer; k++) {
-                    const uint32_t q = tid + (uint32_t)k * kEroWG;
+                    const uint32_t q = tid + (uint32_t)k * WG;
                     idx[k] = q < n ? s_li[q] : 0u;
-                    run[k] = q < n ? s_ls[q] : 0.0;
+                    run[k] = q < n ? (double)s_ls[q] : 0.0;
                 }
                 __syncthreads();
 #pragma unroll
-                for (int k = 0; k < kPer; k++) {
-                    if ((tid & ~63u) + (uint32_t)k * kEroWG >= n) break;   // wave-uniform
-                    const uint32_t q = tid + (uint32_t)k * kEroWG;
-                    bool alive = false, undecided = false;
-                    double sum = run[k];
-                    int cx = 0, cz = 0, sl = 0;
-                    if (q < n) {
-                        unpack_item(s_item[idx[k]], cx, cz, sl);
-                        const double x = (double)(cx * 16 + 8) / 4.0, z = (double)(cz * 16 + 8) / 4.0;
-                        const LPair Q[2] = {LPair{s_tab + (sl * 8 + k0) * 256}, LPair{s_tab + (sl * 8 + k1) * 256}};
-                        const SPar *P[2] = {&s_par[sl * 8 + k0], &s_par[sl * 8 + k1]};
-                        const bool raw = fabs(x) < ed.coord_lim && fabs(z) < ed.coord_lim;
-                        double xs[2] = {x * ed.sc[k0] * ed.lac[k0], x * ed.sc[k1] * ed.lac[k1]};
-                        double zs[2] = {z * ed.sc[k0] * ed.lac[k0], z * ed.sc[k1] * ed.lac[k1]};
-                        if (!raw) {
-                            for (int t = 0; t < 2; t++) { xs[t] = keep_precision(xs[t]); zs[t] = keep_precision(zs[t]); }
-                        }
-                        double v[2];
-                        if (F32) perlin_y0_x2_f32(Q, P, xs, zs, v);
-                        else perlin_y0_x2(Q, P, xs, zs, v);
-                        sum += ed.w[k0] * v[0];
-                        int dcs = tier_decide<F32>(sum * amplitude, ed.thr[k0 - 1], ed.margin[k0]);
+                for (int k = 0; k < kPer; k += IL) {
+                    if ((tid & ~63u) + (uint32_t)k * WG >= n) break;   // wave-uniform
+                    uint32_t sl[IL];
+                    int cx[IL], cz[IL];
+                    bool in[IL], alive[IL], undecided[IL];
+                    double sum[IL], x[IL], z[IL];
+#pragma unroll
+                    for (int u = 0; u < IL; u++) {
+                        in[u] = tid + (uint32_t)(k + u) * WG < n;
+                        alive[u] = undecided[u] = false;
+                        sum[u] = run[k + u];
+                        int s = 0;
+                        cx[u] = cz[u] = 0;
+                        if (in[u]) unpack_item(s_item[idx[k + u]], cx[u], cz[u], s);
+                        sl[u] = (uint32_t)s;
+                        x[u] = (double)(cx[u] * 16 + 8) / 4.0;
+                        z[u] = (double)(cz[u] * 16 + 8) / 4.0;
+                    }
+                    double v[2 * IL];
+                    ero_evals<F32, IL>(s_tab, s_par, ed, k0, sl, x, z, v);
+#pragma unroll
+                    for (int u = 0; u < IL; u++) {
+                        if (!in[u]) continue;
+                        sum[u] += w[k0] * v[2 * u];
+                        int dcs = tier_decide<F32>(sum[u] * amplitude, ed.thr[k0 - 1], ed.margin[k0]);
                         if (dcs == 0) {
                             atomicAdd(&s_cnt[k0 - 1], 1u);
                         } else if (dcs == 1) {
-                            sum += ed.w[k1] * v[1];
-                            dcs = tier_decide<F32>(sum * amplitude, ed.thr[k1 - 1], ed.margin[k1]);
+                            sum[u] += w[k1] * v[2 * u + 1];
+                            dcs = tier_decide<F32>(sum[u] * amplitude, ed.thr[k1 - 1], ed.margin[k1]);
                             if (dcs == 0) atomicAdd(&s_cnt[k1 - 1], 1u);
-                            else if (dcs == 1) alive = true;
-                            else undecided = true;
+                            else if (dcs == 1) alive[u] = true;
+                            else undecided[u] = true;
                         } else {
-                            undecided = true;
+                            undecided[u] = true;
                         }
                     }
-                    if constexpr (F32) push_recheck(undecided, seed0 + (uint32_t)sl, cx, cz, qx, ctl, cnt);
-                    if (L < 3) {
-                        const uint32_t pos = wave_append(alive, &s_n[L]);
-                        if (alive) {
-                            s_li[pos] = (uint16_t)idx[k];
-                            s_ls[pos] = sum;
-                        }
-                    } else {
-                        // survivors of the erosion gate -> Q1
-                        const uint32_t pos = wave_append(alive, &ctl[CTL_Q1]);
-                        if (alive) {
-                            q1[pos] = make_uint4(seed0 + (uint32_t)sl, (uint32_t)cx, (uint32_t)cz, 0u);
-                            needC[seed0 + sl] = 1u;
+#pragma unroll
+                    for (int u = 0; u < IL; u++) {
+                        if constexpr (F32) push_recheck(undecided[u], seed0 + sl[u], cx[u], cz[u], qx, ctl, cnt);
+                        if (L < 3) {
+                            const uint32_t pos = wave_append(alive[u], &s_n[L]);
+                            if (alive[u]) {
+                                s_li[pos] = (uint16_t)idx[k + u];
+                                s_ls[pos] = (Sum)sum[u];
+                            }
+                        } else {
+                            // survivors of the erosion gate -> Q1
+                            const uint32_t pos = wave_append(alive[u], &ctl[CTL_Q1]);
+                            if (alive[u]) {
+                                q1[pos] = make_uint4(seed0 + sl[u], (uint32_t)cx[u], (uint32_t)cz[u], 0u);
+                                needC[seed0 + sl[u]] = 1u;
+                            }
                         }
                     }
                 }
@@ -334,6 +395,7 @@ __global__ __launch_bounds__(kEroWG) void k_erosion(const uint64_t *__restrict__
         if (tid < 4) s_n[tid] = 0;
         __syncthreads();
     }
+    }   // blocks
     if (tid < 7 && s_cnt[tid]) atomicAdd(&cnt[CNT_TIER0 + tid], (unsigned long long)s_cnt[tid]);
 }
 
@@ -402,11 +464,13 @@ __global__ void k_noise_error(const uint8_t *__restrict__ tabE, const SPar *__re
         const SPar *P[2] = {&parE[t0], &parE[t1]};
         const double xs[2] = {keep_precision(x * ed.sc[k0] * ed.lac[k0]), keep_precision(x * ed.sc[k1] * ed.lac[k1])};
         const double zs[2] = {keep_precision(z * ed.sc[k0] * ed.lac[k0]), keep_precision(z * ed.sc[k1] * ed.lac[k1])};
+        const EPar e0 = make_epar(parE[t0]), e1 = make_epar(parE[t1]);
+        const EPar *E[2] = {&e0, &e1};
         double v64[2], v32[2];
         perlin_y0_x2(Q, P, xs, zs, v64);
-        perlin_y0_x2_f32(Q, P, xs, zs, v32);
+        perlin_y0_xn_q<2>(Q, E, xs, zs, v32);
         for (int t = 0; t < 2; t++) {
-            const double err = fabs(v32[t] - v64[t]);
+            const double err = fabs(v32[t] * kFixInv - v64[t]);
             atomicMax(&maxbits[k0 + t], (unsigned long long)__double_as_longlong(err));
         }
     }
@@ -818,6 +882,7 @@ void build_layout(int vflags, Layout &L) {
             e.sc[k] = (k & 1) ? 1.0181268882175227 : 1.0;
             e.lac[k] = d.lac[p];
             e.w[k] = d.amp[p] * d.per[p];
+            e.w23[k] = e.w[k] * kFixInv;
         }
         e.amplitude = d.amplitude;
         const double thr[7] = {0.05, 0.22, 0.42, 0.45, 0.49, 0.51, 0.55};
@@ -829,10 +894,13 @@ void build_layout(int vflags, Layout &L) {
         // x * sc * lac stays below 1e7 in magnitude (maintainPrecision's identity range) when |x| < coord_lim;
         // the 0.999 factor leaves room for the rounding of the two products
         e.coord_lim = 1.0e7 / smax * 0.999;
-        double wsum = 0.0;
+        // and the FP32 gate keeps the running sum as a float between levels: up to three roundings of a sum no larger
+        // than sum |w| x 1.1 (|Perlin| <= 1.07), each within 2^-24 of it
+        double wsum = 0.0, wall = 0.0;
+        for (int k = 0; k < 8; k++) wall += std::fabs(e.w[k]);
         for (int k = 0; k < 8; k++) {
             wsum += std::fabs(e.w[k]);
-            e.margin[k] = e.amplitude * wsum * kFp32PerlinBound + 1e-12;
+            e.margin[k] = e.amplitude * (wsum * kFp32PerlinBound + 3.0 * 0x1p-24 * 1.1 * wall) + 1e-12;
         }
     }
     // build descriptors
@@ -958,7 +1026,7 @@ void launch_reset(whf_ctx *c, uint32_t ns) {
 // Set C for the seeds with erosion survivors (slots by compaction, then tables) and gate 2.
 void stage_c(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags, hipStream_t st) {
     const uint32_t blocks_n = std::max(1u, (ns + 255) / 256);
-    const uint32_t persist = (uint32_t)c->n_cu * 8;
+    const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune.gate_wg_per_cu;
     const uint32_t nb = std::min(ns, kMaxSeeds);   // at most kMaxSeeds seeds get tables; gates count the rest
     {
         Prof p(c, F_COMPACT, st);
@@ -983,7 +1051,7 @@ void stage_c(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags,
 // Set L for the seeds with gate-2 survivors, and gate 3 -> hits.
 void stage_l(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags, int max_height, hipStream_t st) {
     const uint32_t blocks_n = std::max(1u, (ns + 255) / 256);
-    const uint32_t persist = (uint32_t)c->n_cu * 8;
+    const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune.gate_wg_per_cu;
     const uint32_t nb = std::min(ns, kMaxSeeds);
     {
         Prof p(c, F_COMPACT, st);
@@ -1097,16 +1165,32 @@ int stage_e(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_
         const uint32_t chunk = std::min<uint32_t>(fn, 1u << 16);
         const uint32_t n_chunks = (fn + chunk - 1) / chunk;
         const uint32_t tiles = (ns + ST - 1) / ST;
-        const dim3 grid(tiles * n_chunks);
+        // persistent: ero_wg_per_cu workgroups per compute unit loop over the blocks (-1: one workgroup per block)
+        const uint32_t blocks = tiles * n_chunks;
+        const int per_cu = c->tune.ero_wg_per_cu;
+        const dim3 grid(per_cu > 0 ? std::min<uint32_t>(blocks, (uint32_t)c->n_cu * (uint32_t)per_cu) : blocks);
         const bool f32 = req->precision == WHF_PREC_FP32_GATE;
         const FastDiv Wd = make_fastdiv(W), Rd = make_fastdiv(chunk);
-#define WHF_EROSION_ARGS                                                                                          \
-    grid, dim3(kEroWG), 0, st, b.seeds, ns, c->d_tab[SET_E], c->d_par[SET_E], gt, req->rx0, req->rz0, Wd, Rd, f0, fn,  \
+        const int wg = c->tune.ero_wg == 512 ? 512 : 256;
+        const int il = c->tune.ero_ilp == 1 ? 1 : 2;   // items per lane of the FP32 gate
+#define WHF_EROSION_ARGS(WG)                                                                                      \
+    grid, dim3(WG), 0, st, b.seeds, ns, c->d_tab[SET_E], c->d_par[SET_E], gt, req->rx0, req->rz0, Wd, Rd, f0, fn,      \
         chunk, n_chunks, salt, single, b.q1, b.ctl, b.needC, c->d_qx, c->d_cnt
-        if (wide && f32) hipLaunchKernelGGL((k_erosion<4, true>), WHF_EROSION_ARGS);
-        else if (wide) hipLaunchKernelGGL((k_erosion<4, false>), WHF_EROSION_ARGS);
-        else if (f32) hipLaunchKernelGGL((k_erosion<8, true>), WHF_EROSION_ARGS);
-        else hipLaunchKernelGGL((k_erosion<8, false>), WHF_EROSION_ARGS);
+        if (!f32) {
+            if (wide && wg == 512) hipLaunchKernelGGL((k_erosion<4, false, 512, 1>), WHF_EROSION_ARGS(512));
+            else if (wide) hipLaunchKernelGGL((k_erosion<4, false, 256, 1>), WHF_EROSION_ARGS(256));
+            else if (wg == 512) hipLaunchKernelGGL((k_erosion<8, false, 512, 1>), WHF_EROSION_ARGS(512));
+            else hipLaunchKernelGGL((k_erosion<8, false, 256, 1>), WHF_EROSION_ARGS(256));
+        } else if (wg == 512) {
+            if (wide) hipLaunchKernelGGL((k_erosion<4, true, 512, 1>), WHF_EROSION_ARGS(512));
+            else hipLaunchKernelGGL((k_erosion<8, true, 512, 1>), WHF_EROSION_ARGS(512));
+        } else if (il == 2) {
+            if (wide) hipLaunchKernelGGL((k_erosion<4, true, 256, 2>), WHF_EROSION_ARGS(256));
+            else hipLaunchKernelGGL((k_erosion<8, true, 256, 2>), WHF_EROSION_ARGS(256));
+        } else {
+            if (wide) hipLaunchKernelGGL((k_erosion<4, true, 256, 1>), WHF_EROSION_ARGS(256));
+            else hipLaunchKernelGGL((k_erosion<8, true, 256, 1>), WHF_EROSION_ARGS(256));
+        }
 #undef WHF_EROSION_ARGS
         if (f32 && !single)
             hipLaunchKernelGGL(k_erosion_exact, dim3((uint32_t)c->n_cu * 2), dim3(256), 0, st, c->d_tab[SET_E],
@@ -1162,6 +1246,152 @@ int phase2_filter(whf_ctx *c, const whf_request *req, whf_hit *hits, uint64_t n,
 
 // ================================================================================================ C ABI
 
+namespace {
+
+#ifndef WHF_BUILD_ID
+#define WHF_BUILD_ID "unknown"
+#endif
+
+void tune_defaults(whf_tuning &t) {
+    std::memset(&t, 0, sizeof t);
+    t.ero_wg = 256;
+    t.ero_ilp = 2;
+    t.ero_wg_per_cu = -1;
+    t.gate_wg_per_cu = 8;
+    t.chunk_checks = 1ull << 24;
+}
+
+// Fills the zero fields of t with the defaults; false when a field is out of range.
+bool tune_complete(whf_tuning &t) {
+    whf_tuning d;
+    tune_defaults(d);
+    if (!t.ero_wg) t.ero_wg = d.ero_wg;
+    if (!t.ero_ilp) t.ero_ilp = d.ero_ilp;
+    if (!t.ero_wg_per_cu) t.ero_wg_per_cu = d.ero_wg_per_cu;
+    if (!t.gate_wg_per_cu) t.gate_wg_per_cu = d.gate_wg_per_cu;
+    if (!t.chunk_checks) t.chunk_checks = d.chunk_checks;
+    return (t.ero_wg == 256 || t.ero_wg == 512) && (t.ero_ilp == 1 || t.ero_ilp == 2) && t.ero_wg_per_cu >= -1 &&
+           t.ero_wg_per_cu <= 64 && t.gate_wg_per_cu >= 1 && t.gate_wg_per_cu <= 32 &&
+           t.chunk_checks >= 1024 && t.chunk_checks <= (1ull << 32);
+}
+
+// A small reader for the tuned-configuration file: a JSON object whose keys are device names, each an object of
+// numeric launch-shape fields plus "build" (the library build it was measured with) and anything else, skipped.
+struct Json {
+    const std::string &s;
+    size_t i = 0;
+    explicit Json(const std::string &t) : s(t) {}
+    void ws() { while (i < s.size() && std::isspace((unsigned char)s[i])) i++; }
+    bool str(std::string &out) {
+        ws();
+        if (i >= s.size() || s[i] != '"') return false;
+        out.clear();
+        for (i++; i < s.size() && s[i] != '"'; i++) {
+            if (s[i] == '\\' && i + 1 < s.size()) i++;
+            out += s[i];
+        }
+        if (i >= s.size()) return false;
+        i++;
+        return true;
+    }
+    bool skip() {   // any value
+        ws();
+        if (i >= s.size()) return false;
+        if (s[i] == '"') { std::string t; return str(t); }
+        if (s[i] == '{' || s[i] == '[') {
+            const char open = s[i], close = open == '{' ? '}' : ']';
+            int depth = 0;
+            for (; i < s.size(); i++) {
+                if (s[i] == '"') { std::string t; if (!str(t)) return false; i--; continue; }
+                if (s[i] == open) depth++;
+                else if (s[i] == close && --depth == 0) { i++; return true; }
+            }
+            return false;
+        }
+        while (i < s.size() && s[i] != ',' && s[i] != '}' && s[i] != ']') i++;
+        return true;
+    }
+    // Calls f(key) at each member of the object at the cursor, with the cursor on the member's value; f consumes it.
+    template <typename F>
+    bool object(F f) {
+        ws();
+        if (i >= s.size() || s[i] != '{') return false;
+        i++;
+        for (;;) {
+            ws();
+            if (i < s.size() && s[i] == '}') { i++; return true; }
+            std::string k;
+            if (!str(k)) return false;
+            ws();
+            if (i >= s.size() || s[i] != ':') return false;
+            i++;
+            if (!f(k)) return false;
+            ws();
+            if (i < s.size() && s[i] == ',') i++;
+        }
+    }
+    bool number(double &v) {
+        ws();
+        char *end = nullptr;
+        v = std::strtod(s.c_str() + i, &end);
+        if (end == s.c_str() + i) return false;
+        i = (size_t)(end - s.c_str());
+        return true;
+    }
+};
+
+// The path of the tuned-configuration file: $WHF_TUNED, else tuned.json beside this library.
+std::string tuned_path() {
+    if (const char *e = std::getenv("WHF_TUNED")) return e;
+    Dl_info info;
+    if (dladdr((void *)&tune_defaults, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        const size_t k = p.rfind('/');
+        return (k == std::string::npos ? std::string(".") : p.substr(0, k)) + "/tuned.json";
+    }
+    return "tuned.json";
+}
+
+// Applies the file's entry for `device` (else its "default" entry) when it was measured with this build.
+void tune_load(whf_tuning &t, const char *device) {
+    FILE *f = std::fopen(tuned_path().c_str(), "rb");
+    if (!f) return;
+    std::string text;
+    char buf[4096];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
+    std::fclose(f);
+    whf_tuning found[2];
+    bool have[2] = {false, false};
+    Json j(text);
+    j.object([&](const std::string &dev) {
+        const int which = dev == device ? 0 : (dev == "default" ? 1 : -1);
+        if (which < 0) return j.skip();
+        whf_tuning e;
+        std::memset(&e, 0, sizeof e);
+        std::string build;
+        const bool ok = j.object([&](const std::string &k) {
+            if (k == "build") return j.str(build);
+            double v = 0.0;
+            if (k == "ero_wg") { if (!j.number(v)) return false; e.ero_wg = (int)v; return true; }
+            if (k == "ero_ilp") { if (!j.number(v)) return false; e.ero_ilp = (int)v; return true; }
+            if (k == "ero_wg_per_cu") { if (!j.number(v)) return false; e.ero_wg_per_cu = (int)v; return true; }
+            if (k == "gate_wg_per_cu") { if (!j.number(v)) return false; e.gate_wg_per_cu = (int)v; return true; }
+            if (k == "chunk_checks") { if (!j.number(v)) return false; e.chunk_checks = (uint64_t)v; return true; }
+            return j.skip();
+        });
+        if (ok && build == WHF_BUILD_ID && tune_complete(e)) {
+            found[which] = e;
+            have[which] = true;
+        }
+        return ok;
+    });
+    if (have[0]) t = found[0];
+    else if (have[1]) t = found[1];
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *whf_strerror(int code) {
@@ -1210,9 +1440,11 @@ int whf_open(const whf_config *cfg, whf_ctx **out) {
     c->profile = cfg && (cfg->flags & WHF_CFG_PROFILE);
     c->one_stream = cfg && (cfg->flags & WHF_CFG_ONE_STREAM);
     hipDeviceProp_t prop;
+    tune_defaults(c->tune);
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
         std::snprintf(c->name, sizeof c->name, "%s (%s)", prop.name, prop.gcnArchName);
         c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        tune_load(c->tune, prop.name);
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -1261,6 +1493,22 @@ void whf_close(whf_ctx *c) {
 }
 
 const char *whf_device_name(whf_ctx *c) { return c ? c->name : ""; }
+
+const char *whf_build_id(void) { return WHF_BUILD_ID; }
+
+int whf_get_tuning(whf_ctx *c, whf_tuning *out) {
+    if (!c || !out) return WHF_EINVAL;
+    *out = c->tune;
+    return WHF_OK;
+}
+
+int whf_set_tuning(whf_ctx *c, const whf_tuning *t) {
+    if (!c || !t) return WHF_EINVAL;
+    whf_tuning v = *t;
+    if (!tune_complete(v)) return WHF_EINVAL;
+    c->tune = v;
+    return WHF_OK;
+}
 
 int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stats) {
     if (!c || !req) return WHF_EINVAL;

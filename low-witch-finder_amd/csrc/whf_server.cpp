@@ -337,6 +337,11 @@ int whf_server_open(const whf_server_config *cfg, whf_server **out) {
             return rc;
         }
         s->ctx.push_back(c);
+        if (i == 0 && !(cfg && cfg->chunk_checks)) {
+            // the device's tuned chunk size (whf_tuning.chunk_checks)
+            whf_tuning t;
+            if (whf_get_tuning(c, &t) == WHF_OK && t.chunk_checks) s->chunk_checks = t.chunk_checks;
+        }
         // admission by memory: stop adding slots once the next one would exceed the budget
         if (budget && (uint64_t)(i + 2) * whf_ctx_device_bytes(c) > budget) break;
     }

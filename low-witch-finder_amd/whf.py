@@ -27,13 +27,22 @@ SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags"
            "whf_device_name", "whf_noise_error", "whf_fp32_bound", "whf_deepdive", "whf_deepdive_noalloc", "whf_dd_terrain",
            "whf_room_floors",
            "whf_server_open", "whf_server_close", "whf_submit", "whf_submit_deepdive", "whf_poll",
-           "whf_biome_tree", "whf_biome", "whf_surface_heights", "whf_post_carver_heights", "whf_hut_eval")
+           "whf_biome_tree", "whf_biome", "whf_surface_heights", "whf_post_carver_heights", "whf_hut_eval",
+           "whf_get_tuning", "whf_set_tuning", "whf_build_id")
 HIT_DEEPDIVE, HIT_PHASE2 = 1, 2
 P2_NONE, P2_OR_LOWER, P2_EXACTLY = 0, 1, 2
 BIOME_SWAMP, BIOME_QY0, BIOME_QY_N = 6, -17, 97
 BIO_OCC_ONLY = 0x100
 OCC_SURFACE = -(1 << 31)
 BLOCK_SOLID, BLOCK_AIR, BLOCK_WATER, BLOCK_LAVA = 0, 1, 2, 3
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("ero_wg", ctypes.c_int), ("ero_ilp", ctypes.c_int), ("ero_wg_per_cu", ctypes.c_int),
+                ("gate_wg_per_cu", ctypes.c_int), ("chunk_checks", ctypes.c_uint64), ("reserved", ctypes.c_int * 8)]
+
+
+TUNING_FIELDS = ("ero_wg", "ero_ilp", "ero_wg_per_cu", "gate_wg_per_cu", "chunk_checks")
 
 
 class Config(ctypes.Structure):
@@ -196,6 +205,12 @@ def load():
         L.whf_hut_eval.restype = ctypes.c_int
         L.whf_hut_eval.argtypes = [ctypes.c_void_p, ctypes.POINTER(HutQuery), ctypes.c_uint64, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.POINTER(HutResult)]
+        L.whf_get_tuning.restype = ctypes.c_int
+        L.whf_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
+        L.whf_set_tuning.restype = ctypes.c_int
+        L.whf_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
+        L.whf_build_id.restype = ctypes.c_char_p
+        L.whf_build_id.argtypes = []
         L.whf_biome_tree.restype = ctypes.c_int
         L.whf_biome_tree.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         L.whf_biome.restype = ctypes.c_int
@@ -252,6 +267,23 @@ class Context:
     @property
     def device_name(self):
         return self.L.whf_device_name(self.h).decode()
+
+    def tuning(self):
+        """The context's launch shapes (whf_get_tuning) as a dict."""
+        t = Tuning()
+        rc = self.L.whf_get_tuning(self.h, ctypes.byref(t))
+        if rc:
+            raise WhfError(rc, "whf_get_tuning")
+        return {f: getattr(t, f) for f in TUNING_FIELDS}
+
+    def set_tuning(self, **kw):
+        """Sets launch shapes (whf_set_tuning); fields not given keep their current value."""
+        cur = self.tuning()
+        cur.update(kw)
+        t = Tuning(*[cur[f] for f in TUNING_FIELDS])
+        rc = self.L.whf_set_tuning(self.h, ctypes.byref(t))
+        if rc:
+            raise WhfError(rc, "whf_set_tuning %r" % kw)
 
     def scan(self, rect, seeds=None, splitmix=None, counter=None, max_height=-50, vflags=0, capacity=1 << 16,
              salt=0, precision=PREC_FP64, phase2=None):
@@ -573,3 +605,7 @@ class Server:
 
 def fp32_bound():
     return load().whf_fp32_bound()
+
+
+def build_id():
+    return load().whf_build_id().decode()
