@@ -149,6 +149,30 @@ def cpu_baseline(n_seeds_max, seconds, host):
                 if r["phase2_evaluated"] else None)
 
 
+def autotune(seeds):
+    """Runs the autotuner (low-witch-finder_amd/tune.py) for versions 1.18, 1.20 and 1.21 into a file of its own and
+    points whf_open at it (WHF_TUNED), before any timed context opens; returns what it chose and measured."""
+    import importlib.util
+    import tempfile
+    spec = importlib.util.spec_from_file_location("whf_tune", os.path.join(REPO, "low-witch-finder_amd", "tune.py"))
+    tune = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(tune)
+    fd, path = tempfile.mkstemp(prefix="whf_tuned_", suffix=".json")
+    os.close(fd)
+    os.unlink(path)
+    t0 = time.perf_counter()
+    device, entry = tune.main(["--out", path, "--seeds", str(seeds), "--versions", "1.18,1.20,1.21"])
+    secs = time.perf_counter() - t0
+    os.environ["WHF_TUNED"] = path
+    per = entry["measured"]["per_version"]
+    return dict(seconds=round(secs, 2), device=device, build=entry["build"],
+                chosen={v: p["chosen"] for v, p in per.items()},
+                default_ms={v: p["default_ms"] for v, p in per.items()},
+                tuned_ms={v: p["tuned_ms"] for v, p in per.items()},
+                chunk_checks=entry.get("chunk_checks"), workload=entry["measured"]["workload"],
+                serving_capacity=entry["measured"].get("serving_capacity_checks_per_s"))
+
+
 def cpu_parity(whf, cpu, precisions):
     """The GPU scan of exactly the seeds the CPU baseline scanned (its stream prefix), with the same threshold and
     version, compared with the reference scanner's own counters and finds: checks, the reject funnel, the erosion tier
@@ -410,6 +434,8 @@ def main():
     ap.add_argument("--seeds", type=int, default=1 << 26)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="use the library's default launch shapes")
+    ap.add_argument("--tune-seeds", type=int, default=1 << 23, help="seeds of the autotuner's timing workload")
     ap.add_argument("--product-seeds", type=int, default=1024,
                     help="seeds of the product-mode CPU timing (one lysh process per seed)")
     ap.add_argument("--dd-seeds", type=int, default=1 << 28, help="structure seeds of the Deep Dive sweep")
@@ -428,6 +454,10 @@ def main():
         return 2
 
     whf = __graft_entry__.load_whf()
+    tuning = None
+    if not args.no_tune:
+        tuning = autotune(args.tune_seeds)
+        note("autotuned in %.1f s: %s" % (tuning["seconds"], tuning["chosen"]))
     rect = (-8, 8, -8, 8)
     vflags = whf.version_flags("1.20")
     kw = dict(splitmix=(2, args.seeds), max_height=-50, vflags=vflags, capacity=1 << 20,
@@ -586,7 +616,7 @@ def main():
                        "bound_used": whf.fp32_bound()},
         "roofline": roofline, "roofline_build": roofline_build, "step_roofline": step_roofline,
         "cpu_baseline": cpu and {k: v for k, v in cpu.items() if k != "finds"},
-        "cpu_parity": parity, "cpu_product_mode": product,
+        "cpu_parity": parity, "cpu_product_mode": product, "tuning": tuning,
         "funnel": last["funnel"], "tiers": last["tiers"], "accepted": last["accepted"],
         "kernel_ms": {k: round(v, 3) for k, v in fam.items()}, "rooflines": rooflines,
         "rooflines_alone": rooflines_alone, "samplers_built": built,

@@ -382,7 +382,7 @@ int whf_poll(whf_server *srv, whf_event *events, int max_events, int timeout_ms)
 /* ---- Tuning: launch shapes chosen per device ---- */
 
 /* The launch shapes the library leaves open. whf_open starts from the built-in defaults, then applies the entry for
- * its device (whf_device_name without the arch suffix, or "default") from the JSON file named by the WHF_TUNED
+ * its device (whf_device_name up to " (", or "default") from the JSON file named by the WHF_TUNED
  * environment variable, or else from tuned.json beside libwhf_hip.so, if one exists; low-witch-finder_amd/tune.py
  * measures the candidates on the device and writes that file. No counterpart in the reference's engine, whose launch
  * shapes are constants; hunt.py keeps a tuned configuration per GPU the same way (hunt.py:927-945, 959-1021). Every
@@ -400,10 +400,20 @@ typedef struct {
     int reserved[8];
 } whf_tuning;
 
-/* The context's launch shapes. */
+/* The context's launch shapes for the version rules of vflags 0 (1.19 - 26.1, NORMAL preset). */
 int whf_get_tuning(whf_ctx *ctx, whf_tuning *out);
-/* Sets the context's launch shapes (0 fields take the defaults); WHF_EINVAL for a value out of range. */
+/* Sets the context's launch shapes for every rule layout (0 fields take the defaults); WHF_EINVAL for a value out of
+ * range. */
 int whf_set_tuning(whf_ctx *ctx, const whf_tuning *t);
+/* The same for the rule layout of one set of WHF_V_* flags (0..15): tuned.json may give each game version its own. */
+int whf_get_tuning_for(whf_ctx *ctx, int vflags, whf_tuning *out);
+int whf_set_tuning_for(whf_ctx *ctx, int vflags, const whf_tuning *t);
+/* Reads a tuned-configuration file without a device: the shapes whf_open would apply on `device` for vflags, from
+ * `path` (NULL: $WHF_TUNED, else tuned.json beside the library). File format: {"<device name>" or "default": {"build":
+ * "<whf_build_id>", "ero_wg": ..., "ero_ilp": ..., "ero_wg_per_cu": ..., "gate_wg_per_cu": ..., "chunk_checks": ...,
+ * "versions": {"<version name>": {<fields>}}, <other keys, ignored>}}. Returns WHF_OK when an entry measured with this
+ * build applied, else WHF_EINVAL with the defaults in *out. */
+int whf_load_tuning(const char *path, const char *device, int vflags, whf_tuning *out);
 
 /* Version name -> WHF_V_* flags, accepting lysh's version names (main.c:104-125, biome_trees.h:70499-70526). */
 int whf_version_flags(const char *name, int *vflags);

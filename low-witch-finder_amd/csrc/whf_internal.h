@@ -134,7 +134,7 @@ struct whf_ctx {
     bool one_stream = false;              // WHF_CFG_ONE_STREAM: stages stay on the library's stream
     whf::BatchSet batch_set[3] = {};
     hipEvent_t ev_stage[3][3] = {};
-    whf_tuning tune{};                // launch shapes (defaults filled by whf_open, then tuned.json)
+    whf_tuning tune[16] = {};         // launch shapes per rule layout (defaults, then tuned.json, at whf_open)
     std::vector<hipEvent_t> prof_ev;  // pairs per kernel launch when profiling
     std::vector<int> prof_fam;
     int n_cu = 256;

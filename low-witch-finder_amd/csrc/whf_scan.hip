@@ -1026,7 +1026,7 @@ void launch_reset(whf_ctx *c, uint32_t ns) {
 // Set C for the seeds with erosion survivors (slots by compaction, then tables) and gate 2.
 void stage_c(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags, hipStream_t st) {
     const uint32_t blocks_n = std::max(1u, (ns + 255) / 256);
-    const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune.gate_wg_per_cu;
+    const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune[layout].gate_wg_per_cu;
     const uint32_t nb = std::min(ns, kMaxSeeds);   // at most kMaxSeeds seeds get tables; gates count the rest
     {
         Prof p(c, F_COMPACT, st);
@@ -1051,7 +1051,7 @@ void stage_c(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags,
 // Set L for the seeds with gate-2 survivors, and gate 3 -> hits.
 void stage_l(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags, int max_height, hipStream_t st) {
     const uint32_t blocks_n = std::max(1u, (ns + 255) / 256);
-    const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune.gate_wg_per_cu;
+    const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune[layout].gate_wg_per_cu;
     const uint32_t nb = std::min(ns, kMaxSeeds);
     {
         Prof p(c, F_COMPACT, st);
@@ -1167,12 +1167,12 @@ int stage_e(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_
         const uint32_t tiles = (ns + ST - 1) / ST;
         // persistent: ero_wg_per_cu workgroups per compute unit loop over the blocks (-1: one workgroup per block)
         const uint32_t blocks = tiles * n_chunks;
-        const int per_cu = c->tune.ero_wg_per_cu;
+        const int per_cu = c->tune[layout].ero_wg_per_cu;
         const dim3 grid(per_cu > 0 ? std::min<uint32_t>(blocks, (uint32_t)c->n_cu * (uint32_t)per_cu) : blocks);
         const bool f32 = req->precision == WHF_PREC_FP32_GATE;
         const FastDiv Wd = make_fastdiv(W), Rd = make_fastdiv(chunk);
-        const int wg = c->tune.ero_wg == 512 ? 512 : 256;
-        const int il = c->tune.ero_ilp == 1 ? 1 : 2;   // items per lane of the FP32 gate
+        const int wg = c->tune[layout].ero_wg == 512 ? 512 : 256;
+        const int il = c->tune[layout].ero_ilp == 1 ? 1 : 2;   // items per lane of the FP32 gate
 #define WHF_EROSION_ARGS(WG)                                                                                      \
     grid, dim3(WG), 0, st, b.seeds, ns, c->d_tab[SET_E], c->d_par[SET_E], gt, req->rx0, req->rz0, Wd, Rd, f0, fn,      \
         chunk, n_chunks, salt, single, b.q1, b.ctl, b.needC, c->d_qx, c->d_cnt
@@ -1352,42 +1352,79 @@ std::string tuned_path() {
     return "tuned.json";
 }
 
-// Applies the file's entry for `device` (else its "default" entry) when it was measured with this build.
-void tune_load(whf_tuning &t, const char *device) {
-    FILE *f = std::fopen(tuned_path().c_str(), "rb");
-    if (!f) return;
+// The tuning of every rule layout for `device` from the file at `path`: the device's entry, else the "default"
+// entry, when it was measured with this build; its "versions" object overrides fields per game version (for every
+// preset of that version's rules). Returns whether an entry applied; out[] holds the defaults otherwise.
+bool tune_load(whf_tuning out[kLayouts], const std::string &path, const char *device) {
+    for (int l = 0; l < kLayouts; l++) tune_defaults(out[l]);
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
     std::string text;
     char buf[4096];
     size_t n;
     while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
     std::fclose(f);
-    whf_tuning found[2];
-    bool have[2] = {false, false};
+    struct Entry {
+        whf_tuning base;
+        std::vector<std::pair<int, whf_tuning>> per_version;   // (rule flags, fields given)
+        bool ok = false;
+    } found[2];
     Json j(text);
-    j.object([&](const std::string &dev) {
+    auto fields = [&](whf_tuning &e, const std::string &k) -> int {   // 1 parsed, 0 not a field, -1 error
+        double v = 0.0;
+        int *ip = k == "ero_wg" ? &e.ero_wg : k == "ero_ilp" ? &e.ero_ilp : k == "ero_wg_per_cu" ? &e.ero_wg_per_cu
+                  : k == "gate_wg_per_cu" ? &e.gate_wg_per_cu : nullptr;
+        if (ip) { if (!j.number(v)) return -1; *ip = (int)v; return 1; }
+        if (k == "chunk_checks") { if (!j.number(v)) return -1; e.chunk_checks = (uint64_t)v; return 1; }
+        return 0;
+    };
+    const bool parsed = j.object([&](const std::string &dev) {
         const int which = dev == device ? 0 : (dev == "default" ? 1 : -1);
         if (which < 0) return j.skip();
-        whf_tuning e;
-        std::memset(&e, 0, sizeof e);
+        Entry e;
+        std::memset(&e.base, 0, sizeof e.base);
         std::string build;
         const bool ok = j.object([&](const std::string &k) {
             if (k == "build") return j.str(build);
-            double v = 0.0;
-            if (k == "ero_wg") { if (!j.number(v)) return false; e.ero_wg = (int)v; return true; }
-            if (k == "ero_ilp") { if (!j.number(v)) return false; e.ero_ilp = (int)v; return true; }
-            if (k == "ero_wg_per_cu") { if (!j.number(v)) return false; e.ero_wg_per_cu = (int)v; return true; }
-            if (k == "gate_wg_per_cu") { if (!j.number(v)) return false; e.gate_wg_per_cu = (int)v; return true; }
-            if (k == "chunk_checks") { if (!j.number(v)) return false; e.chunk_checks = (uint64_t)v; return true; }
-            return j.skip();
+            if (k == "versions") {
+                return j.object([&](const std::string &ver) {
+                    int vf = 0;
+                    if (whf_version_flags(ver.c_str(), &vf) != WHF_OK) return j.skip();
+                    whf_tuning t;
+                    std::memset(&t, 0, sizeof t);
+                    const bool vok = j.object([&](const std::string &k2) {
+                        const int r = fields(t, k2);
+                        return r < 0 ? false : (r == 0 ? j.skip() : true);
+                    });
+                    if (vok) e.per_version.emplace_back(vf, t);
+                    return vok;
+                });
+            }
+            const int r = fields(e.base, k);
+            return r < 0 ? false : (r == 0 ? j.skip() : true);
         });
-        if (ok && build == WHF_BUILD_ID && tune_complete(e)) {
-            found[which] = e;
-            have[which] = true;
-        }
+        e.ok = ok && build == WHF_BUILD_ID;
+        if (e.ok) found[which] = e;
         return ok;
     });
-    if (have[0]) t = found[0];
-    else if (have[1]) t = found[1];
+    if (!parsed) return false;
+    const Entry *e = found[0].ok ? &found[0] : (found[1].ok ? &found[1] : nullptr);
+    if (!e) return false;
+    for (int l = 0; l < kLayouts; l++) {
+        whf_tuning t = e->base;
+        for (const auto &pv : e->per_version) {
+            if ((l & (WHF_V_118 | WHF_V_262)) != pv.first) continue;
+            const whf_tuning &o = pv.second;
+            if (o.ero_wg) t.ero_wg = o.ero_wg;
+            if (o.ero_ilp) t.ero_ilp = o.ero_ilp;
+            if (o.ero_wg_per_cu) t.ero_wg_per_cu = o.ero_wg_per_cu;
+            if (o.gate_wg_per_cu) t.gate_wg_per_cu = o.gate_wg_per_cu;
+            if (o.chunk_checks) t.chunk_checks = o.chunk_checks;
+        }
+        if (!tune_complete(t)) return false;
+        out[l] = t;
+    }
+    return true;
 }
 
 }  // namespace
@@ -1440,11 +1477,15 @@ int whf_open(const whf_config *cfg, whf_ctx **out) {
     c->profile = cfg && (cfg->flags & WHF_CFG_PROFILE);
     c->one_stream = cfg && (cfg->flags & WHF_CFG_ONE_STREAM);
     hipDeviceProp_t prop;
-    tune_defaults(c->tune);
+    for (int l = 0; l < kLayouts; l++) tune_defaults(c->tune[l]);
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
-        std::snprintf(c->name, sizeof c->name, "%s (%s)", prop.name, prop.gcnArchName);
+        // the device's key in tuned.json: its marketing name, or "AMD <arch>" where the runtime reports none
+        std::string arch = prop.gcnArchName;
+        arch = arch.substr(0, arch.find(':'));
+        const std::string key = prop.name[0] ? std::string(prop.name) : "AMD " + arch;
+        std::snprintf(c->name, sizeof c->name, "%s (%s)", key.c_str(), prop.gcnArchName);
         c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-        tune_load(c->tune, prop.name);
+        tune_load(c->tune, tuned_path(), key.c_str());
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -1496,18 +1537,36 @@ const char *whf_device_name(whf_ctx *c) { return c ? c->name : ""; }
 
 const char *whf_build_id(void) { return WHF_BUILD_ID; }
 
-int whf_get_tuning(whf_ctx *c, whf_tuning *out) {
-    if (!c || !out) return WHF_EINVAL;
-    *out = c->tune;
-    return WHF_OK;
-}
+int whf_get_tuning(whf_ctx *c, whf_tuning *out) { return whf_get_tuning_for(c, 0, out); }
 
 int whf_set_tuning(whf_ctx *c, const whf_tuning *t) {
     if (!c || !t) return WHF_EINVAL;
     whf_tuning v = *t;
     if (!tune_complete(v)) return WHF_EINVAL;
-    c->tune = v;
+    for (int l = 0; l < kLayouts; l++) c->tune[l] = v;
     return WHF_OK;
+}
+
+int whf_get_tuning_for(whf_ctx *c, int vflags, whf_tuning *out) {
+    if (!c || !out || vflags < 0 || vflags >= kLayouts) return WHF_EINVAL;
+    *out = c->tune[vflags];
+    return WHF_OK;
+}
+
+int whf_set_tuning_for(whf_ctx *c, int vflags, const whf_tuning *t) {
+    if (!c || !t || vflags < 0 || vflags >= kLayouts) return WHF_EINVAL;
+    whf_tuning v = *t;
+    if (!tune_complete(v)) return WHF_EINVAL;
+    c->tune[vflags] = v;
+    return WHF_OK;
+}
+
+int whf_load_tuning(const char *path, const char *device, int vflags, whf_tuning *out) {
+    if (!device || !out || vflags < 0 || vflags >= kLayouts) return WHF_EINVAL;
+    whf_tuning all[kLayouts];
+    const bool ok = tune_load(all, path ? std::string(path) : tuned_path(), device);
+    *out = all[vflags];
+    return ok ? WHF_OK : WHF_EINVAL;
 }
 
 int whf_scan(whf_ctx *c, const whf_request *req, whf_sink *sink, whf_stats *stats) {

@@ -28,7 +28,8 @@ SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags"
            "whf_room_floors",
            "whf_server_open", "whf_server_close", "whf_submit", "whf_submit_deepdive", "whf_poll",
            "whf_biome_tree", "whf_biome", "whf_surface_heights", "whf_post_carver_heights", "whf_hut_eval",
-           "whf_get_tuning", "whf_set_tuning", "whf_build_id")
+           "whf_get_tuning", "whf_set_tuning", "whf_get_tuning_for", "whf_set_tuning_for", "whf_load_tuning",
+           "whf_build_id")
 HIT_DEEPDIVE, HIT_PHASE2 = 1, 2
 P2_NONE, P2_OR_LOWER, P2_EXACTLY = 0, 1, 2
 BIOME_SWAMP, BIOME_QY0, BIOME_QY_N = 6, -17, 97
@@ -209,6 +210,12 @@ def load():
         L.whf_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
         L.whf_set_tuning.restype = ctypes.c_int
         L.whf_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
+        L.whf_get_tuning_for.restype = ctypes.c_int
+        L.whf_get_tuning_for.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Tuning)]
+        L.whf_set_tuning_for.restype = ctypes.c_int
+        L.whf_set_tuning_for.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Tuning)]
+        L.whf_load_tuning.restype = ctypes.c_int
+        L.whf_load_tuning.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(Tuning)]
         L.whf_build_id.restype = ctypes.c_char_p
         L.whf_build_id.argtypes = []
         L.whf_biome_tree.restype = ctypes.c_int
@@ -268,20 +275,22 @@ class Context:
     def device_name(self):
         return self.L.whf_device_name(self.h).decode()
 
-    def tuning(self):
-        """The context's launch shapes (whf_get_tuning) as a dict."""
+    def tuning(self, vflags=0):
+        """The context's launch shapes for a rule layout (whf_get_tuning_for) as a dict."""
         t = Tuning()
-        rc = self.L.whf_get_tuning(self.h, ctypes.byref(t))
+        rc = self.L.whf_get_tuning_for(self.h, vflags, ctypes.byref(t))
         if rc:
-            raise WhfError(rc, "whf_get_tuning")
+            raise WhfError(rc, "whf_get_tuning_for")
         return {f: getattr(t, f) for f in TUNING_FIELDS}
 
-    def set_tuning(self, **kw):
-        """Sets launch shapes (whf_set_tuning); fields not given keep their current value."""
-        cur = self.tuning()
+    def set_tuning(self, vflags=None, **kw):
+        """Sets launch shapes: for one rule layout (whf_set_tuning_for), or every layout when vflags is None
+        (whf_set_tuning); fields not given keep their current value."""
+        cur = self.tuning(vflags or 0)
         cur.update(kw)
         t = Tuning(*[cur[f] for f in TUNING_FIELDS])
-        rc = self.L.whf_set_tuning(self.h, ctypes.byref(t))
+        rc = (self.L.whf_set_tuning(self.h, ctypes.byref(t)) if vflags is None else
+              self.L.whf_set_tuning_for(self.h, vflags, ctypes.byref(t)))
         if rc:
             raise WhfError(rc, "whf_set_tuning %r" % kw)
 
@@ -609,3 +618,10 @@ def fp32_bound():
 
 def build_id():
     return load().whf_build_id().decode()
+
+
+def load_tuning(path, device, vflags=0):
+    """The launch shapes whf_open would apply on `device` from the file at `path`: (applied, dict)."""
+    t = Tuning()
+    rc = load().whf_load_tuning(path.encode() if path else None, device.encode(), vflags, ctypes.byref(t))
+    return rc == OK, {f: getattr(t, f) for f in TUNING_FIELDS}
