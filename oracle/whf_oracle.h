@@ -232,6 +232,10 @@ void   wo_p2_fluid(const wo_phase2 *p, int x, int y, int z, int *level, int *typ
 int    wo_p2_substance(const wo_phase2 *p, int x, int y, int z, double density);
 /* pre-carver surface height (phase2.c:318-334); *top_state = state of the block below it */
 int    wo_p2_height(const wo_phase2 *p, int x, int z, int *top_state);
+/* The pillar cube: 0 = pow(t, 3.0) as the reference (default), 1 = the device's compensated cube; wo_cube_stats reads
+ * and clears (cubes evaluated in mode 1, cubes differing from pow). Process-wide. */
+void   wo_set_cube_mode(int mode);
+void   wo_cube_stats(unsigned long long out[2]);
 
 /* ---- carvers and hut evaluation (whf_oracle_hut.c) ---- */
 enum { WO_HUT_OK = 0, WO_HUT_REJECT_Y = 1, WO_HUT_REJECT_FLOOD = 2, WO_HUT_REJECT_BIOME = 3 };   /* eval.h:88-93 */

@@ -226,11 +226,35 @@ static double spaghetti2d(const wo_phase2 *p, int x, int y, int z) {
     return j_clamp(j_max(m, j), -1.0, 1.0);
 }
 
+/* The pillar cube. Mode 0: the C library's pow, as the reference computes it. Mode 1: the cube the device computes
+ * (the exact square and cube error terms by FMA, summed before the final rounding), counting the cubes it evaluates
+ * and those that differ from pow's, so that a test can measure what the difference does to the terrain. */
+static int g_cube_mode = 0;
+static unsigned long long g_cube_stats[2];   /* cubes evaluated in mode 1, cubes differing from pow */
+
+void wo_set_cube_mode(int mode) { g_cube_mode = mode; }
+
+void wo_cube_stats(unsigned long long out[2]) {
+    out[0] = __atomic_exchange_n(&g_cube_stats[0], 0ull, __ATOMIC_RELAXED);
+    out[1] = __atomic_exchange_n(&g_cube_stats[1], 0ull, __ATOMIC_RELAXED);
+}
+
+static double pillar_cube(double t) {
+    const double pw = pow(t, 3.0);
+    if (!g_cube_mode) return pw;
+    const double h = t * t, e1 = fma(t, t, -h);
+    const double q = h * t, e2 = fma(h, t, -q);
+    const double c = q + fma(e1, t, e2);
+    __atomic_add_fetch(&g_cube_stats[0], 1ull, __ATOMIC_RELAXED);
+    if (c != pw) __atomic_add_fetch(&g_cube_stats[1], 1ull, __ATOMIC_RELAXED);
+    return c;
+}
+
 static double pillar(const wo_phase2 *p, int x, int y, int z) {
-    /* density.c:151-162 (the cube is the C library's pow, as the reference computes it) */
+    /* density.c:151-162 */
     const double rare = from_noise(N(p, P_PILLAR_RARE), (double)x, (double)y, (double)z, 0.0, 2.0);
     double thick = from_noise(N(p, P_PILLAR_THICK), (double)x, (double)y, (double)z, 0.0, 1.1);
-    thick = pow(thick, 3.0);
+    thick = pillar_cube(thick);
     const double f = sample3(N(p, P_PILLAR), (double)x * 25.0, (double)y * 0.3, (double)z * 25.0);
     const double g = thick * (f * 2.0 - rare);
     return g > 0.03 ? g : -INFINITY;

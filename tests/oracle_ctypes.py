@@ -169,6 +169,10 @@ def lib():
         L.wo_p2_substance.argtypes = [ctypes.c_void_p, i32, i32, i32, dbl]
         L.wo_p2_height.restype = i32
         L.wo_p2_height.argtypes = [ctypes.c_void_p, i32, i32, ctypes.POINTER(i32)]
+        L.wo_set_cube_mode.restype = None
+        L.wo_set_cube_mode.argtypes = [ctypes.c_int]
+        L.wo_cube_stats.restype = None
+        L.wo_cube_stats.argtypes = [ctypes.POINTER(ctypes.c_ulonglong)]
         L.wo_post_carver_height.restype = i32
         L.wo_post_carver_height.argtypes = [ctypes.c_void_p, u64, i32, i32]
         L.wo_hut_orientation.restype = i32
@@ -464,3 +468,15 @@ WO_HUT_OK, WO_HUT_REJECT_Y, WO_HUT_REJECT_FLOOD, WO_HUT_REJECT_BIOME = 0, 1, 2, 
 
 def hut_orientation(seed, hut_x, hut_z):
     return lib().wo_hut_orientation(u64(seed), hut_x, hut_z)
+
+
+def set_cube_mode(mode):
+    """The oracle's pillar cube: 0 = the C library's pow as the reference, 1 = the device's compensated cube."""
+    lib().wo_set_cube_mode(mode)
+
+
+def cube_stats():
+    """(cubes evaluated in mode 1, cubes differing from pow) since the last call."""
+    out = (ctypes.c_ulonglong * 2)()
+    lib().wo_cube_stats(out)
+    return out[0], out[1]
