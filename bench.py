@@ -267,6 +267,46 @@ def deep_dive_cpu(s0, seconds):
                        % (r["structure_seeds"], seconds), starts=r["starts"], spots=r["spots"])
 
 
+def full_c3_from_profiles():
+    """The whole 2^32-structure-seed Deep Dive as one timed call (bench.py --dd-full), from its committed summary."""
+    try:
+        with open(os.path.join(REPO, "profiles", "deepdive_c3_full.json")) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def deep_dive_full(whf):
+    """The Deep Dive workload in full: 2^32 structure seeds from the first SplitMix64(3) output, regions [-2,1]^2,
+    room floor <= -55, every spot expanded to its 2^16 world seeds with the phase-1 chain and the swamp check (version
+    1.20), as one whf_deepdive call, wall-timed."""
+    s0 = splitmix_first(3) & ((1 << 48) - 1)
+    piece = 1 << 26    # calls of 2^26 structure seeds, so that progress shows between them
+    keys = ("starts", "spots", "expanded_seeds", "accepted", "biome_rejected", "overflow")
+    st = dict.fromkeys(keys, 0)
+    finds = []
+    with whf.Context(0) as ctx:
+        ctx.deepdive(s0, 1 << 20, DD_RECT, DD_FLOOR)   # warm-up
+        t0 = time.perf_counter()
+        for k in range(DD_TOTAL // piece):
+            _, f, s = ctx.deepdive((s0 + k * piece) & ((1 << 48) - 1), piece, DD_RECT, DD_FLOOR, expand=True,
+                                   max_height=-50, spot_capacity=1 << 21, find_capacity=1 << 16)
+            finds += f
+            for key in keys:
+                st[key] += int(s[key])
+            if k % 8 == 7:
+                note("full Deep Dive: %d of %d calls, %.1f s" % (k + 1, DD_TOTAL // piece, time.perf_counter() - t0))
+        secs = time.perf_counter() - t0
+    note("full Deep Dive of 2^32 structure seeds: %.1f s" % secs)
+    return dict(workload="2^32 structure seeds S = first SplitMix64(3) output + i (mod 2^48), hut regions [-2,1]^2, "
+                         "room floor <= -55, spots expanded to 2^16 world seeds (phase-1 chain + swamp check, 1.20), "
+                         "as 64 whf_deepdive calls of 2^26",
+                structure_seeds=DD_TOTAL, seconds=round(secs, 2), structure_seeds_per_s=round(DD_TOTAL / secs, 1),
+                starts=st["starts"], spots=st["spots"], world_seeds=st["expanded_seeds"],
+                phase1_survivors=st["accepted"], biome_rejected=st["biome_rejected"], finds=len(finds),
+                finds_list=[list(f) for f in finds[:64]], overflow=st["overflow"])
+
+
 def deep_dive(whf, sweep_n, expand_n, cpu_seconds):
     """Deep Dive: room-floor sweep over sweep_n structure seeds, and the 2^16 expansion of the spots of the first
     expand_n; both from the first SplitMix64(3) output, hut regions [-2,1]^2, room floor <= -55, version 1.20."""
@@ -304,14 +344,17 @@ def deep_dive(whf, sweep_n, expand_n, cpu_seconds):
         expansion=dict(structure_seeds=expand_n, spots=xs["spots"], world_seeds=xs["expanded_seeds"],
                        seconds_total=round(t_exp_total, 4), stage_seconds=round(x_sec, 4),
                        world_seeds_per_s=round(x_rate, 1) if x_rate else None, finds=len(finds),
+                       phase1_survivors=xs["accepted"], biome_checked=xs["biome_checked"],
+                       biome_rejected=xs["biome_rejected"],
                        funnel=xs["funnel"], tiers=xs["tiers"], samplers_built=built,
                        roofline=dict(bound="valu", unit="Tops/s", peak=round(PEAK_VALU_OPS / 1e12, 2),
                                      achieved=round(x_ops / x_sec / 1e12, 3) if x_sec > 0 else None,
                                      frac=round(x_ops / x_sec / PEAK_VALU_OPS, 4) if x_sec > 0 else None,
-                                     work_unit="Perlin sampler built (all sets) = 271 Xoroshiro128++ steps x 21 "
+                                     work_unit="Perlin sampler built (all sets) = 259 Xoroshiro128++ draws x 25 "
                                                "int32 lane-ops; evaluations not counted")),
         full_workload=dict(structure_seeds=DD_TOTAL, projected_seconds=round(total_s, 1),
-                           structure_seeds_per_s=round(DD_TOTAL / total_s, 1) if total_s else None),
+                           structure_seeds_per_s=round(DD_TOTAL / total_s, 1) if total_s else None,
+                           measured=full_c3_from_profiles()),
         cpu_baseline=None if cpu_seconds <= 0 else deep_dive_cpu(s0, cpu_seconds),
     )
 
@@ -441,6 +484,8 @@ def main():
     ap.add_argument("--dd-seeds", type=int, default=1 << 28, help="structure seeds of the Deep Dive sweep")
     ap.add_argument("--dd-expand", type=int, default=1 << 22, help="structure seeds whose spots are expanded")
     ap.add_argument("--no-deep-dive", action="store_true")
+    ap.add_argument("--dd-full", action="store_true",
+                    help="only time the whole 2^32-structure-seed Deep Dive (minutes) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -454,6 +499,9 @@ def main():
         return 2
 
     whf = __graft_entry__.load_whf()
+    if args.dd_full:
+        print(json.dumps(deep_dive_full(whf)), flush=True)
+        return 0
     tuning = None
     if not args.no_tune:
         tuning = autotune(args.tune_seeds)

@@ -155,7 +155,10 @@ typedef struct {
     int max_height;             /* phase-1 maxHeight of the expansion's check (as in whf_request) */
     int vflags;                 /* WHF_V_* rules of the expansion's check */
     int salt;                   /* structure salt; 0 = swamp hut 14357620 */
-    int reserved;
+    int biome_tree;             /* expand = 1: the swamp check of the expansion's finds, lysh's biome test at the
+                                   chunk middle's pre-carver surface (biome.c:94-156) with tree biome_tree - 1
+                                   (whf_biome_tree); 0 = the rule flags' default tree (eval.c:104-106); -1 = none,
+                                   the finds are the phase-1 survivors */
 } whf_dd_request;
 
 /* A mineshaft start in a hut chunk with a deep room: the solver's hit (S, cx, cz, X) plus the room floor. */
@@ -188,14 +191,16 @@ typedef struct {
     uint64_t samplers_built[3];
     double device_ms;
     double stage_ms[3];         /* sweep (start tests), layout walks, expansion (device time per stage) */
+    uint64_t biome_checked;     /* expansion survivors given the swamp check (req->biome_tree >= 0) */
+    uint64_t biome_rejected;    /* of those, not swamp at the chunk middle's surface (not among the finds) */
 } whf_dd_stats;
 
 /* Deep Dive over a range of structure seeds: for every S and hut region, the hut chunk's carver seed X
  * (room_core.h:55-58), the mineshaft start test nextDouble() < 0.004 (mstree.cpp:10-15) and, for starts, the
  * layout walk that fixes the room floor (mineshaft_tree.h:529-565). Starts with a room floor <= floor_max go
  * to `spots` (the hits of the reference solver, hut_lookup_v4_source.cu:1341 `S,cx,cz,X,1`, here with the
- * floor); with expand = 1 each is expanded to its 2^16 world seeds and checked by the phase-1 chain, finds in
- * `finds` (floor_y = the room floor, flags = 1). Replaces the solve + terrain stages hunt.py drives through
+ * floor); with expand = 1 each is expanded to its 2^16 world seeds and checked by the phase-1 chain, and the
+ * survivors by the swamp biome check (req->biome_tree), finds in `finds` (floor_y = the room floor, flags = 1). Replaces the solve + terrain stages hunt.py drives through
  * SolveRun (hunt.py:947-957). Not re-entrant per context. */
 int whf_deepdive(whf_ctx *ctx, const whf_dd_request *req, whf_dd_sink *spots, whf_sink *finds, whf_dd_stats *stats);
 

@@ -433,7 +433,34 @@ int dd_expand(whf_ctx *c, const whf_dd_request *req, const whf_dd_spot *d_spots,
 
 // After the last batch of a Deep Dive call: the finds (floor_y = the room floor of their spot) and the counters
 // that the sweep and the spots entry share.
-int dd_finish(whf_ctx *c, bool expand, uint64_t spots_total,
+// The swamp check of the expansion's survivors (lysh's biome test at the chunk middle's pre-carver surface,
+// biome.c:94-156): keeps, in place and in order, the survivors whose chunk middle is swamp with the tree `tree`.
+int dd_biome_filter(whf_ctx *c, std::vector<DevHit> &hh, int tree, int vflags, uint64_t *rejected) {
+    *rejected = 0;
+    if (hh.empty()) return WHF_OK;
+    std::vector<whf_biome_query> q(hh.size());
+    for (size_t i = 0; i < hh.size(); i++) q[i] = whf_biome_query{hh[i].seed, hh[i].cx * 16, hh[i].cz * 16, WHF_OCC_SURFACE, 0};
+    std::vector<whf_biome_result> r(hh.size());
+    const int rc = whf_biome(c, q.data(), q.size(), tree, (vflags & WHF_V_LARGE) | WHF_BIO_OCC_ONLY, r.data(), nullptr);
+    if (rc != WHF_OK) return rc;
+    size_t k = 0;
+    for (size_t i = 0; i < hh.size(); i++)
+        if (r[i].ok) hh[k++] = hh[i];
+    *rejected = hh.size() - k;
+    hh.resize(k);
+    return WHF_OK;
+}
+
+// The tree of a Deep Dive request's swamp check: -1 for none (req->biome_tree < 0).
+int dd_tree(const whf_dd_request *req, int *tree) {
+    if (req->biome_tree < 0) { *tree = -1; return WHF_OK; }
+    if (req->biome_tree > 0) { *tree = req->biome_tree - 1; return WHF_OK; }
+    // the rule flags' default tree, as lysh picks it (eval.c:104-106)
+    const char *dflt = (req->vflags & WHF_V_118) ? "1.18.2" : ((req->vflags & WHF_V_262) ? "26.2" : "1.21.5");
+    return whf_biome_tree(dflt, tree);
+}
+
+int dd_finish(whf_ctx *c, bool expand, int tree, int vflags, uint64_t spots_total,
               const std::map<std::tuple<uint64_t, int, int>, int> &floor_of, whf_sink *finds, whf_dd_stats *stats) {
     const hipStream_t st = c->stream;
     HIP_TRY(hipEventRecord(c->ev1, st));
@@ -448,12 +475,23 @@ int dd_finish(whf_ctx *c, bool expand, uint64_t spots_total,
         return WHF_EOVERFLOW;
     }
     int rc = WHF_OK;
-    const uint64_t hits_total = expand ? ctl[CTL_HITS] : 0;
-    if (finds) {
+    uint64_t hits_total = expand ? ctl[CTL_HITS] : 0;
+    uint64_t checked = 0, rejected = 0;
+    if (finds || (expand && tree >= 0)) {
         std::vector<DevHit> hh(std::min<uint64_t>(hits_total, kHitCap));
         if (!hh.empty()) HIP_TRY(hipMemcpy(hh.data(), c->d_hits, hh.size() * sizeof(DevHit), hipMemcpyDeviceToHost));
-        finds->count = hits_total;
-        const uint64_t m = std::min<uint64_t>(finds->capacity, hh.size());
+        if (expand && tree >= 0) {
+            if (hh.size() < hits_total) {
+                std::snprintf(g_last_error, sizeof g_last_error, "%llu expansion survivors beyond the %u-survivor buffer",
+                              (unsigned long long)hits_total, kHitCap);
+                return WHF_EOVERFLOW;
+            }
+            checked = hh.size();
+            if ((rc = dd_biome_filter(c, hh, tree, vflags, &rejected)) != WHF_OK) return rc;
+            hits_total = hh.size();
+        }
+        if (finds) finds->count = hits_total;
+        const uint64_t m = finds ? std::min<uint64_t>(finds->capacity, hh.size()) : 0;
         for (uint64_t i = 0; i < m; i++) {
             whf_hit h;
             std::memcpy(&h, &hh[i], sizeof h);
@@ -462,7 +500,7 @@ int dd_finish(whf_ctx *c, bool expand, uint64_t spots_total,
             h.flags = WHF_HIT_DEEPDIVE;
             finds->hits[i] = h;
         }
-        if (hits_total > finds->capacity || hh.size() < hits_total) rc = WHF_EOVERFLOW;
+        if (finds && (hits_total > finds->capacity || hh.size() < hits_total)) rc = WHF_EOVERFLOW;
     }
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
@@ -474,6 +512,8 @@ int dd_finish(whf_ctx *c, bool expand, uint64_t spots_total,
         stats->draws = cnt[CNT_DD_DRAWS];
         stats->pieces = cnt[CNT_DD_PIECES];
         if (expand) {
+            stats->biome_checked = checked;
+            stats->biome_rejected = rejected;
             stats->expanded_seeds = spots_total << 16;
             uint64_t ero = 0;
             for (int t = 0; t < 7; t++) { stats->tiers[t] = cnt[CNT_TIER0 + t]; ero += cnt[CNT_TIER0 + t]; }
@@ -538,6 +578,7 @@ int whf_deepdive(whf_ctx *c, const whf_dd_request *req, whf_dd_sink *spots, whf_
     if (req->expand != 0 && req->expand != 1) return WHF_EINVAL;
     if (req->vflags < 0 || req->vflags >= kLayouts) return WHF_EINVAL;
     if (req->expand && (req->max_height < -140 || req->max_height > 100000)) return WHF_EINVAL;
+    if (req->expand && (req->biome_tree < -1 || req->biome_tree > 8)) return WHF_EINVAL;
     if (spots && spots->capacity && !spots->spots) return WHF_EINVAL;
     if (finds && finds->capacity && !finds->hits) return WHF_EINVAL;
     HIP_TRY(hipSetDevice(c->device));
@@ -582,7 +623,9 @@ int whf_deepdive(whf_ctx *c, const whf_dd_request *req, whf_dd_sink *spots, whf_
             }
         }
     }
-    rc = dd_finish(c, req->expand != 0, spots_total, floor_of, finds, stats);
+    int tree = -1;
+    if (req->expand && (rc = dd_tree(req, &tree)) != WHF_OK) return rc;
+    rc = dd_finish(c, req->expand != 0, tree, req->vflags, spots_total, floor_of, finds, stats);
     if (rc != WHF_OK && rc != WHF_EOVERFLOW) return rc;
     if (spots) {
         spots->count = spots_total;
@@ -739,7 +782,9 @@ int whf_dd_terrain(whf_ctx *c, const whf_dd_request *req, whf_dd_spot *spots, ui
             if ((rc = dd_expand(c, req, b.spots, (uint32_t)n_starts))) return rc;
         }
     }
-    rc = dd_finish(c, true, n_starts, floor_of, finds, stats);
+    // the terrain stage's survivors are the phase-1 chain's, as the reference's --spots mode reports them (hunt.py
+    // confirms them with lysh hut, which includes the biome test)
+    rc = dd_finish(c, true, -1, req->vflags, n_starts, floor_of, finds, stats);
     if (stats) {
         stats->structure_seeds = n;
         stats->tests = n;

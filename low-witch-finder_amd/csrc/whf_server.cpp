@@ -85,6 +85,8 @@ void add_dd_stats(whf_dd_stats &a, const whf_dd_stats &b) {
     for (int k = 0; k < 3; k++) a.samplers_built[k] += b.samplers_built[k];
     a.device_ms += b.device_ms;
     for (int k = 0; k < 3; k++) a.stage_ms[k] += b.stage_ms[k];
+    a.biome_checked += b.biome_checked;
+    a.biome_rejected += b.biome_rejected;
 }
 
 bool spot_less(const whf_dd_spot &x, const whf_dd_spot &y) {

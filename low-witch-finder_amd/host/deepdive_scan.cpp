@@ -219,6 +219,9 @@ int main(int argc, char **argv) {
         usage();
         return 2;
     }
+    // the expansion's swamp check uses the version's own tree (a rule-set alias takes its default)
+    int tree = 0;
+    if (whf_biome_tree(version.c_str(), &tree) == WHF_OK) req.biome_tree = tree + 1;
     whf_ctx *ctx = nullptr;
     whf_config cfg;
     std::memset(&cfg, 0, sizeof cfg);

@@ -81,7 +81,7 @@ class DdRequest(ctypes.Structure):
     _fields_ = [("s_first", ctypes.c_uint64), ("count", ctypes.c_uint64), ("rx0", ctypes.c_int), ("rx1", ctypes.c_int),
                 ("rz0", ctypes.c_int), ("rz1", ctypes.c_int), ("floor_max", ctypes.c_int), ("expand", ctypes.c_int),
                 ("max_height", ctypes.c_int), ("vflags", ctypes.c_int), ("salt", ctypes.c_int),
-                ("reserved", ctypes.c_int)]
+                ("biome_tree", ctypes.c_int)]
 
 
 class DdSpot(ctypes.Structure):
@@ -100,7 +100,8 @@ class DdStats(ctypes.Structure):
                 ("pieces", ctypes.c_uint64), ("expanded_seeds", ctypes.c_uint64), ("accepted", ctypes.c_uint64),
                 ("funnel", ctypes.c_uint64 * KIND_N), ("tiers", ctypes.c_uint64 * 8),
                 ("samplers_built", ctypes.c_uint64 * 3), ("device_ms", ctypes.c_double),
-                ("stage_ms", ctypes.c_double * 3)]
+                ("stage_ms", ctypes.c_double * 3), ("biome_checked", ctypes.c_uint64),
+                ("biome_rejected", ctypes.c_uint64)]
 
 
 class BiomeQuery(ctypes.Structure):
@@ -246,6 +247,7 @@ def _dd_stats(st, rc, fsink):
                 spots=st.spots, draws=st.draws, pieces=st.pieces, expanded_seeds=st.expanded_seeds,
                 accepted=st.accepted, funnel=list(st.funnel), tiers=list(st.tiers)[:7],
                 samplers_built=list(st.samplers_built), device_ms=st.device_ms, stage_ms=list(st.stage_ms),
+                biome_checked=st.biome_checked, biome_rejected=st.biome_rejected,
                 overflow=rc == EOVERFLOW, find_count=fsink.count)
 
 
@@ -345,12 +347,13 @@ class Context:
 
 
     def deepdive(self, s_first, count, rect, floor_max=-55, expand=False, max_height=-50, vflags=0, salt=0,
-                 spot_capacity=1 << 16, find_capacity=1 << 16):
-        """Deep Dive over structure seeds S = s_first + i (mod 2^48), i < count, hut regions rect. Returns
-        (spots as tuples (s48, rx, rz, cx, cz, carver, room_floor, pieces), finds as tuples
+                 spot_capacity=1 << 16, find_capacity=1 << 16, biome=True):
+        """Deep Dive over structure seeds S = s_first + i (mod 2^48), i < count, hut regions rect. biome: True for the
+        rule flags' default tree, an int for tree `biome`, False for no swamp check of the expansion's survivors.
+        Returns (spots as tuples (s48, rx, rz, cx, cz, carver, room_floor, pieces), finds as tuples
         (seed, rx, rz, cx, cz, room_floor), stats dict)."""
         req = DdRequest(s_first & 0xFFFFFFFFFFFFFFFF, count, rect[0], rect[1], rect[2], rect[3], floor_max,
-                        1 if expand else 0, max_height, vflags, salt, 0)
+                        1 if expand else 0, max_height, vflags, salt, _dd_biome(biome))
         sbuf = (DdSpot * spot_capacity)()
         ssink = DdSink(sbuf, spot_capacity, 0)
         fbuf = (Hit * find_capacity)()
@@ -488,6 +491,15 @@ def biome_tree(version):
     return t.value
 
 
+def _dd_biome(biome):
+    """whf_dd_request.biome_tree of a `biome` argument: True = the rules' default tree, False = none, N = tree N."""
+    if biome is True:
+        return 0
+    if biome is False or biome is None:
+        return -1
+    return int(biome) + 1
+
+
 def _s64(v):
     v &= 0xFFFFFFFFFFFFFFFF
     return v - (1 << 64) if v >= 1 << 63 else v
@@ -551,10 +563,10 @@ class Server:
         return rid.value
 
     def submit_deepdive(self, s_first, count, rect, floor_max=-55, expand=True, max_height=-50, vflags=0, salt=0,
-                        spot_capacity=1 << 20, find_capacity=1 << 16):
+                        spot_capacity=1 << 20, find_capacity=1 << 16, biome=True):
         """Queues a background Deep Dive; its event (from poll) carries spots, finds and dd_stats."""
         req = DdRequest(s_first & 0xFFFFFFFFFFFFFFFF, count, rect[0], rect[1], rect[2], rect[3], floor_max,
-                        1 if expand else 0, max_height, vflags, salt, 0)
+                        1 if expand else 0, max_height, vflags, salt, _dd_biome(biome))
         sbuf = (DdSpot * spot_capacity)()
         ssink = DdSink(sbuf, spot_capacity, 0)
         fbuf = (Hit * find_capacity)()
@@ -586,7 +598,8 @@ class Server:
                 out.append(dict(id=e.id, status=e.status, deepdive=True, spots=spots, finds=finds,
                                 dd_stats=dict(structure_seeds=st.structure_seeds, starts=st.starts, walks=list(st.walks),
                                               spots=st.spots, expanded_seeds=st.expanded_seeds, accepted=st.accepted,
-                                              funnel=list(st.funnel), tiers=list(st.tiers)[:7]),
+                                              funnel=list(st.funnel), tiers=list(st.tiers)[:7],
+                                              biome_checked=st.biome_checked, biome_rejected=st.biome_rejected),
                                 submit_s=e.submit_s, start_s=e.start_s, done_s=e.done_s))
                 continue
             req, keep, buf, sink, p2 = p

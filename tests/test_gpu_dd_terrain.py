@@ -55,7 +55,8 @@ def test_spots_match_sweep_and_oracle(ctx):
     starts, shuffled: every spot filled as the oracle says, finds equal to the sweep's expansion."""
     s0 = O.splitmix(5, 0) & M48
     print("sweep", flush=True)
-    spots, finds, st = ctx.deepdive(s0, 1 << 12, (-2, 2, -2, 2), floor_max=63, expand=True, max_height=-50)
+    spots, finds, st = ctx.deepdive(s0, 1 << 12, (-2, 2, -2, 2), floor_max=63, expand=True, max_height=-50,
+                                    biome=False)
     assert len(spots) > 100 and st["walks"][2] == 0
     bad = non_starts(50, 7)
     order = [(s[0], s[3], s[4]) for s in spots] + bad

@@ -7,6 +7,8 @@ Bar: integer work, so everything is exact: start counts, walk outcomes, LCG draw
 import json
 import os
 
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 
@@ -97,10 +99,15 @@ def test_known_deep_huts_are_found(ctx, mst):
         rx, rz = f["cx"] >> 5, f["cz"] >> 5
         print("find seed=%d chunk=(%d,%d)" % (f["seed"], f["cx"], f["cz"]), flush=True)
         spots, finds, st = ctx.deepdive(f["seed"] & M48, 1, (rx, rx + 1, rz, rz + 1), floor_max=-55, expand=True,
-                                        max_height=-50)
+                                        max_height=-50, biome=False)
         assert [(s[3], s[4], s[6]) for s in spots] == [(f["cx"], f["cz"], f["room_floor"])]
         assert st["expanded_seeds"] == 1 << 16
         assert (f["seed"], rx, rz, f["cx"], f["cz"], f["room_floor"]) in finds
+        # they are witch huts in the game, so the swamp check keeps them
+        _, finds_b, st_b = ctx.deepdive(f["seed"] & M48, 1, (rx, rx + 1, rz, rz + 1), floor_max=-55, expand=True,
+                                        max_height=-50)
+        assert (f["seed"], rx, rz, f["cx"], f["cz"], f["room_floor"]) in finds_b
+        assert st_b["biome_checked"] == len(finds) and st_b["biome_rejected"] == len(finds) - len(finds_b)
 
 
 def expansion_reference(spots, max_y=-50, vflags=0):
@@ -118,8 +125,30 @@ def expansion_reference(spots, max_y=-50, vflags=0):
     return tot
 
 
+def default_tree(vflags):
+    """The rule flags' default biome tree, as lysh picks it (eval.c:104-106)."""
+    return whf.biome_tree("1.18.2" if vflags & whf.V_118 else ("26.2" if vflags & whf.V_262 else "1.21.5"))
+
+
+def biome_filter(hits, vflags):
+    """The oracle's swamp check of phase-1 survivors (lysh's biome test at the chunk middle's pre-carver surface,
+    biome.c:94-156): the survivors it keeps, and how many it rejects."""
+    tree = default_tree(vflags)
+
+    def ok(h):
+        seed, cx, cz = h[0] & 0xFFFFFFFFFFFFFFFF, h[3], h[4]
+        height, _ = O.Terrain(seed, vflags & whf.V_LARGE).height(cx * 16 + 8, cz * 16 + 8)
+        return O.Biome(seed, vflags & whf.V_LARGE).check(tree, cx * 16, cz * 16, height - 1)["winner"] == 6
+
+    with ThreadPoolExecutor(16) as ex:
+        keep = list(ex.map(ok, hits))
+    return [h for h, k in zip(hits, keep) if k], keep.count(False)
+
+
 @pytest.mark.timeout(600)
 def test_expansion_matches_oracle_scan(ctx):
+    """Phase-1 chain of the 2^16 expansion equal to the oracle's scan; then the swamp check of the survivors equal to
+    the oracle's biome test, the finds being the survivors it keeps."""
     s0 = sweep_start()
     spots, finds, st = ctx.deepdive(s0, 1 << 16, (-2, 2, -2, 2), floor_max=-55, expand=True, max_height=-50)
     assert len(spots) > 16   # more than one expansion batch
@@ -129,7 +158,36 @@ def test_expansion_matches_oracle_scan(ctx):
     assert st["funnel"] == ref["funnel"]
     assert st["tiers"] == ref["tiers"]
     assert st["accepted"] == len(ref["hits"])
-    assert finds == ref["hits"]
+    print("biome check of %d survivors" % len(ref["hits"]), flush=True)
+    want, rejected = biome_filter(ref["hits"], 0)
+    assert st["biome_checked"] == len(ref["hits"]) and st["biome_rejected"] == rejected
+    assert finds == want
+    # without the check the finds are the phase-1 survivors
+    _, finds1, st1 = ctx.deepdive(s0, 1 << 16, (-2, 2, -2, 2), floor_max=-55, expand=True, max_height=-50,
+                                  biome=False)
+    assert finds1 == ref["hits"] and st1["biome_checked"] == 0
+
+
+@pytest.mark.timeout(300)
+def test_expansion_swamp_check_keeps_and_rejects_as_oracle(ctx):
+    """Enough expansion survivors for the swamp check to both keep and reject: a 2^20-seed sweep (518 spots), phase 1
+    at maxHeight 60, gives 136 survivors of which the check rejects 6. The survivors come from the same call without
+    the check (phase 1's parity with the oracle is the test above); each is put to the oracle's biome test."""
+    s0 = sweep_start()
+    rect = (-2, 2, -2, 2)
+    spots, surv, st0 = ctx.deepdive(s0, 1 << 20, rect, floor_max=-55, expand=True, max_height=60, biome=False)
+    _, finds, st = ctx.deepdive(s0, 1 << 20, rect, floor_max=-55, expand=True, max_height=60)
+    print("%d spots, %d survivors" % (len(spots), len(surv)), flush=True)
+    assert st["funnel"] == st0["funnel"] and st0["biome_checked"] == 0
+    want, rejected = biome_filter(surv, 0)
+    print("oracle keeps %d, rejects %d" % (len(want), rejected), flush=True)
+    assert st["biome_checked"] == len(surv) and st["biome_rejected"] == rejected
+    assert finds == want
+    assert want and rejected, "both outcomes exercised"
+    # the first spots' survivors are the oracle's phase-1 scan at maxHeight 60
+    ref = expansion_reference(spots[:4], max_y=60)
+    first = {(s[0], s[3], s[4]) for s in spots[:4]}
+    assert [h for h in surv if (h[0] & M48, h[3], h[4]) in first] == ref["hits"]
 
 
 @pytest.mark.parametrize("version", ["1.18", "26.2"])
@@ -141,7 +199,9 @@ def test_expansion_versions_match_oracle(ctx, mst, version):
                                     max_height=-50, vflags=vflags)
     ref = expansion_reference(spots, vflags=vflags)
     assert st["funnel"] == ref["funnel"] and st["tiers"] == ref["tiers"]
-    assert finds == ref["hits"]
+    want, rejected = biome_filter(ref["hits"], vflags)
+    assert st["biome_checked"] == len(ref["hits"]) and st["biome_rejected"] == rejected
+    assert finds == want
 
 
 def test_invalid_requests_are_rejected(ctx):

@@ -164,7 +164,8 @@ def test_background_deepdive_and_requests_equal_alone():
     assert d["status"] == whf.OK
     assert d["spots"] == dd_spots
     assert d["finds"] == dd_finds
-    for k in ("structure_seeds", "starts", "walks", "spots", "expanded_seeds", "accepted", "funnel", "tiers"):
+    for k in ("structure_seeds", "starts", "walks", "spots", "expanded_seeds", "accepted", "funnel", "tiers",
+              "biome_checked", "biome_rejected"):
         assert d["dd_stats"][k] == dd_st[k], k
     # the interactive requests were not held behind the whole Deep Dive
     assert min(got[i]["done_s"] for i in ids) < d["done_s"]
