@@ -40,6 +40,9 @@ PEAK_VALU_OPS = 256 * 4 * 16 * 2.4e9
 # kernel does all 59 in FP64.
 FLOP_BLEND_PER_EVAL = 51
 FLOP_COORD_PER_EVAL = 8
+# One 3D Perlin evaluation: 58 FLOP (8 gradients x 2, 3 fades x 7, 7 lerps x 3) and 12 FLOP of coordinates.
+FLOP_3D_PER_EVAL = 58
+FLOP_3D_COORD = 12
 # Algorithmic integer work of one Perlin sampler build: 259 Xoroshiro128++ draws of the permutation and origin
 # (noise.c:61-78), each about 25 32-bit lane operations (64-bit state update and output on 32-bit halves, the
 # bounded draw's multiply-high), plus the 256 byte swaps.
@@ -65,14 +68,48 @@ def erosion_evals(st):
     return sum((t + 2) * n for t, n in enumerate(st["tiers"])) + 8 * (st["checks"] - st["funnel"][0])
 
 
-def family_work(st, fp32_gate):
-    """Algorithmic work per kernel family of one scan: (fp32 FLOP, fp64 FLOP, int32 lane-ops)."""
+def gate_evals(st, max_height, single_biome=False):
+    """Perlin evaluations of gates 2 and 3 in one scan, as lysh's phase-1 chain makes them (phase1.c:115-200), from
+    the scan's funnel, per kernel family: {"gate2": (y0, 3d), "gate3": (y0, 3d)}. Octaves per double noise (noise_table.h, non-zero
+    amplitudes, two samplers each): temperature 4, ridge 6, continentalness 18 y = 0 evaluations; Entrance 18,
+    Entrance2 12, Cheese 16, floodedness 2 3D evaluations (caves.c:99-131). A Cheese runs only where the Entrance or
+    Entrance2 beside it is >= 0; the funnel shows that only for the regions the pair rejects, so the other Cheese
+    calls are not counted and the figures are lower bounds. A region rejected inside a ladder is counted at its first
+    level."""
+    f = st["funnel"]
+    r_temp = st["checks"] - f[0]
+    r_ridge = r_temp - f[1]
+    r_e50 = r_ridge - f[2]
+    r_e60 = r_e50 - f[3]
+    r_maxh = r_e60 - f[4]
+    # gate 2 (k_gate2a, k_gate2b): temperature, ridge, Entrance@50 and @60; gate 3 (k_gate3): cave@maxHeight, the
+    # ladders, continentalness and floodedness
+    g2 = (4 * r_temp + 6 * r_ridge, 18 * r_e50 + 18 * r_e60)
+    ladder_floor = max(-50, min(-40, max_height))
+    neg_levels = sum(1 for y in range(0, ladder_floor - 1, -10) if max_height < y)
+    flood_levels = len(range(max_height, 61, 10))
+    r_neg = r_maxh - f[5]
+    r_pos = r_neg - f[6]
+    r_cont = r_pos - f[7]
+    r_flood = r_cont - (0 if single_biome else f[8])
+    g3_3d = 12 * r_maxh + 16 * f[5] + (12 * neg_levels * (r_neg - f[6]) + (12 + 16) * f[6] if neg_levels else 0) + \
+        18 * 4 * (r_pos - f[7]) + (18 + 16) * f[7] + 2 * flood_levels * (r_flood - f[9]) + 2 * f[9]
+    g3_y0 = 0 if single_biome else 18 * r_cont
+    return {"gate2": g2, "gate3": (g3_y0, g3_3d)}
+
+
+def family_work(st, fp32_gate, max_height=-50):
+    """Algorithmic work per kernel family of one scan: (fp32 FLOP, fp64 FLOP, int32 lane-ops). Gates 2 and 3 run in
+    FP64: 59 FLOP per y = 0 evaluation, 70 per 3D evaluation."""
     built = st["samplers_built"]
     ev = erosion_evals(st)
     ero = (ev * FLOP_BLEND_PER_EVAL, ev * FLOP_COORD_PER_EVAL, 0) if fp32_gate else \
         (0, ev * (FLOP_BLEND_PER_EVAL + FLOP_COORD_PER_EVAL), 0)
-    return {"build_erosion": (0, 0, built[0] * OPS_PER_SAMPLER), "build_climate": (0, 0, built[1] * OPS_PER_SAMPLER),
-            "build_ladder": (0, 0, built[2] * OPS_PER_SAMPLER), "erosion_gate": ero}
+    out = {"build_erosion": (0, 0, built[0] * OPS_PER_SAMPLER), "build_climate": (0, 0, built[1] * OPS_PER_SAMPLER),
+           "build_ladder": (0, 0, built[2] * OPS_PER_SAMPLER), "erosion_gate": ero}
+    for fam, (y0, d3) in gate_evals(st, max_height).items():
+        out[fam] = (0, y0 * (FLOP_BLEND_PER_EVAL + FLOP_COORD_PER_EVAL) + d3 * (FLOP_3D_PER_EVAL + FLOP_3D_COORD), 0)
+    return out
 
 
 def roofline_of(work, seconds):
@@ -554,6 +591,7 @@ def main():
     fp32 = best == whf.PREC_FP32_GATE
     with whf.Context(0, profile=True) as ctx:
         _, prof = ctx.scan(rect, precision=best, **kw)
+        shape = ctx.tuning(vflags)
     fam = prof["kernel_ms"]
     n_batches = -(-args.seeds // (1 << 17))
     built = prof["samplers_built"]
@@ -573,7 +611,10 @@ def main():
             continue
         rooflines_alone[name] = dict(roofline_of(work[name], ms / 1e3), launch_ms=round(ms / n_batches, 4))
     # the noise kernel: the erosion gate of the faster precision
-    kernel = "k_erosion<4, true>" if fp32 else "k_erosion<4, false>"
+    # the template instance whf_scan launches for this rect (16 regions wide: 4 seeds per tile) and shape; the plain
+    # FP64 kernel and the 512-wide workgroups take one item per lane
+    ilp = shape["ero_ilp"] if fp32 and shape["ero_wg"] == 256 else 1
+    kernel = "k_erosion<4, %s, %d, %d>" % ("true" if fp32 else "false", shape["ero_wg"], ilp)
     evals = erosion_evals(prof)
     r = rooflines["erosion_gate"]
     ra = rooflines_alone["erosion_gate"]
