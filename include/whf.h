@@ -232,6 +232,63 @@ int whf_dd_terrain(whf_ctx *ctx, const whf_dd_request *req, whf_dd_spot *spots, 
 int whf_room_floors(whf_ctx *ctx, const uint64_t *carvers, uint64_t n, int32_t *status, int32_t *room_floor,
                     int32_t *pieces);
 
+/* ---- Deep Dive solver: structure seeds whose hut chunk carries a catalogued mineshaft layout ---- */
+
+/* A plan line (hut_lookup_v4_source.cu:1297-1305, data/hunt/58/plan.txt): a layout's carver seed X, the regime of
+ * the hut regions on each axis (0: |r| <= 16384, 1: 16384..32768, 2: 32768..58593, taskspace.h:13-19) and the tier,
+ * which chunk offsets (a, b) in [0,24)^2 are searched by their 2-adic valuations: 1 "odd" (different, the smaller
+ * 0), 2 "u2" (different, the smaller >= 1), 3 "eq" (equal) (solver_fused.cuh:25-33). A bundle is 2^20 groups. */
+typedef struct {
+    uint64_t layout;            /* X, 48 bits */
+    int regime_x, regime_z;     /* 0..2 */
+    int tier;                   /* 1..3 */
+    int reserved;
+} whf_solve_bundle;
+
+/* A solver hit: the reference's dump line "S,cx,cz,X,1" (hut_lookup_v4_source.cu:1341). */
+typedef struct {
+    uint64_t s48;               /* structure seed */
+    uint64_t layout;            /* X */
+    int32_t cx, cz;             /* hut chunk = mineshaft chunk */
+    uint32_t bundle;            /* index of the plan line (whf_solve), or of the chunk (whf_solve_chunks) */
+    uint32_t reserved;
+} whf_solve_hit;
+
+typedef struct {
+    whf_solve_hit *hits;        /* caller-owned array */
+    uint64_t capacity;
+    uint64_t count;             /* out: hits found (may exceed capacity -> WHF_EOVERFLOW; the first `capacity`
+                                   are stored, in no particular order then) */
+} whf_solve_sink;
+
+typedef struct {
+    uint64_t groups;            /* groups solved */
+    uint64_t regions;           /* hut regions of those groups */
+    uint64_t tasks;             /* (region, chunk offset) tasks */
+    uint64_t candidates;        /* low-20-bit states that pass the hut and low-bit carver filters, summed per task */
+    uint64_t leaves;            /* full 48-bit states the lift reaches (carver equation checked in full) */
+    uint64_t hits;
+    double device_ms;
+} whf_solve_stats;
+
+/* The solve5 solver over groups [first_group, first_group + n_groups) of the bundles (group g is bundle g >> 20,
+ * residue permGroup(g & 0xFFFFF), taskspace.h:24; hut_lookup_v4_source.cu:1361-1397): for every hut region of the
+ * group (taskspace.h:27-37) and chunk offset of the bundle's tier, every structure seed S whose chunk carver seed is
+ * the layout (room_core.h:55-58) and whose region's hut is at that chunk (solver_kernels.cuh:78-82, with the
+ * low-20-bit draw filter the reference's candidate pools apply, solver_kernels.cuh:241-255). The hits equal the
+ * reference solver's (solver_fused.cuh:37-129); they come back sorted by (bundle, S, cx, cz). Replaces
+ * `hut_lookup_v4 --solve-only --solve5` (hut_lookup_v4_source.cu:1293-1403), which hunt.py runs per chunk of groups
+ * (hunt.py:947-952, 1031-1059). Not re-entrant per context. */
+int whf_solve(whf_ctx *ctx, const whf_solve_bundle *bundles, uint64_t n_bundles, uint64_t first_group,
+              uint64_t n_groups, whf_solve_sink *sink, whf_solve_stats *stats);
+
+/* The same search for given chunks of one layout (the reference's --tasks-file mode of the solver,
+ * hut_lookup_v4_source.cu:1417-1419): chunk i is (cx[i], cz[i]), region (cx >> 5, cz >> 5); an offset outside
+ * [0,24)^2, or (0, 0), which no tier holds, is WHF_EINVAL. Hits carry the chunk index in `bundle`, sorted by
+ * (chunk, S). n <= 2^20. */
+int whf_solve_chunks(whf_ctx *ctx, uint64_t layout, const int32_t *cx, const int32_t *cz, uint64_t n,
+                     whf_solve_sink *sink, whf_solve_stats *stats);
+
 /* ---- Biome stage: the swamp-hut biome check of the multi-noise biome source ---- */
 
 #define WHF_BIOME_SWAMP 6     /* the trees' biome id of minecraft:swamp (biome.h:77) */

@@ -124,6 +124,8 @@ struct whf_ctx {
     size_t bio_bytes = 0;
     void *d_ter = nullptr;                // terrain stage buffers and tables (whf_terrain.hip), allocated on first use
     size_t ter_bytes = 0;
+    void *d_solve = nullptr;              // Deep Dive solver buffers and tables (whf_solve.hip), allocated on first use
+    size_t solve_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Seed Check pipelining: a batch runs in three stages on three streams (the library's stream: seeds, set E and
     // the erosion gate; stage_stream[0]: set C and gate 2; stage_stream[1]: set L and gate 3, both created by the

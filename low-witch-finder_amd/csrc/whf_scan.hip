@@ -1510,7 +1510,7 @@ void whf_close(whf_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_seeds, c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_par[0], c->d_par[1], c->d_par[2],
                     c->d_q1, c->d_q2, c->d_qx, c->d_needC, c->d_needL, c->d_slotC, c->d_slotL, c->d_seedC, c->d_seedL,
-                    c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd, c->d_dd, c->d_bio, c->d_ter};
+                    c->d_ctl, c->d_cnt, c->d_hits, c->d_gt, c->d_bd, c->d_dd, c->d_bio, c->d_ter, c->d_solve};
     for (hipStream_t s : c->stage_stream)
         if (s) (void)hipStreamSynchronize(s);
     for (void *b : bufs)

@@ -29,7 +29,7 @@ SYMBOLS = ("whf_open", "whf_close", "whf_scan", "whf_probe", "whf_version_flags"
            "whf_server_open", "whf_server_close", "whf_submit", "whf_submit_deepdive", "whf_poll",
            "whf_biome_tree", "whf_biome", "whf_surface_heights", "whf_post_carver_heights", "whf_hut_eval",
            "whf_get_tuning", "whf_set_tuning", "whf_get_tuning_for", "whf_set_tuning_for", "whf_load_tuning",
-           "whf_build_id")
+           "whf_build_id", "whf_solve", "whf_solve_chunks")
 HIT_DEEPDIVE, HIT_PHASE2 = 1, 2
 P2_NONE, P2_OR_LOWER, P2_EXACTLY = 0, 1, 2
 BIOME_SWAMP, BIOME_QY0, BIOME_QY_N = 6, -17, 97
@@ -139,6 +139,47 @@ class Event(ctypes.Structure):
                 ("submit_s", ctypes.c_double), ("start_s", ctypes.c_double), ("done_s", ctypes.c_double)]
 
 
+class SolveBundle(ctypes.Structure):
+    _fields_ = [("layout", ctypes.c_uint64), ("regime_x", ctypes.c_int), ("regime_z", ctypes.c_int),
+                ("tier", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
+class SolveHit(ctypes.Structure):
+    _fields_ = [("s48", ctypes.c_uint64), ("layout", ctypes.c_uint64), ("cx", ctypes.c_int32), ("cz", ctypes.c_int32),
+                ("bundle", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class SolveSink(ctypes.Structure):
+    _fields_ = [("hits", ctypes.POINTER(SolveHit)), ("capacity", ctypes.c_uint64), ("count", ctypes.c_uint64)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("groups", ctypes.c_uint64), ("regions", ctypes.c_uint64), ("tasks", ctypes.c_uint64),
+                ("candidates", ctypes.c_uint64), ("leaves", ctypes.c_uint64), ("hits", ctypes.c_uint64),
+                ("device_ms", ctypes.c_double)]
+
+
+TIERS = {"odd": 1, "u2": 2, "eq": 3}
+
+
+def parse_plan(text):
+    """Plan lines "X XZ class ..." (hut_lookup_v4_source.cu:1297-1305): [(X, regime_x, regime_z, tier)]. Lines
+    starting with '#' and blank lines are skipped; the class defaults to odd."""
+    out = []
+    for ln in text.splitlines():
+        if not ln.strip() or ln.startswith("#"):
+            continue
+        f = ln.split()
+        rg = f[1]
+        if len(rg) != 2 or rg[0] not in "012" or rg[1] not in "012":
+            raise ValueError("bad plan line: %r" % ln)
+        cls = f[2] if len(f) > 2 else "odd"
+        if cls not in TIERS:
+            raise ValueError("plan class %r unknown (odd | u2 | eq)" % cls)
+        out.append((int(f[0]) & ((1 << 48) - 1), int(rg[0]), int(rg[1]), TIERS[cls]))
+    return out
+
+
 KERNEL_FAMILIES = ("seeds", "build_erosion", "erosion_gate", "compact", "build_climate", "gate2", "build_ladder",
                    "gate3")
 
@@ -219,6 +260,13 @@ def load():
         L.whf_load_tuning.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(Tuning)]
         L.whf_build_id.restype = ctypes.c_char_p
         L.whf_build_id.argtypes = []
+        L.whf_solve.restype = ctypes.c_int
+        L.whf_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(SolveBundle), ctypes.c_uint64, ctypes.c_uint64,
+                                ctypes.c_uint64, ctypes.POINTER(SolveSink), ctypes.POINTER(SolveStats)]
+        L.whf_solve_chunks.restype = ctypes.c_int
+        L.whf_solve_chunks.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32), ctypes.c_uint64, ctypes.POINTER(SolveSink),
+                                       ctypes.POINTER(SolveStats)]
         L.whf_biome_tree.restype = ctypes.c_int
         L.whf_biome_tree.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         L.whf_biome.restype = ctypes.c_int
@@ -240,6 +288,10 @@ def version_flags(name):
     if rc:
         raise WhfError(rc, "version %r" % name)
     return v.value
+
+
+def _solve_stats(st):
+    return {f: getattr(st, f) for f, _ in SolveStats._fields_}
 
 
 def _dd_stats(st, rc, fsink):
@@ -402,6 +454,36 @@ class Context:
         out = [(p.s48, p.rx, p.rz, p.cx, p.cz, p.carver, p.room_floor, p.pieces) for p in sbuf[:n]]
         finds = sorted((h.seed, h.rx, h.rz, h.cx, h.cz, h.floor_y) for h in fbuf[:min(fsink.count, find_capacity)])
         return out, finds, _dd_stats(st, rc, fsink)
+
+    def solve(self, bundles, first_group, n_groups, capacity=1 << 22):
+        """The Deep Dive solver (whf_solve) over groups [first_group, first_group + n_groups) of the plan bundles
+        [(X, regime_x, regime_z, tier)]: (hits as sorted tuples (bundle, S, cx, cz, X), stats dict)."""
+        arr = (SolveBundle * len(bundles))()
+        for i, (x, rgx, rgz, tier) in enumerate(bundles):
+            arr[i] = SolveBundle(x, rgx, rgz, tier, 0)
+        buf = (SolveHit * capacity)()
+        sink = SolveSink(buf, capacity, 0)
+        st = SolveStats()
+        rc = self.L.whf_solve(self.h, arr, len(bundles), first_group, n_groups, ctypes.byref(sink), ctypes.byref(st))
+        if rc:
+            raise WhfError(rc, "whf_solve")
+        return [(h.bundle, h.s48, h.cx, h.cz, h.layout) for h in buf[:sink.count]], _solve_stats(st)
+
+    def solve_chunks(self, layout, chunks, capacity=1 << 20):
+        """The solver's search at given chunks [(cx, cz)] of one layout (whf_solve_chunks): (hits as tuples
+        (chunk index, S, cx, cz) sorted, stats dict)."""
+        n = len(chunks)
+        cx = np.ascontiguousarray([c[0] for c in chunks], dtype=np.int32)
+        cz = np.ascontiguousarray([c[1] for c in chunks], dtype=np.int32)
+        buf = (SolveHit * capacity)()
+        sink = SolveSink(buf, capacity, 0)
+        st = SolveStats()
+        p32 = ctypes.POINTER(ctypes.c_int32)
+        rc = self.L.whf_solve_chunks(self.h, layout, cx.ctypes.data_as(p32), cz.ctypes.data_as(p32), n,
+                                     ctypes.byref(sink), ctypes.byref(st))
+        if rc:
+            raise WhfError(rc, "whf_solve_chunks")
+        return [(h.bundle, h.s48, h.cx, h.cz) for h in buf[:sink.count]], _solve_stats(st)
 
     def room_floors(self, carvers):
         """Full layout walks of the given carver seeds: arrays (status, room_floor, pieces)."""

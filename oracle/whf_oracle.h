@@ -262,6 +262,16 @@ void   wo_probe_free(void *p);
 double wo_noise_amplitude(const void *noise);
 void   wo_noise_perm(const void *noise, int sub, int slot, uint8_t out[256]);
 
+/* ---- Deep Dive solver (whf_oracle_solve.c; hut_lookup_v4 --solve5, taskspace.h, solver_fused.cuh) ---- */
+uint32_t wo_solve_perm_group(uint32_t p);                                       /* taskspace.h:24 */
+uint64_t wo_solve_creg(int32_t rx, int32_t rz);                                 /* taskspace.h:25 */
+int      wo_solve_regions(uint32_t c20, int regime_x, int regime_z, int32_t *rx, int32_t *rz, int cap);
+int      wo_solve_offsets(int tier, uint8_t *ox, uint8_t *oz);                  /* solver_fused.cuh:25-33 */
+int      wo_solve_task(uint64_t X, int32_t cx, int32_t cz, uint64_t *out, int cap);
+int      wo_solve_prefixes(uint64_t X, int32_t cx, int32_t cz, int width, uint64_t *out, int cap);
+int64_t  wo_solve_tasks(uint64_t X, const int32_t *cx, const int32_t *cz, int n, int per_task_cap, int threads,
+                        uint64_t *seeds, int *counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -179,6 +179,22 @@ def lib():
         L.wo_hut_orientation.argtypes = [u64, i32, i32]
         L.wo_hut_eval.restype = i32
         L.wo_hut_eval.argtypes = [ctypes.c_void_p, i32, u64, i32, i32, i32, ctypes.POINTER(HutResult)]
+        L.wo_solve_perm_group.restype = ctypes.c_uint32
+        L.wo_solve_perm_group.argtypes = [ctypes.c_uint32]
+        L.wo_solve_creg.restype = u64
+        L.wo_solve_creg.argtypes = [ctypes.c_int32, ctypes.c_int32]
+        L.wo_solve_regions.restype = i32
+        L.wo_solve_regions.argtypes = [ctypes.c_uint32, i32, i32, ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32), i32]
+        L.wo_solve_offsets.restype = i32
+        L.wo_solve_offsets.argtypes = [i32, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint8)]
+        L.wo_solve_task.restype = i32
+        L.wo_solve_task.argtypes = [u64, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(u64), i32]
+        L.wo_solve_prefixes.restype = i32
+        L.wo_solve_prefixes.argtypes = [u64, ctypes.c_int32, ctypes.c_int32, i32, ctypes.POINTER(u64), i32]
+        L.wo_solve_tasks.restype = i64
+        L.wo_solve_tasks.argtypes = [u64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), i32, i32, i32,
+                                     ctypes.POINTER(u64), ctypes.POINTER(i32)]
         _lib = L
     return _lib
 
@@ -480,3 +496,64 @@ def cube_stats():
     out = (ctypes.c_ulonglong * 2)()
     lib().wo_cube_stats(out)
     return out[0], out[1]
+
+
+# ---- Deep Dive solver (whf_oracle_solve.c) ----
+
+def perm_group(p):
+    """The group index's 20-bit region-constant residue (taskspace.h:24)."""
+    return lib().wo_solve_perm_group(p)
+
+
+def creg(rx, rz):
+    return lib().wo_solve_creg(rx, rz)
+
+
+def solve_regions(c20, regime_x, regime_z):
+    """The hut regions [(rx, rz)] whose region constant is c20 modulo 2^20, in the regimes (taskspace.h:27-37)."""
+    cap = 60000
+    rx = (ctypes.c_int32 * cap)()
+    rz = (ctypes.c_int32 * cap)()
+    n = lib().wo_solve_regions(c20, regime_x, regime_z, rx, rz, cap)
+    return [(rx[i], rz[i]) for i in range(min(n, cap))]
+
+
+def solve_offsets(tier):
+    """The chunk offsets [(a, b)] of a tier, a-major (solver_fused.cuh:25-33)."""
+    ox = (ctypes.c_uint8 * 576)()
+    oz = (ctypes.c_uint8 * 576)()
+    n = lib().wo_solve_offsets(tier, ox, oz)
+    return [(ox[i], oz[i]) for i in range(n)]
+
+
+def solve_task(layout, cx, cz, cap=256):
+    """Every structure seed of one chunk task, ascending; None for an offset no tier holds."""
+    out = (ctypes.c_uint64 * cap)()
+    n = lib().wo_solve_task(layout, cx, cz, out, cap)
+    if n < 0:
+        return None
+    assert n <= cap
+    return [out[i] for i in range(n)]
+
+
+def solve_tasks(layout, chunks, threads=16, per_task_cap=64):
+    """wo_solve_task over chunks [(cx, cz)] on host threads: [sorted seed list per chunk]."""
+    n = len(chunks)
+    cx = (ctypes.c_int32 * n)(*[c[0] for c in chunks])
+    cz = (ctypes.c_int32 * n)(*[c[1] for c in chunks])
+    seeds = (ctypes.c_uint64 * (n * per_task_cap))()
+    counts = (ctypes.c_int * n)()
+    lib().wo_solve_tasks(layout, cx, cz, n, per_task_cap, threads, seeds, counts)
+    out = []
+    for i in range(n):
+        assert counts[i] <= per_task_cap
+        out.append(None if counts[i] < 0 else [seeds[i * per_task_cap + k] for k in range(counts[i])])
+    return out
+
+
+def solve_prefixes(layout, cx, cz, width, cap=1 << 16):
+    """The solver's search stopped at `width` bits: the kept states t mod 2^width, ascending."""
+    out = (ctypes.c_uint64 * cap)()
+    n = lib().wo_solve_prefixes(layout, cx, cz, width, out, cap)
+    assert 0 <= n <= cap
+    return [out[i] for i in range(n)]
