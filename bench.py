@@ -304,6 +304,92 @@ def deep_dive_cpu(s0, seconds):
                        % (r["structure_seeds"], seconds), starts=r["starts"], spots=r["spots"])
 
 
+# The Deep Dive solver's workload: the -58 search's plan (data/hunt/58/plan.txt: one bundle, layout 231564844655014,
+# regime 22, tier "odd"), groups from 2^17 on (hunt.py skips the first eighth of the first bundle, hunt.py:886-891).
+SOLVER_PLAN = [(231564844655014, 2, 2, 1)]
+SOLVER_FIRST = 1 << 17
+# Algorithmic integer work of the solver, per candidate (a low-20-bit state that passes the pool filters): the lift
+# of its 28 - m remaining bits, each a bit test, two conditional 32-bit adds and the xor of three running values
+# (6 lane-ops), and per full state the carver equation's bits 32..47 (two 48-bit and two 32-bit LCG products, two
+# high and two low 32-bit multiplies, xors: about 30 lane-ops).
+SOLVER_OPS_PER_LEVEL = 6
+SOLVER_OPS_PER_LEAF = 30
+
+
+def solver_cpu(seconds, threads):
+    """The oracle's solver (oracle/whf_oracle_solve.c) on host threads over chunks of the workload's first group,
+    time-bounded: tasks per second."""
+    import ctypes
+    so = os.path.join(REPO, "oracle", "_build", "libwhf_oracle.so")
+    if not os.path.exists(so):
+        return None
+    L = ctypes.CDLL(so)
+    L.wo_solve_regions.restype = ctypes.c_int
+    L.wo_solve_regions.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
+    L.wo_solve_perm_group.restype = ctypes.c_uint32
+    L.wo_solve_perm_group.argtypes = [ctypes.c_uint32]
+    L.wo_solve_offsets.restype = ctypes.c_int
+    L.wo_solve_offsets.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint8)]
+    L.wo_solve_tasks.restype = ctypes.c_int64
+    L.wo_solve_tasks.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                                 ctypes.POINTER(ctypes.c_int)]
+    X, rgx, rgz, tier = SOLVER_PLAN[0]
+    rx = (ctypes.c_int32 * 60000)()
+    rz = (ctypes.c_int32 * 60000)()
+    nr = L.wo_solve_regions(L.wo_solve_perm_group(SOLVER_FIRST & 0xFFFFF), rgx, rgz, rx, rz, 60000)
+    ox = (ctypes.c_uint8 * 576)()
+    oz = (ctypes.c_uint8 * 576)()
+    nj = L.wo_solve_offsets(tier, ox, oz)
+    done, hits = 0, 0
+    t0 = time.perf_counter()
+    batch = 64 * threads
+    while time.perf_counter() - t0 < seconds and done < nr * nj:
+        n = min(batch, nr * nj - done)
+        cx = (ctypes.c_int32 * n)(*[32 * rx[(done + i) // nj] + ox[(done + i) % nj] for i in range(n)])
+        cz = (ctypes.c_int32 * n)(*[32 * rz[(done + i) // nj] + oz[(done + i) % nj] for i in range(n)])
+        seeds = (ctypes.c_uint64 * (n * 16))()
+        counts = (ctypes.c_int * n)()
+        hits += L.wo_solve_tasks(X, cx, cz, n, 16, threads, seeds, counts)
+        done += n
+    secs = time.perf_counter() - t0
+    return dict(value=round(done / secs, 1), unit="tasks/s", cores=threads, kind="port",
+                sample="the first %d (region, offset) tasks of group %d of the -58 plan, time-bounded %.0f s: the "
+                       "oracle's solver (every low-20-bit state, then a bit-serial lift), one task per host thread"
+                       % (done, SOLVER_FIRST, seconds), tasks=done, hits=int(hits))
+
+
+def solver(whf, groups, cpu_seconds, host):
+    """The Deep Dive solver (whf_solve) over `groups` groups of the -58 plan, as one call after a one-group warm-up."""
+    with whf.Context(0) as ctx:
+        ctx.solve(SOLVER_PLAN, SOLVER_FIRST, 1)
+        t0 = time.perf_counter()
+        hits, st = ctx.solve(SOLVER_PLAN, SOLVER_FIRST, groups)
+        secs = time.perf_counter() - t0
+    note("solver: %d groups in %.3f s, %d hits" % (groups, secs, len(hits)))
+    work = st["candidates"] * 28 * SOLVER_OPS_PER_LEVEL + st["leaves"] * SOLVER_OPS_PER_LEAF
+    dev_s = st["device_ms"] / 1e3
+    cpu = None if cpu_seconds <= 0 else solver_cpu(cpu_seconds, host["threads"])
+    out = dict(workload="the -58 plan's bundle (layout 231564844655014, regime 22, tier odd), groups [%d, %d): "
+                        "hunt.py's solver stage (hut_lookup_v4 --solve-only --solve5)" % (SOLVER_FIRST,
+                                                                                        SOLVER_FIRST + groups),
+               groups=groups, regions=st["regions"], tasks=st["tasks"], candidates=st["candidates"],
+               leaves=st["leaves"], hits=len(hits), seconds=round(secs, 4), device_seconds=round(dev_s, 4),
+               groups_per_s=round(groups / secs, 2), tasks_per_s=round(st["tasks"] / secs, 1),
+               ms_per_group=round(1e3 * secs / groups, 3),
+               bundle_hours=round((1 << 20) * secs / groups / 3600, 2),
+               roofline=dict(bound="valu", unit="Tops/s", peak=round(PEAK_VALU_OPS / 1e12, 2),
+                             achieved=round(work / dev_s / 1e12, 3), frac=round(work / dev_s / PEAK_VALU_OPS, 4),
+                             kernel="k_sv_lift_groups (with k_sv_regions and k_sv_pool in the same device time)",
+                             work_unit="candidate = %d int32 lane-ops per lifted bit x 28 bits, + %d per full state"
+                                       % (SOLVER_OPS_PER_LEVEL, SOLVER_OPS_PER_LEAF)),
+               cpu_baseline=cpu)
+    if cpu:
+        out["gpu_over_cpu_tasks"] = round(out["tasks_per_s"] / cpu["value"], 1)
+    return out
+
+
 def full_c3_from_profiles():
     """The whole 2^32-structure-seed Deep Dive as one timed call (bench.py --dd-full), from its committed summary."""
     try:
@@ -521,6 +607,8 @@ def main():
     ap.add_argument("--dd-seeds", type=int, default=1 << 28, help="structure seeds of the Deep Dive sweep")
     ap.add_argument("--dd-expand", type=int, default=1 << 22, help="structure seeds whose spots are expanded")
     ap.add_argument("--no-deep-dive", action="store_true")
+    ap.add_argument("--solver-groups", type=int, default=256,
+                    help="groups of the Deep Dive solver's timing (0: none)")
     ap.add_argument("--dd-full", action="store_true",
                     help="only time the whole 2^32-structure-seed Deep Dive (minutes) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=10000)
@@ -676,6 +764,8 @@ def main():
         note("mixed serving done")
     dd = None if args.no_deep_dive else deep_dive(whf, args.dd_seeds, args.dd_expand,
                                                   0 if args.no_cpu else min(10.0, args.cpu_seconds))
+    sol = None if args.solver_groups <= 0 else solver(whf, args.solver_groups,
+                                                      0 if args.no_cpu else min(10.0, args.cpu_seconds), host)
     line = {
         "metric": METRIC, "value": round(checks / secs, 1), "unit": "checks/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(secs * 1e3, 3), "higher_is_better": True, "scaling": "weak",
@@ -714,6 +804,7 @@ def main():
         "serving": srv,
         "serving_mixed": mixed,
         "deep_dive": dd,
+        "deep_dive_solver": sol,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -204,34 +204,68 @@ __device__ __forceinline__ void leaf(const TaskK &T, uint64_t t, SvHit *__restri
     }
 }
 
-// Lift where the valuations differ: one choice per level, then the m free top bits. Returns the leaves checked.
-__device__ __forceinline__ uint32_t lift_unique(const TaskK &T, uint32_t t20, SvHit *__restrict__ hits,
-                                                uint32_t *__restrict__ hit_ctr, uint32_t hit_cap) {
+// A full 48-bit state t = th:tlo whose carver equation is zero in its low 32 bits (the lift makes it so): the bits
+// 32..47 from the draws' halves, then, for the 2^-16 that pass, the exact checks. With a = h1 * 2^32 + sext(h2),
+// bits 32..47 of cx * a are mulhi(cx, h2) + cx * (h1 mod 2^16); h2, h4 are bits 16..47 of the second and fourth
+// states (48-bit products), h1, h3 mod 2^16 bits 16..31 of the first and third (32-bit products).
+__device__ __forceinline__ uint32_t bits16_47(uint64_t mul, uint64_t add, uint32_t tlo, uint32_t th) {
+    const uint64_t p = (uint64_t)tlo * (uint32_t)mul + add;   // wraps modulo 2^64, exact modulo 2^48
+    const uint32_t hi = (uint32_t)(p >> 32) + th * (uint32_t)mul + tlo * (uint32_t)(mul >> 32);
+    return (hi << 16) | ((uint32_t)p >> 16);
+}
+
+constexpr uint64_t kLA3 = (kLA2 * kLA) & kM48;
+constexpr uint64_t kLK3 = (kLC * (kLA * kLA + kLA + 1)) & kM48;
+
+__device__ __forceinline__ void leaf_fast(const TaskK &T, uint32_t tlo, uint32_t th, SvHit *__restrict__ hits,
+                                          uint32_t *__restrict__ hit_ctr, uint32_t hit_cap) {
+    const int32_t h2 = (int32_t)bits16_47(kLA2, kLK2, tlo, th), h4 = (int32_t)bits16_47(kLA4, kLK4, tlo, th);
+    const int32_t h1lo = (int32_t)((tlo * (uint32_t)kLA + (uint32_t)kLC) >> 16);
+    const int32_t h3lo = (int32_t)((tlo * (uint32_t)kLA3 + (uint32_t)kLK3) >> 16);
+    const uint32_t ha = (uint32_t)(__mulhi(T.cx, h2) + T.cx * h1lo);
+    const uint32_t hb = (uint32_t)(__mulhi(T.cz, h4) + T.cz * h3lo);
+    const uint32_t e = ha ^ hb ^ th ^ (uint32_t)(kLA >> 32) ^ (uint32_t)(T.X >> 32);
+    if (e & 0xFFFFu) return;
+    leaf(T, ((uint64_t)th << 32) | tlo, hits, hit_ctr, hit_cap);
+}
+
+// Lift where the valuations differ (m = M): one choice per level, then the M free top bits. The running values are
+// SX = (S xor X) mod 2^32, P = cx * h2 and Q = cz * h4 mod 2^32; setting t's bit j adds cx * a2 * 2^(j-16) to P
+// and cz * a4 * 2^(j-16) to Q (h2 grows by a2 * 2^(j-16): the LCG is affine and j >= 16) and flips S's bit j.
+// Returns the leaves checked.
+template <int M>
+__device__ __forceinline__ uint32_t lift_unique_m(const TaskK &T, uint32_t t20, SvHit *__restrict__ hits,
+                                                  uint32_t *__restrict__ hit_ctr, uint32_t hit_cap) {
     const uint32_t cx = (uint32_t)T.cx, cz = (uint32_t)T.cz, x32 = (uint32_t)T.X;
-    const int m = T.m;
-    uint32_t S = t20 ^ (uint32_t)kLA;
+    const uint32_t A2 = cx * (uint32_t)kLA2, A4 = cz * (uint32_t)kLA4;
+    uint32_t SX = t20 ^ (uint32_t)kLA ^ x32;
     uint32_t P = cx * h_bits(kLA2, kLK2, t20);
     uint32_t Q = cz * h_bits(kLA4, kLK4, t20);
-    const uint32_t A2 = cx * (uint32_t)kLA2, A4 = cz * (uint32_t)kLA4;
-    uint32_t E = S ^ P ^ Q ^ x32;
-    uint32_t u = 0;   // t bits 20..47
+    uint32_t th = 0;   // t bits 32..47
 #pragma unroll
-    for (int j = 20; j < 48; j++) {
-        if (j > 47 - m) break;   // m is the same for the whole wave (one task per wave)
-        const uint32_t bit = (E >> (uint32_t)(j + m - 16)) & 1u;
+    for (int j = 20; j <= 47 - M; j++) {
+        const uint32_t bit = ((SX ^ P ^ Q) >> (j + M - 16)) & 1u;
         const uint32_t msk = 0u - bit;
         P += (A2 << (j - 16)) & msk;
         Q += (A4 << (j - 16)) & msk;
-        if (j < 32) S ^= (1u << j) & msk;
-        E = S ^ P ^ Q ^ x32;
-        u |= bit << (j - 20);
+        if (j < 32) SX ^= (1u << j) & msk;
+        else th |= bit << (j - 32);
     }
-    const uint32_t n_leaves = 1u << m;
-    for (uint32_t tl = 0; tl < n_leaves; tl++) {
-        const uint64_t t = (uint64_t)t20 | ((uint64_t)(u | (tl << (28 - m))) << 20);
-        leaf(T, t, hits, hit_ctr, hit_cap);
+    const uint32_t tlo = t20 | ((SX ^ x32 ^ (uint32_t)kLA) & 0xFFF00000u);
+#pragma unroll
+    for (uint32_t tl = 0; tl < (1u << M); tl++) leaf_fast(T, tlo, th | (tl << (16 - M)), hits, hit_ctr, hit_cap);
+    return 1u << M;
+}
+
+__device__ __forceinline__ uint32_t lift_unique(const TaskK &T, uint32_t t20, SvHit *__restrict__ hits,
+                                                uint32_t *__restrict__ hit_ctr, uint32_t hit_cap) {
+    switch (T.m) {   // the same for the whole wave (one task per wave)
+    case 0: return lift_unique_m<0>(T, t20, hits, hit_ctr, hit_cap);
+    case 1: return lift_unique_m<1>(T, t20, hits, hit_ctr, hit_cap);
+    case 2: return lift_unique_m<2>(T, t20, hits, hit_ctr, hit_cap);
+    case 3: return lift_unique_m<3>(T, t20, hits, hit_ctr, hit_cap);
+    default: return lift_unique_m<4>(T, t20, hits, hit_ctr, hit_cap);
     }
-    return n_leaves;
 }
 
 // Lift where the valuations are equal: a level's bit does not decide the equation bit it reaches, so both values
@@ -255,7 +289,7 @@ __device__ __forceinline__ uint32_t lift_branch(const TaskK &T, uint32_t t20, Sv
         }
         bool dead = false;
         if (j == 48) {
-            leaf(T, t, hits, hit_ctr, hit_cap);
+            leaf_fast(T, (uint32_t)t, (uint32_t)(t >> 32), hits, hit_ctr, hit_cap);
             leaves++;
             dead = true;
         } else {
@@ -397,14 +431,14 @@ __global__ __launch_bounds__(kSvWG) void k_sv_lift_groups(const SvRegion *__rest
     for (uint32_t i = threadIdx.x; i < 2 * nJ; i += kSvWG) s_off[i] = offs[i];
     __syncthreads();
     const uint64_t n_tasks = (uint64_t)(*n_regions) * nJ;
-    const uint32_t waves = kSvWG / kWave, wave = threadIdx.x / kWave;
+    const uint32_t waves = kSvWG / kWave, wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     uint32_t cand = 0, leaves = 0, tasks = 0;
     for (uint64_t task = (uint64_t)blockIdx.x * waves + wave; task < n_tasks; task += (uint64_t)gridDim.x * waves) {
         const uint32_t r = (uint32_t)(task / nJ), j = (uint32_t)(task - (uint64_t)r * nJ);
         const SvRegion R = regions[r];
         TaskK T;
-        T.ox = s_off[j];
-        T.oz = s_off[nJ + j];
+        T.ox = __builtin_amdgcn_readfirstlane(s_off[j]);
+        T.oz = __builtin_amdgcn_readfirstlane(s_off[nJ + j]);
         T.cx = 32 * R.rx + T.ox;
         T.cz = 32 * R.rz + T.oz;
         const int vx = val32(T.cx), vz = val32(T.cz);
@@ -437,7 +471,7 @@ __global__ __launch_bounds__(kSvWG) void k_sv_lift_chunks(const int32_t *__restr
     __shared__ unsigned long long s_acc[3];
     if (threadIdx.x < 3) s_acc[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t waves = kSvWG / kWave, wave = threadIdx.x / kWave;
+    const uint32_t waves = kSvWG / kWave, wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     uint32_t cand = 0, leaves = 0, tasks = 0;
     for (uint32_t task = blockIdx.x * waves + wave; task < n; task += gridDim.x * waves) {
         TaskK T;

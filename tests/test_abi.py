@@ -115,3 +115,49 @@ def test_biome_map_cli_errors_exit_2_before_touching_the_gpu():
                  base + ["--seed", "x"], ["--seed", "5", "--x0", str(1 << 31), "--z0", "0", "--w", "4", "--h", "4"]):
         r = subprocess.run([whf.BIOME_MAP, *args], capture_output=True, text=True, timeout=60)
         assert r.returncode == 2, (args, r.returncode, r.stderr)
+
+
+@pytest.mark.skipif(not os.path.exists(whf.DEEPDIVE_SCAN), reason="deepdive_scan not built")
+def test_solver_command_line_errors_exit_2_before_touching_the_gpu(tmp_path):
+    """deepdive_scan --solve-only --solve5 refuses what the reference's solver refuses, with status 2 and before
+    opening a device (hut_lookup_v4_source.cu:805, 841, 1297-1315)."""
+    plan = tmp_path / "plan.txt"
+    plan.write_text("231564844655014 22 odd 1 7.7e11 1 1\n")
+    cat = tmp_path / "cat.txt"
+    cat.write_text("231564844655014 331 331\n")
+    base = ["--catalog", str(cat), "--solve-only", "--solve5"]
+    cases = [
+        ["--solve-only", "--solve5", "--plan", str(plan), "--first-index", "0", "--count", "1"],   # no --catalog
+        base + ["--plan", str(plan), "--first-index", "0", "--count", "1", "--fused-blk", "100"],
+        base + ["--plan", str(plan), "--first-index", "0", "--count", "1", "--fused-gpb", "0"],
+        base + ["--plan", str(plan), "--first-index", "0", "--count", "1", "--fused-slots", "9"],
+        base + ["--plan", str(plan), "--first-index", "0", "--count", "1", "--task-batch", "0"],
+        base + ["--plan", str(plan), "--first-index", str((1 << 20) - 1), "--count", "2"],       # past the bundle
+        base + ["--plan", str(plan), "--first-index", "-1", "--count", "1"],
+        base + ["--plan", str(plan), "--first-index", "0", "--count", "0"],
+        base + ["--first-index", "0", "--count", "1"],                                            # no plan, no regime
+        base + ["--regime", "32", "--first-index", "0", "--count", "1"],
+        ["--catalog", str(cat), "--solve-only", "--plan", str(plan), "--first-index", "0", "--count", "1"],
+    ]
+    bad_plan = tmp_path / "bad.txt"
+    bad_plan.write_text("231564844655014 22 weird\n")
+    cases.append(base + ["--plan", str(bad_plan), "--first-index", "0", "--count", "1"])
+    bad_plan2 = tmp_path / "bad2.txt"
+    bad_plan2.write_text("231564844655014 2\n")
+    cases.append(base + ["--plan", str(bad_plan2), "--first-index", "0", "--count", "1"])
+    for args in cases:
+        r = subprocess.run([whf.DEEPDIVE_SCAN, *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2, (args, r.returncode, r.stderr)
+        assert "whf_open" not in r.stderr
+    # an unreadable plan is status 1, as in the reference
+    r = subprocess.run([whf.DEEPDIVE_SCAN, *base, "--plan", str(tmp_path / "absent.txt"), "--first-index", "0",
+                        "--count", "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "cannot read plan" in r.stderr
+
+
+def test_plan_parsing():
+    assert whf.parse_plan("# comment\n\n231564844655014 22 odd 1 7.7e11 1 1\n156186219368766 10 u2\n1 02\n") == [
+        (231564844655014, 2, 2, 1), (156186219368766, 1, 0, 2), (1, 0, 2, 1)]
+    for bad in ("1 3x odd\n", "1 22 weird\n", "1 222\n"):
+        with pytest.raises(ValueError):
+            whf.parse_plan(bad)
