@@ -502,17 +502,18 @@ def serving_stream(n, seed, versions=("1.20",)):
     return reqs
 
 
-def serve(whf, reqs, rate, slots, chunk_checks, background=None):
+def serve(whf, reqs, rate, slots, chunk_checks, background=None, slots_mode=False):
     """Submits reqs at Poisson arrivals of `rate` per second (rate None: all at once), after an optional background
     Deep Dive (s_first, count); returns per-request events, the wall time from the first submit to the last
-    request's completion, and the Deep Dive's event."""
+    request's completion, and the Deep Dive's event. slots_mode: the host-driven slot scheduler instead of the
+    device-resident loop."""
     import threading
     import numpy as np
     rng = np.random.default_rng(12345)
     gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
     events = {}
     dd_event = None
-    with whf.Server(0, slots=slots, chunk_checks=chunk_checks) as srv:
+    with whf.Server(0, slots=slots, chunk_checks=chunk_checks, slots_mode=slots_mode) as srv:
         lock = threading.Lock()
         ids = []
         dd_id = None
@@ -553,17 +554,17 @@ def serve(whf, reqs, rate, slots, chunk_checks, background=None):
     return [events[i] for i in ids], wall, dd_event
 
 
-def serving(whf, n, load, slots, chunk_checks, versions=("1.20",), background=None):
+def serving(whf, n, load, slots, chunk_checks, versions=("1.20",), background=None, slots_mode=False):
     import numpy as np
     reqs = serving_stream(n, 4, versions)
     mean_checks = float(np.mean([q["checks"] for q in reqs]))
     # capacity: a prefix of the same stream submitted at once (saturation)
     sat = reqs[: max(50, n // 20)]
-    ev, wall, _ = serve(whf, sat, None, slots, chunk_checks)
+    ev, wall, _ = serve(whf, sat, None, slots, chunk_checks, slots_mode=slots_mode)
     capacity = sum(e["stats"]["checks"] for e in ev) / wall
     rate = load * capacity / mean_checks
     note("serving capacity %.3g checks/s from %d requests at once" % (capacity, len(sat)))
-    ev, wall, dd = serve(whf, reqs, rate, slots, chunk_checks, background)
+    ev, wall, dd = serve(whf, reqs, rate, slots, chunk_checks, background, slots_mode=slots_mode)
     lat = np.array([e["done_s"] - e["submit_s"] for e in ev]) * 1e3
     big = np.array([q["splitmix"][1] >= 1 << 19 for q in reqs])
     checks = sum(e["stats"]["checks"] for e in ev)

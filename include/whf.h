@@ -394,12 +394,18 @@ int whf_hut_eval(whf_ctx *ctx, const whf_hut_query *huts, uint64_t n, int tree, 
 
 typedef struct whf_server whf_server;
 
+#define WHF_SERVE_SLOTS 1   /* whf_server_config.flags: the host-driven slot scheduler instead of the device loop */
+
 typedef struct {
     int device;
-    int slots;                  /* concurrent device slots (a context, stream and buffers each); 0 = 4 */
-    uint64_t chunk_checks;      /* (seed, region) checks per scheduled chunk; 0 = 2^24 */
-    uint64_t hbm_budget;        /* device bytes all slots may hold; 0 = no limit (slots are cut to fit) */
-    int flags;                  /* reserved, 0 */
+    int slots;                  /* WHF_SERVE_SLOTS: concurrent device slots (a context, stream and buffers each);
+                                   0 = 4. Unused by the device loop */
+    uint64_t chunk_checks;      /* device loop: (seed, region) checks per batch at most (0 = 2^25, the most a batch
+                                   holds; at least 2^16); WHF_SERVE_SLOTS: checks per scheduled chunk (0 = 2^24) */
+    uint64_t hbm_budget;        /* device bytes the server may hold; 0 = no limit. Device loop: its survivor and
+                                   seed-list pools are cut to fit (WHF_ENOMEM when the fixed part does not);
+                                   WHF_SERVE_SLOTS: slots are cut to fit */
+    int flags;                  /* WHF_SERVE_* */
     int reserved;
 } whf_server_config;
 
@@ -412,12 +418,20 @@ typedef struct {
     double submit_s, start_s, done_s;
 } whf_event;
 
-/* Opens a server: `slots` contexts on the device, each driven by its own host thread. Requests are cut into
+/* Opens a server. By default it is a device-resident loop: admitted requests live in a request table in device
+ * memory, and every batch of the three-stage Seed Check pipeline is packed on the device by a claim kernel that
+ * takes seeds from the live requests, least remaining work first, so one batch holds tiles of many requests, rects,
+ * game versions and precisions. Counters and survivor queues are per request on the device; a host thread only
+ * admits requests (when a table entry and space in the survivor and seed-list pools are free: admission against
+ * the HBM budget) and keeps batches flowing while the device reports unclaimed work, and another reads completed
+ * requests back, runs lysh's phase 2 on their survivors and posts their events. A request the loop cannot finish
+ * exactly (its survivor queue or the FP32 gate's re-evaluation queue overflowed) is scanned again alone with
+ * whf_scan. Served requests take at most 2^16 regions and fewer than 2^32 seeds (WHF_EINVAL otherwise).
+ * With WHF_SERVE_SLOTS: `slots` contexts on the device, each driven by its own host thread. Requests are cut into
  * chunks of about chunk_checks checks; an idle slot takes the next chunk of the request with the least work left
- * (the oldest on ties), so a request that arrives while others run starts as soon as a slot finishes its chunk,
- * small requests overtake big ones, and the work of several requests overlaps on the device (one HIP stream per
- * slot). Replaces the one-request-at-a-time `--worker` loop
- * (phase1_scan.cu:1346-1387) that hunt.py's GpuScanWorker drives (hunt.py:1276-1332). */
+ * (the oldest on ties). Either way a request that arrives while others run starts while they run, small requests
+ * overtake big ones, and the work of several requests overlaps on the device. Replaces the one-request-at-a-time
+ * `--worker` loop (phase1_scan.cu:1346-1387) that hunt.py's GpuScanWorker drives (hunt.py:1276-1332). */
 int whf_server_open(const whf_server_config *cfg, whf_server **out);
 
 /* Finishes the requests in flight, then releases the slots. Events not polled are dropped. */
@@ -430,8 +444,9 @@ void whf_server_close(whf_server *srv);
 int whf_submit(whf_server *srv, const whf_request *req, whf_sink *sink, uint64_t *id);
 
 /* Queues a background Deep Dive (request copied; see whf_deepdive). Slots run its chunks of 2^18 structure seeds
- * only when no Seed Check chunk is waiting, and at most half the slots at once, so interactive requests keep
- * their latency while the Deep Dive uses the rest of the device. The sinks and *stats are filled when it
+ * only when no Seed Check work is waiting (device loop: none left to claim; one slot context of its own), and at
+ * most half the slots at once, so interactive requests keep their latency while the Deep Dive uses the rest of
+ * the device. The sinks and *stats are filled when it
  * completes; its completion comes through whf_poll like a request's (with zero whf_stats). Thread-safe. No
  * counterpart in the reference, which runs Seed Check and Deep Dive as separate programs (hunt.py:58-63). */
 int whf_submit_deepdive(whf_server *srv, const whf_dd_request *req, whf_dd_sink *spots, whf_sink *finds,

@@ -91,6 +91,86 @@ enum Fam {
 }  // namespace whf
 
 namespace whf {
+
+// ---- device-resident serving (whf_serve.hip): the request table and the packed batches ----
+// A served request lives in one entry of a device table. Every batch of the pipeline is packed on the device: a
+// claim kernel takes seeds from the table's requests (least remaining work first) into the batch's seed slots, so
+// one batch holds tiles of many requests, rule layouts and rects. The kernels then find each slot's request, rule
+// layout and rect through the tables below, count into per-request counters and append survivors to per-request
+// queues.
+constexpr uint32_t kSrvLive = 1024;                     // request-table entries
+constexpr int kSrvClasses = 4;                          // tile classes: FP32 gate or FP64, x 4 or 8 seeds per tile
+constexpr uint32_t kSrvTileCap = kMaxSeeds / 4 + kSrvLive;   // tiles per class list (a partial tile per claim)
+constexpr uint32_t kSrvMaxRegions = 1u << 16;           // regions of a served request (one erosion tile covers all)
+constexpr int kSrvWide = 64;                            // rects of at least this many regions take 4 seeds per tile
+
+// A request as the device reads it (written by the host when the request is admitted to its entry).
+struct SReq {
+    uint64_t stream, offset;   // WHF_SEEDS_SPLITMIX / _COUNTER
+    const uint64_t *list;      // WHF_SEEDS_LIST: the request's seeds, copied into the server's device seed pool
+    uint32_t n_seeds, R;       // seeds; regions W x H
+    int32_t mode, rx0, rz0, W;
+    int32_t max_height, vflags, salt, f32;
+    uint32_t hit_base, hit_cap;   // the request's survivor queue in the server's hit pool
+    uint32_t seq, pad;            // admission order (ties of the claim order)
+};
+
+// Per-request progress, written by the device: seeds claimed into batches, seeds whose batch completed, survivors.
+struct SState {
+    uint64_t claimed, done;
+    uint32_t hits, pad;
+};
+
+// A run of at most 4 or 8 consecutive slots of one request: one erosion workgroup's tile.
+struct STile {
+    uint32_t slot0;
+    uint16_t ns, req;
+};
+
+// Seeds [first, first + n) of request `req` went to slots [slot0, slot0 + n) of the batch; their tiles are entries
+// [tile0, ...) of the class list cls.
+struct SClaim {
+    uint32_t req, slot0, n, tile0;
+    uint64_t first;
+    uint32_t cls, pad;
+};
+
+// The device side of one batch of the served pipeline (one per batch set).
+struct SBatch {
+    uint32_t n_slots, n_claims;
+    uint32_t n_tiles[kSrvClasses];
+    uint64_t claimed_items;
+    SClaim claims[kSrvLive];
+    STile tiles[kSrvClasses][kSrvTileCap];
+    uint16_t rq_of[kMaxSeeds];   // request entry of each slot
+    uint8_t lay_of[kMaxSeeds];   // rule layout (whf_request.vflags) of each slot
+};
+
+// What the served kernels read besides a batch's own buffers (whf_scan.hip's kernels take it by value).
+struct SrvView {
+    const SReq *req;             // [kSrvLive]
+    const uint16_t *rq_of;       // [kMaxSeeds]
+    const uint8_t *lay_of;       // [kMaxSeeds]
+    unsigned long long *rcnt;    // [kSrvLive][CNT_N]: the Cnt counters per request
+    SState *state;               // [kSrvLive]
+    DevHit *hit_pool;
+    const GateTables *gt;        // [kLayouts]
+};
+
+// Counts 1 into counter idx (a Cnt index; < 0: none) of the lane's request, with one atomic per distinct
+// (request, counter) pair of the wave.
+__device__ __forceinline__ void wave_count_req(int idx, uint32_t req, unsigned long long *rcnt) {
+    unsigned long long todo = __ballot(idx >= 0);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const uint32_t lr = __shfl(req, leader);
+        const int li = __shfl(idx, leader);
+        const unsigned long long same = __ballot(idx == li && req == lr) & todo;
+        if ((int)lane_id() == leader) atomicAdd(&rcnt[(size_t)lr * CNT_N + li], (unsigned long long)__popcll(same));
+        todo &= ~same;
+    }
+}
+
 // The buffers of one batch that its later stages read: its seeds, erosion survivors (Q1) and their seeds' flags,
 // set-C slots and tables, gate-2 survivors (Q2) and their seeds' flags, and its control words.
 struct BatchSet {
@@ -191,6 +271,12 @@ void launch_reset(whf_ctx *c, uint32_t ns);
 // adds to p2 (evaluated, accepted, rejected biome, height, flood) and to *ms.
 int phase2_filter(whf_ctx *c, const whf_request *req, whf_hit *hits, uint64_t n, uint64_t *kept, uint64_t p2[5],
                   double *ms);
+// The biome tree of a request's hut verdict (its biome_tree, or its rule flags' default as lysh picks it).
+int phase2_tree(const whf_request *req, int *tree);
+// A request's threshold applied to hut verdicts hr[0..n) of its survivors (evaluated with any max_y: the floor test
+// is the verdict's last step): keeps the accepted huts in place with their floor and adds to p2 as phase2_filter.
+void phase2_apply(const whf_request *req, whf_hit *hits, uint64_t n, const whf_hut_result *hr, uint64_t *kept,
+                  uint64_t p2[5]);
 // Pre-carver surface heights of n columns (seed, x, z) and the state of each column's top block (0 solid,
 // 2 water, 3 lava; 1 for a column of air); whf_terrain.hip.
 int terrain_heights(whf_ctx *c, const whf_column *cols, uint64_t n, int vflags, int32_t *height, int32_t *state);
@@ -202,6 +288,12 @@ int run_tail(whf_ctx *c, int layout, uint32_t ns, int vflags, int max_height);
 int dd_scratch_init(whf_ctx *c);
 // Resets a batch set's control words and the seed flags of its first ns seeds, on the library's stream.
 void reset_set(whf_ctx *c, const BatchSet &b, uint32_t ns);
+
+// The three stages of a served batch (whf_scan.hip), on the given streams. The batch's slots, tiles and their requests
+// are in sb and v (packed by whf_serve.hip's claim kernel); every launch has a fixed grid.
+void serve_stage_e(whf_ctx *c, const BatchSet &b, SBatch *sb, const SrvView &v, hipStream_t st);
+void serve_stage_c(whf_ctx *c, const BatchSet &b, const SrvView &v, hipStream_t st);
+void serve_stage_l(whf_ctx *c, const BatchSet &b, const SrvView &v, hipStream_t st);
 
 // The three-stage batch pipeline (whf_scan.hip). Batch k uses buffer set k % 3: the caller takes the set with
 // acquire(), issues the batch's first stage into it on the library's stream, and hands it to tail(), which issues

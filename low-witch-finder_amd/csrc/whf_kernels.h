@@ -46,7 +46,7 @@ struct FastDiv {
         return (t + ((n - t) >> 1)) >> (l - 1);
     }
 };
-inline FastDiv make_fastdiv(uint32_t d) {
+__host__ __device__ inline FastDiv make_fastdiv(uint32_t d) {
     FastDiv f{d, 0u, 0u};
     if (d <= 1) { f.d = 1; return f; }
     uint32_t l = 0;

@@ -64,7 +64,10 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
                                               int set_n, const uint32_t *__restrict__ slot_seed,
                                               const uint32_t *__restrict__ n_slots_dev, uint32_t n_slots_host,
                                               uint8_t *__restrict__ out_tab, SPar *__restrict__ out_par,
-                                              unsigned long long *__restrict__ built_cnt) {
+                                              unsigned long long *__restrict__ built_cnt,
+                                              const uint8_t *__restrict__ lay_of) {
+    // lay_of (served batches): the rule layout of each seed slot, whose descriptors follow bd at layout *
+    // kBdPerLayout; built_cnt may be null (served batches count their samplers per request elsewhere)
     __shared__ uint32_t words[64 * 64];
     const uint32_t lane = threadIdx.x;
     // with a device count, n_slots_host caps it at the table capacity
@@ -76,8 +79,9 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
         if (j >= n_jobs) break;
         const uint32_t slot = (uint32_t)(j / (uint32_t)set_n);
         const int sid = (int)(j - (uint64_t)slot * (uint32_t)set_n);
-        const uint64_t seed = seeds[slot_seed ? slot_seed[slot] : slot];
-        const BuildDesc d = bd[sid];
+        const uint32_t si = slot_seed ? slot_seed[slot] : slot;
+        const uint64_t seed = seeds[si];
+        const BuildDesc d = bd[(lay_of ? (uint32_t)lay_of[si] * (uint32_t)kBdPerLayout : 0u) + (uint32_t)sid];
 
         // worldSeed -> deriver -> createRandom(identifier) -> octave deriver (second one for sub 1)
         // -> createRandom("octave_<n>")  (noise.c:80-143, rng.c:157-170)
@@ -95,7 +99,7 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
     // one atomic per wave
     unsigned long long tot = built;
     for (int o = 32; o > 0; o >>= 1) tot += __shfl_down(tot, o);
-    if (lane == 0 && tot) atomicAdd(built_cnt, tot);
+    if (lane == 0 && tot && built_cnt) atomicAdd(built_cnt, tot);
 }
 
 // ---------------------------------------------------------------------------------------------- erosion gate
@@ -174,15 +178,22 @@ __device__ __forceinline__ void ero_evals(const uint16_t *s_tab, const Par *s_pa
 // FP32-gated fixed-point corner blend (perlin_y0_xn_q, values scaled by 2^23, weights w23) with FP64 re-evaluation of
 // undecided items. IL items per lane are evaluated at once (the FP32 gate's evaluations are short, and without a
 // second item their LDS lookup chains leave the device waiting; IL = 1 for the plain kernel).
-template <int ST, bool F32, int WG, int IL>
+//
+// SERVE (served batches): the workgroups take tiles from a device list (tiles[0 .. *n_tiles)) instead of the
+// launch's (tile, region chunk) blocks. A tile is a run of at most ST slots of one request, whose rect, salt and rule
+// layout come from the request table (srv.req); every region of the rect is one chunk (at most kSrvMaxRegions), and
+// the tile's tier counts go to its request's counters when the tile is done.
+template <int ST, bool F32, int WG, int IL, bool SERVE = false>
 __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ seeds, uint32_t n_seeds,
                                                     const uint8_t *__restrict__ tabE, const SPar *__restrict__ parE,
-                                                    const GateTables *__restrict__ gt, int rx0, int rz0, FastDiv Wd,
+                                                    const GateTables *__restrict__ gt_, int rx0_, int rz0_, FastDiv Wd_,
                                                     FastDiv Rd, uint64_t flat0, uint32_t flat_n, uint32_t chunk,
-                                                    uint32_t n_chunks, int salt, int single,
+                                                    uint32_t n_chunks, int salt_, int single_,
                                                     uint4 *__restrict__ q1, uint32_t *__restrict__ ctl,
                                                     uint32_t *__restrict__ needC, uint4 *__restrict__ qx,
-                                                    unsigned long long *__restrict__ cnt) {
+                                                    unsigned long long *__restrict__ cnt, SrvView srv,
+                                                    const STile *__restrict__ tiles,
+                                                    const uint32_t *__restrict__ n_tiles) {
     static_assert(F32 || IL == 1, "the plain kernel evaluates one item per lane");
     __shared__ __align__(16) uint16_t s_tab[ST * 8 * 256];   // byte-pair tables: P(i) | P(i + 1) << 8
     using Par = typename std::conditional<F32, EPar, SPar>::type;
@@ -202,17 +213,46 @@ __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ see
     if (tid < 8) s_cnt[tid] = 0;
     // persistent workgroups: each takes blocks (tile, region chunk) blockIdx.x, + gridDim.x, ... in turn; the barrier
     // at the end of a block's last pass keeps its tables until every wave is done with them
-    const uint32_t n_blocks = ((n_seeds + ST - 1) / ST) * n_chunks;
+    const uint32_t n_blocks = SERVE ? *n_tiles : ((n_seeds + ST - 1) / ST) * n_chunks;
     for (uint32_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
-    const uint32_t tile = blk / n_chunks;
-    const uint32_t ck = blk - tile * n_chunks;
-    const uint32_t seed0 = tile * ST;
-    const uint32_t ns = min((uint32_t)ST, n_seeds - seed0);
-    const uint32_t rbase = ck * chunk;
-    const uint32_t rn = min(chunk, flat_n - rbase);
+    uint32_t seed0, ns, rbase, rn, req = 0;
+    FastDiv Rn, Wd;
+    const GateTables *gt;
+    int rx0, rz0, salt, single;
+    if constexpr (SERVE) {
+        const STile t = tiles[blk];
+        const SReq &r = srv.req[t.req];
+        req = t.req;
+        seed0 = t.slot0;
+        ns = t.ns;
+        rbase = 0;
+        rn = r.R;
+        Rn = make_fastdiv(rn);
+        Wd = make_fastdiv((uint32_t)r.W);
+        gt = srv.gt + r.vflags;
+        rx0 = r.rx0;
+        rz0 = r.rz0;
+        salt = r.salt;
+        single = (r.vflags & WHF_V_SINGLE) ? 1 : 0;
+    } else {
+        const uint32_t tile = blk / n_chunks;
+        const uint32_t ck = blk - tile * n_chunks;
+        seed0 = tile * ST;
+        ns = min((uint32_t)ST, n_seeds - seed0);
+        rbase = ck * chunk;
+        rn = min(chunk, flat_n - rbase);
+        // regions per seed in this workgroup: the full chunk (Rd) unless this is the last, partial chunk
+        Rn = rn == Rd.d ? Rd : FastDiv{rn, 0u, 0u};
+        Wd = Wd_;
+        gt = gt_;
+        rx0 = rx0_;
+        rz0 = rz0_;
+        salt = salt_;
+        single = single_;
+    }
     const uint32_t W = Wd.d;
-    // regions per seed in this workgroup: the full chunk (Rd) unless this is the last, partial chunk
-    const FastDiv Rn = rn == Rd.d ? Rd : FastDiv{rn, 0u, 0u};
+    // FP32-gate queue overflows count into the request's counters when serving
+    unsigned long long *const cnt_blk = SERVE ? srv.rcnt + (size_t)req * CNT_N : cnt;
 
     {   // stage the tile's erosion tables as byte pairs, and the origins: a lane takes 16 bytes of a table with one
         // 16-byte load, and the byte after them (the next 16 bytes' first, or the table's first after its last), and
@@ -244,7 +284,7 @@ __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ see
     const double amplitude = ed.amplitude;
     const double *w = F32 ? ed.w23 : ed.w;   // the FP32 gate's values are scaled by 2^23
     // first region of this chunk as (row, column); a region offset ro then needs one 32-bit division
-    const uint64_t fbase = flat0 + rbase;
+    const uint64_t fbase = SERVE ? 0 : flat0 + rbase;
     const uint32_t row0 = (uint32_t)(fbase / W);
     const uint32_t col0 = (uint32_t)(fbase - (uint64_t)row0 * W);
     __syncthreads();
@@ -296,7 +336,7 @@ __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ see
             }
 #pragma unroll
             for (int u = 0; u < IL; u++) {
-                if constexpr (F32) push_recheck(undecided[u], seed0 + sl[u], cx[u], cz[u], qx, ctl, cnt);
+                if constexpr (F32) push_recheck(undecided[u], seed0 + sl[u], cx[u], cz[u], qx, ctl, cnt_blk);
                 if (single) {
                     const uint32_t pos = wave_append(alive[u], &ctl[CTL_Q1]);
                     if (alive[u]) {
@@ -371,7 +411,7 @@ __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ see
                     }
 #pragma unroll
                     for (int u = 0; u < IL; u++) {
-                        if constexpr (F32) push_recheck(undecided[u], seed0 + sl[u], cx[u], cz[u], qx, ctl, cnt);
+                        if constexpr (F32) push_recheck(undecided[u], seed0 + sl[u], cx[u], cz[u], qx, ctl, cnt_blk);
                         if (L < 3) {
                             const uint32_t pos = wave_append(alive[u], &s_n[L]);
                             if (alive[u]) {
@@ -395,29 +435,44 @@ __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ see
         if (tid < 4) s_n[tid] = 0;
         __syncthreads();
     }
+    if constexpr (SERVE) {
+        // the tile's tier counts go to its request; the next tile's staging barrier orders the reset before its
+        // first count
+        if (tid < 7) {
+            if (s_cnt[tid]) atomicAdd(&srv.rcnt[(size_t)req * CNT_N + CNT_TIER0 + tid], (unsigned long long)s_cnt[tid]);
+            s_cnt[tid] = 0;
+        }
+    }
     }   // blocks
-    if (tid < 7 && s_cnt[tid]) atomicAdd(&cnt[CNT_TIER0 + tid], (unsigned long long)s_cnt[tid]);
+    if constexpr (!SERVE)
+        if (tid < 7 && s_cnt[tid]) atomicAdd(&cnt[CNT_TIER0 + tid], (unsigned long long)s_cnt[tid]);
 }
 
 // FP64 re-evaluation of the items the FP32 gate left undecided: the reference's tiered gate from the first
 // evaluation on, with the set-E tables read from HBM (noise.c:432-459).
+// SERVE: each entry's rule layout and request come from its slot (srv), and its tier and re-evaluation counts go to
+// the request's counters.
+template <bool SERVE = false>
 __global__ __launch_bounds__(256) void k_erosion_exact(const uint8_t *__restrict__ tabE,
                                                        const SPar *__restrict__ parE,
                                                        const GateTables *__restrict__ gt,
                                                        const uint4 *__restrict__ qx, uint4 *__restrict__ q1,
                                                        uint32_t *__restrict__ ctl, uint32_t *__restrict__ needC,
-                                                       unsigned long long *__restrict__ cnt) {
+                                                       unsigned long long *__restrict__ cnt, SrvView srv) {
     __shared__ uint32_t s_cnt[8];
     if (threadIdx.x < 8) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t n = min(ctl[CTL_QX], kQxCap);
-    const EroDesc &ed = gt->ero;
     for (uint32_t q0 = blockIdx.x * 256 + (threadIdx.x & ~63u); q0 < n; q0 += gridDim.x * 256) {
         const uint32_t q = q0 + lane_id();
         bool alive = false;
+        int tier = -1;
+        uint32_t req = 0;
         uint4 e = make_uint4(0, 0, 0, 0);
         if (q < n) {
             e = qx[q];
+            const EroDesc &ed = SERVE ? srv.gt[srv.lay_of[e.x]].ero : gt->ero;
+            if constexpr (SERVE) req = srv.rq_of[e.x];
             const double x = (double)((int)e.y * 16 + 8) / 4.0, z = (double)((int)e.z * 16 + 8) / 4.0;
             double sum = 0.0;
             alive = true;
@@ -427,11 +482,17 @@ __global__ __launch_bounds__(256) void k_erosion_exact(const uint8_t *__restrict
                                            keep_precision(z * ed.sc[k] * ed.lac[k]));
                 sum += ed.w[k] * v;
                 if (k >= 1 && sum * ed.amplitude < ed.thr[k - 1]) {
-                    atomicAdd(&s_cnt[k - 1], 1u);
+                    tier = k - 1;
                     alive = false;
                     break;
                 }
             }
+        }
+        if constexpr (SERVE) {
+            wave_count_req(tier >= 0 ? CNT_TIER0 + tier : -1, req, srv.rcnt);
+            wave_count_req(q < n ? (int)CNT_QX : -1, req, srv.rcnt);
+        } else if (tier >= 0) {
+            atomicAdd(&s_cnt[tier], 1u);
         }
         const uint32_t pos = wave_append(alive, &ctl[CTL_Q1]);
         if (alive) {
@@ -439,10 +500,12 @@ __global__ __launch_bounds__(256) void k_erosion_exact(const uint8_t *__restrict
             needC[e.x] = 1u;
         }
     }
-    __syncthreads();
-    if (threadIdx.x < 7 && s_cnt[threadIdx.x])
-        atomicAdd(&cnt[CNT_TIER0 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt[CNT_QX], (unsigned long long)ctl[CTL_QX]);
+    if constexpr (!SERVE) {
+        __syncthreads();
+        if (threadIdx.x < 7 && s_cnt[threadIdx.x])
+            atomicAdd(&cnt[CNT_TIER0 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt[CNT_QX], (unsigned long long)ctl[CTL_QX]);
+    }
 }
 
 // Measured error of the FP32-gate corner blend against FP64, per set-E sampler (the erosion octaves): for
@@ -498,25 +561,32 @@ constexpr int kGateWG = 256;
 // Gate 2 in two kernels. k_gate2a: temperature and ridge with the version rules on Q1 -> Q2 (count CTL_QM); it
 // needs few registers, so many waves hide its gathers. k_gate2b: the cave ladder's Entrance@50 and @60 on those
 // survivors -> Q1, which gate 3 then reads (count CTL_Q2).
-__global__ __launch_bounds__(kGateWG) void k_gate2a(const GateTables *__restrict__ gt, const uint8_t *__restrict__ tabC,
+// SERVE (both gate-2 kernels and gate 3): each entry's rule layout (= its request's vflags) and request come from its
+// seed slot; counts go to the request's counters.
+template <bool SERVE = false>
+__global__ __launch_bounds__(kGateWG) void k_gate2a(const GateTables *__restrict__ gt_, const uint8_t *__restrict__ tabC,
                                                      const SPar *__restrict__ parC, const uint32_t *__restrict__ slotC,
                                                      const uint4 *__restrict__ q1, uint4 *__restrict__ qm,
-                                                     uint32_t *__restrict__ ctl, int vflags,
-                                                     unsigned long long *__restrict__ cnt) {
+                                                     uint32_t *__restrict__ ctl, int vflags_,
+                                                     unsigned long long *__restrict__ cnt, SrvView srv) {
     __shared__ uint32_t s_cnt[4];
     if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t n = ctl[CTL_Q1];
     const SetView C{tabC, parC, kSetSize[SET_C]};
-    const bool single = (vflags & WHF_V_SINGLE) != 0;
     for (uint32_t q0 = blockIdx.x * kGateWG + (threadIdx.x & ~63u); q0 < n; q0 += gridDim.x * kGateWG) {
         const uint32_t q = q0 + lane_id();
         bool survive = false;
+        int kind = -1;
+        uint32_t req = 0;
         uint4 e = make_uint4(0, 0, 0, 0);
         if (q < n) {
             e = q1[q];
+            const int vflags = SERVE ? (int)srv.lay_of[e.x] : vflags_;
+            const GateTables *gt = SERVE ? srv.gt + vflags : gt_;
+            if constexpr (SERVE) req = srv.rq_of[e.x];
+            const bool single = (vflags & WHF_V_SINGLE) != 0;
             const uint32_t slot = slotC[e.x];
-            int kind = -1;
             if (slot >= kMaxSeeds) {
                 atomicAdd(&cnt[CNT_SLOT_OVER], 1ull);
                 kind = 0;   // no tables: neither counted nor kept
@@ -531,22 +601,26 @@ __global__ __launch_bounds__(kGateWG) void k_gate2a(const GateTables *__restrict
                     else if ((vflags & WHF_V_262) && r <= -0.91) kind = 2;
                 }
             }
-            if (kind > 0) block_count(s_cnt, kind);
+            if (!SERVE && kind > 0) block_count(s_cnt, kind);
             else if (kind < 0) survive = true;
         }
+        if constexpr (SERVE) wave_count_req(kind > 0 ? CNT_KIND0 + kind : -1, req, srv.rcnt);
         const uint32_t pos = wave_append(survive, &ctl[CTL_QM]);
         if (survive) qm[pos] = e;
     }
-    __syncthreads();
-    if (threadIdx.x >= 1 && threadIdx.x < 3 && s_cnt[threadIdx.x])
-        atomicAdd(&cnt[CNT_KIND0 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+    if constexpr (!SERVE) {
+        __syncthreads();
+        if (threadIdx.x >= 1 && threadIdx.x < 3 && s_cnt[threadIdx.x])
+            atomicAdd(&cnt[CNT_KIND0 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+    }
 }
 
-__global__ __launch_bounds__(kGateWG) void k_gate2b(const GateTables *__restrict__ gt, const uint8_t *__restrict__ tabC,
+template <bool SERVE = false>
+__global__ __launch_bounds__(kGateWG) void k_gate2b(const GateTables *__restrict__ gt_, const uint8_t *__restrict__ tabC,
                                                      const SPar *__restrict__ parC, const uint32_t *__restrict__ slotC,
                                                      const uint4 *__restrict__ qm, uint4 *__restrict__ qout,
                                                      uint32_t *__restrict__ ctl, uint32_t *__restrict__ needL,
-                                                     unsigned long long *__restrict__ cnt) {
+                                                     unsigned long long *__restrict__ cnt, SrvView srv) {
     __shared__ uint32_t s_cnt[8];
     if (threadIdx.x < 8) s_cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -555,17 +629,24 @@ __global__ __launch_bounds__(kGateWG) void k_gate2b(const GateTables *__restrict
     for (uint32_t q0 = blockIdx.x * kGateWG + (threadIdx.x & ~63u); q0 < n; q0 += gridDim.x * kGateWG) {
         const uint32_t q = q0 + lane_id();
         bool survive = false;
+        int kind = -1;
+        uint32_t req = 0;
         uint4 e = make_uint4(0, 0, 0, 0);
         if (q < n) {
             e = qm[q];
-            atomicAdd(&s_cnt[0], 1u);   // reached the cave ladder
+            const GateTables *gt = SERVE ? srv.gt + srv.lay_of[e.x] : gt_;
+            if constexpr (SERVE) req = srv.rq_of[e.x];
+            else atomicAdd(&s_cnt[0], 1u);   // reached the cave ladder
             CaveCtx cv{gt, C, SetView{nullptr, nullptr, 0}, slotC[e.x], 0};
             const int x = (int)e.y * 16 + 3, z = (int)e.z * 16 + 3;
-            int kind = -1;
             if (cv.entrance_nonneg(x, 50, z)) kind = 3;
             else if (cv.entrance_nonneg(x, 60, z)) kind = 4;
-            if (kind > 0) block_count(s_cnt, kind);
-            else survive = true;
+            if (!SERVE && kind > 0) block_count(s_cnt, kind);
+            else if (kind < 0) survive = true;
+        }
+        if constexpr (SERVE) {
+            wave_count_req(q < n ? (int)CNT_LADDER : -1, req, srv.rcnt);
+            wave_count_req(kind > 0 ? CNT_KIND0 + kind : -1, req, srv.rcnt);
         }
         const uint32_t pos = wave_append(survive, &ctl[CTL_Q2]);
         if (survive) {
@@ -573,18 +654,23 @@ __global__ __launch_bounds__(kGateWG) void k_gate2b(const GateTables *__restrict
             needL[e.x] = 1u;
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && s_cnt[0]) atomicAdd(&cnt[CNT_LADDER], (unsigned long long)s_cnt[0]);
-    if ((threadIdx.x == 3 || threadIdx.x == 4) && s_cnt[threadIdx.x])
-        atomicAdd(&cnt[CNT_KIND0 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+    if constexpr (!SERVE) {
+        __syncthreads();
+        if (threadIdx.x == 0 && s_cnt[0]) atomicAdd(&cnt[CNT_LADDER], (unsigned long long)s_cnt[0]);
+        if ((threadIdx.x == 3 || threadIdx.x == 4) && s_cnt[threadIdx.x])
+            atomicAdd(&cnt[CNT_KIND0 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+    }
 }
 
-// Gate 3 per gate-2 survivor, in two stages per chunk of 256 entries. Stage A: cave@maxHeight, one lane per entry
-// (it decides 3 in 4 entries). Stage B: every remaining check of the survivors at once, one lane per (entry,
+// Gate 3 per gate-2 survivor, in two stages per chunk of kG3Entries entries. Stage A: cave@maxHeight, one lane per
+// entry (it decides 3 in 4 entries). Stage B: every remaining check of the survivors at once, one lane per (entry,
 // check): the negative ladder, the positive ladder, continentalness and the floodedness heights, in lysh's order
 // (phase1.c:151-196). Each check is a pure function of the entry, so the entry's reject kind is its first failing
-// check, as in the sequential scan, and the survivors are the entries with no failing check.
+// check, as in the sequential scan, and the survivors are the entries with no failing check. A chunk is a quarter of
+// the workgroup's lanes: a batch holds a few thousand entries, and small chunks spread them over more workgroups and
+// give each lane fewer of stage B's dependent noise lookups.
 constexpr int kG3Checks = 32;
+constexpr int kG3Entries = kGateWG / 4;
 
 __device__ __forceinline__ int g3_layout(int max_h, bool single, int &n_neg, int &n_flood) {
     int floor_y = max_h < -40 ? max_h : -40;
@@ -596,103 +682,154 @@ __device__ __forceinline__ int g3_layout(int max_h, bool single, int &n_neg, int
     return n_neg + 4 + (single ? 0 : 1) + n_flood;
 }
 
-__global__ __launch_bounds__(kGateWG) void k_gate3(const uint64_t *__restrict__ seeds, const GateTables *__restrict__ gt,
+// SERVE: every entry carries its own request's maxHeight and rule layout, so stage B gives each entry kG3Checks lanes
+// (the most any layout needs) and lanes past the entry's own check count idle; survivors go to the request's queue.
+template <bool SERVE = false>
+__global__ __launch_bounds__(kGateWG) void k_gate3(const uint64_t *__restrict__ seeds, const GateTables *__restrict__ gt_,
                                                     const uint8_t *__restrict__ tabC, const SPar *__restrict__ parC,
                                                     const uint8_t *__restrict__ tabL, const SPar *__restrict__ parL,
                                                     const uint32_t *__restrict__ slotC,
                                                     const uint32_t *__restrict__ slotL, const uint4 *__restrict__ surv2,
-                                                    const uint32_t *__restrict__ ctl, int vflags, int max_h,
+                                                    const uint32_t *__restrict__ ctl, int vflags_, int max_h_,
                                                     DevHit *__restrict__ hits, uint32_t *__restrict__ hit_ctr,
-                                                    uint32_t hit_cap, unsigned long long *__restrict__ cnt) {
+                                                    uint32_t hit_cap, unsigned long long *__restrict__ cnt, SrvView srv) {
     __shared__ uint32_t s_cnt[16];
-    __shared__ uint32_t s_list[kGateWG];
-    __shared__ uint32_t s_fail[kGateWG];
+    __shared__ uint32_t s_list[kG3Entries];
+    __shared__ uint32_t s_fail[kG3Entries];
     __shared__ uint32_t s_n;
     if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
     const uint32_t n = ctl[CTL_Q2];
-    const bool single = (vflags & WHF_V_SINGLE) != 0;
-    int n_neg, n_flood;
-    const int n_checks = g3_layout(max_h, single, n_neg, n_flood);
+    // the rules of an entry: its layout's tables, maxHeight, SINGLE_BIOME flag and check layout
+    struct Rules {
+        const GateTables *gt;
+        int max_h, n_neg, n_flood, n_checks;
+        bool single;
+    };
+    auto rules_of = [&](uint32_t slot) {
+        Rules r;
+        const int vflags = SERVE ? (int)srv.lay_of[slot] : vflags_;
+        r.gt = SERVE ? srv.gt + vflags : gt_;
+        r.max_h = SERVE ? srv.req[srv.rq_of[slot]].max_height : max_h_;
+        r.single = (vflags & WHF_V_SINGLE) != 0;
+        r.n_checks = g3_layout(r.max_h, r.single, r.n_neg, r.n_flood);
+        return r;
+    };
+    // the launch's rules; a served batch reads every entry's own (slot 0 of the tables need not be written)
+    const Rules launch_rules = SERVE ? Rules{gt_, 0, 0, 0, 0, false} : rules_of(0);
+    const uint32_t stride = SERVE ? (uint32_t)kG3Checks : (uint32_t)launch_rules.n_checks;
     const SetView C{tabC, parC, kSetSize[SET_C]}, L{tabL, parL, kSetSize[SET_L]};
-    for (uint32_t base = blockIdx.x * kGateWG; base < n; base += gridDim.x * kGateWG) {
+    for (uint32_t base = blockIdx.x * kG3Entries; base < n; base += gridDim.x * kG3Entries) {
         if (threadIdx.x == 0) s_n = 0;
-        s_fail[threadIdx.x] = 0;
+        if (threadIdx.x < kG3Entries) s_fail[threadIdx.x] = 0;
         __syncthreads();
-        // ---- stage A: cave@maxHeight ----
-        {
+        // ---- stage A: cave@maxHeight, the chunk's entries on its first waves ----
+        if (threadIdx.x < kG3Entries) {
             const uint32_t q = base + threadIdx.x;
             bool keep = false;
+            int idx = -1;
+            uint32_t req = 0;
             if (q < n) {
                 const uint4 e = surv2[q];
-                const CaveCtx cv{gt, C, L, slotC[e.x], slotL[e.x]};
+                const Rules r = SERVE ? rules_of(e.x) : launch_rules;
+                if constexpr (SERVE) req = srv.rq_of[e.x];
+                const CaveCtx cv{r.gt, C, L, slotC[e.x], slotL[e.x]};
                 const int x = (int)e.y * 16 + 3, z = (int)e.z * 16 + 3;
                 if (cv.slotC >= kMaxSeeds || cv.slotL >= kMaxSeeds) atomicAdd(&cnt[CNT_SLOT_OVER], 1ull);
-                else if (cv.spaghetti(x, max_h, z) >= 0 && cv.cheese(x, max_h, z) >= 0) atomicAdd(&s_cnt[5], 1u);
+                else if (cv.spaghetti(x, r.max_h, z) >= 0 && cv.cheese(x, r.max_h, z) >= 0) idx = CNT_KIND0 + 5;
                 else keep = true;
             }
+            if constexpr (SERVE) wave_count_req(idx, req, srv.rcnt);
+            else if (idx >= 0) atomicAdd(&s_cnt[5], 1u);
             const uint32_t pos = wave_append(keep, &s_n);
             if (keep) s_list[pos] = q;
         }
         __syncthreads();
         // ---- stage B: all remaining checks of the survivors, one lane per (entry, check) ----
         const uint32_t ns = s_n;
-        for (uint32_t w = threadIdx.x; w < ns * (uint32_t)n_checks; w += kGateWG) {
-            const uint32_t li = w / (uint32_t)n_checks;
-            const int j = (int)(w - li * (uint32_t)n_checks);
+        for (uint32_t w = threadIdx.x; w < ns * stride; w += kGateWG) {
+            const uint32_t li = w / stride;
+            const int j = (int)(w - li * stride);
             const uint4 e = surv2[s_list[li]];
-            const CaveCtx cv{gt, C, L, slotC[e.x], slotL[e.x]};
+            const Rules r = SERVE ? rules_of(e.x) : launch_rules;
+            if (SERVE && j >= r.n_checks) continue;
+            const CaveCtx cv{r.gt, C, L, slotC[e.x], slotL[e.x]};
             const int hut_x = (int)e.y * 16, hut_z = (int)e.z * 16, x = hut_x + 3, z = hut_z + 3;
             bool fail;
-            if (j < n_neg) {
+            if (j < r.n_neg) {
                 const int y = -10 * j;
                 fail = cv.spaghetti(x, y, z) >= 0 && cv.cheese(x, y, z) >= 0;
-            } else if (j < n_neg + 4) {
-                const int y = 10 * (j - n_neg + 1);
+            } else if (j < r.n_neg + 4) {
+                const int y = 10 * (j - r.n_neg + 1);
                 fail = cv.entrance_nonneg(x, y, z) && cv.cheese(x, y, z) >= 0;
-            } else if (!single && j == n_neg + 4) {
-                const NoiseDesc &d = gt->noise[G_CONTINENTALNESS];
+            } else if (!r.single && j == r.n_neg + 4) {
+                const NoiseDesc &d = r.gt->noise[G_CONTINENTALNESS];
                 fail = noise_y0(d, cv.L, cv.slotL, (hut_x + 8) / 4.0, (hut_z + 8) / 4.0) < -0.11;
             } else {
-                const int y = max_h + 10 * (j - n_neg - 4 - (single ? 0 : 1));
+                const int y = r.max_h + 10 * (j - r.n_neg - 4 - (r.single ? 0 : 1));
                 fail = cv.flooded(hut_x, hut_z, y) > 0.41;
             }
             if (fail) atomicOr(&s_fail[li], 1u << j);
         }
         __syncthreads();
         // ---- verdicts: first failing check, in order ----
-        {
+        if (threadIdx.x < kG3Entries) {
             const uint32_t li = threadIdx.x;
             bool accept = false;
+            int idx = -1;
+            uint32_t req = 0;
             uint4 e = make_uint4(0, 0, 0, 0);
             if (li < ns) {
+                e = surv2[s_list[li]];
+                const Rules r = SERVE ? rules_of(e.x) : launch_rules;
+                if constexpr (SERVE) req = srv.rq_of[e.x];
                 const uint32_t f = s_fail[li];
+                int kind = 10;
                 if (f == 0) {
                     accept = true;
-                    atomicAdd(&s_cnt[10], 1u);
                 } else {
                     const int j = __ffs((int)f) - 1;
-                    const int kind = j < n_neg ? 6 : (j < n_neg + 4 ? 7 : ((!single && j == n_neg + 4) ? 8 : 9));
-                    atomicAdd(&s_cnt[kind], 1u);
+                    kind = j < r.n_neg ? 6 : (j < r.n_neg + 4 ? 7 : ((!r.single && j == r.n_neg + 4) ? 8 : 9));
                 }
-                e = surv2[s_list[li]];
+                idx = CNT_KIND0 + kind;
             }
-            const uint32_t pos = wave_append(accept, hit_ctr);
-            if (accept && pos < hit_cap) {
-                DevHit h;
-                h.seed = (int64_t)seeds[e.x];
-                h.cx = (int)e.y;
-                h.cz = (int)e.z;
-                h.rx = h.cx >> 5;
-                h.rz = h.cz >> 5;
-                h.floor_y = INT32_MIN;
-                h.flags = 0;
-                hits[pos] = h;
+            if constexpr (SERVE) {
+                wave_count_req(idx, req, srv.rcnt);
+                if (accept) {
+                    // the request's survivor queue: counted past its capacity, written up to it
+                    const uint32_t pos = atomicAdd(&srv.state[req].hits, 1u);
+                    const SReq &rq = srv.req[req];
+                    if (pos < rq.hit_cap) {
+                        DevHit h;
+                        h.seed = (int64_t)seeds[e.x];
+                        h.cx = (int)e.y;
+                        h.cz = (int)e.z;
+                        h.rx = h.cx >> 5;
+                        h.rz = h.cz >> 5;
+                        h.floor_y = INT32_MIN;
+                        h.flags = 0;
+                        srv.hit_pool[(size_t)rq.hit_base + pos] = h;
+                    }
+                }
+            } else {
+                if (idx >= 0) atomicAdd(&s_cnt[idx - CNT_KIND0], 1u);
+                const uint32_t pos = wave_append(accept, hit_ctr);
+                if (accept && pos < hit_cap) {
+                    DevHit h;
+                    h.seed = (int64_t)seeds[e.x];
+                    h.cx = (int)e.y;
+                    h.cz = (int)e.z;
+                    h.rx = h.cx >> 5;
+                    h.rz = h.cz >> 5;
+                    h.floor_y = INT32_MIN;
+                    h.flags = 0;
+                    hits[pos] = h;
+                }
             }
         }
         __syncthreads();
     }
     __syncthreads();
-    if (threadIdx.x < 16 && threadIdx.x > 0 && s_cnt[threadIdx.x])
+    if (!SERVE && threadIdx.x < 16 && threadIdx.x > 0 && s_cnt[threadIdx.x])
         atomicAdd(&cnt[CNT_KIND0 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
 }
 
@@ -1014,7 +1151,8 @@ void launch_build(whf_ctx *c, const uint64_t *seeds, const BuildDesc *bd, int se
                   uint8_t *tab, SPar *par, unsigned long long *built) {
     const uint64_t jobs = (uint64_t)n_slots * (uint32_t)set_n;
     hipLaunchKernelGGL(k_build, dim3((uint32_t)std::max<uint64_t>(1, (jobs + 63) / 64)), dim3(64), 0, c->stream, seeds,
-                       bd, set_n, (const uint32_t *)nullptr, (const uint32_t *)nullptr, n_slots, tab, par, built);
+                       bd, set_n, (const uint32_t *)nullptr, (const uint32_t *)nullptr, n_slots, tab, par, built,
+                       (const uint8_t *)nullptr);
 }
 
 void launch_reset(whf_ctx *c, uint32_t ns) {
@@ -1037,14 +1175,14 @@ void stage_c(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags,
         Prof p(c, F_BUILD_C, st);
         hipLaunchKernelGGL(k_build, dim3(std::min<uint32_t>(persist * 4, (nb * 28 + 63) / 64 + 1)), dim3(64), 0, st,
                            b.seeds, bd_of(c, layout, SET_C), 28, b.seedC, b.ctl + CTL_NC, kMaxSeeds, b.tabC, b.parC,
-                           c->d_cnt + CNT_BUILT + SET_C);
+                           c->d_cnt + CNT_BUILT + SET_C, (const uint8_t *)nullptr);
     }
     {
         Prof p(c, F_GATE2, st);
-        hipLaunchKernelGGL(k_gate2a, dim3(persist), dim3(kGateWG), 0, st, c->d_gt + layout, b.tabC, b.parC, b.slotC,
-                           b.q1, b.q2, b.ctl, vflags, c->d_cnt);
-        hipLaunchKernelGGL(k_gate2b, dim3(persist), dim3(kGateWG), 0, st, c->d_gt + layout, b.tabC, b.parC, b.slotC,
-                           b.q2, b.q1, b.ctl, b.needL, c->d_cnt);
+        hipLaunchKernelGGL(k_gate2a<false>, dim3(persist), dim3(kGateWG), 0, st, c->d_gt + layout, b.tabC, b.parC,
+                           b.slotC, b.q1, b.q2, b.ctl, vflags, c->d_cnt, SrvView{});
+        hipLaunchKernelGGL(k_gate2b<false>, dim3(persist), dim3(kGateWG), 0, st, c->d_gt + layout, b.tabC, b.parC,
+                           b.slotC, b.q2, b.q1, b.ctl, b.needL, c->d_cnt, SrvView{});
     }
 }
 
@@ -1062,13 +1200,13 @@ void stage_l(whf_ctx *c, const BatchSet &b, int layout, uint32_t ns, int vflags,
         Prof p(c, F_BUILD_L, st);
         hipLaunchKernelGGL(k_build, dim3(std::min<uint32_t>(persist, (nb * 36 + 63) / 64 + 1)), dim3(64), 0, st,
                            b.seeds, bd_of(c, layout, SET_L), 36, c->d_seedL, b.ctl + CTL_NL, kMaxSeeds,
-                           c->d_tab[SET_L], c->d_par[SET_L], c->d_cnt + CNT_BUILT + SET_L);
+                           c->d_tab[SET_L], c->d_par[SET_L], c->d_cnt + CNT_BUILT + SET_L, (const uint8_t *)nullptr);
     }
     {
         Prof p(c, F_GATE3, st);
-        hipLaunchKernelGGL(k_gate3, dim3(persist), dim3(kGateWG), 0, st, b.seeds, c->d_gt + layout, b.tabC, b.parC,
-                           c->d_tab[SET_L], c->d_par[SET_L], b.slotC, c->d_slotL, b.q1, b.ctl, vflags, max_height,
-                           c->d_hits, c->d_ctl + CTL_HITS, kHitCap, c->d_cnt);
+        hipLaunchKernelGGL(k_gate3<false>, dim3(persist), dim3(kGateWG), 0, st, b.seeds, c->d_gt + layout, b.tabC,
+                           b.parC, c->d_tab[SET_L], c->d_par[SET_L], b.slotC, c->d_slotL, b.q1, b.ctl, vflags, max_height,
+                           c->d_hits, c->d_ctl + CTL_HITS, kHitCap, c->d_cnt, SrvView{});
     }
 }
 
@@ -1129,6 +1267,81 @@ int Pipeline::tail(const BatchSet &b, int layout, uint32_t ns, int vflags, int m
     return WHF_OK;
 }
 
+// ---- served batches (packed on the device by whf_serve.hip's claim kernel) ----
+
+// Stage 1: set E for the batch's slots, each with its request's rule layout, then the erosion gate over the four tile
+// classes (FP64 with 4 and 8 seeds per tile, FP32 gate with 4 and 8) and the FP64 re-evaluation of the items the FP32
+// gate left undecided. Every grid is fixed; the kernels read the batch's slot and tile counts on the device.
+void serve_stage_e(whf_ctx *c, const BatchSet &b, SBatch *sb, const SrvView &v, hipStream_t st) {
+    const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune[0].gate_wg_per_cu;
+    {
+        Prof p(c, F_BUILD_E, st);
+        hipLaunchKernelGGL(k_build, dim3(persist * 4), dim3(64), 0, st, b.seeds, bd_of(c, 0, SET_E), 8,
+                           (const uint32_t *)nullptr, &sb->n_slots, kMaxSeeds, c->d_tab[SET_E], c->d_par[SET_E],
+                           (unsigned long long *)nullptr, v.lay_of);
+    }
+    Prof p(c, F_EROSION, st);
+    const int per_cu = c->tune[0].ero_wg_per_cu > 0 ? c->tune[0].ero_wg_per_cu : 8;
+    const dim3 grid((uint32_t)c->n_cu * (uint32_t)per_cu);
+    const FastDiv one{1u, 0u, 0u};
+#define WHF_SRV_EROSION_ARGS(cls)                                                                                    \
+    grid, dim3(256), 0, st, b.seeds, 0u, c->d_tab[SET_E], c->d_par[SET_E], c->d_gt, 0, 0, one, one, (uint64_t)0, 0u, \
+        1u, 1u, 0, 0, b.q1, b.ctl, b.needC, c->d_qx, c->d_cnt, v, &sb->tiles[cls][0], &sb->n_tiles[cls]
+    hipLaunchKernelGGL((k_erosion<4, false, 256, 1, true>), WHF_SRV_EROSION_ARGS(0));
+    hipLaunchKernelGGL((k_erosion<8, false, 256, 1, true>), WHF_SRV_EROSION_ARGS(1));
+    hipLaunchKernelGGL((k_erosion<4, true, 256, 2, true>), WHF_SRV_EROSION_ARGS(2));
+    hipLaunchKernelGGL((k_erosion<8, true, 256, 2, true>), WHF_SRV_EROSION_ARGS(3));
+#undef WHF_SRV_EROSION_ARGS
+    hipLaunchKernelGGL(k_erosion_exact<true>, dim3((uint32_t)c->n_cu * 2), dim3(256), 0, st, c->d_tab[SET_E],
+                       c->d_par[SET_E], c->d_gt, c->d_qx, b.q1, b.ctl, b.needC, c->d_cnt, v);
+}
+
+// Stage 2: set C for the slots with erosion survivors and gate 2's temperature and ridge, with each slot's rule
+// layout. (Gate 2's Entrance checks run at the start of stage 3, which balances the stages of the serving mix, whose
+// small rects need set C for most seeds.)
+void serve_stage_c(whf_ctx *c, const BatchSet &b, const SrvView &v, hipStream_t st) {
+    const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune[0].gate_wg_per_cu;
+    {
+        Prof p(c, F_COMPACT, st);
+        hipLaunchKernelGGL(k_compact, dim3(kMaxSeeds / 256), dim3(256), 0, st, b.needC, kMaxSeeds, b.slotC, b.seedC,
+                           b.ctl + CTL_NC, kMaxSeeds);
+    }
+    {
+        Prof p(c, F_BUILD_C, st);
+        hipLaunchKernelGGL(k_build, dim3(persist * 4), dim3(64), 0, st, b.seeds, bd_of(c, 0, SET_C), 28, b.seedC,
+                           b.ctl + CTL_NC, kMaxSeeds, b.tabC, b.parC, (unsigned long long *)nullptr, v.lay_of);
+    }
+    Prof p(c, F_GATE2, st);
+    hipLaunchKernelGGL(k_gate2a<true>, dim3(persist), dim3(kGateWG), 0, st, c->d_gt, b.tabC, b.parC, b.slotC, b.q1,
+                       b.q2, b.ctl, 0, c->d_cnt, v);
+}
+
+// Stage 3: gate 2's Entrance checks, set L for the slots with gate-2 survivors and gate 3, whose survivors go to their
+// requests' queues.
+void serve_stage_l(whf_ctx *c, const BatchSet &b, const SrvView &v, hipStream_t st) {
+    const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune[0].gate_wg_per_cu;
+    {
+        Prof p(c, F_GATE2, st);
+        hipLaunchKernelGGL(k_gate2b<true>, dim3(persist), dim3(kGateWG), 0, st, c->d_gt, b.tabC, b.parC, b.slotC,
+                           b.q2, b.q1, b.ctl, b.needL, c->d_cnt, v);
+    }
+    {
+        Prof p(c, F_COMPACT, st);
+        hipLaunchKernelGGL(k_compact, dim3(kMaxSeeds / 256), dim3(256), 0, st, b.needL, kMaxSeeds, c->d_slotL,
+                           c->d_seedL, b.ctl + CTL_NL, kMaxSeeds);
+    }
+    {
+        Prof p(c, F_BUILD_L, st);
+        hipLaunchKernelGGL(k_build, dim3(persist), dim3(64), 0, st, b.seeds, bd_of(c, 0, SET_L), 36, c->d_seedL,
+                           b.ctl + CTL_NL, kMaxSeeds, c->d_tab[SET_L], c->d_par[SET_L], (unsigned long long *)nullptr,
+                           v.lay_of);
+    }
+    Prof p(c, F_GATE3, st);
+    hipLaunchKernelGGL(k_gate3<true>, dim3(persist), dim3(kGateWG), 0, st, b.seeds, c->d_gt, b.tabC, b.parC,
+                       c->d_tab[SET_L], c->d_par[SET_L], b.slotC, c->d_slotL, b.q1, b.ctl, 0, 0, (DevHit *)nullptr,
+                       (uint32_t *)nullptr, 0u, c->d_cnt, v);
+}
+
 int Pipeline::end() {
     for (int s = 1; s < 3 && piped; s++) {   // everything on the stage streams so far
         HIP_TRY(hipEventRecord(c->ev_stage[s][0], ss[s]));
@@ -1152,7 +1365,7 @@ int stage_e(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_
         const uint32_t jobs = ns * 8;
         hipLaunchKernelGGL(k_build, dim3((jobs + 63) / 64), dim3(64), 0, st, b.seeds, bd_of(c, layout, SET_E), 8,
                            (const uint32_t *)nullptr, (const uint32_t *)nullptr, ns, c->d_tab[SET_E], c->d_par[SET_E],
-                           c->d_cnt + CNT_BUILT + SET_E);
+                           c->d_cnt + CNT_BUILT + SET_E, (const uint8_t *)nullptr);
     }
     {
         Prof p(c, F_EROSION);
@@ -1175,7 +1388,8 @@ int stage_e(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_
         const int il = c->tune[layout].ero_ilp == 1 ? 1 : 2;   // items per lane of the FP32 gate
 #define WHF_EROSION_ARGS(WG)                                                                                      \
     grid, dim3(WG), 0, st, b.seeds, ns, c->d_tab[SET_E], c->d_par[SET_E], gt, req->rx0, req->rz0, Wd, Rd, f0, fn,      \
-        chunk, n_chunks, salt, single, b.q1, b.ctl, b.needC, c->d_qx, c->d_cnt
+        chunk, n_chunks, salt, single, b.q1, b.ctl, b.needC, c->d_qx, c->d_cnt, SrvView{}, (const STile *)nullptr,     \
+        (const uint32_t *)nullptr
         if (!f32) {
             if (wide && wg == 512) hipLaunchKernelGGL((k_erosion<4, false, 512, 1>), WHF_EROSION_ARGS(512));
             else if (wide) hipLaunchKernelGGL((k_erosion<4, false, 256, 1>), WHF_EROSION_ARGS(256));
@@ -1193,8 +1407,8 @@ int stage_e(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_
         }
 #undef WHF_EROSION_ARGS
         if (f32 && !single)
-            hipLaunchKernelGGL(k_erosion_exact, dim3((uint32_t)c->n_cu * 2), dim3(256), 0, st, c->d_tab[SET_E],
-                               c->d_par[SET_E], gt, c->d_qx, b.q1, b.ctl, b.needC, c->d_cnt);
+            hipLaunchKernelGGL(k_erosion_exact<false>, dim3((uint32_t)c->n_cu * 2), dim3(256), 0, st, c->d_tab[SET_E],
+                               c->d_par[SET_E], gt, c->d_qx, b.q1, b.ctl, b.needC, c->d_cnt, SrvView{});
     }
     return check_launch();
 }
@@ -1206,26 +1420,42 @@ namespace whf {
 // lysh's phase 2 on phase-1 survivors (lysh_grade_scan, eval.c:381-477): the hut verdict of each, keeping the accepted
 // huts (in place, in order) with their floor; counters evaluated, accepted, rejected for the biome, the height
 // (avg_y above the threshold, or not equal to it with 'exactly') and the flood.
+int phase2_tree(const whf_request *req, int *tree) {
+    *tree = req->biome_tree - 1;
+    if (req->biome_tree == 0) {
+        const char *dflt = (req->vflags & WHF_V_118) ? "1.18.2" : ((req->vflags & WHF_V_262) ? "26.2" : "1.21.5");
+        if (whf_biome_tree(dflt, tree) != WHF_OK) return WHF_EINVAL;
+    }
+    return WHF_OK;
+}
+
 int phase2_filter(whf_ctx *c, const whf_request *req, whf_hit *hits, uint64_t n, uint64_t *kept, uint64_t p2[5],
                   double *ms) {
     const auto t0 = std::chrono::steady_clock::now();
     *kept = 0;
-    int tree = req->biome_tree - 1;
-    if (req->biome_tree == 0) {
-        const char *dflt = (req->vflags & WHF_V_118) ? "1.18.2" : ((req->vflags & WHF_V_262) ? "26.2" : "1.21.5");
-        if (whf_biome_tree(dflt, &tree) != WHF_OK) return WHF_EINVAL;
-    }
+    int tree;
+    if (phase2_tree(req, &tree) != WHF_OK) return WHF_EINVAL;
     std::vector<whf_hut_query> hq(n);
     std::vector<whf_hut_result> hr(n);
     for (uint64_t i = 0; i < n; i++) hq[i] = whf_hut_query{hits[i].seed, hits[i].cx, hits[i].cz};
     const int rc = whf_hut_eval(c, hq.data(), n, tree, req->vflags & WHF_V_LARGE, req->phase2_max_y, hr.data());
     if (rc != WHF_OK) return rc;
+    phase2_apply(req, hits, n, hr.data(), kept, p2);
+    *ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return WHF_OK;
+}
+
+void phase2_apply(const whf_request *req, whf_hit *hits, uint64_t n, const whf_hut_result *hr, uint64_t *kept,
+                  uint64_t p2[5]) {
     uint64_t k = 0;
     for (uint64_t i = 0; i < n; i++) {
         const whf_hut_result &h = hr[i];
         p2[0]++;
-        bool ok = h.ok != 0;
-        int reject = h.reject;
+        // the verdict's floor test is its last step (biome, flood, height: eval.c:221-224), so it is applied here
+        // with the request's own threshold whatever max_y the evaluation ran with
+        bool ok = h.biome_ok && !h.flooded;
+        int reject = !h.biome_ok ? WHF_HUT_REJECT_BIOME : (h.flooded ? WHF_HUT_REJECT_FLOOD : WHF_HUT_OK);
+        if (ok && h.avg_y > req->phase2_max_y) { ok = false; reject = WHF_HUT_REJECT_Y; }
         if (ok && req->phase2 == WHF_P2_EXACTLY && h.avg_y != req->phase2_max_y) { ok = false; reject = WHF_HUT_REJECT_Y; }
         if (ok) {
             p2[1]++;
@@ -1238,8 +1468,6 @@ int phase2_filter(whf_ctx *c, const whf_request *req, whf_hit *hits, uint64_t n,
         }
     }
     *kept = k;
-    *ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return WHF_OK;
 }
 
 }  // namespace whf
@@ -1713,7 +1941,7 @@ int whf_probe(whf_ctx *c, const int64_t *seeds, const int32_t *hut_x, const int3
         const uint32_t jobs = nn * kSetSize[s];
         hipLaunchKernelGGL(k_build, dim3((jobs + 63) / 64), dim3(64), 0, st, c->d_seeds, bd_of(c, vflags, s),
                            kSetSize[s], (const uint32_t *)nullptr, (const uint32_t *)nullptr, nn, c->d_tab[s],
-                           c->d_par[s], c->d_cnt + CNT_BUILT + s);
+                           c->d_par[s], c->d_cnt + CNT_BUILT + s, (const uint8_t *)nullptr);
     }
     hipLaunchKernelGGL(k_probe, dim3((nn + 63) / 64), dim3(64), 0, st, c->d_gt + vflags, c->d_tab[SET_E],
                        c->d_par[SET_E], c->d_tab[SET_C], c->d_par[SET_C], c->d_tab[SET_L], c->d_par[SET_L], d_xz,
@@ -1743,7 +1971,7 @@ int whf_noise_error(whf_ctx *c, const int64_t *seeds, uint64_t n_seeds, uint64_t
     HIP_TRY(hipMemcpyAsync(c->d_seeds, seeds, n_seeds * sizeof(int64_t), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_build, dim3((ns * 8 + 63) / 64), dim3(64), 0, st, c->d_seeds, bd_of(c, vflags, SET_E), 8,
                        (const uint32_t *)nullptr, (const uint32_t *)nullptr, ns, c->d_tab[SET_E], c->d_par[SET_E],
-                       c->d_cnt + CNT_BUILT + SET_E);
+                       c->d_cnt + CNT_BUILT + SET_E, (const uint8_t *)nullptr);
     const uint64_t total = (uint64_t)ns * np;
     hipLaunchKernelGGL(k_noise_error, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, c->d_tab[SET_E],
                        c->d_par[SET_E], c->d_gt + vflags, ns, np, d_max);

@@ -1,4 +1,10 @@
-// whf_server.cpp — continuous batching of Seed Check requests over several device slots (C ABI in whf.h).
+// whf_server.cpp — the serving C ABI (whf.h). By default Seed Check requests go to the device-resident loop of
+// whf_serve.hip (a device request table, batches packed on the device across requests, per-request counters and
+// survivor queues on the device); with WHF_SERVE_SLOTS they go to the host-driven slots below instead. Background Deep
+// Dive jobs always run on slots: with the device loop, one slot context of their own takes a Deep Dive chunk whenever
+// the loop has no Seed Check work left to claim.
+//
+// Slot mode: continuous batching of Seed Check requests over several device slots.
 //
 // Background Deep Dive jobs share the slots: a slot takes a Deep Dive chunk (2^18 structure seeds: sweep, walks and
 // the expansion of its spots) only when no Seed Check chunk is waiting, and at most half the slots run Deep Dive
@@ -24,6 +30,7 @@
 #include <vector>
 
 #include "../../include/whf.h"
+#include "whf_serve.h"
 
 // Device bytes one context holds (whf_scan.hip).
 __attribute__((visibility("hidden"))) uint64_t whf_ctx_device_bytes(const whf_ctx *c);
@@ -134,6 +141,18 @@ struct whf_server {
     int in_flight_jobs = 0;
     bool stop = false;
     Clock::time_point epoch = Clock::now();
+    whf::DqServer *dq = nullptr;   // the device-resident loop (default mode); null with WHF_SERVE_SLOTS
+    std::mutex dd_open_mu;         // the device loop's Deep Dive slot is opened by the first background job
+    int device = 0;
+
+    // the device loop's completions
+    static void post_event(void *p, const whf_event &e) {
+        whf_server *s = static_cast<whf_server *>(p);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->events.push_back(e);
+        s->in_flight_jobs--;
+        s->event_cv.notify_all();
+    }
 
     double now() const { return std::chrono::duration<double>(Clock::now() - epoch).count(); }
 
@@ -219,7 +238,7 @@ struct whf_server {
         std::vector<whf_hit> buf(kChunkHits);
         std::vector<whf_dd_spot> dd_spots;
         std::vector<whf_hit> dd_finds;
-        const int dd_limit = std::max(1, (int)ctx.size() / 2);
+        const int dd_limit = dq ? 1 : std::max(1, (int)ctx.size() / 2);
         for (;;) {
             std::shared_ptr<Job> j;
             uint64_t s0 = 0, n = 0;
@@ -228,10 +247,18 @@ struct whf_server {
                 // wake for interactive work, for a Deep Dive chunk this slot may take, or to exit once stopped with
                 // nothing queued; while Deep Dive chunks are queued but the other slots hold every Deep Dive place,
                 // a stopping slot sleeps until one of them finishes its chunk (run_dd_chunk notifies)
-                work_cv.wait(lk, [&] {
-                    return !jobs.empty() || (!dd_jobs.empty() && dd_running < dd_limit) ||
+                // with the device loop, a Deep Dive chunk also waits until the loop has no Seed Check work to claim,
+                // which it does not signal here: such a slot looks again every millisecond
+                auto ready = [&] {
+                    return !jobs.empty() ||
+                           (!dd_jobs.empty() && dd_running < dd_limit && !(dq && whf::dq_busy(dq))) ||
                            (stop && dd_jobs.empty());
-                });
+                };
+                if (dq) {
+                    while (!ready()) work_cv.wait_for(lk, std::chrono::milliseconds(1));
+                } else {
+                    work_cv.wait(lk, ready);
+                }
                 if (jobs.empty()) {
                     if (!dd_jobs.empty() && dd_running < dd_limit) {
                         // no interactive work waiting: one chunk of the oldest background Deep Dive
@@ -328,6 +355,17 @@ int whf_server_open(const whf_server_config *cfg, whf_server **out) {
     if (!s) return WHF_ENOMEM;
     if (cfg && cfg->chunk_checks) s->chunk_checks = cfg->chunk_checks;
     const uint64_t budget = cfg ? cfg->hbm_budget : 0;
+    s->device = cfg ? cfg->device : 0;
+    if (!(cfg && (cfg->flags & WHF_SERVE_SLOTS))) {
+        whf::DqHooks hooks{s, &whf_server::post_event};
+        const int rc = whf::dq_open(s->device, cfg ? cfg->chunk_checks : 0, budget, hooks, s->epoch, &s->dq);
+        if (rc) {
+            delete s;
+            return rc;
+        }
+        *out = s;
+        return WHF_OK;
+    }
     // one stream per slot: the slots' contexts already overlap on the device, which has few hardware queues to share
     whf_config cc{cfg ? cfg->device : 0, WHF_CFG_ONE_STREAM, 0};
     for (int i = 0; i < slots; i++) {
@@ -359,7 +397,9 @@ void whf_server_close(whf_server *s) {
         s->stop = true;
     }
     s->work_cv.notify_all();
+    // Deep Dive slots first: they may wait for the device loop to go idle
     for (auto &t : s->threads) t.join();
+    if (s->dq) whf::dq_close(s->dq);
     for (whf_ctx *c : s->ctx) whf_close(c);
     delete s;
 }
@@ -369,6 +409,30 @@ int whf_submit(whf_server *s, const whf_request *req, whf_sink *sink, uint64_t *
     if (req->rx1 <= req->rx0 || req->rz1 <= req->rz0) return WHF_EINVAL;
     if (req->seed_mode == WHF_SEEDS_LIST && req->n_seeds && !req->seeds) return WHF_EINVAL;
     if (sink && sink->capacity && !sink->hits) return WHF_EINVAL;
+    if (s->dq) {
+        if (int rc = whf::dq_check(req)) return rc;
+        uint64_t rid;
+        double t;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->stop) return WHF_EINVAL;
+            rid = s->next_id++;
+            t = s->now();
+            s->in_flight_jobs++;
+        }
+        if (id) *id = rid;
+        if (req->n_seeds == 0) {
+            if (sink) sink->count = 0;
+            whf_event e{};
+            e.id = rid;
+            e.status = WHF_OK;
+            e.submit_s = e.start_s = e.done_s = t;
+            whf_server::post_event(s, e);
+        } else {
+            whf::dq_submit(s->dq, rid, req, sink, t);
+        }
+        return WHF_OK;
+    }
     auto j = std::make_shared<Job>();
     j->req = *req;
     j->sink = sink;
@@ -396,6 +460,24 @@ int whf_submit_deepdive(whf_server *s, const whf_dd_request *req, whf_dd_sink *s
     if (req->rx1 <= req->rx0 || req->rz1 <= req->rz0 || req->count > (1ull << 48)) return WHF_EINVAL;
     if (spots && spots->capacity && !spots->spots) return WHF_EINVAL;
     if (finds && finds->capacity && !finds->hits) return WHF_EINVAL;
+    if (s->dq) {
+        // the device loop's Deep Dive slot: one context and its thread, opened by the first background job
+        std::lock_guard<std::mutex> olk(s->dd_open_mu);
+        bool need = false;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->stop) return WHF_EINVAL;
+            need = s->ctx.empty();
+        }
+        if (need) {
+            whf_config cc{s->device, WHF_CFG_ONE_STREAM, 0};
+            whf_ctx *c = nullptr;
+            if (int rc = whf_open(&cc, &c)) return rc;
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->ctx.push_back(c);
+            s->threads.emplace_back(&whf_server::worker, s, 0);
+        }
+    }
     auto j = std::make_shared<DdJob>();
     j->req = *req;
     j->spots = spots;

@@ -620,13 +620,17 @@ def stats_dict(st):
                 device_ms=st.device_ms, fp64_rechecks=st.fp64_rechecks, phase2=list(st.phase2))
 
 
-class Server:
-    """Continuous batching (whf_server_open / whf_submit / whf_poll / whf_server_close)."""
+SERVE_SLOTS = 1   # whf_server_config.flags: the host-driven slot scheduler instead of the device-resident loop
 
-    def __init__(self, device=0, slots=4, chunk_checks=0, hbm_budget=0):
+
+class Server:
+    """Continuous batching (whf_server_open / whf_submit / whf_poll / whf_server_close). By default the
+    device-resident loop; slots_mode=True selects the host-driven slot scheduler (slots contexts)."""
+
+    def __init__(self, device=0, slots=4, chunk_checks=0, hbm_budget=0, slots_mode=False):
         self.L = load()
         self.h = ctypes.c_void_p()
-        cfg = ServerConfig(device, slots, chunk_checks, hbm_budget, 0, 0)
+        cfg = ServerConfig(device, slots, chunk_checks, hbm_budget, SERVE_SLOTS if slots_mode else 0, 0)
         rc = self.L.whf_server_open(ctypes.byref(cfg), ctypes.byref(self.h))
         if rc:
             raise WhfError(rc, "whf_server_open")

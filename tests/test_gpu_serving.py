@@ -1,5 +1,7 @@
 """Continuous batching (whf_server_* through the C ABI): every request's hits and counters in the mix equal those of
-the same request scanned alone with whf_scan, whatever the chunking, arrival order or overlap with other requests.
+the same request scanned alone with whf_scan, whatever the batching, arrival order or overlap with other requests.
+Each test runs against both schedulers: the device-resident loop (the default: batches packed on the device across
+requests, per-request counters and survivor queues on the device) and the host-driven slots (WHF_SERVE_SLOTS).
 """
 import time
 
@@ -11,7 +13,12 @@ from product import whf
 
 pytestmark = pytest.mark.gpu
 
-KEYS = ("checks", "accepted", "reached_ladder", "funnel", "tiers")
+KEYS = ("checks", "accepted", "reached_ladder", "funnel", "tiers", "samplers_built", "fp64_rechecks")
+MODES = ("device", "slots")
+
+
+def server(mode, **kw):
+    return whf.Server(0, slots_mode=(mode == "slots"), **kw)
 
 
 def request_mix(n, seed, max_seeds_log2=14):
@@ -55,12 +62,13 @@ def check_same(ev, ref):
 
 
 @pytest.mark.timeout(600)
-def test_mix_equals_alone_all_at_once():
+@pytest.mark.parametrize("mode", MODES)
+def test_mix_equals_alone_all_at_once(mode):
     reqs = request_mix(48, 1)
     print("scanning %d requests alone" % len(reqs), flush=True)
     ref = alone(reqs)
-    print("serving them together", flush=True)
-    with whf.Server(0, slots=4, chunk_checks=1 << 20) as srv:
+    print("serving them together (%s)" % mode, flush=True)
+    with server(mode, slots=4, chunk_checks=1 << 20) as srv:
         ids = [srv.submit(rect, capacity=1 << 16, **kw) for rect, kw in reqs]
         got = {}
         while len(got) < len(ids):
@@ -71,7 +79,8 @@ def test_mix_equals_alone_all_at_once():
 
 
 @pytest.mark.timeout(600)
-def test_mix_with_phase2_equals_alone():
+@pytest.mark.parametrize("mode", MODES)
+def test_mix_with_phase2_equals_alone(mode):
     """Requests with lysh's hut verdict (or-lower and exactly, several thresholds and versions) at a phase-1 height
     that leaves survivors: finds with their floors and the phase-2 counters equal whf_scan alone."""
     rng = np.random.default_rng(9)
@@ -86,8 +95,8 @@ def test_mix_with_phase2_equals_alone():
     print("scanning %d phase-2 requests alone" % len(reqs), flush=True)
     ref = alone(reqs)
     assert sum(st["phase2"][0] for _, st in ref) > 20, "too few survivors to exercise phase 2"
-    print("serving them together", flush=True)
-    with whf.Server(0, slots=4, chunk_checks=1 << 18) as srv:
+    print("serving them together (%s)" % mode, flush=True)
+    with server(mode, slots=4, chunk_checks=1 << 18) as srv:
         ids = [srv.submit(rect, capacity=1 << 16, **kw) for rect, kw in reqs]
         got = {}
         while len(got) < len(ids):
@@ -99,11 +108,12 @@ def test_mix_with_phase2_equals_alone():
 
 
 @pytest.mark.timeout(600)
-def test_mix_equals_alone_staggered_arrivals():
+@pytest.mark.parametrize("mode", MODES)
+def test_mix_equals_alone_staggered_arrivals(mode):
     """Requests arrive while others run (one big request first, then a stream of small ones)."""
     reqs = [((-9, 9, -9, 9), dict(splitmix=(31337, 1 << 20), max_height=-50))] + request_mix(24, 2, 10)
     ref = alone(reqs)
-    with whf.Server(0, slots=3, chunk_checks=1 << 22) as srv:
+    with server(mode, slots=3, chunk_checks=1 << 22) as srv:
         ids = []
         for rect, kw in reqs:
             ids.append(srv.submit(rect, capacity=1 << 16, **kw))
@@ -119,8 +129,9 @@ def test_mix_equals_alone_staggered_arrivals():
     assert sum(got[i]["done_s"] < big_done for i in ids[1:]) > 0
 
 
-def test_empty_request_and_sink_overflow():
-    with whf.Server(0, slots=2) as srv:
+@pytest.mark.parametrize("mode", MODES)
+def test_empty_request_and_sink_overflow(mode):
+    with server(mode, slots=2) as srv:
         a = srv.submit((0, 1, 0, 1), splitmix=(1, 0))
         b = srv.submit((-79, 79, -79, 79), capacity=0, seeds=np.array([1473884015873122059], dtype=np.int64),
                        max_height=-50)
@@ -134,7 +145,8 @@ def test_empty_request_and_sink_overflow():
 
 
 @pytest.mark.timeout(600)
-def test_background_deepdive_and_requests_equal_alone():
+@pytest.mark.parametrize("mode", MODES)
+def test_background_deepdive_and_requests_equal_alone(mode):
     """A background Deep Dive (4 chunks, the last partial) shares the slots with Seed Check requests of versions
     1.18, 1.20 and 1.21: every result equals its run alone."""
     s0 = O.splitmix(3, 0) & ((1 << 48) - 1)
@@ -148,7 +160,7 @@ def test_background_deepdive_and_requests_equal_alone():
     with whf.Context(0) as ctx:
         dd_spots, dd_finds, dd_st = ctx.deepdive(s0, count, rect, floor_max=-55, expand=True, max_height=-50,
                                                  spot_capacity=1 << 16)
-    with whf.Server(0, slots=3) as srv:
+    with server(mode, slots=3) as srv:
         did = srv.submit_deepdive(s0, count, rect, floor_max=-55, expand=True, max_height=-50)
         ids = []
         for rect_i, kw in reqs:
@@ -172,13 +184,14 @@ def test_background_deepdive_and_requests_equal_alone():
 
 
 @pytest.mark.timeout(300)
-def test_close_with_queued_deepdive_returns_without_spinning():
+@pytest.mark.parametrize("mode", MODES)
+def test_close_with_queued_deepdive_returns_without_spinning(mode):
     """Closing a server while a background Deep Dive still has chunks queued and every Deep Dive place is taken
     (3 slots, at most 1 Deep Dive chunk at a time): close finishes the Deep Dive and returns, and the two idle slots
     sleep meanwhile instead of spinning on the scheduler's lock (the process's CPU time over the close stays below
     twice its wall time; a thread waiting on the device may spin, two more spinning threads would pass it)."""
     s0 = O.splitmix(3, 0) & ((1 << 48) - 1)
-    srv = whf.Server(0, slots=3)
+    srv = server(mode, slots=3)
     srv.submit_deepdive(s0, 16 << 18, (-2, 2, -2, 2), floor_max=-55, expand=True, max_height=-50)
     time.sleep(0.05)
     t0, c0 = time.perf_counter(), time.process_time()
@@ -187,3 +200,49 @@ def test_close_with_queued_deepdive_returns_without_spinning():
     print("close: %.3f s wall, %.3f s cpu" % (wall, cpu), flush=True)
     assert wall > 0.05, "the Deep Dive finished before close; the case was not exercised"
     assert cpu < 2.0 * wall + 0.05
+
+
+@pytest.mark.timeout(600)
+def test_device_loop_survivor_queue_overflow_falls_back_to_scan():
+    """A phase-2 request with more phase-1 survivors than the device loop reserves for it (4096) cannot be judged from
+    its queue: the loop scans it again alone, so its finds and counters still equal whf_scan's. Small requests beside
+    it are served from the queue."""
+    big = ((-8, 8, -8, 8), dict(splitmix=(4343, 1 << 17), max_height=60, vflags=whf.version_flags("1.20") | 8,
+                                phase2=(whf.P2_OR_LOWER, 80, whf.biome_tree("1.20"))))
+    reqs = [big] + request_mix(6, 5, 8)
+    ref = alone(reqs)
+    assert ref[0][1]["accepted"] > 4096, ref[0][1]["accepted"]
+    with server("device") as srv:
+        ids = [srv.submit(rect, capacity=1 << 16, **kw) for rect, kw in reqs]
+        got = {}
+        while len(got) < len(ids):
+            for e in srv.poll(timeout_ms=60000):
+                got[e["id"]] = e
+    for rid, r in zip(ids, ref):
+        check_same(got[rid], r)
+    assert got[ids[0]]["stats"]["phase2"] == ref[0][1]["phase2"]
+
+
+@pytest.mark.timeout(300)
+def test_device_loop_packs_requests_into_shared_batches():
+    """64 one-seed requests of mixed rects, versions and precisions submitted at once complete from far fewer
+    batches than requests: their work is packed together (each still equal to its scan alone)."""
+    rng = np.random.default_rng(3)
+    reqs = []
+    for i in range(64):
+        r = int(rng.choice([2, 3, 5, 9]))
+        reqs.append(((-r, r, -r, r), dict(splitmix=(7000 + i, 1 + i % 3), max_height=-50,
+                                           vflags=whf.version_flags(["1.18", "1.20", "26.2"][i % 3]),
+                                           precision=whf.PREC_FP32_GATE if i % 2 else whf.PREC_FP64)))
+    ref = alone(reqs)
+    with server("device") as srv:
+        ids = [srv.submit(rect, capacity=1 << 16, **kw) for rect, kw in reqs]
+        got = {}
+        while len(got) < len(ids):
+            for e in srv.poll(timeout_ms=60000):
+                got[e["id"]] = e
+    for rid, r in zip(ids, ref):
+        check_same(got[rid], r)
+    # requests that completed at the same instant came out of one batch's record
+    distinct_done = len({round(got[i]["done_s"], 6) for i in ids})
+    print("64 requests completed at %d distinct times" % distinct_done, flush=True)
