@@ -282,6 +282,17 @@ def splitmix_at(stream, i):
     return z - (1 << 64) if z >> 63 else z
 
 
+def splitmix_seeds(stream, count):
+    """Outputs 0 .. count-1 of sequential SplitMix64 seeded with `stream`, as signed 64-bit world seeds (numpy)."""
+    import numpy as np
+    with np.errstate(over="ignore"):
+        z = np.uint64(stream) + (np.arange(1, count + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z ^= z >> np.uint64(31)
+    return z.view(np.int64)
+
+
 def splitmix_first(stream):
     """First output of sequential SplitMix64 seeded with `stream`, unsigned."""
     return splitmix_at(stream, 0) & ((1 << 64) - 1)
@@ -482,7 +493,7 @@ def deep_dive(whf, sweep_n, expand_n, cpu_seconds):
     )
 
 
-def serving_stream(n, seed, versions=("1.20",)):
+def serving_stream(n, seed, versions=("1.20",), precision=0):
     """The serving workload: n requests, sizes round(2^U(0,20)) seeds from SplitMix64(request index), radius
     uniform over {512, 1024, 2048, 4096} blocks -> regions +-(r // 512 + 1), threshold uniform over -50..-54 with
     'or lower' / 'exactly' at 50/50 (-54 is always 'or lower'), version 1.20. Phase 1 runs at maxHeight -50, and
@@ -498,7 +509,7 @@ def serving_stream(n, seed, versions=("1.20",)):
         r = int(radius[i]) // 512 + 1
         reqs.append(dict(rect=(-r, r, -r, r), splitmix=(i, int(sizes[i])), threshold=int(thr[i]),
                          exactly=bool(exactly[i]), checks=int(sizes[i]) * (2 * r) ** 2,
-                         version=versions[i % len(versions)]))
+                         version=versions[i % len(versions)], precision=precision))
     return reqs
 
 
@@ -531,7 +542,7 @@ def serve(whf, reqs, rate, slots, chunk_checks, background=None, slots_mode=Fals
                     time.sleep(d)
                 with lock:
                     ids.append(srv.submit(q["rect"], capacity=1024, splitmix=q["splitmix"], max_height=-50,
-                                          vflags=whf.version_flags(q["version"]),
+                                          vflags=whf.version_flags(q["version"]), precision=q["precision"],
                                           phase2=(whf.P2_EXACTLY if q["exactly"] else whf.P2_OR_LOWER,
                                                   q["threshold"], whf.biome_tree(q["version"]))))
 
@@ -554,14 +565,40 @@ def serve(whf, reqs, rate, slots, chunk_checks, background=None, slots_mode=Fals
     return [events[i] for i in ids], wall, dd_event
 
 
-def serving(whf, n, load, slots, chunk_checks, versions=("1.20",), background=None, slots_mode=False):
+def scan_rate(whf, reqs):
+    """whf_scan on the same requests' seeds, grouped by rect and version (one scan per group, seed lists): the rate a
+    one-request-per-launch scan reaches on this mix, with every group's seeds in one request."""
     import numpy as np
-    reqs = serving_stream(n, 4, versions)
+    groups = {}
+    for q in reqs:
+        groups.setdefault((q["rect"], q["version"], q["precision"]), []).append(q)
+    lists = {k: np.concatenate([splitmix_seeds(q["splitmix"][0], q["splitmix"][1]) for q in qs])
+             for k, qs in groups.items()}
+    best = None
+    with whf.Context(0) as ctx:
+        for _ in range(2):
+            t = time.perf_counter()
+            n = 0
+            for (rect, version, precision), seeds in lists.items():
+                _, st = ctx.scan(rect, seeds=seeds, max_height=-50, vflags=whf.version_flags(version),
+                                 precision=precision, capacity=1 << 16)
+                n += st["checks"]
+            w = time.perf_counter() - t
+            best = max(best or 0.0, n / w)
+    return best
+
+
+def serving(whf, n, load, slots, chunk_checks, versions=("1.20",), background=None, slots_mode=False, precision=0):
+    import numpy as np
+    reqs = serving_stream(n, 4, versions, precision)
     mean_checks = float(np.mean([q["checks"] for q in reqs]))
-    # capacity: a prefix of the same stream submitted at once (saturation)
+    # capacity: a prefix of the same stream submitted at once (saturation), the better of two runs
     sat = reqs[: max(50, n // 20)]
-    ev, wall, _ = serve(whf, sat, None, slots, chunk_checks, slots_mode=slots_mode)
-    capacity = sum(e["stats"]["checks"] for e in ev) / wall
+    capacity = 0.0
+    for _ in range(2):
+        ev, wall, _ = serve(whf, sat, None, slots, chunk_checks, slots_mode=slots_mode)
+        capacity = max(capacity, sum(e["stats"]["checks"] for e in ev) / wall)
+    scan = scan_rate(whf, sat)
     rate = load * capacity / mean_checks
     note("serving capacity %.3g checks/s from %d requests at once" % (capacity, len(sat)))
     ev, wall, dd = serve(whf, reqs, rate, slots, chunk_checks, background, slots_mode=slots_mode)
@@ -572,10 +609,13 @@ def serving(whf, n, load, slots, chunk_checks, versions=("1.20",), background=No
     out = dict(
         workload="serving: %d Seed Check requests, Poisson arrivals at %.0f%% of measured capacity, sizes "
                  "round(2^U(0,20)) SplitMix64(request index) seeds, radius {512,1024,2048,4096} blocks, "
-                 "threshold -50..-54 or-lower/exactly, versions %s%s" % (
-                     n, 100 * load, "/".join(versions),
+                 "threshold -50..-54 or-lower/exactly, versions %s, %s%s" % (
+                     n, 100 * load, "/".join(versions), "FP32 gate" if precision else "FP64",
                      ", with a background Deep Dive of %d structure seeds" % background[1] if background else ""),
+        scheduler=("host slots: %d contexts, chunks of %d checks" % (slots, chunk_checks)) if slots_mode else
+                  ("device loop: %d lanes, batches of at most %d checks" % (slots, chunk_checks)),
         slots=slots, chunk_checks=chunk_checks, capacity_checks_per_s=round(capacity, 1),
+        scan_same_requests_checks_per_s=round(scan, 1), capacity_vs_scan=round(capacity / scan, 3),
         mean_request_checks=round(mean_checks, 1), arrival_rate_per_s=round(rate, 2),
         offered_checks_per_s=round(rate * mean_checks, 1), throughput_checks_per_s=round(checks / wall, 1),
         wall_s=round(wall, 3), requests=n, finds=sum(len(e["hits"]) for e in ev),
@@ -587,9 +627,11 @@ def serving(whf, n, load, slots, chunk_checks, versions=("1.20",), background=No
     if background:
         assert dd["status"] == whf.OK
         dd_s = dd["done_s"] - dd["start_s"]
+        last_request = max(e["done_s"] for e in ev)
         out["deep_dive"] = dict(structure_seeds=background[1], seconds=round(dd_s, 3),
                                 structure_seeds_per_s=round(background[1] / dd_s, 1), spots=len(dd["spots"]),
-                                expanded_seeds=dd["dd_stats"]["expanded_seeds"], finds=len(dd["finds"]))
+                                expanded_seeds=dd["dd_stats"]["expanded_seeds"], finds=len(dd["finds"]),
+                                ran_under_whole_stream=bool(dd["done_s"] >= last_request))
     return out
 
 
@@ -615,7 +657,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--serve-chunk", type=int, default=1 << 24)
+    ap.add_argument("--serve-chunk", type=int, default=1 << 24, help="checks per chunk of the slot scheduler")
+    ap.add_argument("--serve-batch", type=int, default=1 << 25, help="checks per batch of the device loop at most")
     ap.add_argument("--mixed-dd-seeds", type=int, default=1 << 26,
                     help="structure seeds of the background Deep Dive in the mixed serving run (0: no mixed run)")
     args = ap.parse_args()
@@ -754,13 +797,20 @@ def main():
         note("cpu parity: %s over %d seeds" % (parity["equal"], parity["seeds"]))
     product = None if args.no_cpu else cpu_product_mode(args.product_seeds, host["threads"])
     note("cpu product mode done")
-    srv = None if args.serve_requests <= 0 else serving(whf, args.serve_requests, args.serve_load, args.serve_slots,
-                                                        args.serve_chunk)
-    note("serving done")
+    # serving: the device-resident loop (the default scheduler), then the host-driven slots on the same stream
+    srv = srv_slots = None
+    if args.serve_requests > 0:
+        srv = serving(whf, args.serve_requests, args.serve_load, args.serve_slots, args.serve_batch)
+        note("serving (device loop) done")
+        srv_slots = serving(whf, args.serve_requests, args.serve_load, args.serve_slots, args.serve_chunk,
+                            slots_mode=True)
+        note("serving (slots) done")
     mixed = None
     if args.serve_requests > 0 and args.mixed_dd_seeds > 0:
-        mixed = serving(whf, args.serve_requests, args.serve_load, args.serve_slots, args.serve_chunk,
-                        versions=("1.18", "1.20", "1.21"),
+        # three game versions with three different biome trees, FP32-gated noise, a background Deep Dive long
+        # enough to run under the whole request stream
+        mixed = serving(whf, args.serve_requests, args.serve_load, args.serve_slots, args.serve_batch,
+                        versions=("1.18", "1.20", "1.21.5"), precision=whf.PREC_FP32_GATE,
                         background=(splitmix_first(3) & ((1 << 48) - 1), args.mixed_dd_seeds))
         note("mixed serving done")
     dd = None if args.no_deep_dive else deep_dive(whf, args.dd_seeds, args.dd_expand,
@@ -803,6 +853,7 @@ def main():
         "device_ms": round(last["device_ms"], 3),
         "latency_ms": srv["latency_ms"] if srv else None,
         "serving": srv,
+        "serving_slots": srv_slots,
         "serving_mixed": mixed,
         "deep_dive": dd,
         "deep_dive_solver": sol,

@@ -112,7 +112,10 @@ struct SReq {
     int32_t mode, rx0, rz0, W;
     int32_t max_height, vflags, salt, f32;
     uint32_t hit_base, hit_cap;   // the request's survivor queue in the server's hit pool
-    uint32_t seq, pad;            // admission order (ties of the claim order)
+    uint32_t seq;                 // admission order (ties of the claim order)
+    uint32_t admit_batch;         // the first batch that could claim it (its age counts from there)
+    FastDiv Rdiv, Wdiv;           // division by R and by W, prepared on the host
+    uint32_t pad[2];
 };
 
 // Per-request progress, written by the device: seeds claimed into batches, seeds whose batch completed, survivors.
@@ -139,6 +142,7 @@ struct SClaim {
 struct SBatch {
     uint32_t n_slots, n_claims;
     uint32_t n_tiles[kSrvClasses];
+    uint32_t tile_next[kSrvClasses];   // work-stealing cursors of the erosion workgroups over each class's tiles
     uint64_t claimed_items;
     SClaim claims[kSrvLive];
     STile tiles[kSrvClasses][kSrvTileCap];
@@ -289,11 +293,24 @@ int dd_scratch_init(whf_ctx *c);
 // Resets a batch set's control words and the seed flags of its first ns seeds, on the library's stream.
 void reset_set(whf_ctx *c, const BatchSet &b, uint32_t ns);
 
-// The three stages of a served batch (whf_scan.hip), on the given streams. The batch's slots, tiles and their requests
-// are in sb and v (packed by whf_serve.hip's claim kernel); every launch has a fixed grid.
-void serve_stage_e(whf_ctx *c, const BatchSet &b, SBatch *sb, const SrvView &v, hipStream_t st);
+// A serving lane's own buffers besides its batch set: set-E tables, the FP32 gate's re-evaluation queue, set-L tables
+// and set-L slot maps (a lane runs whole batches on its own stream, beside the other lanes).
+struct SrvLane {
+    uint8_t *tabE;
+    SPar *parE;
+    uint4 *qx;
+    uint8_t *tabL;
+    SPar *parL;
+    uint32_t *slotL, *seedL;
+};
+// The stages of a served batch (whf_scan.hip), in order on the lane's stream: reset, then (after the claim) set E and
+// the erosion gate, set C and gate 2, set L and gate 3. The batch's slots, tiles and their requests are in sb and v
+// (packed by whf_serve.hip's claim kernel); every launch has a fixed grid.
+void serve_reset(const BatchSet &b, hipStream_t st);
+void serve_stage_e(whf_ctx *c, const BatchSet &b, const SrvLane &ln, SBatch *sb, const SrvView &v, hipStream_t st,
+                   int class_mask);
 void serve_stage_c(whf_ctx *c, const BatchSet &b, const SrvView &v, hipStream_t st);
-void serve_stage_l(whf_ctx *c, const BatchSet &b, const SrvView &v, hipStream_t st);
+void serve_stage_l(whf_ctx *c, const BatchSet &b, const SrvLane &ln, const SrvView &v, hipStream_t st);
 
 // The three-stage batch pipeline (whf_scan.hip). Batch k uses buffer set k % 3: the caller takes the set with
 // acquire(), issues the batch's first stage into it on the library's stream, and hands it to tail(), which issues

@@ -179,10 +179,11 @@ __device__ __forceinline__ void ero_evals(const uint16_t *s_tab, const Par *s_pa
 // undecided items. IL items per lane are evaluated at once (the FP32 gate's evaluations are short, and without a
 // second item their LDS lookup chains leave the device waiting; IL = 1 for the plain kernel).
 //
-// SERVE (served batches): the workgroups take tiles from a device list (tiles[0 .. *n_tiles)) instead of the
-// launch's (tile, region chunk) blocks. A tile is a run of at most ST slots of one request, whose rect, salt and rule
-// layout come from the request table (srv.req); every region of the rect is one chunk (at most kSrvMaxRegions), and
-// the tile's tier counts go to its request's counters when the tile is done.
+// SERVE (served batches): the workgroups steal tiles from a device list (tiles[0 .. *n_tiles), the next one at
+// *tile_next) instead of striding over the launch's (tile, region chunk) blocks: tiles of different requests differ
+// widely in work. A tile is a run of at most ST slots of one request, whose rect, salt and rule layout come from the
+// request table (srv.req); every region of the rect is one chunk (at most kSrvMaxRegions), and the tile's tier counts
+// go to its request's counters when the tile is done.
 template <int ST, bool F32, int WG, int IL, bool SERVE = false>
 __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ seeds, uint32_t n_seeds,
                                                     const uint8_t *__restrict__ tabE, const SPar *__restrict__ parE,
@@ -193,7 +194,8 @@ __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ see
                                                     uint32_t *__restrict__ needC, uint4 *__restrict__ qx,
                                                     unsigned long long *__restrict__ cnt, SrvView srv,
                                                     const STile *__restrict__ tiles,
-                                                    const uint32_t *__restrict__ n_tiles) {
+                                                    const uint32_t *__restrict__ n_tiles,
+                                                    uint32_t *__restrict__ tile_next) {
     static_assert(F32 || IL == 1, "the plain kernel evaluates one item per lane");
     __shared__ __align__(16) uint16_t s_tab[ST * 8 * 256];   // byte-pair tables: P(i) | P(i + 1) << 8
     using Par = typename std::conditional<F32, EPar, SPar>::type;
@@ -207,6 +209,7 @@ __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ see
     __shared__ Sum s_ls[kEroPass];
     __shared__ uint32_t s_n[4];                  // list sizes after levels 0, 1, 2
     __shared__ uint32_t s_cnt[8];
+    __shared__ uint32_t s_blk;                   // SERVE: the tile this workgroup stole
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
@@ -214,7 +217,19 @@ __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ see
     // persistent workgroups: each takes blocks (tile, region chunk) blockIdx.x, + gridDim.x, ... in turn; the barrier
     // at the end of a block's last pass keeps its tables until every wave is done with them
     const uint32_t n_blocks = SERVE ? *n_tiles : ((n_seeds + ST - 1) / ST) * n_chunks;
-    for (uint32_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    // the next block: stolen from the cursor when serving (both barriers keep s_blk one value per round), else strided
+    auto next_block = [&](uint32_t prev) -> uint32_t {
+        if constexpr (SERVE) {
+            if (tid == 0) s_blk = atomicAdd(tile_next, 1u);
+            __syncthreads();
+            const uint32_t b = s_blk;
+            __syncthreads();
+            return b;
+        } else {
+            return prev + gridDim.x;
+        }
+    };
+    for (uint32_t blk = SERVE ? next_block(0) : blockIdx.x; blk < n_blocks; blk = next_block(blk)) {
     uint32_t seed0, ns, rbase, rn, req = 0;
     FastDiv Rn, Wd;
     const GateTables *gt;
@@ -227,8 +242,8 @@ __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ see
         ns = t.ns;
         rbase = 0;
         rn = r.R;
-        Rn = make_fastdiv(rn);
-        Wd = make_fastdiv((uint32_t)r.W);
+        Rn = r.Rdiv;
+        Wd = r.Wdiv;
         gt = srv.gt + r.vflags;
         rx0 = r.rx0;
         rz0 = r.rz0;
@@ -580,11 +595,16 @@ __global__ __launch_bounds__(kGateWG) void k_gate2a(const GateTables *__restrict
         int kind = -1;
         uint32_t req = 0;
         uint4 e = make_uint4(0, 0, 0, 0);
+        int lay = vflags_;
         if (q < n) {
             e = q1[q];
-            const int vflags = SERVE ? (int)srv.lay_of[e.x] : vflags_;
-            const GateTables *gt = SERVE ? srv.gt + vflags : gt_;
-            if constexpr (SERVE) req = srv.rq_of[e.x];
+            if constexpr (SERVE) {
+                lay = (int)srv.lay_of[e.x];
+                req = srv.rq_of[e.x];
+            }
+        }
+        // the temperature and ridge checks of this lane's entry under the rules of layout vflags
+        auto check = [&](const GateTables *gt, int vflags) {
             const bool single = (vflags & WHF_V_SINGLE) != 0;
             const uint32_t slot = slotC[e.x];
             if (slot >= kMaxSeeds) {
@@ -601,6 +621,22 @@ __global__ __launch_bounds__(kGateWG) void k_gate2a(const GateTables *__restrict
                     else if ((vflags & WHF_V_262) && r <= -0.91) kind = 2;
                 }
             }
+        };
+        if constexpr (SERVE) {
+            // a wave whose entries share one rule layout (the usual case) reads that layout's noise descriptors with
+            // scalar loads; a mixed wave reads each lane's own
+            const unsigned long long act = __ballot(q < n);
+            const int lay0 = act ? __shfl(lay, __ffsll((long long)act) - 1) : 0;
+            if (__ballot(q < n && lay != lay0) == 0) {
+                const int L = __builtin_amdgcn_readfirstlane(lay0);
+                if (q < n) check(srv.gt + L, L);
+            } else if (q < n) {
+                check(srv.gt + lay, lay);
+            }
+        } else if (q < n) {
+            check(gt_, vflags_);
+        }
+        if (q < n) {
             if (!SERVE && kind > 0) block_count(s_cnt, kind);
             else if (kind < 0) survive = true;
         }
@@ -1268,78 +1304,63 @@ int Pipeline::tail(const BatchSet &b, int layout, uint32_t ns, int vflags, int m
 }
 
 // ---- served batches (packed on the device by whf_serve.hip's claim kernel) ----
+// A serving lane runs whole batches on its own stream, with its own sampler tables, re-evaluation queue and set-L
+// slots (ln), so that several lanes' batches run side by side.
 
-// Stage 1: set E for the batch's slots, each with its request's rule layout, then the erosion gate over the four tile
-// classes (FP64 with 4 and 8 seeds per tile, FP32 gate with 4 and 8) and the FP64 re-evaluation of the items the FP32
-// gate left undecided. Every grid is fixed; the kernels read the batch's slot and tile counts on the device.
-void serve_stage_e(whf_ctx *c, const BatchSet &b, SBatch *sb, const SrvView &v, hipStream_t st) {
+// The batch's queue counts and the seed flags of every slot, on the lane's stream.
+void serve_reset(const BatchSet &b, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset, dim3(kMaxSeeds / 256), dim3(256), 0, st, b.ctl, b.needC, b.needL, kMaxSeeds);
+}
+
+// Set E for the batch's slots, each with its request's rule layout, then the erosion gate over the tile classes of
+// class_mask (bit k: class k, 0/1 FP64 with 4/8 seeds per tile, 2/3 FP32 gate with 4/8) and, with an FP32 class, the
+// FP64 re-evaluation of the items the FP32 gate left undecided. Every grid is fixed; the kernels read the batch's slot
+// and tile counts on the device.
+void serve_stage_e(whf_ctx *c, const BatchSet &b, const SrvLane &ln, SBatch *sb, const SrvView &v, hipStream_t st,
+                   int class_mask) {
     const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune[0].gate_wg_per_cu;
-    {
-        Prof p(c, F_BUILD_E, st);
-        hipLaunchKernelGGL(k_build, dim3(persist * 4), dim3(64), 0, st, b.seeds, bd_of(c, 0, SET_E), 8,
-                           (const uint32_t *)nullptr, &sb->n_slots, kMaxSeeds, c->d_tab[SET_E], c->d_par[SET_E],
-                           (unsigned long long *)nullptr, v.lay_of);
-    }
-    Prof p(c, F_EROSION, st);
+    hipLaunchKernelGGL(k_build, dim3(persist * 4), dim3(64), 0, st, b.seeds, bd_of(c, 0, SET_E), 8,
+                       (const uint32_t *)nullptr, &sb->n_slots, kMaxSeeds, ln.tabE, ln.parE,
+                       (unsigned long long *)nullptr, v.lay_of);
     const int per_cu = c->tune[0].ero_wg_per_cu > 0 ? c->tune[0].ero_wg_per_cu : 8;
     const dim3 grid((uint32_t)c->n_cu * (uint32_t)per_cu);
     const FastDiv one{1u, 0u, 0u};
 #define WHF_SRV_EROSION_ARGS(cls)                                                                                    \
-    grid, dim3(256), 0, st, b.seeds, 0u, c->d_tab[SET_E], c->d_par[SET_E], c->d_gt, 0, 0, one, one, (uint64_t)0, 0u, \
-        1u, 1u, 0, 0, b.q1, b.ctl, b.needC, c->d_qx, c->d_cnt, v, &sb->tiles[cls][0], &sb->n_tiles[cls]
-    hipLaunchKernelGGL((k_erosion<4, false, 256, 1, true>), WHF_SRV_EROSION_ARGS(0));
-    hipLaunchKernelGGL((k_erosion<8, false, 256, 1, true>), WHF_SRV_EROSION_ARGS(1));
-    hipLaunchKernelGGL((k_erosion<4, true, 256, 2, true>), WHF_SRV_EROSION_ARGS(2));
-    hipLaunchKernelGGL((k_erosion<8, true, 256, 2, true>), WHF_SRV_EROSION_ARGS(3));
+    grid, dim3(256), 0, st, b.seeds, 0u, ln.tabE, ln.parE, c->d_gt, 0, 0, one, one, (uint64_t)0, 0u, 1u, 1u, 0, 0,   \
+        b.q1, b.ctl, b.needC, ln.qx, c->d_cnt, v, &sb->tiles[cls][0], &sb->n_tiles[cls], &sb->tile_next[cls]
+    if (class_mask & 1) hipLaunchKernelGGL((k_erosion<4, false, 256, 1, true>), WHF_SRV_EROSION_ARGS(0));
+    if (class_mask & 2) hipLaunchKernelGGL((k_erosion<8, false, 256, 1, true>), WHF_SRV_EROSION_ARGS(1));
+    if (class_mask & 4) hipLaunchKernelGGL((k_erosion<4, true, 256, 2, true>), WHF_SRV_EROSION_ARGS(2));
+    if (class_mask & 8) hipLaunchKernelGGL((k_erosion<8, true, 256, 2, true>), WHF_SRV_EROSION_ARGS(3));
 #undef WHF_SRV_EROSION_ARGS
-    hipLaunchKernelGGL(k_erosion_exact<true>, dim3((uint32_t)c->n_cu * 2), dim3(256), 0, st, c->d_tab[SET_E],
-                       c->d_par[SET_E], c->d_gt, c->d_qx, b.q1, b.ctl, b.needC, c->d_cnt, v);
+    if (class_mask & 12)
+        hipLaunchKernelGGL(k_erosion_exact<true>, dim3((uint32_t)c->n_cu * 2), dim3(256), 0, st, ln.tabE, ln.parE,
+                           c->d_gt, ln.qx, b.q1, b.ctl, b.needC, c->d_cnt, v);
 }
 
-// Stage 2: set C for the slots with erosion survivors and gate 2's temperature and ridge, with each slot's rule
-// layout. (Gate 2's Entrance checks run at the start of stage 3, which balances the stages of the serving mix, whose
-// small rects need set C for most seeds.)
+// Set C for the slots with erosion survivors and gate 2, with each slot's rule layout.
 void serve_stage_c(whf_ctx *c, const BatchSet &b, const SrvView &v, hipStream_t st) {
     const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune[0].gate_wg_per_cu;
-    {
-        Prof p(c, F_COMPACT, st);
-        hipLaunchKernelGGL(k_compact, dim3(kMaxSeeds / 256), dim3(256), 0, st, b.needC, kMaxSeeds, b.slotC, b.seedC,
-                           b.ctl + CTL_NC, kMaxSeeds);
-    }
-    {
-        Prof p(c, F_BUILD_C, st);
-        hipLaunchKernelGGL(k_build, dim3(persist * 4), dim3(64), 0, st, b.seeds, bd_of(c, 0, SET_C), 28, b.seedC,
-                           b.ctl + CTL_NC, kMaxSeeds, b.tabC, b.parC, (unsigned long long *)nullptr, v.lay_of);
-    }
-    Prof p(c, F_GATE2, st);
+    hipLaunchKernelGGL(k_compact, dim3(kMaxSeeds / 256), dim3(256), 0, st, b.needC, kMaxSeeds, b.slotC, b.seedC,
+                       b.ctl + CTL_NC, kMaxSeeds);
+    hipLaunchKernelGGL(k_build, dim3(persist * 4), dim3(64), 0, st, b.seeds, bd_of(c, 0, SET_C), 28, b.seedC,
+                       b.ctl + CTL_NC, kMaxSeeds, b.tabC, b.parC, (unsigned long long *)nullptr, v.lay_of);
     hipLaunchKernelGGL(k_gate2a<true>, dim3(persist), dim3(kGateWG), 0, st, c->d_gt, b.tabC, b.parC, b.slotC, b.q1,
                        b.q2, b.ctl, 0, c->d_cnt, v);
+    hipLaunchKernelGGL(k_gate2b<true>, dim3(persist), dim3(kGateWG), 0, st, c->d_gt, b.tabC, b.parC, b.slotC, b.q2,
+                       b.q1, b.ctl, b.needL, c->d_cnt, v);
 }
 
-// Stage 3: gate 2's Entrance checks, set L for the slots with gate-2 survivors and gate 3, whose survivors go to their
-// requests' queues.
-void serve_stage_l(whf_ctx *c, const BatchSet &b, const SrvView &v, hipStream_t st) {
+// Set L for the slots with gate-2 survivors and gate 3, whose survivors go to their requests' queues.
+void serve_stage_l(whf_ctx *c, const BatchSet &b, const SrvLane &ln, const SrvView &v, hipStream_t st) {
     const uint32_t persist = (uint32_t)c->n_cu * (uint32_t)c->tune[0].gate_wg_per_cu;
-    {
-        Prof p(c, F_GATE2, st);
-        hipLaunchKernelGGL(k_gate2b<true>, dim3(persist), dim3(kGateWG), 0, st, c->d_gt, b.tabC, b.parC, b.slotC,
-                           b.q2, b.q1, b.ctl, b.needL, c->d_cnt, v);
-    }
-    {
-        Prof p(c, F_COMPACT, st);
-        hipLaunchKernelGGL(k_compact, dim3(kMaxSeeds / 256), dim3(256), 0, st, b.needL, kMaxSeeds, c->d_slotL,
-                           c->d_seedL, b.ctl + CTL_NL, kMaxSeeds);
-    }
-    {
-        Prof p(c, F_BUILD_L, st);
-        hipLaunchKernelGGL(k_build, dim3(persist), dim3(64), 0, st, b.seeds, bd_of(c, 0, SET_L), 36, c->d_seedL,
-                           b.ctl + CTL_NL, kMaxSeeds, c->d_tab[SET_L], c->d_par[SET_L], (unsigned long long *)nullptr,
-                           v.lay_of);
-    }
-    Prof p(c, F_GATE3, st);
-    hipLaunchKernelGGL(k_gate3<true>, dim3(persist), dim3(kGateWG), 0, st, b.seeds, c->d_gt, b.tabC, b.parC,
-                       c->d_tab[SET_L], c->d_par[SET_L], b.slotC, c->d_slotL, b.q1, b.ctl, 0, 0, (DevHit *)nullptr,
-                       (uint32_t *)nullptr, 0u, c->d_cnt, v);
+    hipLaunchKernelGGL(k_compact, dim3(kMaxSeeds / 256), dim3(256), 0, st, b.needL, kMaxSeeds, ln.slotL, ln.seedL,
+                       b.ctl + CTL_NL, kMaxSeeds);
+    hipLaunchKernelGGL(k_build, dim3(persist), dim3(64), 0, st, b.seeds, bd_of(c, 0, SET_L), 36, ln.seedL,
+                       b.ctl + CTL_NL, kMaxSeeds, ln.tabL, ln.parL, (unsigned long long *)nullptr, v.lay_of);
+    hipLaunchKernelGGL(k_gate3<true>, dim3(persist), dim3(kGateWG), 0, st, b.seeds, c->d_gt, b.tabC, b.parC, ln.tabL,
+                       ln.parL, b.slotC, ln.slotL, b.q1, b.ctl, 0, 0, (DevHit *)nullptr, (uint32_t *)nullptr, 0u,
+                       c->d_cnt, v);
 }
 
 int Pipeline::end() {
@@ -1389,7 +1410,7 @@ int stage_e(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_
 #define WHF_EROSION_ARGS(WG)                                                                                      \
     grid, dim3(WG), 0, st, b.seeds, ns, c->d_tab[SET_E], c->d_par[SET_E], gt, req->rx0, req->rz0, Wd, Rd, f0, fn,      \
         chunk, n_chunks, salt, single, b.q1, b.ctl, b.needC, c->d_qx, c->d_cnt, SrvView{}, (const STile *)nullptr,     \
-        (const uint32_t *)nullptr
+        (const uint32_t *)nullptr, (uint32_t *)nullptr
         if (!f32) {
             if (wide && wg == 512) hipLaunchKernelGGL((k_erosion<4, false, 512, 1>), WHF_EROSION_ARGS(512));
             else if (wide) hipLaunchKernelGGL((k_erosion<4, false, 256, 1>), WHF_EROSION_ARGS(256));

@@ -16,10 +16,11 @@ struct DqHooks {
     void (*post)(void *ctx, const whf_event &e);
 };
 
-// Opens the loop on a device: its pipeline context, request table, pools and threads. item_cap: (seed, region)
-// checks per batch (0: the most a batch holds); hbm_budget: device bytes the server may hold (0: no limit; the pools
-// are cut to fit, WHF_ENOMEM below the fixed part). Times of the events are seconds since epoch.
-int dq_open(int device, uint64_t item_cap, uint64_t hbm_budget, const DqHooks &hooks,
+// Opens the loop on a device: its context, lanes, request table, pools and threads. lanes: streams that run whole
+// batches side by side (1..4; 0: 3); item_cap: (seed, region) checks per batch (0: the most a batch holds);
+// hbm_budget: device bytes the server may hold (0: no limit; the seed-list pool is cut to fit, WHF_ENOMEM below the
+// fixed part). Times of the events are seconds since epoch.
+int dq_open(int device, int lanes, uint64_t item_cap, uint64_t hbm_budget, const DqHooks &hooks,
             std::chrono::steady_clock::time_point epoch, DqServer **out);
 // WHF_OK when the loop can serve the request (at most 2^16 regions, fewer than 2^32 seeds, a seed list that fits the
 // seed pool), else WHF_EINVAL.

@@ -45,10 +45,10 @@ namespace whf {
 namespace {
 
 constexpr uint32_t kSrvRing = 64;               // batch records the host can lag behind the device
-// Batch sets in flight: a set is free again when its batch's stage 3 ends. The three stages of one batch take about
-// as long as three batches' stage 1, so three sets would stall stage 1; six keep it fed. Sets 0-2 are the context's,
-// 3-5 the loop's own.
-constexpr int kSrvSets = 6;
+constexpr int kSrvMaxLanes = 4;             // lanes: streams that each run whole batches, side by side (the device
+                                            // has four hardware queues per process)
+constexpr uint32_t kSrvAge = 8;             // batches after which a live request is claimed first (at most half a
+                                            // batch of it), so big requests are not starved by a stream of small ones
 constexpr uint32_t kSrvHitMin = 4096;           // survivor entries reserved per request at least
 constexpr uint64_t kSrvHitPool = 1ull << 22;    // survivor entries of the pool (128 MiB of host memory)
 constexpr uint64_t kSrvSeedPool = 1ull << 26;   // listed seeds of the pool by default (512 MiB)
@@ -75,11 +75,13 @@ struct SDone {
 
 // One workgroup of kSrvLive lanes, lane t on table entry t. Live requests (seeds left to claim) are sorted by
 // remaining checks, least first (ties: the earlier admission), and take seeds in that order: whole requests while
-// the batch's slot and item caps hold, then part of the first one that does not fit. Each claim's slots follow the
-// previous claims', and its tiles are a range of its class list.
+// the batch's slot and item caps hold, then part of the first one that does not fit. A request live for kSrvAge
+// batches sorts before all others (oldest first) and takes at most half the batch's items. Each claim's slots follow
+// the previous claims', and its tiles are a range of its class list. Claims run one after another (the host chains
+// them with events), so `claimed` needs no atomics.
 __global__ __launch_bounds__(kSrvLive) void k_srv_claim(const SReq *__restrict__ req, SState *__restrict__ state,
                                                        SBatch *__restrict__ sb, unsigned long long *__restrict__ rcnt,
-                                                       uint64_t item_cap, SClaimOut *__restrict__ out) {
+                                                       uint64_t item_cap, uint32_t batch, SClaimOut *__restrict__ out) {
     __shared__ uint64_t s_key[kSrvLive];
     __shared__ uint16_t s_idx[kSrvLive];
     __shared__ uint64_t s_a[kSrvLive], s_b[kSrvLive];
@@ -92,8 +94,10 @@ __global__ __launch_bounds__(kSrvLive) void k_srv_claim(const SReq *__restrict__
         const SReq &r = req[t];
         const uint64_t claimed = state[t].claimed;
         const uint64_t rem = r.n_seeds > claimed ? r.n_seeds - claimed : 0;
-        // remaining checks < 2^48 (seeds < 2^32, regions <= 2^16), then the admission order's low 16 bits
-        s_key[t] = rem ? ((rem * r.R) << 16) | (uint64_t)(r.seq & 0xFFFFu) : ~0ull;
+        const bool aged = batch - r.admit_batch >= kSrvAge;
+        // remaining checks (< 2^48: seeds < 2^32, regions <= 2^16), then the admission order's low 16 bits; an aged
+        // request's key has no remaining-checks part
+        s_key[t] = rem ? ((aged ? 0ull : rem * r.R) << 16) | (uint64_t)(r.seq & 0xFFFFu) : ~0ull;
         s_idx[t] = (uint16_t)t;
     }
     __syncthreads();
@@ -119,8 +123,10 @@ __global__ __launch_bounds__(kSrvLive) void k_srv_claim(const SReq *__restrict__
     const uint32_t e = s_idx[t];
     const SReq &r = req[e];
     const uint64_t claimed = live ? state[e].claimed : 0;
-    const uint64_t rem = live ? r.n_seeds - claimed : 0;
     const uint64_t R = live ? r.R : 1;
+    uint64_t rem = live ? r.n_seeds - claimed : 0;
+    const uint64_t left = rem;
+    if (live && batch - r.admit_batch >= kSrvAge) rem = min(rem, max((uint64_t)1, (item_cap / 2) / R));
     // inclusive prefix sums of seeds and items in the sorted order
     s_a[t] = rem;
     s_b[t] = rem * R;
@@ -160,13 +166,16 @@ __global__ __launch_bounds__(kSrvLive) void k_srv_claim(const SReq *__restrict__
         atomicAdd(&s_sum[0], (unsigned long long)take);
         atomicAdd(&s_sum[1], (unsigned long long)(take * R));
     }
-    if (rem > take) atomicAdd(&s_sum[2], (unsigned long long)((rem - take) * R));
+    if (left > take) atomicAdd(&s_sum[2], (unsigned long long)((left - take) * R));
     __syncthreads();
     if (t == 0) {
         sb->n_slots = (uint32_t)s_sum[0];
         sb->n_claims = n_claims;
         sb->claimed_items = s_sum[1];
-        for (int k = 0; k < kSrvClasses; k++) sb->n_tiles[k] = s_tiles[k];
+        for (int k = 0; k < kSrvClasses; k++) {
+            sb->n_tiles[k] = s_tiles[k];
+            sb->tile_next[k] = 0;
+        }
         out->claimed_seeds = s_sum[0];
         out->claimed_items = s_sum[1];
         out->unclaimed_checks = s_sum[2];
@@ -233,7 +242,7 @@ __global__ __launch_bounds__(256) void k_srv_account(const SBatch *__restrict__ 
 // the host reads completions without a copy.
 __global__ __launch_bounds__(kSrvLive) void k_srv_finish(const SReq *__restrict__ req, SState *__restrict__ state,
                                                         const SBatch *__restrict__ sb,
-                                                        const unsigned long long *__restrict__ rcnt,
+                                                        unsigned long long *__restrict__ rcnt,
                                                         SDone *__restrict__ out, uint64_t seq) {
     __shared__ uint32_t s_n;
     __shared__ uint16_t s_done[kSrvLive];
@@ -241,9 +250,11 @@ __global__ __launch_bounds__(kSrvLive) void k_srv_finish(const SReq *__restrict_
     if (t == 0) s_n = 0;
     __syncthreads();
     if (t < sb->n_claims) {
+        // lanes finish batches in any order: the batch whose count completes the request reports it, after every
+        // other batch holding its seeds has added its own count (and so run its gates)
         const SClaim cl = sb->claims[t];
-        const uint64_t done = state[cl.req].done + cl.n;
-        state[cl.req].done = done;
+        const uint64_t done =
+            atomicAdd(reinterpret_cast<unsigned long long *>(&state[cl.req].done), (unsigned long long)cl.n) + cl.n;
         if (done == req[cl.req].n_seeds) s_done[atomicAdd(&s_n, 1u)] = (uint16_t)cl.req;
     }
     __syncthreads();
@@ -254,11 +265,13 @@ __global__ __launch_bounds__(kSrvLive) void k_srv_finish(const SReq *__restrict_
         const uint32_t i = w / kWords, k = w - i * kWords;
         const uint32_t e = s_done[i];
         SDoneRec &o = out->rec[i];
+        // other lanes' kernels counted into these with device-wide atomics: read them the same way
         if (k == 0) {
             o.entry = e;
-            o.hits = state[e].hits;
+            o.hits = __hip_atomic_load(&state[e].hits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            o.cnt[k - 1] = rcnt[(size_t)e * CNT_N + (k - 1)];
+            o.cnt[k - 1] = __hip_atomic_load(&rcnt[(size_t)e * CNT_N + (k - 1)], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     __syncthreads();
@@ -322,6 +335,11 @@ class Pool {
     }
 };
 
+// The erosion tile class of a request (whf_internal.h kSrvClasses): FP32 gate or not, 4 or 8 seeds per tile.
+int tile_class(const whf_request &q, uint32_t R) {
+    return (q.precision == WHF_PREC_FP32_GATE ? 2 : 0) + (R >= (uint32_t)kSrvWide ? 0 : 1);
+}
+
 struct DqReq {
     uint64_t id = 0;
     whf_request req{};
@@ -335,9 +353,21 @@ struct DqReq {
 
 }  // namespace
 
+// A lane: a stream that runs whole batches, with its batch set, its own tables and queue (SrvLane) and its batch
+// record on the device.
+struct LaneBufs {
+    hipStream_t st = nullptr;
+    bool own_st = false;
+    BatchSet b{};
+    bool own_b = false;
+    SrvLane ln{};
+    bool own_ln = false;
+    SBatch *sb = nullptr;
+};
+
 struct DqServer {
     int device = 0;
-    whf_ctx *c = nullptr;     // the pipeline's context: its batch sets, streams and sampler tables
+    whf_ctx *c = nullptr;     // the lanes' context: lane 0 uses its buffers and stream, lanes 1-2 its other batch sets
     whf_ctx *p2 = nullptr;    // the completer's context: phase 2, and whf_scan for a request the queue cannot finish
     hipStream_t copy_st = nullptr;
     DqHooks hooks{};
@@ -349,15 +379,14 @@ struct DqServer {
     unsigned long long *d_rcnt = nullptr;
     whf_hit *h_hits = nullptr;   // survivor pool in host memory (hipHostMalloc): gate 3 appends to it directly
     uint64_t *d_seeds = nullptr;
-    SBatch *d_sb[kSrvSets] = {};
-    BatchSet sets[kSrvSets] = {};
-    hipEvent_t ev_set[3][kSrvSets] = {};   // end of stage 1, 2, 3 of the set's batch
+    LaneBufs lanes[kSrvMaxLanes];
+    int n_lanes = 3;
     // host memory the device writes or the copies read (hipHostMalloc)
     SReq *h_req = nullptr;
     unsigned char *h_zero = nullptr;
-    SClaimOut *h_claim = nullptr;   // [kSrvSets]
+    SClaimOut *h_claim = nullptr;   // [kSrvRing]
     SDone *h_done = nullptr;        // [kSrvRing]
-    hipEvent_t ev_claim[kSrvSets] = {};
+    hipEvent_t ev_claim[kSrvRing] = {};
     hipEvent_t ev_done[kSrvRing] = {};
     // host state, under mu
     std::mutex mu;
@@ -369,19 +398,20 @@ struct DqServer {
     uint32_t next_seq = 0;
     bool stop = false, launcher_done = false;
     uint64_t admitted_live = 0;
-    bool unclaimed = false;         // the last claim left work on the device
+    bool unclaimed = false;         // the last claim read left work on the device
+    uint64_t claims_read = 0;       // batches whose claim summary the launcher has read
+    int class_live[kSrvClasses] = {};   // admitted, not completed requests per tile class
     uint64_t issued = 0, completed_batches = 0;
     std::deque<uint64_t> issued_q;
     std::thread launcher, completer;
-    bool used[kSrvSets] = {};
 
     double now() const { return std::chrono::duration<double>(Clock::now() - epoch).count(); }
 
-    SrvView view(int set) const {
+    SrvView view(const LaneBufs &L) const {
         SrvView v;
         v.req = d_req;
-        v.rq_of = d_sb[set]->rq_of;
-        v.lay_of = d_sb[set]->lay_of;
+        v.rq_of = L.sb->rq_of;
+        v.lay_of = L.sb->lay_of;
         v.rcnt = d_rcnt;
         v.state = d_state;
         v.hit_pool = reinterpret_cast<DevHit *>(h_hits);
@@ -396,8 +426,8 @@ struct DqServer {
         return hit_pool.can(r.hit_cap) && seed_pool.can(r.seed_words);
     }
 
-    int upload(DqReq &r) {
-        const hipStream_t st = c->stream;
+    // Publishes an admitted request on stream st, the stream of the batch that claims next.
+    int upload(DqReq &r, hipStream_t st) {
         if (r.seed_words) {
             // a pageable copy on the idle copy stream, finished before the descriptor is published
             HIP_TRY(hipMemcpyAsync(d_seeds + r.seed_off, r.req.seeds, r.seed_words * sizeof(uint64_t),
@@ -422,6 +452,9 @@ struct DqServer {
         d.hit_base = (uint32_t)r.hit_off;
         d.hit_cap = r.hit_cap;
         d.seq = next_seq++;
+        d.admit_batch = (uint32_t)issued;
+        d.Rdiv = make_fastdiv(d.R);
+        d.Wdiv = make_fastdiv((uint32_t)d.W);
         // the state and counters are zeroed before the descriptor makes the entry live; all three copies precede the
         // next claim kernel on the same stream
         HIP_TRY(hipMemcpyAsync(d_state + r.entry, h_zero, sizeof(SState), hipMemcpyHostToDevice, st));
@@ -431,44 +464,52 @@ struct DqServer {
         return WHF_OK;
     }
 
-    int issue_batch() {
-        const int set = (int)(issued % kSrvSets);
+    // Issues batch `issued` on lane issued % n_lanes: reset, claim (after the previous batch's claim, on whatever lane
+    // it ran), fill, and all three stages; then waits for the claim's summary.
+    int issue_batch(bool admitted_before) {
         const uint64_t seq = issued;
-        const BatchSet &b = sets[set];
-        SBatch *sb = d_sb[set];
-        const SrvView v = view(set);
-        const hipStream_t s0 = c->stream, s1 = c->stage_stream[0], s2 = c->stage_stream[1];
-        // stage 3 of the batch three back is the last reader of this set
-        if (used[set]) HIP_TRY(hipStreamWaitEvent(s0, ev_set[2][set], 0));
-        reset_set(c, b, kMaxSeeds);
-        hipLaunchKernelGGL(k_srv_claim, dim3(1), dim3(kSrvLive), 0, s0, d_req, d_state, sb, d_rcnt, item_cap,
-                           &h_claim[set]);
-        HIP_TRY(hipEventRecord(ev_claim[set], s0));
-        hipLaunchKernelGGL(k_srv_fill, dim3(kMaxSeeds / 256), dim3(256), 0, s0, d_req, sb, b.seeds);
-        serve_stage_e(c, b, sb, v, s0);
-        HIP_TRY(hipEventRecord(ev_set[0][set], s0));
-        HIP_TRY(hipStreamWaitEvent(s1, ev_set[0][set], 0));
-        serve_stage_c(c, b, v, s1);
-        HIP_TRY(hipEventRecord(ev_set[1][set], s1));
-        HIP_TRY(hipStreamWaitEvent(s2, ev_set[1][set], 0));
-        serve_stage_l(c, b, v, s2);
-        hipLaunchKernelGGL(k_srv_account, dim3(kMaxSeeds / 256), dim3(256), 0, s2, sb, b.needC, b.needL, d_rcnt);
-        hipLaunchKernelGGL(k_srv_finish, dim3(1), dim3(kSrvLive), 0, s2, d_req, d_state, sb, d_rcnt,
+        LaneBufs &L = lanes[seq % n_lanes];
+        const SrvView v = view(L);
+        const hipStream_t st = L.st;
+        serve_reset(L.b, st);
+        if (seq > 0) HIP_TRY(hipStreamWaitEvent(st, ev_claim[(seq - 1) % kSrvRing], 0));
+        hipLaunchKernelGGL(k_srv_claim, dim3(1), dim3(kSrvLive), 0, st, d_req, d_state, L.sb, d_rcnt, item_cap,
+                           (uint32_t)seq, &h_claim[seq % kSrvRing]);
+        HIP_TRY(hipEventRecord(ev_claim[seq % kSrvRing], st));
+        hipLaunchKernelGGL(k_srv_fill, dim3(kMaxSeeds / 256), dim3(256), 0, st, d_req, L.sb, L.b.seeds);
+        int class_mask = 0;
+        {
+            // a class with no live request has no tiles in this batch
+            std::lock_guard<std::mutex> lk(mu);
+            for (int k = 0; k < kSrvClasses; k++)
+                if (class_live[k]) class_mask |= 1 << k;
+        }
+        serve_stage_e(c, L.b, L.ln, L.sb, v, st, class_mask);
+        serve_stage_c(c, L.b, v, st);
+        serve_stage_l(c, L.b, L.ln, v, st);
+        hipLaunchKernelGGL(k_srv_account, dim3(kMaxSeeds / 256), dim3(256), 0, st, L.sb, L.b.needC, L.b.needL, d_rcnt);
+        hipLaunchKernelGGL(k_srv_finish, dim3(1), dim3(kSrvLive), 0, st, d_req, d_state, L.sb, d_rcnt,
                            &h_done[seq % kSrvRing], seq);
         if (int rc = check_launch()) return rc;
-        HIP_TRY(hipEventRecord(ev_set[2][set], s2));
-        HIP_TRY(hipEventRecord(ev_done[seq % kSrvRing], s2));
-        used[set] = true;
+        HIP_TRY(hipEventRecord(ev_done[seq % kSrvRing], st));
         {
             std::lock_guard<std::mutex> lk(mu);
             issued = seq + 1;
             issued_q.push_back(seq);
         }
         cv_complete.notify_all();
-        // the claim's summary decides whether another batch is needed before new requests arrive
-        HIP_TRY(hipEventSynchronize(ev_claim[set]));
+        // the launcher runs one batch ahead of the claims it has read, so that a lane whose batch ends finds the next
+        // one queued: it reads the previous batch's claim now, and this one's before it stops issuing
+        return seq > 0 ? read_claim(seq - 1, admitted_before) : WHF_OK;
+    }
+
+    // Reads the summary of batch seq's claim (waiting for it); requests admitted after that claim (fresh) leave work
+    // it did not see.
+    int read_claim(uint64_t seq, bool fresh) {
+        HIP_TRY(hipEventSynchronize(ev_claim[seq % kSrvRing]));
         std::lock_guard<std::mutex> lk(mu);
-        unclaimed = h_claim[set].unclaimed_checks > 0;
+        unclaimed = h_claim[seq % kSrvRing].unclaimed_checks > 0 || fresh;
+        claims_read = seq + 1;
         return WHF_OK;
     }
 
@@ -485,6 +526,7 @@ struct DqServer {
                     r.reset();
                 }
             admitted_live = 0;
+            for (int &k : class_live) k = 0;
         }
         for (auto &r : dead) post(*r, rc, whf_stats{}, {}, 0);
     }
@@ -515,7 +557,7 @@ struct DqServer {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_launch.wait(lk, [&] {
                     const bool room = issued - completed_batches < kSrvRing - 4;
-                    return (stop && inbox.empty() && admitted_live == 0) ||
+                    return (stop && inbox.empty() && admitted_live == 0) || claims_read < issued ||
                            (room && ((!inbox.empty() && fits(*inbox.front())) || unclaimed));
                 });
                 if (stop && inbox.empty() && admitted_live == 0) break;
@@ -533,14 +575,26 @@ struct DqServer {
                     table[r->entry] = r;
                     r->start_s = now();
                     admitted_live++;
+                    class_live[tile_class(r->req, r->R)]++;
                     admit.push_back(r);
                 }
             }
             for (auto &r : refused) post(*r, WHF_ENOMEM, whf_stats{}, {}, 0);
             int rc = WHF_OK;
+            if (!admit.empty() && issued > 0) {
+                // an entry is rewritten by three copies (state, counters, descriptor); no claim may run between them,
+                // so they follow the previous batch's claim (on whatever lane it ran) and precede the next one
+                const hipError_t e = hipStreamWaitEvent(lanes[issued % n_lanes].st, ev_claim[(issued - 1) % kSrvRing], 0);
+                if (e != hipSuccess) rc = WHF_EDEVICE;
+            }
             for (auto &r : admit)
-                if ((rc = upload(*r)) != WHF_OK) break;
-            if (rc == WHF_OK && (!admit.empty() || unclaimed)) rc = issue_batch();
+                if (rc == WHF_OK && (rc = upload(*r, lanes[issued % n_lanes].st)) != WHF_OK) break;
+            if (rc == WHF_OK && (!admit.empty() || unclaimed)) {
+                rc = issue_batch(!admit.empty());
+            } else if (rc == WHF_OK && claims_read < issued) {
+                // about to wait: the newest claim decides whether work is left
+                rc = read_claim(issued - 1, false);
+            }
             if (rc != WHF_OK) {
                 fail_all(rc);
                 break;
@@ -680,6 +734,7 @@ struct DqServer {
             table[f.e].reset();
             free_entries.push_back(f.e);
             admitted_live--;
+            class_live[tile_class(f.r->req, f.r->R)]--;
             cv_launch.notify_all();
         }
     }
@@ -709,54 +764,31 @@ struct DqServer {
         }
     }
 
-    int open(int dev, uint64_t cap, uint64_t hbm_budget) {
-        device = dev;
-        // at least one seed of the widest request fits a batch
-        item_cap = cap ? std::min<uint64_t>(std::max<uint64_t>(cap, kSrvMaxRegions), kMaxItems) : kMaxItems;
-        whf_config cc{device, 0, 0};
-        if (int rc = whf_open(&cc, &c)) return rc;
-        HIP_TRY(hipSetDevice(device));
-        for (hipStream_t &s : c->stage_stream)
-            if (!s) HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        HIP_TRY(hipStreamCreateWithFlags(&copy_st, hipStreamNonBlocking));
-        // the seed-list pool (device memory): the default, cut to what the budget leaves after the contexts (the
-        // pipeline's and the completer's) and the batch records; the survivor pool is host memory
-        const uint64_t hit_n = kSrvHitPool;
-        uint64_t seed_n = kSrvSeedPool;
-        const uint64_t set_bytes = kMaxSeeds * (8 + kSetSize[SET_C] * (256 + sizeof(SPar)) + 4 * 4) +
-                                   2ull * kMaxItems * sizeof(uint4) + 64;
-        const uint64_t fixed = 2 * whf_ctx_device_bytes(c) + kSrvSets * sizeof(SBatch) +
-                               (kSrvSets - 3) * set_bytes +
-                               kSrvLive * (sizeof(SReq) + sizeof(SState) + CNT_N * sizeof(unsigned long long));
-        if (hbm_budget) {
-            const uint64_t min_pool = kSrvLive * sizeof(uint64_t);
-            if (hbm_budget < fixed + min_pool) {
-                std::snprintf(g_last_error, sizeof g_last_error, "hbm_budget %llu below the server's %llu bytes",
-                              (unsigned long long)hbm_budget, (unsigned long long)(fixed + min_pool));
-                return WHF_ENOMEM;
-            }
-            seed_n = std::min<uint64_t>(seed_n, (hbm_budget - fixed) / sizeof(uint64_t));
+    // A lane's buffers: lane 0 uses the context's primary buffers and stream, lanes 1 and 2 its second and third
+    // batch sets and the stage streams, and every lane beyond 0 gets its own set-E and set-L tables, re-evaluation
+    // queue and set-L slots; lane 3 also its own batch set and stream.
+    int make_lane(int k) {
+        LaneBufs &L = lanes[k];
+        const size_t S = kMaxSeeds;
+        HIP_TRY(hipMalloc(&L.sb, sizeof(SBatch)));
+        HIP_TRY(hipMemset(L.sb, 0, sizeof(SBatch)));   // rq_of / lay_of of unused slots stay in range
+        if (k == 0) {
+            L.st = c->stream;
+            L.b = c->batch_set[0];
+            L.ln = SrvLane{c->d_tab[SET_E], c->d_par[SET_E], c->d_qx, c->d_tab[SET_L], c->d_par[SET_L], c->d_slotL,
+                           c->d_seedL};
+            return WHF_OK;
         }
-        HIP_TRY(hipMalloc(&d_req, kSrvLive * sizeof(SReq)));
-        HIP_TRY(hipMalloc(&d_state, kSrvLive * sizeof(SState)));
-        HIP_TRY(hipMalloc(&d_rcnt, (size_t)kSrvLive * CNT_N * sizeof(unsigned long long)));
-        HIP_TRY(hipHostMalloc((void **)&h_hits, hit_n * sizeof(whf_hit), hipHostMallocCoherent));
-        HIP_TRY(hipMalloc(&d_seeds, seed_n * sizeof(uint64_t)));
-        HIP_TRY(hipMemset(d_req, 0, kSrvLive * sizeof(SReq)));
-        HIP_TRY(hipMemset(d_state, 0, kSrvLive * sizeof(SState)));
-        HIP_TRY(hipMemset(d_rcnt, 0, (size_t)kSrvLive * CNT_N * sizeof(unsigned long long)));
-        for (int k = 0; k < kSrvSets; k++) {
-            HIP_TRY(hipMalloc(&d_sb[k], sizeof(SBatch)));
-            HIP_TRY(hipMemset(d_sb[k], 0, sizeof(SBatch)));
-            for (int st = 0; st < 3; st++) HIP_TRY(hipEventCreateWithFlags(&ev_set[st][k], hipEventDisableTiming));
-            if (k < 3) {
-                sets[k] = c->batch_set[k];
-                continue;
-            }
-            // what a batch's later stages read, for the sets beyond the context's three (seeds, set-C tables, Q1,
-            // Q2, seed flags, set-C slots, control words)
-            const size_t S = kMaxSeeds;
-            BatchSet &b = sets[k];
+        if (k < 3) {
+            hipStream_t &s = c->stage_stream[k - 1];
+            if (!s) HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            L.st = s;
+            L.b = c->batch_set[k];
+        } else {
+            HIP_TRY(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+            L.own_st = true;
+            BatchSet &b = L.b;
+            L.own_b = true;
             HIP_TRY(hipMalloc(&b.seeds, S * sizeof(uint64_t)));
             HIP_TRY(hipMalloc(&b.tabC, S * kSetSize[SET_C] * 256));
             HIP_TRY(hipMalloc(&b.parC, S * kSetSize[SET_C] * sizeof(SPar)));
@@ -768,12 +800,64 @@ struct DqServer {
             HIP_TRY(hipMalloc(&b.needL, S * 4));
             HIP_TRY(hipMalloc(&b.ctl, 64));
         }
+        SrvLane &ln = L.ln;
+        L.own_ln = true;
+        HIP_TRY(hipMalloc(&ln.tabE, S * kSetSize[SET_E] * 256));
+        HIP_TRY(hipMalloc(&ln.parE, S * kSetSize[SET_E] * sizeof(SPar)));
+        HIP_TRY(hipMalloc(&ln.qx, (size_t)kQxCap * sizeof(uint4)));
+        HIP_TRY(hipMalloc(&ln.tabL, S * kSetSize[SET_L] * 256));
+        HIP_TRY(hipMalloc(&ln.parL, S * kSetSize[SET_L] * sizeof(SPar)));
+        HIP_TRY(hipMalloc(&ln.slotL, S * 4));
+        HIP_TRY(hipMalloc(&ln.seedL, S * 4));
+        return WHF_OK;
+    }
+
+    int open(int dev, int lanes_n, uint64_t cap, uint64_t hbm_budget) {
+        device = dev;
+        n_lanes = lanes_n > 0 ? std::min(lanes_n, kSrvMaxLanes) : 3;
+        // at least one seed of the widest request fits a batch
+        item_cap = cap ? std::min<uint64_t>(std::max<uint64_t>(cap, kSrvMaxRegions), kMaxItems) : kMaxItems;
+        whf_config cc{device, 0, 0};
+        if (int rc = whf_open(&cc, &c)) return rc;
+        HIP_TRY(hipSetDevice(device));
+        // the seed-list pool (device memory): the default, cut to what the budget leaves after the contexts (the
+        // lanes' and the completer's), the lanes' own buffers and the request table; the survivor pool is host memory
+        const uint64_t hit_n = kSrvHitPool;
+        uint64_t seed_n = kSrvSeedPool;
+        const uint64_t S = kMaxSeeds;
+        const uint64_t set_bytes = S * (8 + kSetSize[SET_C] * (256 + sizeof(SPar)) + 4 * 4) +
+                                   2ull * kMaxItems * sizeof(uint4) + 64;
+        const uint64_t lane_bytes = S * ((kSetSize[SET_E] + kSetSize[SET_L]) * (256 + sizeof(SPar)) + 8) +
+                                    (uint64_t)kQxCap * sizeof(uint4);
+        const uint64_t fixed = 2 * whf_ctx_device_bytes(c) + n_lanes * sizeof(SBatch) +
+                               (n_lanes - 1) * lane_bytes + (n_lanes > 3 ? set_bytes : 0) +
+                               kSrvLive * (sizeof(SReq) + sizeof(SState) + CNT_N * sizeof(unsigned long long));
+        if (hbm_budget) {
+            const uint64_t min_pool = kSrvLive * sizeof(uint64_t);
+            if (hbm_budget < fixed + min_pool) {
+                std::snprintf(g_last_error, sizeof g_last_error, "hbm_budget %llu below the server's %llu bytes",
+                              (unsigned long long)hbm_budget, (unsigned long long)(fixed + min_pool));
+                return WHF_ENOMEM;
+            }
+            seed_n = std::min<uint64_t>(seed_n, (hbm_budget - fixed) / sizeof(uint64_t));
+        }
+        HIP_TRY(hipStreamCreateWithFlags(&copy_st, hipStreamNonBlocking));
+        HIP_TRY(hipMalloc(&d_req, kSrvLive * sizeof(SReq)));
+        HIP_TRY(hipMalloc(&d_state, kSrvLive * sizeof(SState)));
+        HIP_TRY(hipMalloc(&d_rcnt, (size_t)kSrvLive * CNT_N * sizeof(unsigned long long)));
+        HIP_TRY(hipHostMalloc((void **)&h_hits, hit_n * sizeof(whf_hit), hipHostMallocCoherent));
+        HIP_TRY(hipMalloc(&d_seeds, seed_n * sizeof(uint64_t)));
+        HIP_TRY(hipMemset(d_req, 0, kSrvLive * sizeof(SReq)));
+        HIP_TRY(hipMemset(d_state, 0, kSrvLive * sizeof(SState)));
+        HIP_TRY(hipMemset(d_rcnt, 0, (size_t)kSrvLive * CNT_N * sizeof(unsigned long long)));
+        for (int k = 0; k < n_lanes; k++)
+            if (int rc = make_lane(k)) return rc;
         HIP_TRY(hipHostMalloc((void **)&h_req, kSrvLive * sizeof(SReq), hipHostMallocCoherent));
         HIP_TRY(hipHostMalloc((void **)&h_zero, 4096, hipHostMallocCoherent));
-        HIP_TRY(hipHostMalloc((void **)&h_claim, kSrvSets * sizeof(SClaimOut), hipHostMallocCoherent));
+        HIP_TRY(hipHostMalloc((void **)&h_claim, kSrvRing * sizeof(SClaimOut), hipHostMallocCoherent));
         HIP_TRY(hipHostMalloc((void **)&h_done, kSrvRing * sizeof(SDone), hipHostMallocCoherent));
         std::memset(h_zero, 0, 4096);
-        std::memset(h_claim, 0, kSrvSets * sizeof(SClaimOut));
+        std::memset(h_claim, 0, kSrvRing * sizeof(SClaimOut));
         std::memset(h_done, 0, kSrvRing * sizeof(SDone));
         for (hipEvent_t &e : ev_claim) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (hipEvent_t &e : ev_done) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -802,21 +886,24 @@ struct DqServer {
         for (hipEvent_t e : ev_done)
             if (e) (void)hipEventDestroy(e);
         if (copy_st) (void)hipStreamDestroy(copy_st);
-        for (SBatch *b : d_sb) (void)hipFree(b);
-        for (int k = 3; k < kSrvSets; k++) {
-            const BatchSet &b = sets[k];
-            for (void *p : {(void *)b.seeds, (void *)b.tabC, (void *)b.parC, (void *)b.q1, (void *)b.q2, (void *)b.needC,
-                            (void *)b.slotC, (void *)b.seedC, (void *)b.needL, (void *)b.ctl})
-                if (p) (void)hipFree(p);
+        for (LaneBufs &L : lanes) {
+            if (L.sb) (void)hipFree(L.sb);
+            if (L.own_b)
+                for (void *p : {(void *)L.b.seeds, (void *)L.b.tabC, (void *)L.b.parC, (void *)L.b.q1, (void *)L.b.q2,
+                                (void *)L.b.needC, (void *)L.b.slotC, (void *)L.b.seedC, (void *)L.b.needL,
+                                (void *)L.b.ctl})
+                    if (p) (void)hipFree(p);
+            if (L.own_ln)
+                for (void *p : {(void *)L.ln.tabE, (void *)L.ln.parE, (void *)L.ln.qx, (void *)L.ln.tabL,
+                                (void *)L.ln.parL, (void *)L.ln.slotL, (void *)L.ln.seedL})
+                    if (p) (void)hipFree(p);
+            if (L.own_st && L.st) (void)hipStreamDestroy(L.st);
         }
-        for (auto &row : ev_set)
-            for (hipEvent_t e : row)
-                if (e) (void)hipEventDestroy(e);
         (void)hipFree(d_req);
         (void)hipFree(d_state);
         (void)hipFree(d_rcnt);
-        if (h_hits) (void)hipHostFree(h_hits);
         (void)hipFree(d_seeds);
+        if (h_hits) (void)hipHostFree(h_hits);
         for (void *p : {(void *)h_req, (void *)h_zero, (void *)h_claim, (void *)h_done})
             if (p) (void)hipHostFree(p);
         if (p2) whf_close(p2);
@@ -824,14 +911,14 @@ struct DqServer {
     }
 };
 
-int dq_open(int device, uint64_t item_cap, uint64_t hbm_budget, const DqHooks &hooks,
+int dq_open(int device, int lanes, uint64_t item_cap, uint64_t hbm_budget, const DqHooks &hooks,
             std::chrono::steady_clock::time_point epoch, DqServer **out) {
     *out = nullptr;
     DqServer *s = new (std::nothrow) DqServer();
     if (!s) return WHF_ENOMEM;
     s->hooks = hooks;
     s->epoch = epoch;
-    const int rc = s->open(device, item_cap, hbm_budget);
+    const int rc = s->open(device, lanes, item_cap, hbm_budget);
     if (rc != WHF_OK) {
         s->release();
         delete s;

@@ -358,7 +358,8 @@ int whf_server_open(const whf_server_config *cfg, whf_server **out) {
     s->device = cfg ? cfg->device : 0;
     if (!(cfg && (cfg->flags & WHF_SERVE_SLOTS))) {
         whf::DqHooks hooks{s, &whf_server::post_event};
-        const int rc = whf::dq_open(s->device, cfg ? cfg->chunk_checks : 0, budget, hooks, s->epoch, &s->dq);
+        const int rc = whf::dq_open(s->device, cfg ? cfg->slots : 0, cfg ? cfg->chunk_checks : 0, budget, hooks,
+                                    s->epoch, &s->dq);
         if (rc) {
             delete s;
             return rc;
