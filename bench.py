@@ -659,9 +659,21 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-chunk", type=int, default=1 << 24, help="checks per chunk of the slot scheduler")
     ap.add_argument("--serve-batch", type=int, default=1 << 25, help="checks per batch of the device loop at most")
+    ap.add_argument("--serve-only", choices=("device", "slots"),
+                    help="only serve the first 150 requests of the serving stream at once with this scheduler, once "
+                         "(a short run to profile), and print the rate")
     ap.add_argument("--mixed-dd-seeds", type=int, default=1 << 26,
                     help="structure seeds of the background Deep Dive in the mixed serving run (0: no mixed run)")
     args = ap.parse_args()
+    if args.serve_only:
+        whf = __graft_entry__.load_whf()
+        reqs = serving_stream(3000, 4)[:150]
+        mode = args.serve_only
+        ev, wall, _ = serve(whf, reqs, None, args.serve_slots,
+                            args.serve_batch if mode == "device" else args.serve_chunk, slots_mode=(mode == "slots"))
+        print(json.dumps(dict(scheduler=mode, requests=len(reqs), checks=sum(e["stats"]["checks"] for e in ev),
+                              seconds=round(wall, 4), checks_per_s=round(sum(e["stats"]["checks"] for e in ev) / wall))))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would scan its own seed "
               "range independently); run with --gpus 1", file=sys.stderr)

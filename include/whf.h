@@ -398,8 +398,9 @@ typedef struct whf_server whf_server;
 
 typedef struct {
     int device;
-    int slots;                  /* WHF_SERVE_SLOTS: concurrent device slots (a context, stream and buffers each);
-                                   0 = 4. Unused by the device loop */
+    int slots;                  /* device loop: lanes, streams that run whole batches side by side (1..4; 0 = 3);
+                                   WHF_SERVE_SLOTS: concurrent device slots (a context, stream and buffers each);
+                                   0 = 4 */
     uint64_t chunk_checks;      /* device loop: (seed, region) checks per batch at most (0 = 2^25, the most a batch
                                    holds; at least 2^16); WHF_SERVE_SLOTS: checks per scheduled chunk (0 = 2^24) */
     uint64_t hbm_budget;        /* device bytes the server may hold; 0 = no limit. Device loop: its survivor and
@@ -419,9 +420,10 @@ typedef struct {
 } whf_event;
 
 /* Opens a server. By default it is a device-resident loop: admitted requests live in a request table in device
- * memory, and every batch of the three-stage Seed Check pipeline is packed on the device by a claim kernel that
- * takes seeds from the live requests, least remaining work first, so one batch holds tiles of many requests, rects,
- * game versions and precisions. Counters and survivor queues are per request on the device; a host thread only
+ * memory, and every batch of the Seed Check chain is packed on the device by a claim kernel that takes seeds from
+ * the live requests, least remaining work first (a request live for 8 batches first, at most half a batch of it),
+ * so one batch holds tiles of many requests, rects, game versions and precisions, and the lanes run batches side by
+ * side. Counters and survivor queues are per request on the device; a host thread only
  * admits requests (when a table entry and space in the survivor and seed-list pools are free: admission against
  * the HBM budget) and keeps batches flowing while the device reports unclaimed work, and another reads completed
  * requests back, runs lysh's phase 2 on their survivors and posts their events. A request the loop cannot finish
