@@ -9,11 +9,16 @@
 //     default; --v26-2 still selects the 26.2 ridge rule);
 //   * --splitmix S N [OFFSET] scans N seeds of the sequential SplitMix64 stream seeded with S;
 //   * --stats prints the funnel and erosion tier counters as one JSON line;
-//   * --fp32-gate runs the erosion stage's corner blend in FP32 with FP64 re-evaluation near every threshold.
+//   * --fp32-gate runs the erosion stage's corner blend in FP32 with FP64 re-evaluation near every threshold;
+//   * in --worker mode, a request line with --id K is served asynchronously by the library's serving loop
+//     (whf_server_open / whf_submit / whf_poll): the worker reads the next line at once, and prints
+//     ===SCAN_DONE id=K rc=N=== when that request completes (after writing its --out file), in completion order.
+//     Lines without --id keep the reference's one-at-a-time answer, so hunt.py's sequential use is unchanged.
 // Options that only tuned the reference's own kernels (--block, --climate-gate, --no-climate-gate,
 // --no-density-prefilter) are accepted and have no effect on the answer. Options of searches this engine
 // does not run (--self-test, --quad-bases, --require-all-regions, --max-hut-dist) are errors.
 #include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <cinttypes>
 #include <climits>
@@ -22,8 +27,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "whf.h"
@@ -41,6 +50,7 @@ struct Params {
     std::string out, version;
     bool have_p2 = false, exactly = false;
     int p2_max_y = 0;
+    std::string id;   // --worker: serve this request asynchronously and tag its completion line with the id
 };
 
 bool read_seed_list(const char *path, std::vector<int64_t> &out) {
@@ -87,7 +97,9 @@ void usage() {
         "  --out <file>          write candidates as CSV (seed,chunkX,chunkZ,blockX,blockZ,gpuEst)\n"
         "  --stats               print the funnel counters as JSON\n"
         "  --fp32-gate           FP32-gated erosion stage with FP64 re-evaluation near thresholds (same answers)\n"
-        "  --worker              read one request per stdin line; ===WORKER_READY===, ===SCAN_DONE rc=N===\n");
+        "  --worker              read one request per stdin line; ===WORKER_READY===, ===SCAN_DONE rc=N===\n"
+        "  --id <k>              (a --worker line) serve the request asynchronously beside the following lines;\n"
+        "                        its answer is ===SCAN_DONE id=<k> rc=N===, in completion order\n");
 }
 
 // Parses argv[1..] or one --worker line (phase1_scan.cu:641-759). Returns false on a bad request.
@@ -141,6 +153,7 @@ bool parse(const std::vector<std::string> &t, Params &p) {
         else if (a == "--no-density-prefilter" || a == "--climate-gate" || a == "--no-climate-gate") {}
         else if (a == "--block") next("--block");
         else if (a == "--worker") {}
+        else if (a == "--id") { p.id = next("--id"); }
         else if (a == "--self-test" || a == "--quad-bases" || a == "--quad-base-offset" || a == "--quad-base-limit" ||
                  a == "--require-all-regions" || a == "--max-hut-dist" || a == "--gate-center-x" || a == "--gate-center-z") {
             std::fprintf(stderr, "%s is not supported by this engine\n", a.c_str());
@@ -158,7 +171,7 @@ bool parse(const std::vector<std::string> &t, Params &p) {
     return true;
 }
 
-int run(whf_ctx *ctx, const Params &p) {
+whf_request make_request(const Params &p) {
     whf_request r;
     std::memset(&r, 0, sizeof r);
     r.seed_mode = p.seed_mode;
@@ -181,26 +194,16 @@ int run(whf_ctx *ctx, const Params &p) {
         // a version with a tree of its own selects it, a rule-set alias takes its default (eval.c:104-106)
         r.biome_tree = whf_biome_tree(p.version.empty() ? "1.19.x~26.1" : p.version.c_str(), &tree) == WHF_OK ? tree + 1 : 0;
     }
-    std::vector<whf_hit> hits(1 << 16);
-    whf_sink sink{hits.data(), hits.size(), 0};
-    whf_stats st;
-    int rc = whf_scan(ctx, &r, &sink, &st);
-    if (rc == WHF_EOVERFLOW) {
-        hits.resize(sink.count);
-        sink = whf_sink{hits.data(), hits.size(), 0};
-        rc = whf_scan(ctx, &r, &sink, &st);
-    }
-    if (rc != WHF_OK) {
-        std::fprintf(stderr, "scan failed: %s\n", whf_strerror(rc));
-        return 1;
-    }
-    hits.resize(std::min<uint64_t>(sink.count, hits.size()));
+    return r;
+}
+
+// The progress line, the --stats line and the --out CSV of a finished request (phase1_scan.cu:1328-1339).
+int report(const Params &p, std::vector<whf_hit> &hits, const whf_stats &st, double secs) {
     std::sort(hits.begin(), hits.end(), [](const whf_hit &a, const whf_hit &b) {
         if (a.seed != b.seed) return a.seed < b.seed;
         if (a.cx != b.cx) return a.cx < b.cx;
         return a.cz < b.cz;
     });
-    const double secs = st.device_ms / 1000.0;
     std::printf("kernel: %.3fs, %zu candidates, %.1f M region-seed-checks/s (%" PRIu64 " seeds x %" PRIu64 " regions)\n",
                 secs, hits.size(), secs > 0 ? (double)st.checks / secs / 1e6 : 0.0, st.seeds,
                 st.seeds ? st.checks / st.seeds : 0);
@@ -233,6 +236,128 @@ int run(whf_ctx *ctx, const Params &p) {
     return 0;
 }
 
+int run(whf_ctx *ctx, const Params &p) {
+    const whf_request r = make_request(p);
+    std::vector<whf_hit> hits(1 << 16);
+    whf_sink sink{hits.data(), hits.size(), 0};
+    whf_stats st;
+    int rc = whf_scan(ctx, &r, &sink, &st);
+    if (rc == WHF_EOVERFLOW) {
+        hits.resize(sink.count);
+        sink = whf_sink{hits.data(), hits.size(), 0};
+        rc = whf_scan(ctx, &r, &sink, &st);
+    }
+    if (rc != WHF_OK) {
+        std::fprintf(stderr, "scan failed: %s\n", whf_strerror(rc));
+        return 1;
+    }
+    hits.resize(std::min<uint64_t>(sink.count, hits.size()));
+    return report(p, hits, st, st.device_ms / 1000.0);
+}
+
+// --worker requests with --id: submitted to the serving loop, answered by a poller thread in completion order.
+// Output lines of the two threads go through one lock, each request's lines together.
+struct Async {
+    struct Pending {
+        Params p;
+        whf_request r;
+        std::vector<whf_hit> hits;
+        whf_sink sink;
+    };
+    whf_server *srv = nullptr;
+    std::mutex mu;                                   // pending and stdout
+    std::map<uint64_t, std::unique_ptr<Pending>> pending;
+    std::thread poller;
+    std::atomic<bool> closing{false};
+
+    // Opens the server on first use; false if the device cannot.
+    bool ensure() {
+        if (srv) return true;
+        whf_server_config cfg;
+        std::memset(&cfg, 0, sizeof cfg);
+        const int rc = whf_server_open(&cfg, &srv);
+        if (rc != WHF_OK) {
+            std::fprintf(stderr, "cannot open the serving loop: %s\n", whf_strerror(rc));
+            srv = nullptr;
+            return false;
+        }
+        poller = std::thread(&Async::poll_loop, this);
+        return true;
+    }
+
+    // mu held
+    int submit_locked(std::unique_ptr<Pending> q) {
+        q->sink = whf_sink{q->hits.data(), q->hits.size(), 0};
+        uint64_t sid = 0;
+        const int rc = whf_submit(srv, &q->r, &q->sink, &sid);
+        if (rc == WHF_OK) pending[sid] = std::move(q);
+        return rc;
+    }
+
+    void submit(const Params &p) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!ensure()) {
+            std::printf("===SCAN_DONE id=%s rc=1===\n", p.id.c_str());
+            return;
+        }
+        auto q = std::make_unique<Pending>();
+        q->p = p;
+        q->r = make_request(q->p);   // seeds point into q->p
+        q->hits.resize(1 << 16);
+        const int rc = submit_locked(std::move(q));
+        if (rc != WHF_OK) {
+            std::fprintf(stderr, "request %s refused: %s\n", p.id.c_str(), whf_strerror(rc));
+            std::printf("===SCAN_DONE id=%s rc=1===\n", p.id.c_str());
+        }
+    }
+
+    void poll_loop() {
+        whf_event ev[16];
+        for (;;) {
+            const int n = whf_poll(srv, ev, 16, 50);
+            if (n < 0) return;
+            for (int k = 0; k < n; k++) {
+                std::lock_guard<std::mutex> lk(mu);
+                auto it = pending.find(ev[k].id);
+                if (it == pending.end()) continue;
+                std::unique_ptr<Pending> q = std::move(it->second);
+                pending.erase(it);
+                if (ev[k].status == WHF_EOVERFLOW && q->sink.count > q->hits.size()) {
+                    // more survivors than the first buffer: the same request again with room for all of them
+                    const std::string id = q->p.id;
+                    q->hits.resize(q->sink.count);
+                    if (submit_locked(std::move(q)) == WHF_OK) continue;
+                    std::printf("===SCAN_DONE id=%s rc=1===\n", id.c_str());
+                    std::fflush(stdout);
+                    continue;
+                }
+                int rc = 1;
+                if (ev[k].status == WHF_OK) {
+                    q->hits.resize(std::min<uint64_t>(q->sink.count, q->hits.size()));
+                    rc = report(q->p, q->hits, ev[k].stats, ev[k].done_s - ev[k].start_s);
+                } else {
+                    std::fprintf(stderr, "request %s failed: %s\n", q->p.id.c_str(), whf_strerror(ev[k].status));
+                }
+                std::printf("===SCAN_DONE id=%s rc=%d===\n", q->p.id.c_str(), rc);
+                std::fflush(stdout);
+            }
+            if (closing) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (pending.empty()) return;
+            }
+        }
+    }
+
+    // End of input: every submitted request is answered, then the loop closes.
+    void drain() {
+        if (!srv) return;
+        closing = true;
+        poller.join();
+        whf_server_close(srv);
+        srv = nullptr;
+    }
+};
+
 std::vector<std::string> tokenize(std::string line) {
     if (line.size() >= 3 && (unsigned char)line[0] == 0xEF && (unsigned char)line[1] == 0xBB &&
         (unsigned char)line[2] == 0xBF)
@@ -264,6 +389,7 @@ int main(int argc, char **argv) {
             if (t == "--help" || t == "-h") { usage(); return 0; }
         Params p;
         if (!parse(tokens, p)) { usage(); return 2; }
+        if (!p.id.empty()) { std::fprintf(stderr, "--id is for --worker request lines\n"); usage(); return 2; }
         whf_ctx *ctx = nullptr;
         if (open_ctx(&ctx) != WHF_OK) return 1;
         int rc = run(ctx, p);
@@ -274,19 +400,30 @@ int main(int argc, char **argv) {
     if (open_ctx(&ctx) != WHF_OK) return 1;
     std::printf("===WORKER_READY===\n");
     std::fflush(stdout);
+    Async async;
     std::string line;
     while (std::getline(std::cin, line)) {
         if (line.empty()) continue;
         Params p;
         if (!parse(tokenize(line), p)) {
+            std::lock_guard<std::mutex> lk(async.mu);
             std::printf("===SCAN_ERROR: bad request line===\n");
             std::fflush(stdout);
             continue;
         }
+        if (!p.id.empty()) {
+            async.submit(p);
+            std::fflush(stdout);
+            continue;
+        }
+        // a sequential request runs on the worker's own context; its lines stay together
+        std::unique_lock<std::mutex> lk(async.mu, std::defer_lock);
+        if (async.srv) lk.lock();
         int rc = run(ctx, p);
         std::printf("===SCAN_DONE rc=%d===\n", rc);
         std::fflush(stdout);
     }
+    async.drain();
     whf_close(ctx);
     return 0;
 }

@@ -57,7 +57,8 @@ def run_cli(*args, stdin=None):
 def test_cli_argument_errors_exit_2_before_touching_the_gpu():
     for args in (["--bogus"], ["--seed"], ["--min-region-x", "3", "--max-region-x", "3", "--seed", "1"],
                  ["--self-test", "--seed", "1"], ["--quad-bases", "x.txt"], ["--version", "1.17", "--seed", "1"],
-                 ["--seed-gen", "1", "0"], ["--seed-gen", "5", "10", "--seed", "3"], ["--max-height"]):
+                 ["--seed-gen", "1", "0"], ["--seed-gen", "5", "10", "--seed", "3"], ["--max-height"],
+                 ["--id", "7", "--seed", "1"], ["--seed", "1", "--id"]):
         r = run_cli(*args)
         assert r.returncode == 2, (args, r.returncode, r.stderr)
         assert "usage: phase1_scan" in r.stdout

@@ -119,3 +119,48 @@ def test_worker_protocol_as_hunt_drives_it(tmp_path):
             w.p.kill()
             w.p.wait()
         w.log.close()
+
+
+@pytest.mark.timeout(300)
+def test_worker_async_requests_with_ids(tmp_path):
+    """Request lines with --id K go to the library's serving loop while the worker reads on: three are in flight at
+    once, each answered by ===SCAN_DONE id=K rc=0=== after its own CSV is written, and that CSV holds the oracle's
+    survivors. A sequential line among them keeps its plain ===SCAN_DONE rc=0=== answer. At end of input the worker
+    answers every pending request before it exits 0."""
+    w = Worker(str(tmp_path))
+    try:
+        assert w.wait("===WORKER_READY===") == "===WORKER_READY==="
+        neg = -3339079979306198497
+        jobs = {"a": (KNOWN, (-79, 79, -79, 79)), "b": (neg, (-16, 16, -16, 16)), "c7": (KNOWN, (-20, 5, -3, 30))}
+        for k, (seed, rect) in jobs.items():
+            line = request(seed, rect).replace("--out cands.csv", "--out cands_%s.csv" % k)
+            print("async request %s" % k, flush=True)
+            w.send("--id %s %s" % (k, line))
+        print("sequential request", flush=True)
+        w.send(request(KNOWN, (-8, 8, -8, 8)))
+        done, plain = {}, 0
+        while len(done) < len(jobs) or plain < 1:
+            line = w.wait("===SCAN_")
+            assert not line.startswith("===SCAN_ERROR"), line
+            if line.startswith("===SCAN_DONE id="):
+                k, rc = line[len("===SCAN_DONE id="):-3].split(" rc=")
+                done[k] = int(rc)
+            else:
+                assert line == "===SCAN_DONE rc=0===", line
+                plain += 1
+        assert done == {k: 0 for k in jobs}
+        for k, (seed, rect) in jobs.items():
+            assert read_cands(tmp_path / ("cands_%s.csv" % k)) == oracle_cands(seed, rect), k
+        assert read_cands(tmp_path / "cands.csv") == oracle_cands(KNOWN, (-8, 8, -8, 8))
+        # a request sent last, then end of input: answered before the exit
+        w.send("--id last " + request(neg, (-16, 16, -16, 16)).replace("cands.csv", "cands_last.csv"))
+        w.p.stdin.close()
+        rest = [ln.strip() for ln in w.p.stdout]
+        assert "===SCAN_DONE id=last rc=0===" in rest, rest[-5:]
+        assert w.p.wait(60) == 0
+        assert read_cands(tmp_path / "cands_last.csv") == oracle_cands(neg, (-16, 16, -16, 16))
+    finally:
+        if w.p.poll() is None:
+            w.p.kill()
+            w.p.wait()
+        w.log.close()
