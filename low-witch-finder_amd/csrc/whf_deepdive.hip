@@ -264,19 +264,7 @@ __global__ __launch_bounds__(64) void k_dd_erosion(const uint64_t *__restrict__ 
             s.ox = xr_next_double(pr) * 256.0;
             s.oy = xr_next_double(pr) * 256.0;
             s.oz = xr_next_double(pr) * 256.0;
-#pragma unroll 8
-            for (int w = 0; w < 64; w++) words[w * 64 + lane] = 0x03020100u + (uint32_t)w * 0x04040404u;
-            uint32_t ai = lane * 4;
-            uint32_t jj = xr_next_bounded(pr, 256u);
-            for (int i = 0; i < 256; i++) {
-                const uint32_t aj = ((jj >> 2) << 8) + lane * 4 + (jj & 3);
-                const uint8_t vi = bytes[ai];
-                const uint8_t vj = bytes[aj];
-                if (i < 255) jj = (uint32_t)(i + 1) + xr_next_bounded(pr, 255u - (uint32_t)i);
-                bytes[ai] = vj;
-                bytes[aj] = vi;
-                ai += ((i & 3) == 3) ? 256 - 3 : 1;
-            }
+            shuffle_perm(pr, words, lane);
             const double fj = floor(s.oy);
             s.jy = (int)fj;
             s.hy = s.oy - fj;

@@ -59,6 +59,29 @@ __device__ __forceinline__ uint32_t xr_next_bounded(Xr &s, uint32_t bound) {
     return (uint32_t)(m >> 32);
 }
 
+// The same engine as four 32-bit words (lo = a1:a0, hi = b1:b0) for the permutation shuffle: each 64-bit rotation is
+// two alignbits, and a draw yields only the low word of nextLong, which is what nextInt uses.
+struct Xw {
+    uint32_t a0, a1, b0, b1;
+};
+__device__ __forceinline__ Xw xw_of(const Xr &s) {
+    return Xw{(uint32_t)s.lo, (uint32_t)(s.lo >> 32), (uint32_t)s.hi, (uint32_t)(s.hi >> 32)};
+}
+__device__ __forceinline__ Xr xr_of(const Xw &s) {
+    return Xr{((uint64_t)s.a1 << 32) | s.a0, ((uint64_t)s.b1 << 32) | s.b0};
+}
+__device__ __forceinline__ uint32_t xw_next32(Xw &s) {
+    const uint64_t sum = (((uint64_t)s.a1 << 32) | s.a0) + (((uint64_t)s.b1 << 32) | s.b0);
+    const uint32_t out = __builtin_amdgcn_alignbit((uint32_t)sum, (uint32_t)(sum >> 32), 15) + s.a0;   // rotl 17
+    const uint32_t h0 = s.b0 ^ s.a0, h1 = s.b1 ^ s.a1;                                                 // hi ^= lo
+    const uint32_t r0 = __builtin_amdgcn_alignbit(s.a1, s.a0, 15), r1 = __builtin_amdgcn_alignbit(s.a0, s.a1, 15);
+    s.a0 = r0 ^ h0 ^ (h0 << 21);                                        // rotl(lo, 49) ^ hi ^ (hi << 21)
+    s.a1 = r1 ^ h1 ^ __builtin_amdgcn_alignbit(h1, h0, 11);
+    s.b0 = __builtin_amdgcn_alignbit(h0, h1, 4);                        // rotl(hi, 28)
+    s.b1 = __builtin_amdgcn_alignbit(h1, h0, 4);
+    return out;
+}
+
 __device__ __forceinline__ double xr_next_double(Xr &s) {
     return (double)(xr_next(s) >> 11) * 0x1.0p-53;
 }
@@ -146,30 +169,78 @@ __device__ __forceinline__ double grad(int hash, double x, double y, double z) {
 __device__ __forceinline__ double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
 __device__ __forceinline__ double lerp(double t, double a, double b) { return a + t * (b - a); }
 
-// One Perlin sampler from random pr (PerlinNoiseSampler(random), noise.c:61-78): the origin, then Fisher-Yates over
-// 0..255 with nextInt(256 - i). The permutation is shuffled in the calling lane's column of a 64-lane LDS area laid
-// out lane-interleaved (word w of lane l at words[w * 64 + l]), so the 64 lanes' byte accesses hit 64 distinct
-// banks whatever the swap index; the table then goes to out_tab (256 bytes) and the y constants to out.
-__device__ __forceinline__ void build_sampler(Xr &pr, uint32_t *words, uint32_t lane, uint8_t *out_tab, SPar &out) {
+// Fisher-Yates of PerlinNoiseSampler(random) (noise.c:61-78) over 0..255 with nextInt(256 - i), in the calling lane's
+// column of a 64-lane LDS area laid out lane-interleaved (word w of lane l at words[w * 64 + l]), so the 64 lanes'
+// byte accesses hit 64 distinct banks whatever the swap index. pr ends where the reference's random does.
+__device__ __forceinline__ void shuffle_perm(Xr &pr, uint32_t *words, uint32_t lane) {
     uint8_t *bytes = reinterpret_cast<uint8_t *>(words);
+    const uint32_t base = lane * 4;   // byte address of position 0 in this lane's column
+    auto identity = [&] {
+#pragma unroll 8
+        for (int w = 0; w < 64; w++) words[w * 64 + lane] = 0x03020100u + (uint32_t)w * 0x04040404u;
+    };
+    auto addr = [&](uint32_t j) { return ((j << 6) & 0xFFFFFF00u) | ((j & 3u) | base); };
+    identity();
+    // Fast path: nextInt(b) with no rejection test. A draw can only be rejected when the low word of its product
+    // with b is below b <= 256 (rng.c:124-146), so the lane tracks the smallest such word; if it ends below 256 the
+    // lane replays the shuffle below with the exact nextInt. The draw of step i + 1 is computed between the reads
+    // and the writes of step i: it does not depend on the table, so it fills the LDS read latency. Four steps per
+    // iteration keep the in-word byte position a constant offset.
+    const Xr start = pr;
+    Xw w = xw_of(pr);
+    const uint32_t x0 = xw_next32(w);
+    uint32_t lo_min = x0 << 8;        // the low word of x0 * 256
+    uint32_t jj = x0 >> 24;           // nextInt(256)
+    // step i (i < 255) draws nextInt(b) for step i + 1 with b = 255 - i; the 32-bit product halves are taken
+    // explicitly, so the multiply stays one 32 x 32 -> 64 operation
+    auto step = [&](uint32_t ai, uint32_t b, bool draw) {
+        const uint32_t aj = addr(jj);
+        const uint8_t vi = bytes[ai];
+        const uint8_t vj = bytes[aj];
+        if (draw) {
+            const uint32_t x = xw_next32(w);
+            lo_min = min(lo_min, x * b);
+            jj = (256u - b) + __umulhi(x, b);
+        }
+        bytes[ai] = vj;
+        bytes[aj] = vi;
+    };
+    uint32_t aq = base, bq = 255u;
+    for (int q = 0; q < 63; q++, aq += 256u, bq -= 4u) {
+        // an opaque bound: otherwise the compiler turns the products into 64-bit induction variables and full
+        // 64-bit multiplies
+        asm volatile("" : "+s"(bq));
+#pragma unroll
+        for (int r = 0; r < 4; r++) step(aq + (uint32_t)r, bq - (uint32_t)r, true);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) step(aq + (uint32_t)r, 3u - (uint32_t)r, r < 3);
+    pr = xr_of(w);
+    if (lo_min < 256u) {
+        // exact replay (odds about 2^-16 per sampler): 256 draws nextInt(256 - i), as above but with the rejection
+        pr = start;
+        identity();
+        uint32_t ai = base;
+        uint32_t j = xr_next_bounded(pr, 256u);
+        for (int i = 0; i < 256; i++) {
+            const uint32_t aj = addr(j);
+            const uint8_t vi = bytes[ai];
+            const uint8_t vj = bytes[aj];
+            if (i < 255) j = (uint32_t)(i + 1) + xr_next_bounded(pr, 255u - (uint32_t)i);
+            bytes[ai] = vj;
+            bytes[aj] = vi;
+            ai += ((i & 3) == 3) ? 256 - 3 : 1;
+        }
+    }
+}
+
+// One Perlin sampler from random pr (PerlinNoiseSampler(random), noise.c:61-78): the origin, then the permutation
+// (shuffle_perm); the table then goes to out_tab (256 bytes) and the y constants to out.
+__device__ __forceinline__ void build_sampler(Xr &pr, uint32_t *words, uint32_t lane, uint8_t *out_tab, SPar &out) {
     const double ox = xr_next_double(pr) * 256.0;
     const double oy = xr_next_double(pr) * 256.0;
     const double oz = xr_next_double(pr) * 256.0;
-#pragma unroll 8
-    for (int w = 0; w < 64; w++) words[w * 64 + lane] = 0x03020100u + (uint32_t)w * 0x04040404u;
-    uint32_t ai = lane * 4;   // byte address of position i (i = 0)
-    // The draw of step i + 1 is computed between the reads and the writes of step i: it does not depend on the
-    // table, so it fills the LDS read latency.
-    uint32_t jj = xr_next_bounded(pr, 256u);
-    for (int i = 0; i < 256; i++) {
-        const uint32_t aj = ((jj >> 2) << 8) + lane * 4 + (jj & 3);
-        const uint8_t vi = bytes[ai];
-        const uint8_t vj = bytes[aj];
-        if (i < 255) jj = (uint32_t)(i + 1) + xr_next_bounded(pr, 255u - (uint32_t)i);
-        bytes[ai] = vj;
-        bytes[aj] = vi;
-        ai += ((i & 3) == 3) ? 256 - 3 : 1;
-    }
+    shuffle_perm(pr, words, lane);
     uint4 *dst = reinterpret_cast<uint4 *>(out_tab);
 #pragma unroll
     for (int q = 0; q < 16; q++)
