@@ -75,8 +75,9 @@ __device__ __forceinline__ uint32_t xw_next32(Xw &s) {
     const uint32_t out = __builtin_amdgcn_alignbit((uint32_t)sum, (uint32_t)(sum >> 32), 15) + s.a0;   // rotl 17
     const uint32_t h0 = s.b0 ^ s.a0, h1 = s.b1 ^ s.a1;                                                 // hi ^= lo
     const uint32_t r0 = __builtin_amdgcn_alignbit(s.a1, s.a0, 15), r1 = __builtin_amdgcn_alignbit(s.a0, s.a1, 15);
-    s.a0 = r0 ^ h0 ^ (h0 << 21);                                        // rotl(lo, 49) ^ hi ^ (hi << 21)
-    s.a1 = r1 ^ h1 ^ __builtin_amdgcn_alignbit(h1, h0, 11);
+    // rotl(lo, 49) ^ hi ^ (hi << 21): three-input XOR (truth table 0x96) in one instruction
+    s.a0 = __builtin_amdgcn_bitop3_b32(r0, h0, h0 << 21, 0x96);
+    s.a1 = __builtin_amdgcn_bitop3_b32(r1, h1, __builtin_amdgcn_alignbit(h1, h0, 11), 0x96);
     s.b0 = __builtin_amdgcn_alignbit(h0, h1, 4);                        // rotl(hi, 28)
     s.b1 = __builtin_amdgcn_alignbit(h1, h0, 4);
     return out;
@@ -179,7 +180,8 @@ __device__ __forceinline__ void shuffle_perm(Xr &pr, uint32_t *words, uint32_t l
 #pragma unroll 8
         for (int w = 0; w < 64; w++) words[w * 64 + lane] = 0x03020100u + (uint32_t)w * 0x04040404u;
     };
-    auto addr = [&](uint32_t j) { return ((j << 6) & 0xFFFFFF00u) | ((j & 3u) | base); };
+    // byte j of this lane's column: word j >> 2 (j < 256) at 256-byte stride, byte j & 3 within it
+    auto addr = [&](uint32_t j) { return (__builtin_amdgcn_ubfe(j, 2u, 6u) << 8) + ((j & 3u) | base); };
     identity();
     // Fast path: nextInt(b) with no rejection test. A draw can only be rejected when the low word of its product
     // with b is below b <= 256 (rng.c:124-146), so the lane tracks the smallest such word; if it ends below 256 the
@@ -191,16 +193,15 @@ __device__ __forceinline__ void shuffle_perm(Xr &pr, uint32_t *words, uint32_t l
     const uint32_t x0 = xw_next32(w);
     uint32_t lo_min = x0 << 8;        // the low word of x0 * 256
     uint32_t jj = x0 >> 24;           // nextInt(256)
-    // step i (i < 255) draws nextInt(b) for step i + 1 with b = 255 - i; the 32-bit product halves are taken
-    // explicitly, so the multiply stays one 32 x 32 -> 64 operation
+    // step i (i < 255) draws nextInt(b) for step i + 1 with b = 255 - i
     auto step = [&](uint32_t ai, uint32_t b, bool draw) {
         const uint32_t aj = addr(jj);
         const uint8_t vi = bytes[ai];
         const uint8_t vj = bytes[aj];
         if (draw) {
-            const uint32_t x = xw_next32(w);
-            lo_min = min(lo_min, x * b);
-            jj = (256u - b) + __umulhi(x, b);
+            const uint64_t m = (uint64_t)xw_next32(w) * b;
+            lo_min = min(lo_min, (uint32_t)m);
+            jj = (256u - b) + (uint32_t)(m >> 32);
         }
         bytes[ai] = vj;
         bytes[aj] = vi;
@@ -208,7 +209,7 @@ __device__ __forceinline__ void shuffle_perm(Xr &pr, uint32_t *words, uint32_t l
     uint32_t aq = base, bq = 255u;
     for (int q = 0; q < 63; q++, aq += 256u, bq -= 4u) {
         // an opaque bound: otherwise the compiler turns the products into 64-bit induction variables and full
-        // 64-bit multiplies
+        // 64-bit multiplies; opaque, each product is one 32 x 32 -> 64 multiply
         asm volatile("" : "+s"(bq));
 #pragma unroll
         for (int r = 0; r < 4; r++) step(aq + (uint32_t)r, bq - (uint32_t)r, true);
