@@ -644,7 +644,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="use the library's default launch shapes")
-    ap.add_argument("--tune-seeds", type=int, default=1 << 23, help="seeds of the autotuner's timing workload")
+    ap.add_argument("--tune-seeds", type=int, default=1 << 24, help="seeds of the autotuner's timing workload")
     ap.add_argument("--product-seeds", type=int, default=1024,
                     help="seeds of the product-mode CPU timing (one lysh process per seed)")
     ap.add_argument("--dd-seeds", type=int, default=1 << 28, help="structure seeds of the Deep Dive sweep")

@@ -1,16 +1,18 @@
 """Autotuner: measures the library's launch shapes (whf_tuning) on this device and writes tuned.json.
 
 For each game version's rules (1.18, 1.20 and 1.21 by default; 1.20 and 1.21 share theirs), a greedy search times
-whf_scan on a Seed Check workload of the benchmark's shape (2^23 SplitMix64(2) seeds x regions [-8, 7]^2, threshold
+whf_scan on a Seed Check workload of the benchmark's shape (2^24 SplitMix64(2) seeds x regions [-8, 7]^2, threshold
 -52 or-lower) under each candidate: first the erosion gate's shape (items per lane, workgroup size, persistent grid),
-then the grid of gates 2 and 3 with the best erosion shape. Every candidate must give the default's exact answer
+then the grid of gates 2 and 3 with the best erosion shape. A candidate replaces the best so far only when it is faster
+by more than the margin (default 2%): one shape's time varies by about that much from run to run, and a shape that
+wins a small workload by less has lost the full one. Every candidate must give the default's exact answer
 (finds, funnel, tier histogram), or the tuner stops. Then the server's chunk size: the smallest chunk whose serving
 capacity (a fixed set of requests submitted at once) is within 5% of the best measured, since larger chunks raise
 latency for little capacity. whf_open applies the device's entry when its "build" matches the library
 (whf_build_id). hunt.py keeps a tuned configuration per GPU the same way (hunt.py:927-945, 959-1021).
 
     python tune.py [--out FILE] [--seeds N] [--versions 1.18,1.20,1.21] [--reps 2] [--precision fp32|fp64]
-                   [--no-serving] [--serving-requests N]
+                   [--margin 0.02] [--no-serving] [--serving-requests N]
 
 The default output is tuned.json beside libwhf_hip.so, where whf_open looks unless WHF_TUNED names another file;
 entries of other devices already in the file are kept.
@@ -62,7 +64,7 @@ def time_scan(version, n_seeds, reps, precision, **tuning):
     return best, answer
 
 
-def tune_version(version, n_seeds, reps, precision):
+def tune_version(version, n_seeds, reps, precision, margin=0.02):
     base = whf.Context(0)
     vf = whf.version_flags(version)
     cur = base.tuning(vf)
@@ -81,7 +83,7 @@ def tune_version(version, n_seeds, reps, precision):
             raise SystemExit("tuning candidate %r changed the answer for %s" % (cand, version))
         log.append(dict(shape=dict(cand), ms=round(t * 1e3, 3)))
         note("%s: %r %.1f ms" % (version, (wg, ilp, per_cu), t * 1e3))
-        if t < best_t:
+        if t < best_t * (1.0 - margin):
             best_t, best = t, cand
     for g in GATE_CANDIDATES:
         cand = dict(best, gate_wg_per_cu=g)
@@ -92,7 +94,7 @@ def tune_version(version, n_seeds, reps, precision):
             raise SystemExit("tuning candidate %r changed the answer for %s" % (cand, version))
         log.append(dict(shape=dict(cand), ms=round(t * 1e3, 3)))
         note("%s: gate %d/CU %.1f ms" % (version, g, t * 1e3))
-        if t < best_t:
+        if t < best_t * (1.0 - margin):
             best_t, best = t, cand
     return best, dict(default_ms=round(ref_t * 1e3, 3), tuned_ms=round(best_t * 1e3, 3), candidates=log)
 
@@ -127,10 +129,11 @@ def serving_capacity(chunk, reqs):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--out", default=os.path.join(HERE, "tuned.json"))
-    ap.add_argument("--seeds", type=int, default=1 << 23)
+    ap.add_argument("--seeds", type=int, default=1 << 24)
     ap.add_argument("--versions", default="1.18,1.20,1.21")
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--margin", type=float, default=0.02, help="the least relative gain that replaces the best shape")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=200)
     a = ap.parse_args(argv)
@@ -143,7 +146,7 @@ def main(argv=None):
         layouts.setdefault(whf.version_flags(v), []).append(v)
     results = {}
     for vf, names in layouts.items():
-        best, meas = tune_version(names[0], a.seeds, a.reps, precision)
+        best, meas = tune_version(names[0], a.seeds, a.reps, precision, a.margin)
         results[vf] = (names, best, meas)
     # the entry's base shapes: those of the default rules (vflags 0) if tuned, else the first version's
     base_vf = 0 if 0 in results else next(iter(results))
