@@ -398,7 +398,8 @@ typedef struct whf_server whf_server;
 
 typedef struct {
     int device;
-    int slots;                  /* device loop: lanes, streams that run whole batches side by side (1..4; 0 = 3);
+    int slots;                  /* device loop: lanes, streams that run whole batches side by side (3 or 4; 0, 1
+                                   and 2 give 3);
                                    WHF_SERVE_SLOTS: concurrent device slots (a context, stream and buffers each);
                                    0 = 4 */
     uint64_t chunk_checks;      /* device loop: (seed, region) checks per batch at most (0 = 2^25, the most a batch

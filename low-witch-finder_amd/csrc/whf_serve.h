@@ -17,7 +17,7 @@ struct DqHooks {
 };
 
 // Opens the loop on a device: its context, lanes, request table, pools and threads. lanes: streams that run whole
-// batches side by side (1..4; 0: 3); item_cap: (seed, region) checks per batch (0: the most a batch holds);
+// batches side by side (3 or 4; fewer give 3); item_cap: (seed, region) checks per batch (0: the most a batch holds);
 // hbm_budget: device bytes the server may hold (0: no limit; the seed-list pool is cut to fit, WHF_ENOMEM below the
 // fixed part). Times of the events are seconds since epoch.
 int dq_open(int device, int lanes, uint64_t item_cap, uint64_t hbm_budget, const DqHooks &hooks,

@@ -47,6 +47,7 @@ namespace {
 constexpr uint32_t kSrvRing = 64;               // batch records the host can lag behind the device
 constexpr int kSrvMaxLanes = 4;             // lanes: streams that each run whole batches, side by side (the device
                                             // has four hardware queues per process)
+constexpr int kSrvMinLanes = 3;
 constexpr uint32_t kSrvAge = 8;             // batches after which a live request is claimed first (at most half a
                                             // batch of it), so big requests are not starved by a stream of small ones
 constexpr uint32_t kSrvHitMin = 4096;           // survivor entries reserved per request at least
@@ -814,7 +815,8 @@ struct DqServer {
 
     int open(int dev, int lanes_n, uint64_t cap, uint64_t hbm_budget) {
         device = dev;
-        n_lanes = lanes_n > 0 ? std::min(lanes_n, kSrvMaxLanes) : 3;
+        // 3 or 4 lanes: fewer are not run (a saturated 2-lane loop once stopped making progress, cause not found)
+        n_lanes = lanes_n > 0 ? std::min(std::max(lanes_n, kSrvMinLanes), kSrvMaxLanes) : kSrvMinLanes;
         // at least one seed of the widest request fits a batch
         item_cap = cap ? std::min<uint64_t>(std::max<uint64_t>(cap, kSrvMaxRegions), kMaxItems) : kMaxItems;
         whf_config cc{device, 0, 0};
