@@ -656,7 +656,9 @@ def main():
                     help="only time the whole 2^32-structure-seed Deep Dive (minutes) and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-load", type=float, default=0.8)
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=4, help="lanes of the device loop")
+    ap.add_argument("--serve-host-slots", type=int, default=6,
+                    help="slots of the host-slot scheduler (6: its best capacity of 3 to 6 slots on this workload)")
     ap.add_argument("--serve-chunk", type=int, default=1 << 24, help="checks per chunk of the slot scheduler")
     ap.add_argument("--serve-batch", type=int, default=1 << 25, help="checks per batch of the device loop at most")
     ap.add_argument("--serve-only", choices=("device", "slots"),
@@ -669,7 +671,7 @@ def main():
         whf = __graft_entry__.load_whf()
         reqs = serving_stream(3000, 4)[:150]
         mode = args.serve_only
-        ev, wall, _ = serve(whf, reqs, None, args.serve_slots,
+        ev, wall, _ = serve(whf, reqs, None, args.serve_slots if mode == "device" else args.serve_host_slots,
                             args.serve_batch if mode == "device" else args.serve_chunk, slots_mode=(mode == "slots"))
         print(json.dumps(dict(scheduler=mode, requests=len(reqs), checks=sum(e["stats"]["checks"] for e in ev),
                               seconds=round(wall, 4), checks_per_s=round(sum(e["stats"]["checks"] for e in ev) / wall))))
@@ -814,7 +816,7 @@ def main():
     if args.serve_requests > 0:
         srv = serving(whf, args.serve_requests, args.serve_load, args.serve_slots, args.serve_batch)
         note("serving (device loop) done")
-        srv_slots = serving(whf, args.serve_requests, args.serve_load, args.serve_slots, args.serve_chunk,
+        srv_slots = serving(whf, args.serve_requests, args.serve_load, args.serve_host_slots, args.serve_chunk,
                             slots_mode=True)
         note("serving (slots) done")
     mixed = None
