@@ -1396,9 +1396,14 @@ int stage_e(whf_ctx *c, const whf_request *req, int layout, uint32_t ns, uint64_
         // seeds per tile: narrow rects want many seeds per workgroup to fill the lanes
         const bool wide = fn >= 64;
         const int ST = wide ? 4 : 8;
-        const uint32_t chunk = std::min<uint32_t>(fn, 1u << 16);
-        const uint32_t n_chunks = (fn + chunk - 1) / chunk;
         const uint32_t tiles = (ns + ST - 1) / ST;
+        // regions per block: up to 2^16, but a batch of few seeds (few tiles) is cut finer so that it still has 8
+        // blocks per CU (one seed over the whole world would otherwise be 512 blocks of 2^16 regions a batch)
+        const uint32_t want = (uint32_t)c->n_cu * 8;
+        uint32_t chunk = std::min<uint32_t>(fn, 1u << 16);
+        if ((uint64_t)tiles * ((fn + chunk - 1) / chunk) < want)
+            chunk = std::min(chunk, std::max<uint32_t>((uint32_t)kEroPass, (uint32_t)(((uint64_t)fn * tiles + want - 1) / want)));
+        const uint32_t n_chunks = (fn + chunk - 1) / chunk;
         // persistent: ero_wg_per_cu workgroups per compute unit loop over the blocks (-1: one workgroup per block)
         const uint32_t blocks = tiles * n_chunks;
         const int per_cu = c->tune[layout].ero_wg_per_cu;
