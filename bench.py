@@ -441,6 +441,45 @@ def deep_dive_full(whf):
                 finds_list=[list(f) for f in finds[:64]], overflow=st["overflow"])
 
 
+WORLD_RECT = (-58594, 58594, -58594, 58594)   # hunt.py:72 WORLD: hut regions of the +-30M world, [low, high)
+# docs/HUNT54.txt:19-20 (and hunt.py:86 SCAN_REFERENCE): whole-world seeds with one confirmed hut each at the -54 hunt
+WORLD_SEEDS = [(1473884015873122059, (-15792, -54, -17168)), (1291329195661654798, (2137600, -54, 26493184))]
+
+
+def world_seeds(whf):
+    """The reference's per-seed workload: one seed over the whole world (1.37e10 hut regions), phase 1 and lysh's hut
+    verdict at -54 or-lower, for the two seeds its docs give with their confirmed huts; one scan per seed and precision
+    after a warm-up. The reference quotes about 9.7 s per such seed for its optimised scanner on a 188-SM workstation
+    GPU and 58.4 s on a 20-SM laptop GPU (docs/HUNT54.txt:10-16)."""
+    import numpy as np
+    vf = whf.version_flags("1.20")
+    tree = whf.biome_tree("1.20")
+    regions = (WORLD_RECT[1] - WORLD_RECT[0]) * (WORLD_RECT[3] - WORLD_RECT[2])
+    secs = {}
+    found = True
+    with whf.Context(0) as ctx:
+        ctx.scan((-8, 8, -8, 8), seeds=np.array([1], dtype=np.int64))   # warm-up
+        for prec in (whf.PREC_FP64, whf.PREC_FP32_GATE):
+            t = 0.0
+            for seed, hut in WORLD_SEEDS:
+                t0 = time.perf_counter()
+                hits, st = ctx.scan(WORLD_RECT, seeds=np.array([seed], dtype=np.int64), max_height=-50, vflags=vf,
+                                    capacity=1 << 12, precision=prec, phase2=(whf.P2_OR_LOWER, -54, tree))
+                t += time.perf_counter() - t0
+                found = found and [(h[3] * 16, h[5], h[4] * 16) for h in hits] == [hut]
+            secs["fp64" if prec == whf.PREC_FP64 else "fp32_gate"] = t / len(WORLD_SEEDS)
+    best = min(secs.values())
+    note("whole-world seeds: %.3f s per seed" % best)
+    return dict(workload="one seed over the whole world (hut regions [-58594, 58594)^2, 1.373e10 regions), phase 1 "
+                         "at maxHeight -50, then lysh's hut verdict at -54 or-lower (1.19-26.1 rules); the two seeds "
+                         "of docs/HUNT54.txt with their confirmed huts",
+                seeds=[s for s, _ in WORLD_SEEDS], regions_per_seed=regions,
+                seconds_per_seed={k: round(v, 3) for k, v in secs.items()},
+                checks_per_s=round(regions / best, 1), known_huts_found=found,
+                reference_seconds_per_seed={"188-SM workstation GPU (docs/HUNT54.txt:16)": 9.7,
+                                            "20-SM laptop GPU (docs/HUNT54.txt:10-12)": 58.4})
+
+
 def deep_dive(whf, sweep_n, expand_n, cpu_seconds):
     """Deep Dive: room-floor sweep over sweep_n structure seeds, and the 2^16 expansion of the spots of the first
     expand_n; both from the first SplitMix64(3) output, hut regions [-2,1]^2, room floor <= -55, version 1.20."""
@@ -650,6 +689,7 @@ def main():
     ap.add_argument("--dd-seeds", type=int, default=1 << 28, help="structure seeds of the Deep Dive sweep")
     ap.add_argument("--dd-expand", type=int, default=1 << 22, help="structure seeds whose spots are expanded")
     ap.add_argument("--no-deep-dive", action="store_true")
+    ap.add_argument("--no-world", action="store_true", help="skip the whole-world single-seed scans")
     ap.add_argument("--solver-groups", type=int, default=256,
                     help="groups of the Deep Dive solver's timing (0: none)")
     ap.add_argument("--dd-full", action="store_true",
@@ -827,6 +867,7 @@ def main():
                         versions=("1.18", "1.20", "1.21.5"), precision=whf.PREC_FP32_GATE,
                         background=(splitmix_first(3) & ((1 << 48) - 1), args.mixed_dd_seeds))
         note("mixed serving done")
+    world = None if args.no_world else world_seeds(whf)
     dd = None if args.no_deep_dive else deep_dive(whf, args.dd_seeds, args.dd_expand,
                                                   0 if args.no_cpu else min(10.0, args.cpu_seconds))
     sol = None if args.solver_groups <= 0 else solver(whf, args.solver_groups,
@@ -869,6 +910,7 @@ def main():
         "serving": srv,
         "serving_slots": srv_slots,
         "serving_mixed": mixed,
+        "world_seed": world,
         "deep_dive": dd,
         "deep_dive_solver": sol,
     }
