@@ -331,6 +331,25 @@ __device__ __forceinline__ double grad_sel(uint32_t h, double x, double y, doubl
     return u + v;
 }
 
+// ((byte SEL of v) + a) & 255 in one instruction: an SDWA add that reads byte SEL of its first source and writes
+// byte 0 of the result with the other bits cleared. The permutation's lookup chain is all such index steps.
+template <int SEL>
+__device__ __forceinline__ uint32_t byte_add(uint32_t v, uint32_t a) {
+    static_assert(SEL == 0 || SEL == 1, "byte 0 or 1 of a pair");
+    uint32_t d;
+    if constexpr (SEL == 0)
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:BYTE_0 dst_unused:UNUSED_PAD src0_sel:BYTE_0 src1_sel:DWORD"
+            : "=v"(d) : "v"(v), "v"(a));
+    else
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:BYTE_0 dst_unused:UNUSED_PAD src0_sel:BYTE_1 src1_sel:DWORD"
+            : "=v"(d) : "v"(v), "v"(a));
+    return d;
+}
+
+// The pair tables the y = 0 evaluations read (LDS or HBM) share one interface: Q(i) is the byte pair
+// P(i & 255) | P((i + 1) & 255) << 8 for a lattice coordinate i, and Q.next<SEL>(pair, a) is Q((byte SEL of pair) + a),
+// the step from one level of the lattice hash to the next.
+
 // y = 0 Perlin evaluation of two independent samples at once (two tables, two origins, two points), written as
 // lock-step lanes of a 2-array so the compiler interleaves the two dependency chains. Q(i) returns the byte pair
 // P(i & 255) | P((i + 1) & 255) << 8 of the permutation; every lookup and every arithmetic step is the one of
@@ -355,16 +374,16 @@ __device__ __forceinline__ void perlin_y0_x2(const Pairs *Q, const SPar *const *
 #pragma unroll
     for (int t = 0; t < 2; t++) {
         const int j = s[t]->jy;
-        cd[t] = Q[t]((int)(ab[t] & 255u) + j);
-        ef[t] = Q[t]((int)(ab[t] >> 8) + j);
+        cd[t] = Q[t].template next<0>(ab[t], j);
+        ef[t] = Q[t].template next<1>(ab[t], j);
     }
     uint32_t h0[2], h1[2], h2[2], h3[2];
 #pragma unroll
     for (int t = 0; t < 2; t++) {
-        h0[t] = Q[t]((int)(cd[t] & 255u) + k[t]);   // P(c+k),  P(c+k+1)
-        h1[t] = Q[t]((int)(ef[t] & 255u) + k[t]);   // P(e+k),  P(e+k+1)
-        h2[t] = Q[t]((int)(cd[t] >> 8) + k[t]);     // P(d+k),  P(d+k+1)
-        h3[t] = Q[t]((int)(ef[t] >> 8) + k[t]);     // P(f+k),  P(f+k+1)
+        h0[t] = Q[t].template next<0>(cd[t], k[t]);   // P(c+k),  P(c+k+1)
+        h1[t] = Q[t].template next<0>(ef[t], k[t]);   // P(e+k),  P(e+k+1)
+        h2[t] = Q[t].template next<1>(cd[t], k[t]);   // P(d+k),  P(d+k+1)
+        h3[t] = Q[t].template next<1>(ef[t], k[t]);   // P(f+k),  P(f+k+1)
     }
 #pragma unroll
     for (int t = 0; t < 2; t++) {
@@ -490,6 +509,10 @@ __device__ __forceinline__ void perlin_y0_xn_q(const Pairs *Q, const EPar *const
     }
 #pragma unroll
     for (int t = 0; t < N; t++) ab[t] = Q[t](i[t]);
+    // the index steps stay in plain integer form here: the one-instruction steps (Q.next) take 2 of its 7
+    // instructions per evaluation off this blend but raise the FP32 erosion kernel from 109 to 117-130 VGPRs,
+    // which leaves no room beside its four waves per SIMD for the other stages' waves, and the pipelined step
+    // was 2 % slower
 #pragma unroll
     for (int t = 0; t < N; t++) {
         const int j = s[t]->jy;

@@ -80,6 +80,10 @@ struct GPair {
     __device__ __forceinline__ uint32_t operator()(int i) const {
         return (uint32_t)p[i & 255] | (uint32_t)p[(i + 1) & 255] << 8;
     }
+    template <int SEL>
+    __device__ __forceinline__ uint32_t next(uint32_t pair, int a) const {
+        return (*this)((int)((pair >> (8 * SEL)) & 255u) + a);
+    }
 };
 
 // Double-Perlin 3D sample over tables in HBM (noise.c:263-289).

@@ -113,6 +113,8 @@ __global__ __launch_bounds__(64) void k_build(const uint64_t *__restrict__ seeds
 struct LPair {
     const uint16_t *p;
     __device__ __forceinline__ uint32_t operator()(int i) const { return p[i & 255]; }
+    template <int SEL>
+    __device__ __forceinline__ uint32_t next(uint32_t pair, int a) const { return p[byte_add<SEL>(pair, (uint32_t)a)]; }
 };
 
 // Items per pass: a whole tile of 4 seeds x 256 regions, so that levels 2 and 3 still fill whole waves (measured
