@@ -150,7 +150,8 @@ struct SBatch {
     uint8_t lay_of[kMaxSeeds];   // rule layout (whf_request.vflags) of each slot
 };
 
-// What the served kernels read besides a batch's own buffers (whf_scan.hip's kernels take it by value).
+// What the served kernels read besides a batch's own buffers (whf_scan.hip's kernels take it by value). Their
+// gate-table argument is then the whole per-layout array (whf_ctx.d_gt), indexed by each request's rule layout.
 struct SrvView {
     const SReq *req;             // [kSrvLive]
     const uint16_t *rq_of;       // [kMaxSeeds]
@@ -158,7 +159,6 @@ struct SrvView {
     unsigned long long *rcnt;    // [kSrvLive][CNT_N]: the Cnt counters per request
     SState *state;               // [kSrvLive]
     DevHit *hit_pool;
-    const GateTables *gt;        // [kLayouts]
 };
 
 // Counts 1 into counter idx (a Cnt index; < 0: none) of the lane's request, with one atomic per distinct

@@ -224,7 +224,9 @@ __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ see
         if constexpr (SERVE) {
             if (tid == 0) s_blk = atomicAdd(tile_next, 1u);
             __syncthreads();
-            const uint32_t b = s_blk;
+            // one value for the whole workgroup: read as a scalar, so the tile's and its request's fields load
+            // into scalar registers, as the launch's block index does for the scan
+            const uint32_t b = __builtin_amdgcn_readfirstlane(s_blk);
             __syncthreads();
             return b;
         } else {
@@ -237,20 +239,25 @@ __global__ __launch_bounds__(WG) void k_erosion(const uint64_t *__restrict__ see
     const GateTables *gt;
     int rx0, rz0, salt, single;
     if constexpr (SERVE) {
+        // the request table is written by other kernels of the loop, so its fields load per lane; each is one
+        // value for the workgroup and is kept as a scalar, and the rule layout's tables are read through gt_ (the
+        // server's table array, read-only here) so that they load as scalars, as the scan's do
+        auto u = [](uint32_t v) -> uint32_t { return __builtin_amdgcn_readfirstlane(v); };
         const STile t = tiles[blk];
         const SReq &r = srv.req[t.req];
         req = t.req;
         seed0 = t.slot0;
         ns = t.ns;
         rbase = 0;
-        rn = r.R;
-        Rn = r.Rdiv;
-        Wd = r.Wdiv;
-        gt = srv.gt + r.vflags;
-        rx0 = r.rx0;
-        rz0 = r.rz0;
-        salt = r.salt;
-        single = (r.vflags & WHF_V_SINGLE) ? 1 : 0;
+        rn = u(r.R);
+        Rn = FastDiv{u(r.Rdiv.d), u(r.Rdiv.m), u(r.Rdiv.l)};
+        Wd = FastDiv{u(r.Wdiv.d), u(r.Wdiv.m), u(r.Wdiv.l)};
+        const int vflags = (int)u((uint32_t)r.vflags);
+        gt = gt_ + vflags;
+        rx0 = (int)u((uint32_t)r.rx0);
+        rz0 = (int)u((uint32_t)r.rz0);
+        salt = (int)u((uint32_t)r.salt);
+        single = (vflags & WHF_V_SINGLE) ? 1 : 0;
     } else {
         const uint32_t tile = blk / n_chunks;
         const uint32_t ck = blk - tile * n_chunks;
@@ -488,7 +495,7 @@ __global__ __launch_bounds__(256) void k_erosion_exact(const uint8_t *__restrict
         uint4 e = make_uint4(0, 0, 0, 0);
         if (q < n) {
             e = qx[q];
-            const EroDesc &ed = SERVE ? srv.gt[srv.lay_of[e.x]].ero : gt->ero;
+            const EroDesc &ed = SERVE ? gt[srv.lay_of[e.x]].ero : gt->ero;
             if constexpr (SERVE) req = srv.rq_of[e.x];
             const double x = (double)((int)e.y * 16 + 8) / 4.0, z = (double)((int)e.z * 16 + 8) / 4.0;
             double sum = 0.0;
@@ -631,9 +638,9 @@ __global__ __launch_bounds__(kGateWG) void k_gate2a(const GateTables *__restrict
             const int lay0 = act ? __shfl(lay, __ffsll((long long)act) - 1) : 0;
             if (__ballot(q < n && lay != lay0) == 0) {
                 const int L = __builtin_amdgcn_readfirstlane(lay0);
-                if (q < n) check(srv.gt + L, L);
+                if (q < n) check(gt_ + L, L);
             } else if (q < n) {
-                check(srv.gt + lay, lay);
+                check(gt_ + lay, lay);
             }
         } else if (q < n) {
             check(gt_, vflags_);
@@ -672,7 +679,7 @@ __global__ __launch_bounds__(kGateWG) void k_gate2b(const GateTables *__restrict
         uint4 e = make_uint4(0, 0, 0, 0);
         if (q < n) {
             e = qm[q];
-            const GateTables *gt = SERVE ? srv.gt + srv.lay_of[e.x] : gt_;
+            const GateTables *gt = SERVE ? gt_ + srv.lay_of[e.x] : gt_;
             if constexpr (SERVE) req = srv.rq_of[e.x];
             else atomicAdd(&s_cnt[0], 1u);   // reached the cave ladder
             CaveCtx cv{gt, C, SetView{nullptr, nullptr, 0}, slotC[e.x], 0};
@@ -746,7 +753,7 @@ __global__ __launch_bounds__(kGateWG) void k_gate3(const uint64_t *__restrict__ 
     auto rules_of = [&](uint32_t slot) {
         Rules r;
         const int vflags = SERVE ? (int)srv.lay_of[slot] : vflags_;
-        r.gt = SERVE ? srv.gt + vflags : gt_;
+        r.gt = SERVE ? gt_ + vflags : gt_;
         r.max_h = SERVE ? srv.req[srv.rq_of[slot]].max_height : max_h_;
         r.single = (vflags & WHF_V_SINGLE) != 0;
         r.n_checks = g3_layout(r.max_h, r.single, r.n_neg, r.n_flood);

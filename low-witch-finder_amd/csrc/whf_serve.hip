@@ -416,7 +416,6 @@ struct DqServer {
         v.rcnt = d_rcnt;
         v.state = d_state;
         v.hit_pool = reinterpret_cast<DevHit *>(h_hits);
-        v.gt = c->d_gt;
         return v;
     }
 
