@@ -509,7 +509,7 @@ __device__ __forceinline__ void perlin_y0_xn_q(const Pairs *Q, const EPar *const
     }
 #pragma unroll
     for (int t = 0; t < N; t++) ab[t] = Q[t](i[t]);
-    // the index steps stay in plain integer form here: the one-instruction steps (Q.next) take 2 of its 7
+    // the index steps stay in plain integer form here: the one-instruction steps (Q.next) take about 10
     // instructions per evaluation off this blend but raise the FP32 erosion kernel from 109 to 117-130 VGPRs,
     // which leaves no room beside its four waves per SIMD for the other stages' waves, and the pipelined step
     // was 2 % slower
