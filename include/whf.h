@@ -474,7 +474,7 @@ typedef struct {
     int ero_wg_per_cu;          /* erosion gate: persistent workgroups per compute unit, 1..64, or -1 for one
                                    workgroup per tile (default -1) */
     int gate_wg_per_cu;         /* workgroups per compute unit of the persistent gate-2 and gate-3 kernels, 1..32
-                                   (default 8) */
+                                   (default 16) */
     uint64_t chunk_checks;      /* serving: (seed, region) checks per scheduled chunk when whf_server_config leaves
                                    it 0 (default 2^24) */
     int reserved[8];

@@ -1520,7 +1520,9 @@ void tune_defaults(whf_tuning &t) {
     t.ero_wg = 256;
     t.ero_ilp = 2;
     t.ero_wg_per_cu = -1;
-    t.gate_wg_per_cu = 8;
+    // gates 2 and 3 wait on gathers: more resident workgroups than fit at once keep the CUs fed between batches
+    // (2^24-seed, 256-region scan: 4 per CU 308, 8 299, 12 297, 16 291, 20 292, 24 294, 32 293 ms)
+    t.gate_wg_per_cu = 16;
     t.chunk_checks = 1ull << 24;
 }
 

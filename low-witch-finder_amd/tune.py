@@ -34,7 +34,7 @@ RECT = (-8, 8, -8, 8)
 THRESHOLD = -52
 # (ero_wg, ero_ilp, ero_wg_per_cu): the default first
 ERO_CANDIDATES = [(256, 2, -1), (256, 1, -1), (512, 1, -1), (256, 2, 8), (256, 1, 10)]
-GATE_CANDIDATES = [8, 4, 16]
+GATE_CANDIDATES = [16, 8, 4]
 CHUNK_CANDIDATES = [1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26]
 
 

@@ -4,7 +4,7 @@ import json
 
 from product import whf
 
-DEFAULTS = dict(ero_wg=256, ero_ilp=2, ero_wg_per_cu=-1, gate_wg_per_cu=8, chunk_checks=1 << 24)
+DEFAULTS = dict(ero_wg=256, ero_ilp=2, ero_wg_per_cu=-1, gate_wg_per_cu=16, chunk_checks=1 << 24)
 
 
 def write(tmp_path, doc):
